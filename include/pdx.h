@@ -1,0 +1,196 @@
+/*
+ * pdx.h -- C ABI of libpdx, the MI355X (gfx950) NOSB peridynamics solver step.
+ *
+ * The library owns all device memory of a context; the caller owns its host
+ * arrays and data crosses only at the explicit set/get calls.  Every call
+ * returns PDX_OK (0) or an error code; pdx_last_error() gives the message of
+ * the last failing call on the calling thread.  One context per host thread;
+ * calls are synchronous on the context's own HIP stream.
+ *
+ * Particle arrays are in the caller's order (n rows); vectors are n*dim
+ * doubles, tensors n*dim*dim doubles row-major per particle.  Internally the
+ * particles are Morton-reordered; the reordering never shows at the ABI.
+ *
+ * A context holds one or more cases (independent bodies that share the
+ * launches).  pdx_create makes a single case; pdx_create_batch packs several
+ * cases of uneven size into the same arrays, with one residual, damping and
+ * convergence state per case.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository root, monostylegc/simul).
+ */
+#ifndef PDX_H
+#define PDX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDX_OK 0
+#define PDX_EINVAL 1   /* invalid argument: maps to ValueError / FEAValidationError */
+#define PDX_ESTATE 2   /* call out of order (e.g. forces before bonds): RuntimeError */
+#define PDX_EDEVICE 3  /* HIP runtime error or no usable GPU */
+#define PDX_ENOMEM 4   /* device allocation failed */
+#define PDX_EUNSUP 5   /* unsupported combination (e.g. J2 in 2-D) */
+
+/* constitutive models (pdx_set_materials) */
+#define PDX_MODEL_LINEAR_ELASTIC 0 /* backend/fea/peridynamics/core/nosb.py:211-227 */
+#define PDX_MODEL_NEO_HOOKEAN 1    /* backend/fea/fem/material/neo_hookean.py:117-127,176-180 */
+#define PDX_MODEL_J2 2             /* backend/fea/fem/material/j2_plasticity.py:202-249 */
+
+/* force formulations (pdx_set_formulation) */
+#define PDX_FORM_NOSB 0 /* nosb.py:192-275 correspondence + stabilisation */
+#define PDX_FORM_BOND 1 /* backend/fea/peridynamics/solver/quasi_static.py:119-143 PMB */
+
+/* dynamic-relaxation methods (pdx_dr_params.method) */
+#define PDX_DR_ADAPTIVE 0 /* adaptive DR (Kilic & Madenci 2010); no reference counterpart */
+#define PDX_DR_KINETIC 1  /* velocity Verlet + kinetic damping, nosb_solver.py:156-332 */
+
+/* singular shape-tensor test (pdx_shape_tensor) */
+#define PDX_SING_ABSOLUTE 0 /* |det K| > tol, the reference test nosb.py:89-94 */
+#define PDX_SING_RELATIVE 1 /* |det K| > tol * (tr K / dim)^dim, unit-independent */
+
+/* field ids for pdx_get / pdx_set (f64 unless noted) */
+enum {
+    PDX_FIELD_X = 0,      /* n*dim reference positions (get)                particles.py:37 */
+    PDX_FIELD_x = 1,      /* n*dim current positions X+u (get/set)         particles.py:38 */
+    PDX_FIELD_u = 2,      /* n*dim displacement (get/set)                   particles.py:225-227 */
+    PDX_FIELD_v = 3,      /* n*dim velocity (get/set)                       particles.py:39 */
+    PDX_FIELD_a = 4,      /* n*dim acceleration (kinetic DR) or previous force (adaptive DR) */
+    PDX_FIELD_f = 5,      /* n*dim force density of the last force evaluation particles.py:41 */
+    PDX_FIELD_VOLUME = 6, /* n (get/set)                                    particles.py:44 */
+    PDX_FIELD_MASS = 7,   /* n (get/set)                                    particles.py:46 */
+    PDX_FIELD_BULK = 8,   /* n (get/set)                                    particles.py:52 */
+    PDX_FIELD_SHEAR = 9,  /* n (get/set)                                    particles.py:53 */
+    PDX_FIELD_CSTAB = 10, /* n stabilisation micromodulus c_i (get/set)     nosb.py:335-346 */
+    PDX_FIELD_K = 11,     /* n*dim*dim shape tensor (get)                   particles.py:56 */
+    PDX_FIELD_KINV = 12,  /* n*dim*dim inverse (get)                        particles.py:57 */
+    PDX_FIELD_F = 13,     /* n*dim*dim deformation gradient (get)           particles.py:58 */
+    PDX_FIELD_P = 14,     /* n*dim*dim first Piola-Kirchhoff stress (get)   particles.py:59 */
+    PDX_FIELD_DAMAGE = 15,/* n (get/set)                                    particles.py:49 */
+    PDX_FIELD_FIXED = 16, /* n uint8 DOF bitmask (get/set)                  particles.py:62 */
+    PDX_FIELD_FEXT = 17,  /* n*dim external force density (get/set)         quasi_static.py:344-377 */
+    PDX_FIELD_EP = 18,    /* n*9 committed plastic strain (get/set)         j2_plasticity.py:117 */
+    PDX_FIELD_EPBAR = 19, /* n committed equivalent plastic strain (get/set) j2_plasticity.py:118 */
+    PDX_FIELD_BACK = 20,  /* n*9 committed back stress (get/set)            extension (0 in the reference) */
+    PDX_FIELD_NNB = 21,   /* n int32 bonds per particle (get)               bonds.py:72 */
+    PDX_FIELD_SING = 22,  /* n uint8 singular-K flag (get)                  nosb.py:92-94 */
+    PDX_FIELD_DRMASS = 23,/* n adaptive-DR diagonal mass (get/set) */
+    PDX_FIELD_DENSITY = 24,/* n (get/set)                                   particles.py:45 */
+    PDX_FIELD_COUNT_
+};
+
+typedef struct pdx_ctx pdx_ctx;
+
+typedef struct {
+    int32_t method;       /* PDX_DR_ADAPTIVE or PDX_DR_KINETIC */
+    int32_t use_graph;    /* 1: replay iteration chunks as a captured hipGraph */
+    int32_t chunk;        /* iterations per host convergence check (even; 0 = default) */
+    int32_t reserved;
+    int64_t max_iterations;
+    double tol;           /* stop when residual / ref < tol (strict, nosb_solver.py:318) */
+    double ref;           /* > 0: fixed reference residual; <= 0: residual of the start state (nosb_solver.py:288-293) */
+    double dt;            /* kinetic DR time step (nosb_solver.py:81-83) */
+    double viscous_damping;/* kinetic DR viscous factor (nosb_solver.py:168-174) */
+    double cmax;          /* adaptive DR damping cap (0 = default 1.9) */
+} pdx_dr_params;
+
+typedef struct {
+    int32_t converged;
+    int32_t reserved;
+    int64_t iterations;
+    double residual;
+    double relative_residual;
+    double reference_residual;
+    double damping;        /* last adaptive damping coefficient */
+    int64_t resets;        /* kinetic DR velocity resets */
+    double elapsed_s;      /* wall time of the iteration loop */
+} pdx_dr_result;
+
+/* ---- lifetime -------------------------------------------------------- */
+/* ParticleSystem(n, dim), particles.py:26-62 */
+int pdx_create(int32_t n, int32_t dim, pdx_ctx **out);
+/* several independent bodies in one context: case c owns caller rows
+ * [offsets[c], offsets[c+1]); offsets has ncase+1 entries, offsets[0] = 0 */
+int pdx_create_batch(int32_t ncase, const int64_t *offsets, int32_t dim, pdx_ctx **out);
+int pdx_destroy(pdx_ctx *ctx);
+const char *pdx_last_error(void);
+int pdx_num_cases(pdx_ctx *ctx, int32_t *ncase);
+
+/* ---- inputs ------------------------------------------------------------ */
+/* ParticleSystem.initialize_from_arrays / _init_fields, particles.py:121-173.
+ * X n*dim, vol n, mass n (mass may be NULL: then mass = vol) */
+int pdx_set_particles(pdx_ctx *ctx, const double *X, const double *vol, const double *mass);
+/* per-particle constants: set_material_constants_per_particle, particles.py:211-219;
+ * yield_stress/h_iso/h_kin only for PDX_MODEL_J2 (may be NULL otherwise) */
+int pdx_set_materials(pdx_ctx *ctx, int32_t model, const double *bulk, const double *shear,
+                      const double *yield_stress, const double *h_iso, const double *h_kin);
+/* NOSBCompute(stabilization) + bond constant, nosb.py:35-57, 193-209; c per particle
+ * (bond modulus (c_i+c_j)/2), or c == NULL with c_scalar for the reference scalar */
+int pdx_set_stabilization(pdx_ctx *ctx, double G_s, const double *c, double c_scalar);
+int pdx_set_formulation(pdx_ctx *ctx, int32_t formulation);
+/* fixed DOF bitmask per particle (set_fixed_particles, particles.py:201-209) and
+ * external force density (LoadControl, quasi_static.py:344-377); fext may be NULL */
+int pdx_set_bc(pdx_ctx *ctx, const uint8_t *fixed, const double *fext);
+
+/* ---- bonds --------------------------------------------------------------- */
+/* NeighborSearch.count_neighbors_only, neighbor.py:239-319 (counts in caller order) */
+int pdx_count_neighbors(pdx_ctx *ctx, double horizon, int32_t *counts);
+/* NeighborSearch.build + BondSystem.build_from_neighbor_search,
+ * neighbor.py:208-218, bonds.py:82-137.  Neighbour sets {j != i : |X_j-X_i|^2 < h^2}. */
+int pdx_build_bonds(pdx_ctx *ctx, double horizon, int64_t *n_bonds);
+/* bonds in caller order: row n+1, col/broken n_bonds, each row sorted by j
+ * (BondSystem.neighbors / broken, bonds.py:68-77); any pointer may be NULL */
+int pdx_get_bonds(pdx_ctx *ctx, int64_t *row, int32_t *col, uint8_t *broken);
+/* per-bond reference data xi (nb*dim), |xi| (nb), omega (nb) in pdx_get_bonds order,
+ * bonds.py:116-137; any pointer may be NULL */
+int pdx_get_bond_data(pdx_ctx *ctx, double *xi, double *xi_len, double *omega);
+/* overwrite broken flags (nb, pdx_get_bonds order), BondSystem.broken, bonds.py:76 */
+int pdx_set_broken(pdx_ctx *ctx, const uint8_t *broken);
+
+/* ---- kernels ------------------------------------------------------------- */
+/* NOSBCompute.compute_shape_tensor, nosb.py:59-94 (FP64 K and inverse) */
+int pdx_shape_tensor(pdx_ctx *ctx, int32_t singular_mode, double tol);
+/* compute_deformation_gradient + stress loop of compute_force_state_with_stabilization,
+ * nosb.py:96-126, 211-227; keeps F and P for pdx_get */
+int pdx_compute_F_stress(pdx_ctx *ctx);
+/* NOSBSolver.compute_forces, nosb_solver.py:200-216 (+ LoadControl.apply) */
+int pdx_compute_forces(pdx_ctx *ctx);
+/* _compute_residual_norm, nosb_solver.py:190-198, per case (ncase values) */
+int pdx_residual_norm(pdx_ctx *ctx, double *out);
+/* residual at the cold-start state (free DOFs at u = 0, prescribed DOFs as set),
+ * per case; the state is left unchanged */
+int pdx_cold_residual(pdx_ctx *ctx, double *out);
+/* DamageModel.update_damage + compute_local_damage, damage.py:94-152 (FP64 stretch) */
+int pdx_break_bonds(pdx_ctx *ctx, double critical_stretch, int64_t *n_broken_now);
+int pdx_local_damage(pdx_ctx *ctx);
+/* J2: commit the trial plastic state of the current displacement (once per
+ * converged load increment; the reference commits per call, j2_plasticity.py:242-243) */
+int pdx_commit_state(pdx_ctx *ctx);
+
+/* ---- dynamic relaxation ---------------------------------------------------- */
+/* NOSBSolver._estimate_stable_dt, nosb_solver.py:94-154 */
+int pdx_estimate_dt(pdx_ctx *ctx, double bulk, double shear, double rho_mean, double *dt);
+/* adaptive-DR diagonal mass (Gershgorin bound of the force stiffness) x beta */
+int pdx_adr_mass(pdx_ctx *ctx, double beta);
+/* NOSBSolver.step, nosb_solver.py:218-256: one kinetic-damping step with the
+ * reset decision taken on the host state passed in/out (prev_ke, ke_increasing) */
+int pdx_kd_step(pdx_ctx *ctx, double dt, double viscous_damping, double *prev_ke,
+                int32_t *ke_increasing, double *ke, double *residual, int32_t *reset);
+/* NOSBSolver.solve, nosb_solver.py:262-332, iterations on the device; results per case
+ * (res has ncase entries) */
+int pdx_dr_solve(pdx_ctx *ctx, const pdx_dr_params *params, pdx_dr_result *res);
+
+/* ---- measurement ------------------------------------------------------------ */
+/* times `iters` adaptive-DR iterations with HIP events around every launch on the
+ * context stream; ms[0] = mean F-pass, ms[1] = mean force pass, ms[2] = total */
+int pdx_time_kernels(pdx_ctx *ctx, int32_t iters, double *ms);
+/* n_particles, n_bonds (intact+broken), n_intact, sell_slots (padded bond slots) */
+int pdx_stats(pdx_ctx *ctx, int64_t *out4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PDX_H */

@@ -1,0 +1,1359 @@
+// pdx_api.hip -- C ABI of libpdx (declared in include/pdx.h): context
+// lifetime, Morton reordering, host<->device field transfer, bond build and
+// the dynamic-relaxation driver (device-side convergence, hipGraph chunks).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <numeric>
+
+#include "pdx_internal.h"
+
+static thread_local std::string g_err;
+void pdx_set_error(const std::string &msg) { g_err = msg; }
+extern "C" const char *pdx_last_error(void) { return g_err.c_str(); }
+
+#define PDX_TRY(expr)                 \
+    do {                              \
+        int _r = (expr);              \
+        if (_r != PDX_OK) return _r;  \
+    } while (0)
+
+static int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define CHECK_CTX(c) \
+    if (!(c)) return fail(PDX_EINVAL, "null context")
+
+// ---------------------------------------------------------------------------
+// Morton keys (21 bits per axis in 3-D, 31 in 2-D)
+// ---------------------------------------------------------------------------
+static uint64_t spread3(uint64_t x)
+{
+    x &= 0x1fffff;
+    x = (x | x << 32) & 0x1f00000000ffffULL;
+    x = (x | x << 16) & 0x1f0000ff0000ffULL;
+    x = (x | x << 8) & 0x100f00f00f00f00fULL;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+    x = (x | x << 2) & 0x1249249249249249ULL;
+    return x;
+}
+
+static uint64_t spread2(uint64_t x)
+{
+    x &= 0x7fffffff;
+    x = (x | x << 16) & 0x0000ffff0000ffffULL;
+    x = (x | x << 8) & 0x00ff00ff00ff00ffULL;
+    x = (x | x << 4) & 0x0f0f0f0f0f0f0f0fULL;
+    x = (x | x << 2) & 0x3333333333333333ULL;
+    x = (x | x << 1) & 0x5555555555555555ULL;
+    return x;
+}
+
+// ---------------------------------------------------------------------------
+// allocation helpers
+// ---------------------------------------------------------------------------
+template <typename T>
+static int dalloc(T **p, size_t count)
+{
+    if (*p) {
+        hipFree(*p);
+        *p = nullptr;
+    }
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
+    if (e != hipSuccess) return fail(PDX_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    e = hipMemset(*p, 0, sizeof(T) * count);
+    if (e != hipSuccess) return fail(PDX_EDEVICE, std::string("hipMemset: ") + hipGetErrorString(e));
+    return PDX_OK;
+}
+
+template <typename T>
+static void dfree(T *&p)
+{
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
+static void free_all(pdx_ctx *c)
+{
+    Dev &d = c->d;
+    dfree(d.P); dfree(d.KR); dfree(d.AR); dfree(d.j2p); dfree(d.j2s);
+    dfree(d.u[0]); dfree(d.u[1]); dfree(d.v); dfree(d.fp); dfree(d.f); dfree(d.fext);
+    dfree(d.mass); dfree(d.lam); dfree(d.V64); dfree(d.K64); dfree(d.Kinv64);
+    dfree(d.Fd); dfree(d.Pd); dfree(d.damage); dfree(d.init_nb);
+    dfree(d.sl_ptr); dfree(d.bj); dfree(d.case_of_blk); dfree(d.case_blk0);
+    dfree(d.sc); dfree(d.part); dfree(d.grid); dfree(d.case_of_p);
+    dfree(d.cell_start); dfree(d.cell_idx); dfree(d.counts);
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    if (c->h_sc) hipHostFree(c->h_sc);
+    c->h_sc = nullptr;
+}
+
+static void invalidate_graph(pdx_ctx *c)
+{
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->gkey = -1;
+}
+
+// host mirrors of caller-order inputs
+struct HostMirror {
+    std::vector<double> vol, mass, bulk, shear, cst, rho, fext;
+    std::vector<uint8_t> fixed;
+    std::vector<int32_t> nnb;  // internal order
+};
+static HostMirror &mirror(pdx_ctx *c);
+
+// keyed by context pointer; the context struct stays ABI-stable for the kernels file
+#include <unordered_map>
+#include <mutex>
+static std::mutex g_mx;
+static std::unordered_map<pdx_ctx *, HostMirror> g_mirror;
+static HostMirror &mirror(pdx_ctx *c)
+{
+    std::lock_guard<std::mutex> l(g_mx);
+    return g_mirror[c];
+}
+
+// ---------------------------------------------------------------------------
+// lifetime
+// ---------------------------------------------------------------------------
+static int create_common(int32_t ncase, const int64_t *off, int32_t dim, pdx_ctx **out)
+{
+    if (!out) return fail(PDX_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (dim != 2 && dim != 3) return fail(PDX_EINVAL, "dim must be 2 or 3");
+    if (ncase < 1) return fail(PDX_EINVAL, "ncase must be >= 1");
+    if (off[0] != 0) return fail(PDX_EINVAL, "offsets[0] must be 0");
+    for (int c = 0; c < ncase; ++c)
+        if (off[c + 1] < off[c]) return fail(PDX_EINVAL, "offsets must be non-decreasing");
+    if (off[ncase] <= 0 || off[ncase] > 0x7fffffff / 2) return fail(PDX_EINVAL, "particle count out of range");
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev < 1) return fail(PDX_EDEVICE, "no HIP device available");
+    pdx_ctx *c = new pdx_ctx();
+    c->n = (int32_t)off[ncase];
+    c->dim = dim;
+    c->ncase = ncase;
+    c->case_off.assign(off, off + ncase + 1);
+    hipGetDevice(&c->device);
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(PDX_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    e = hipHostMalloc((void **)&c->h_sc, sizeof(DScal) * ncase, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return fail(PDX_EDEVICE, "hipHostMalloc failed");
+    }
+    // internal layout: each case padded to a multiple of PDX_BLOCK
+    c->case_p0.resize(ncase + 1);
+    int64_t p = 0;
+    for (int k = 0; k < ncase; ++k) {
+        c->case_p0[k] = (int32_t)p;
+        int64_t nk = off[k + 1] - off[k];
+        p += ((nk + PDX_BLOCK - 1) / PDX_BLOCK) * PDX_BLOCK;
+        if (nk == 0) p += PDX_BLOCK;  // empty case keeps one (inactive) block
+    }
+    c->case_p0[ncase] = (int32_t)p;
+    c->ni = (int32_t)p;
+    int nb0 = c->ni / PDX_BLOCK;
+    c->nblk = ((nb0 + PDX_NXCD - 1) / PDX_NXCD) * PDX_NXCD;
+    c->perm.assign(c->ni, -1);
+    c->inv.assign(c->n, -1);
+    HostMirror &m = mirror(c);
+    m.vol.assign(c->n, 0.0);
+    m.mass.assign(c->n, 0.0);
+    m.bulk.assign(c->n, 0.0);
+    m.shear.assign(c->n, 0.0);
+    m.cst.assign(c->n, 0.0);
+    m.rho.assign(c->n, 0.0);
+    m.fext.assign((size_t)c->n * dim, 0.0);
+    m.fixed.assign(c->n, 0);
+    m.nnb.assign(c->ni, 0);
+    Dev &d = c->d;
+    d.ni = c->ni;
+    d.nblk = c->nblk;
+    d.ncase = ncase;
+    d.dim = dim;
+    int r = PDX_OK;
+    size_t ni = c->ni;
+    if ((r = dalloc(&d.P, ni)) || (r = dalloc(&d.KR, ni)) || (r = dalloc(&d.AR, ni)) ||
+        (r = dalloc(&d.j2p, ni)) || (r = dalloc(&d.j2s, ni)) || (r = dalloc(&d.u[0], ni)) ||
+        (r = dalloc(&d.u[1], ni)) || (r = dalloc(&d.v, ni)) || (r = dalloc(&d.fp, ni)) ||
+        (r = dalloc(&d.f, ni)) || (r = dalloc(&d.fext, ni)) || (r = dalloc(&d.mass, ni)) ||
+        (r = dalloc(&d.lam, ni)) || (r = dalloc(&d.V64, ni)) || (r = dalloc(&d.K64, ni * 9)) ||
+        (r = dalloc(&d.Kinv64, ni * 9)) || (r = dalloc(&d.Fd, ni * 9)) || (r = dalloc(&d.Pd, ni * 9)) ||
+        (r = dalloc(&d.damage, ni)) || (r = dalloc(&d.init_nb, ni)) || (r = dalloc(&d.case_of_p, ni)) ||
+        (r = dalloc(&d.counts, ni)) || (r = dalloc(&d.case_of_blk, (size_t)c->nblk)) ||
+        (r = dalloc(&d.case_blk0, (size_t)ncase + 1)) || (r = dalloc(&d.sc, (size_t)ncase)) ||
+        (r = dalloc(&d.part, (size_t)c->nblk * 4)) || (r = dalloc(&d.grid, (size_t)ncase))) {
+        free_all(c);
+        hipStreamDestroy(c->stream);
+        delete c;
+        return r;
+    }
+    // block -> case tables; padding blocks of the grid join the last case
+    std::vector<int32_t> cob(c->nblk), cb0(ncase + 1);
+    for (int k = 0; k < ncase; ++k) {
+        cb0[k] = c->case_p0[k] / PDX_BLOCK;
+        for (int b = c->case_p0[k] / PDX_BLOCK; b < c->case_p0[k + 1] / PDX_BLOCK; ++b) cob[b] = k;
+    }
+    for (int b = nb0; b < c->nblk; ++b) cob[b] = ncase - 1;
+    cb0[ncase] = c->nblk;
+    hipMemcpy(d.case_of_blk, cob.data(), sizeof(int32_t) * cob.size(), hipMemcpyHostToDevice);
+    hipMemcpy(d.case_blk0, cb0.data(), sizeof(int32_t) * cb0.size(), hipMemcpyHostToDevice);
+    std::vector<DScal> sc(ncase);
+    memset(sc.data(), 0, sizeof(DScal) * ncase);
+    for (auto &s : sc) { s.ref = 1.0; s.cmax = 1.9; s.first = 1; s.inc = 1; }
+    hipMemcpy(d.sc, sc.data(), sizeof(DScal) * ncase, hipMemcpyHostToDevice);
+    *out = c;
+    return PDX_OK;
+}
+
+extern "C" int pdx_create(int32_t n, int32_t dim, pdx_ctx **out)
+{
+    int64_t off[2] = {0, n};
+    if (n <= 0) return fail(PDX_EINVAL, "n must be positive");
+    return create_common(1, off, dim, out);
+}
+
+extern "C" int pdx_create_batch(int32_t ncase, const int64_t *offsets, int32_t dim, pdx_ctx **out)
+{
+    if (!offsets) return fail(PDX_EINVAL, "offsets is NULL");
+    return create_common(ncase, offsets, dim, out);
+}
+
+extern "C" int pdx_destroy(pdx_ctx *c)
+{
+    if (!c) return PDX_OK;
+    hipStreamSynchronize(c->stream);
+    free_all(c);
+    hipStreamDestroy(c->stream);
+    {
+        std::lock_guard<std::mutex> l(g_mx);
+        g_mirror.erase(c);
+    }
+    delete c;
+    return PDX_OK;
+}
+
+extern "C" int pdx_num_cases(pdx_ctx *c, int32_t *ncase)
+{
+    CHECK_CTX(c);
+    if (ncase) *ncase = c->ncase;
+    return PDX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// transfer helpers (caller order <-> internal order)
+// ---------------------------------------------------------------------------
+template <typename T>
+static int upload(pdx_ctx *c, T *dst, const std::vector<T> &h)
+{
+    PDX_HIP(hipMemcpyAsync(dst, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, c->stream));
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    return PDX_OK;
+}
+
+template <typename T>
+static int download(pdx_ctx *c, std::vector<T> &h, const T *src, size_t count)
+{
+    h.resize(count);
+    PDX_HIP(hipMemcpyAsync(h.data(), src, sizeof(T) * count, hipMemcpyDeviceToHost, c->stream));
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    return PDX_OK;
+}
+
+// write cnt floats per particle (caller rows of `src`, internal scatter) into KRec words
+static int kr_put(pdx_ctx *c, const std::vector<float> &internal, int off, int cnt)
+{
+    float *tmp = nullptr;
+    PDX_HIP(hipMallocAsync((void **)&tmp, sizeof(float) * internal.size(), c->stream));
+    PDX_HIP(hipMemcpyAsync(tmp, internal.data(), sizeof(float) * internal.size(), hipMemcpyHostToDevice, c->stream));
+    int r = pdx_launch_kr_put(c, tmp, off, cnt);
+    hipFreeAsync(tmp, c->stream);
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    return r;
+}
+
+static const int KR_OFF_BULK = 12, KR_OFF_FLAGS = 14, KR_OFF_NNB = 15;
+
+static int push_flags(pdx_ctx *c)
+{
+    HostMirror &m = mirror(c);
+    std::vector<float> w(c->ni);
+    // keep the singular bit computed on the device
+    std::vector<KRec> kr;
+    PDX_TRY(download(c, kr, c->d.KR, c->ni));
+    for (int i = 0; i < c->ni; ++i) {
+        uint32_t fl = kr[i].flags & PDX_FL_SING;
+        int r = c->perm[i];
+        if (r >= 0) {
+            fl |= PDX_FL_ACTIVE | (m.fixed[r] & PDX_FL_FIXMASK);
+            bool loaded = false;
+            for (int q = 0; q < c->dim; ++q) loaded |= m.fext[(size_t)r * c->dim + q] != 0.0;
+            if (loaded) fl |= PDX_FL_LOADED;
+        }
+        memcpy(&w[i], &fl, 4);
+    }
+    return kr_put(c, w, KR_OFF_FLAGS, 1);
+}
+
+// caller-order n*dim doubles -> internal double4
+static int put_vec(pdx_ctx *c, double4 *dst, const double *src)
+{
+    std::vector<double4> h(c->ni, make_double4(0, 0, 0, 0));
+    for (int i = 0; i < c->ni; ++i) {
+        int r = c->perm[i];
+        if (r < 0) continue;
+        const double *s = src + (size_t)r * c->dim;
+        h[i] = make_double4(s[0], s[1], c->dim == 3 ? s[2] : 0.0, 0.0);
+    }
+    return upload(c, dst, h);
+}
+
+static int get_vec(pdx_ctx *c, const double4 *src, double *dst)
+{
+    std::vector<double4> h;
+    PDX_TRY(download(c, h, src, c->ni));
+    for (int i = 0; i < c->ni; ++i) {
+        int r = c->perm[i];
+        if (r < 0) continue;
+        double *o = dst + (size_t)r * c->dim;
+        o[0] = h[i].x;
+        o[1] = h[i].y;
+        if (c->dim == 3) o[2] = h[i].z;
+    }
+    return PDX_OK;
+}
+
+static int put_scalar(pdx_ctx *c, double *dst, const double *src)
+{
+    std::vector<double> h(c->ni, 0.0);
+    for (int i = 0; i < c->ni; ++i)
+        if (c->perm[i] >= 0) h[i] = src[c->perm[i]];
+    return upload(c, dst, h);
+}
+
+static int get_scalar(pdx_ctx *c, const double *src, double *dst)
+{
+    std::vector<double> h;
+    PDX_TRY(download(c, h, src, c->ni));
+    for (int i = 0; i < c->ni; ++i)
+        if (c->perm[i] >= 0) dst[c->perm[i]] = h[i];
+    return PDX_OK;
+}
+
+// 9-per-particle tensor (dim*dim used) -> caller n*dim*dim
+template <typename T>
+static int get_tensor(pdx_ctx *c, const T *src, double *dst)
+{
+    std::vector<T> h;
+    PDX_TRY(download(c, h, src, (size_t)c->ni * 9));
+    const int D = c->dim;
+    for (int i = 0; i < c->ni; ++i) {
+        int r = c->perm[i];
+        if (r < 0) continue;
+        for (int q = 0; q < D * D; ++q) dst[(size_t)r * D * D + q] = (double)h[(size_t)i * 9 + q];
+    }
+    return PDX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// inputs
+// ---------------------------------------------------------------------------
+// reference geometry records (X, V, c) and the FP64 volumes, internal order
+static int push_prec(pdx_ctx *c)
+{
+    HostMirror &m = mirror(c);
+    const int D = c->dim;
+    std::vector<PRec> hp(c->ni);
+    for (int i = 0; i < c->ni; ++i) {
+        PRec p;
+        memset(&p, 0, sizeof(p));
+        int r = c->perm[i];
+        if (r >= 0) {
+            for (int q = 0; q < D; ++q) p.X[q] = c->hX[(size_t)r * D + q];
+            p.V = (float)m.vol[r];
+            p.c = (float)m.cst[r];
+        }
+        hp[i] = p;
+    }
+    PDX_TRY(upload(c, c->d.P, hp));
+    return put_scalar(c, c->d.V64, m.vol.data());
+}
+
+extern "C" int pdx_set_particles(pdx_ctx *c, const double *X, const double *vol, const double *mass)
+{
+    CHECK_CTX(c);
+    if (!X || !vol) return fail(PDX_EINVAL, "X and vol are required");
+    const int D = c->dim;
+    for (int r = 0; r < c->n; ++r) {
+        if (!(vol[r] > 0.0) || !isfinite(vol[r])) return fail(PDX_EINVAL, "particle volumes must be positive");
+        for (int q = 0; q < D; ++q)
+            if (!isfinite(X[(size_t)r * D + q])) return fail(PDX_EINVAL, "positions must be finite");
+    }
+    c->hX.assign(X, X + (size_t)c->n * D);
+    // Morton order per case
+    for (int k = 0; k < c->ncase; ++k) {
+        const int64_t r0 = c->case_off[k], r1 = c->case_off[k + 1], nk = r1 - r0;
+        if (nk == 0) continue;
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int64_t r = r0; r < r1; ++r)
+            for (int q = 0; q < D; ++q) {
+                lo[q] = std::min(lo[q], X[r * D + q]);
+                hi[q] = std::max(hi[q], X[r * D + q]);
+            }
+        double vol_box = 1.0;
+        for (int q = 0; q < D; ++q) vol_box *= std::max(hi[q] - lo[q], 1e-300);
+        double hq = pow(vol_box / (double)nk, 1.0 / D);
+        for (int q = 0; q < D; ++q) hq = std::max(hq, (hi[q] - lo[q]) / (D == 3 ? 2097151.0 : 2147483647.0));
+        if (!(hq > 0.0)) hq = 1.0;
+        std::vector<std::pair<uint64_t, int32_t>> key(nk);
+        for (int64_t r = r0; r < r1; ++r) {
+            uint64_t qv[3] = {0, 0, 0};
+            for (int q = 0; q < D; ++q) qv[q] = (uint64_t)floor((X[r * D + q] - lo[q]) / hq);
+            uint64_t kk = D == 3 ? (spread3(qv[0]) << 2 | spread3(qv[1]) << 1 | spread3(qv[2]))
+                                 : (spread2(qv[0]) << 1 | spread2(qv[1]));
+            key[r - r0] = {kk, (int32_t)r};
+        }
+        std::sort(key.begin(), key.end());
+        for (int64_t t = 0; t < nk; ++t) {
+            c->perm[c->case_p0[k] + t] = key[t].second;
+            c->inv[key[t].second] = (int32_t)(c->case_p0[k] + t);
+        }
+    }
+    HostMirror &m = mirror(c);
+    for (int r = 0; r < c->n; ++r) {
+        m.vol[r] = vol[r];
+        m.mass[r] = mass ? mass[r] : vol[r];
+        m.rho[r] = m.mass[r] / vol[r];
+    }
+    std::vector<int32_t> cop(c->ni, -1);
+    for (int k = 0; k < c->ncase; ++k)
+        for (int i = c->case_p0[k]; i < c->case_p0[k + 1]; ++i)
+            if (c->perm[i] >= 0) cop[i] = k;
+    PDX_TRY(push_prec(c));
+    PDX_TRY(upload(c, c->d.case_of_p, cop));
+    PDX_TRY(put_scalar(c, c->d.mass, m.mass.data()));
+    // state back to the reference initial values (particles.py:159-173)
+    PDX_HIP(hipMemset(c->d.u[0], 0, sizeof(double4) * c->ni));
+    PDX_HIP(hipMemset(c->d.u[1], 0, sizeof(double4) * c->ni));
+    PDX_HIP(hipMemset(c->d.v, 0, sizeof(double4) * c->ni));
+    PDX_HIP(hipMemset(c->d.fp, 0, sizeof(double4) * c->ni));
+    PDX_HIP(hipMemset(c->d.f, 0, sizeof(double4) * c->ni));
+    PDX_HIP(hipMemset(c->d.damage, 0, sizeof(double) * c->ni));
+    PDX_HIP(hipMemset(c->d.KR, 0, sizeof(KRec) * c->ni));
+    c->ucur = 0;
+    c->have_particles = true;
+    c->have_bonds = c->have_K = c->have_mass = c->have_A = false;
+    invalidate_graph(c);
+    PDX_TRY(push_flags(c));
+    return PDX_OK;
+}
+
+extern "C" int pdx_set_materials(pdx_ctx *c, int32_t model, const double *bulk, const double *shear,
+                                 const double *ys, const double *hi, const double *hk)
+{
+    CHECK_CTX(c);
+    if (!c->have_particles) return fail(PDX_ESTATE, "set particles first");
+    if (model < 0 || model > 2) return fail(PDX_EINVAL, "unknown constitutive model");
+    if (model == PDX_MODEL_J2 && c->dim != 3) return fail(PDX_EUNSUP, "J2 plasticity is 3-D only");
+    if (!bulk || !shear) return fail(PDX_EINVAL, "bulk and shear are required");
+    if (model == PDX_MODEL_J2 && (!ys || !hi)) return fail(PDX_EINVAL, "J2 needs yield_stress and h_iso");
+    HostMirror &m = mirror(c);
+    std::vector<float> w((size_t)c->ni * 2, 0.f);
+    for (int i = 0; i < c->ni; ++i) {
+        int r = c->perm[i];
+        if (r < 0) continue;
+        w[(size_t)i * 2] = (float)bulk[r];
+        w[(size_t)i * 2 + 1] = (float)shear[r];
+    }
+    for (int r = 0; r < c->n; ++r) {
+        m.bulk[r] = bulk[r];
+        m.shear[r] = shear[r];
+    }
+    PDX_TRY(kr_put(c, w, KR_OFF_BULK, 2));
+    if (model == PDX_MODEL_J2) {
+        std::vector<J2Par> jp(c->ni);
+        memset(jp.data(), 0, sizeof(J2Par) * c->ni);
+        for (int i = 0; i < c->ni; ++i) {
+            int r = c->perm[i];
+            if (r < 0) continue;
+            jp[i].sy0 = (float)ys[r];
+            jp[i].hiso = (float)hi[r];
+            jp[i].hkin = hk ? (float)hk[r] : 0.f;
+        }
+        PDX_TRY(upload(c, c->d.j2p, jp));
+    }
+    c->model = model;
+    c->d.model = model;
+    invalidate_graph(c);
+    return PDX_OK;
+}
+
+extern "C" int pdx_set_stabilization(pdx_ctx *c, double G_s, const double *cst, double c_scalar)
+{
+    CHECK_CTX(c);
+    if (!c->have_particles) return fail(PDX_ESTATE, "set particles first");
+    HostMirror &m = mirror(c);
+    for (int r = 0; r < c->n; ++r) m.cst[r] = cst ? cst[r] : c_scalar;
+    int r = push_prec(c);
+    c->G_s = G_s;
+    c->d.G_s = G_s;
+    c->have_mass = false;
+    invalidate_graph(c);
+    return r;
+}
+
+extern "C" int pdx_set_formulation(pdx_ctx *c, int32_t form)
+{
+    CHECK_CTX(c);
+    if (form != PDX_FORM_NOSB && form != PDX_FORM_BOND) return fail(PDX_EINVAL, "unknown formulation");
+    c->form = form;
+    c->d.form = form;
+    c->have_mass = false;
+    invalidate_graph(c);
+    return PDX_OK;
+}
+
+extern "C" int pdx_set_bc(pdx_ctx *c, const uint8_t *fixed, const double *fext)
+{
+    CHECK_CTX(c);
+    if (!c->have_particles) return fail(PDX_ESTATE, "set particles first");
+    HostMirror &m = mirror(c);
+    const uint8_t full = c->dim == 3 ? 7 : 3;
+    for (int r = 0; r < c->n; ++r) m.fixed[r] = fixed ? (fixed[r] & full) : 0;
+    if (fext) {
+        for (size_t k = 0; k < (size_t)c->n * c->dim; ++k) {
+            if (!isfinite(fext[k])) return fail(PDX_EINVAL, "external forces must be finite");
+            m.fext[k] = fext[k];
+        }
+    } else {
+        std::fill(m.fext.begin(), m.fext.end(), 0.0);
+    }
+    PDX_TRY(put_vec(c, c->d.fext, m.fext.data()));
+    return push_flags(c);
+}
+
+// ---------------------------------------------------------------------------
+// bonds
+// ---------------------------------------------------------------------------
+static int build_grid(pdx_ctx *c, double horizon)
+{
+    const int D = c->dim;
+    const double h = horizon * 1.01;  // cell edge >= horizon (neighbor.py:47)
+    std::vector<GridP> gp(c->ncase);
+    std::vector<int32_t> cstart, cidx;
+    cidx.reserve(c->n);
+    int64_t cell_off = 0;
+    for (int k = 0; k < c->ncase; ++k) {
+        GridP g;
+        memset(&g, 0, sizeof(g));
+        g.h = h;
+        g.nc[0] = g.nc[1] = g.nc[2] = 1;
+        const int i0 = c->case_p0[k], i1 = c->case_p0[k + 1];
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        bool any = false;
+        for (int i = i0; i < i1; ++i) {
+            int r = c->perm[i];
+            if (r < 0) continue;
+            any = true;
+            for (int q = 0; q < D; ++q) {
+                lo[q] = std::min(lo[q], c->hX[(size_t)r * D + q]);
+                hi[q] = std::max(hi[q], c->hX[(size_t)r * D + q]);
+            }
+        }
+        if (!any) for (int q = 0; q < 3; ++q) lo[q] = hi[q] = 0.0;
+        int64_t ncell = 1;
+        for (int q = 0; q < D; ++q) {
+            g.lo[q] = lo[q];
+            double span = (hi[q] - lo[q]) / h;
+            if (span > 4.0e6) return fail(PDX_EINVAL, "domain too large for the neighbour grid");
+            g.nc[q] = (int32_t)floor(span) + 1;
+            ncell *= g.nc[q];
+        }
+        if (ncell > (int64_t)1 << 28) return fail(PDX_EINVAL, "neighbour grid too large (sparse domain)");
+        g.cell_off = cell_off;
+        std::vector<int32_t> cnt(ncell + 1, 0);
+        std::vector<int64_t> cell_of(i1 - i0, -1);
+        for (int i = i0; i < i1; ++i) {
+            int r = c->perm[i];
+            if (r < 0) continue;
+            int64_t cc[3] = {0, 0, 0};
+            for (int q = 0; q < D; ++q) {
+                int64_t t = (int64_t)floor((c->hX[(size_t)r * D + q] - g.lo[q]) / g.h);
+                cc[q] = std::max<int64_t>(0, std::min<int64_t>(t, g.nc[q] - 1));
+            }
+            int64_t cell = (cc[0] * g.nc[1] + cc[1]) * g.nc[2] + cc[2];
+            cell_of[i - i0] = cell;
+            cnt[cell + 1]++;
+        }
+        const int32_t base = (int32_t)cidx.size();
+        for (int64_t t = 0; t < ncell; ++t) cnt[t + 1] += cnt[t];
+        std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+        cidx.resize(base + cnt[ncell]);
+        for (int i = i0; i < i1; ++i)
+            if (cell_of[i - i0] >= 0) cidx[base + fill[cell_of[i - i0]]++] = i;
+        for (int64_t t = 0; t <= ncell; ++t) cstart.push_back(base + cnt[t]);
+        cell_off += ncell + 1;
+        gp[k] = g;
+    }
+    PDX_TRY(dalloc(&c->d.cell_start, cstart.size()));
+    PDX_TRY(dalloc(&c->d.cell_idx, std::max<size_t>(cidx.size(), 1)));
+    PDX_TRY(upload(c, c->d.cell_start, cstart));
+    if (!cidx.empty()) PDX_TRY(upload(c, c->d.cell_idx, cidx));
+    PDX_TRY(upload(c, c->d.grid, gp));
+    c->d.h = horizon;
+    c->d.inv_h = 1.0 / horizon;
+    c->d.h2 = horizon * horizon;
+    return PDX_OK;
+}
+
+extern "C" int pdx_count_neighbors(pdx_ctx *c, double horizon, int32_t *counts)
+{
+    CHECK_CTX(c);
+    if (!c->have_particles) return fail(PDX_ESTATE, "set particles first");
+    if (!(horizon > 0.0)) return fail(PDX_EINVAL, "horizon must be positive");
+    PDX_TRY(build_grid(c, horizon));
+    PDX_TRY(pdx_launch_count(c, c->d.counts));
+    std::vector<int32_t> h;
+    PDX_TRY(download(c, h, c->d.counts, c->ni));
+    if (counts)
+        for (int i = 0; i < c->ni; ++i)
+            if (c->perm[i] >= 0) counts[c->perm[i]] = h[i];
+    return PDX_OK;
+}
+
+extern "C" int pdx_build_bonds(pdx_ctx *c, double horizon, int64_t *n_bonds)
+{
+    CHECK_CTX(c);
+    if (!c->have_particles) return fail(PDX_ESTATE, "set particles first");
+    if (!(horizon > 0.0) || !isfinite(horizon)) return fail(PDX_EINVAL, "horizon must be positive");
+    PDX_TRY(build_grid(c, horizon));
+    PDX_TRY(pdx_launch_count(c, c->d.counts));
+    std::vector<int32_t> cnt;
+    PDX_TRY(download(c, cnt, c->d.counts, c->ni));
+    const int nsl = c->ni / PDX_SLICE;
+    c->h_slptr.assign(nsl + 1, 0);
+    int64_t tot = 0;
+    for (int s = 0; s < nsl; ++s) {
+        int w = 0;
+        for (int t = 0; t < PDX_SLICE; ++t) {
+            w = std::max(w, cnt[s * PDX_SLICE + t]);
+            tot += cnt[s * PDX_SLICE + t];
+        }
+        c->h_slptr[s + 1] = c->h_slptr[s] + (int64_t)w * PDX_SLICE;
+    }
+    c->slots = c->h_slptr[nsl];
+    c->nb = tot;
+    PDX_TRY(dalloc(&c->d.sl_ptr, (size_t)nsl + 1));
+    PDX_TRY(upload(c, c->d.sl_ptr, c->h_slptr));
+    PDX_TRY(dalloc(&c->d.bj, (size_t)std::max<int64_t>(c->slots, 1)));
+    PDX_HIP(hipMemset(c->d.bj, 0xff, sizeof(uint32_t) * (size_t)std::max<int64_t>(c->slots, 1)));
+    PDX_HIP(hipMemset(c->d.init_nb, 0, sizeof(int32_t) * c->ni));
+    PDX_TRY(pdx_launch_fill(c));
+    HostMirror &m = mirror(c);
+    m.nnb = cnt;
+    std::vector<float> w(c->ni);
+    for (int i = 0; i < c->ni; ++i) {
+        uint32_t v = (uint32_t)cnt[i];
+        memcpy(&w[i], &v, 4);
+    }
+    PDX_TRY(kr_put(c, w, KR_OFF_NNB, 1));
+    c->horizon = horizon;
+    c->have_bonds = true;
+    c->have_K = c->have_mass = c->have_A = false;
+    invalidate_graph(c);
+    if (n_bonds) *n_bonds = tot;
+    return PDX_OK;
+}
+
+// per caller row: (caller j, internal slot address) sorted by caller j
+struct RowMap {
+    std::vector<int64_t> row;
+    std::vector<int32_t> col;
+    std::vector<int64_t> slot;   // index into bj
+    std::vector<int64_t> csr;    // index into internal CSR order (k_bond_data output)
+};
+
+static int row_map(pdx_ctx *c, RowMap &rm)
+{
+    HostMirror &m = mirror(c);
+    std::vector<uint32_t> bj;
+    PDX_TRY(download(c, bj, c->d.bj, (size_t)std::max<int64_t>(c->slots, 1)));
+    std::vector<int64_t> rowi(c->ni + 1, 0);
+    for (int i = 0; i < c->ni; ++i) rowi[i + 1] = rowi[i] + m.nnb[i];
+    rm.row.assign(c->n + 1, 0);
+    for (int r = 0; r < c->n; ++r) rm.row[r + 1] = rm.row[r] + m.nnb[c->inv[r]];
+    rm.col.resize(rm.row[c->n]);
+    rm.slot.resize(rm.row[c->n]);
+    rm.csr.resize(rm.row[c->n]);
+    std::vector<std::pair<int32_t, int32_t>> tmp;
+    for (int r = 0; r < c->n; ++r) {
+        const int i = c->inv[r];
+        const int64_t base = c->h_slptr[i / PDX_SLICE] + (i % PDX_SLICE);
+        tmp.clear();
+        for (int k = 0; k < m.nnb[i]; ++k) {
+            uint32_t e = bj[base + (int64_t)k * PDX_SLICE] & ~PDX_BROKEN;
+            tmp.push_back({c->perm[e], k});
+        }
+        std::sort(tmp.begin(), tmp.end());
+        for (size_t t = 0; t < tmp.size(); ++t) {
+            rm.col[rm.row[r] + t] = tmp[t].first;
+            rm.slot[rm.row[r] + t] = base + (int64_t)tmp[t].second * PDX_SLICE;
+            rm.csr[rm.row[r] + t] = rowi[i] + tmp[t].second;
+        }
+    }
+    return PDX_OK;
+}
+
+extern "C" int pdx_get_bonds(pdx_ctx *c, int64_t *row, int32_t *col, uint8_t *broken)
+{
+    CHECK_CTX(c);
+    if (!c->have_bonds) return fail(PDX_ESTATE, "bonds not built");
+    RowMap rm;
+    PDX_TRY(row_map(c, rm));
+    if (row) memcpy(row, rm.row.data(), sizeof(int64_t) * rm.row.size());
+    if (col) memcpy(col, rm.col.data(), sizeof(int32_t) * rm.col.size());
+    if (broken) {
+        std::vector<uint32_t> bj;
+        PDX_TRY(download(c, bj, c->d.bj, (size_t)std::max<int64_t>(c->slots, 1)));
+        for (size_t t = 0; t < rm.slot.size(); ++t) broken[t] = (bj[rm.slot[t]] & PDX_BROKEN) ? 1 : 0;
+    }
+    return PDX_OK;
+}
+
+extern "C" int pdx_set_broken(pdx_ctx *c, const uint8_t *broken)
+{
+    CHECK_CTX(c);
+    if (!c->have_bonds) return fail(PDX_ESTATE, "bonds not built");
+    if (!broken) return fail(PDX_EINVAL, "broken is NULL");
+    RowMap rm;
+    PDX_TRY(row_map(c, rm));
+    std::vector<uint32_t> bj;
+    PDX_TRY(download(c, bj, c->d.bj, (size_t)std::max<int64_t>(c->slots, 1)));
+    for (size_t t = 0; t < rm.slot.size(); ++t) {
+        uint32_t e = bj[rm.slot[t]] & ~PDX_BROKEN;
+        bj[rm.slot[t]] = broken[t] ? (e | PDX_BROKEN) : e;
+    }
+    PDX_TRY(upload(c, c->d.bj, bj));
+    c->have_mass = false;
+    return PDX_OK;
+}
+
+extern "C" int pdx_get_bond_data(pdx_ctx *c, double *xi, double *xlen, double *omega)
+{
+    CHECK_CTX(c);
+    if (!c->have_bonds) return fail(PDX_ESTATE, "bonds not built");
+    RowMap rm;
+    PDX_TRY(row_map(c, rm));
+    HostMirror &m = mirror(c);
+    const int D = c->dim;
+    std::vector<int64_t> rowi(c->ni + 1, 0);
+    for (int i = 0; i < c->ni; ++i) rowi[i + 1] = rowi[i] + m.nnb[i];
+    const size_t nb = std::max<int64_t>(rowi[c->ni], 1);
+    double *dxi = nullptr, *dl = nullptr, *dw = nullptr;
+    int64_t *drow = nullptr;
+    PDX_HIP(hipMalloc((void **)&dxi, sizeof(double) * nb * D));
+    PDX_HIP(hipMalloc((void **)&dl, sizeof(double) * nb));
+    PDX_HIP(hipMalloc((void **)&dw, sizeof(double) * nb));
+    PDX_HIP(hipMalloc((void **)&drow, sizeof(int64_t) * (c->ni + 1)));
+    int r = upload(c, drow, rowi);
+    if (r == PDX_OK) r = pdx_launch_bond_data(c, dxi, dl, dw, drow);
+    std::vector<double> hxi, hl, hw;
+    if (r == PDX_OK) r = download(c, hxi, dxi, nb * D);
+    if (r == PDX_OK) r = download(c, hl, dl, nb);
+    if (r == PDX_OK) r = download(c, hw, dw, nb);
+    hipFree(dxi); hipFree(dl); hipFree(dw); hipFree(drow);
+    if (r != PDX_OK) return r;
+    for (size_t t = 0; t < rm.csr.size(); ++t) {
+        const int64_t s = rm.csr[t];
+        if (xi) for (int q = 0; q < D; ++q) xi[t * D + q] = hxi[s * D + q];
+        if (xlen) xlen[t] = hl[s];
+        if (omega) omega[t] = hw[s];
+    }
+    return PDX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+extern "C" int pdx_shape_tensor(pdx_ctx *c, int32_t mode, double tol)
+{
+    CHECK_CTX(c);
+    if (!c->have_bonds) return fail(PDX_ESTATE, "bonds not built");
+    if (mode != PDX_SING_ABSOLUTE && mode != PDX_SING_RELATIVE) return fail(PDX_EINVAL, "unknown singular test");
+    PDX_TRY(pdx_launch_shape(c, mode, tol));
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    c->have_K = true;
+    c->have_mass = false;
+    return PDX_OK;
+}
+
+static int need_K(pdx_ctx *c)
+{
+    if (!c->have_bonds) return fail(PDX_ESTATE, "bonds not built");
+    if (!c->have_K && c->form == PDX_FORM_NOSB) return fail(PDX_ESTATE, "shape tensor not computed");
+    return PDX_OK;
+}
+
+extern "C" int pdx_compute_F_stress(pdx_ctx *c)
+{
+    CHECK_CTX(c);
+    PDX_TRY(need_K(c));
+    PDX_TRY(pdx_launch_fpass(c, c->ucur, true, false, false));
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    c->have_A = true;
+    return PDX_OK;
+}
+
+extern "C" int pdx_compute_forces(pdx_ctx *c)
+{
+    CHECK_CTX(c);
+    PDX_TRY(need_K(c));
+    if (c->form == PDX_FORM_NOSB) PDX_TRY(pdx_launch_fpass(c, c->ucur, true, false, false));
+    PDX_TRY(pdx_launch_force(c, c->ucur, 0, false, 0.0, 0.0));
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    c->have_A = true;
+    return PDX_OK;
+}
+
+static int read_sc(pdx_ctx *c)
+{
+    PDX_HIP(hipMemcpyAsync(c->h_sc, c->d.sc, sizeof(DScal) * c->ncase, hipMemcpyDeviceToHost, c->stream));
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    return PDX_OK;
+}
+
+static int write_sc(pdx_ctx *c)
+{
+    PDX_HIP(hipMemcpyAsync(c->d.sc, c->h_sc, sizeof(DScal) * c->ncase, hipMemcpyHostToDevice, c->stream));
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    return PDX_OK;
+}
+
+extern "C" int pdx_residual_norm(pdx_ctx *c, double *out)
+{
+    CHECK_CTX(c);
+    if (!c->have_particles) return fail(PDX_ESTATE, "set particles first");
+    PDX_TRY(pdx_launch_resid(c));
+    PDX_TRY(read_sc(c));
+    for (int k = 0; k < c->ncase; ++k) out[k] = c->h_sc[k].res;
+    return PDX_OK;
+}
+
+extern "C" int pdx_cold_residual(pdx_ctx *c, double *out)
+{
+    CHECK_CTX(c);
+    PDX_TRY(need_K(c));
+    double4 *save = nullptr;
+    PDX_HIP(hipMalloc((void **)&save, sizeof(double4) * c->ni));
+    PDX_HIP(hipMemcpyAsync(save, c->d.u[c->ucur], sizeof(double4) * c->ni, hipMemcpyDeviceToDevice, c->stream));
+    int r = pdx_launch_zero_free(c, c->ucur);
+    if (r == PDX_OK && c->form == PDX_FORM_NOSB) r = pdx_launch_fpass(c, c->ucur, false, false, false);
+    if (r == PDX_OK) r = pdx_launch_force(c, c->ucur, 0, false, 0.0, 0.0);
+    if (r == PDX_OK) r = read_sc(c);
+    if (r == PDX_OK)
+        for (int k = 0; k < c->ncase; ++k) out[k] = c->h_sc[k].res;
+    hipMemcpyAsync(c->d.u[c->ucur], save, sizeof(double4) * c->ni, hipMemcpyDeviceToDevice, c->stream);
+    hipStreamSynchronize(c->stream);
+    hipFree(save);
+    c->have_A = false;
+    return r;
+}
+
+extern "C" int pdx_break_bonds(pdx_ctx *c, double s_c, int64_t *n_new)
+{
+    CHECK_CTX(c);
+    if (!c->have_bonds) return fail(PDX_ESTATE, "bonds not built");
+    unsigned long long *cnt = nullptr;
+    PDX_HIP(hipMalloc((void **)&cnt, sizeof(unsigned long long)));
+    PDX_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
+    int r = pdx_launch_damage(c, s_c, cnt);
+    unsigned long long h = 0;
+    if (r == PDX_OK) {
+        hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+        hipStreamSynchronize(c->stream);
+    }
+    hipFree(cnt);
+    if (n_new) *n_new = (int64_t)h;
+    if (h) c->have_mass = false;
+    return r;
+}
+
+extern "C" int pdx_local_damage(pdx_ctx *c)
+{
+    CHECK_CTX(c);
+    if (!c->have_bonds) return fail(PDX_ESTATE, "bonds not built");
+    PDX_TRY(pdx_launch_local_damage(c));
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    return PDX_OK;
+}
+
+extern "C" int pdx_commit_state(pdx_ctx *c)
+{
+    CHECK_CTX(c);
+    PDX_TRY(need_K(c));
+    if (c->model != PDX_MODEL_J2) return PDX_OK;
+    PDX_TRY(pdx_launch_fpass(c, c->ucur, false, true, false));
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    return PDX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// dynamic relaxation
+// ---------------------------------------------------------------------------
+extern "C" int pdx_estimate_dt(pdx_ctx *c, double bulk, double shear, double rho_mean, double *dt)
+{
+    CHECK_CTX(c);
+    PDX_TRY(need_K(c));
+    const double lam = c->dim == 3 ? bulk - 2.0 * shear / 3.0 : bulk - shear;
+    const double modulus = lam + 2.0 * shear;
+    double *dl = nullptr;
+    PDX_HIP(hipMalloc((void **)&dl, sizeof(double) * c->ni));
+    int r = pdx_launch_dt_est(c, modulus, dl);
+    std::vector<double> h;
+    if (r == PDX_OK) r = download(c, h, dl, c->ni);
+    hipFree(dl);
+    if (r != PDX_OK) return r;
+    double lmax = 0.0;
+    for (double v : h) lmax = std::max(lmax, v);
+    if (lmax > 0) {
+        *dt = 0.5 * (2.0 / sqrt(lmax));
+        return PDX_OK;
+    }
+    HostMirror &m = mirror(c);
+    double avg = 0.0;
+    for (double v : m.vol) avg += v;
+    avg /= c->n;
+    const double cw = sqrt((bulk + 4 * shear / 3) / (rho_mean + 1e-20));
+    const double sp = c->dim == 3 ? cbrt(avg) : sqrt(avg);
+    *dt = 0.1 * sp / (cw + 1e-20);
+    return PDX_OK;
+}
+
+extern "C" int pdx_adr_mass(pdx_ctx *c, double beta)
+{
+    CHECK_CTX(c);
+    PDX_TRY(need_K(c));
+    if (!(beta > 0.0)) return fail(PDX_EINVAL, "beta must be positive");
+    PDX_TRY(pdx_launch_adr_mass(c, beta));
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    c->have_mass = true;
+    return PDX_OK;
+}
+
+static int zero_v_kernel(pdx_ctx *c)
+{
+    PDX_HIP(hipMemsetAsync(c->d.v, 0, sizeof(double4) * c->ni, c->stream));
+    return PDX_OK;
+}
+
+extern "C" int pdx_kd_step(pdx_ctx *c, double dt, double visc, double *prev_ke, int32_t *inc, double *ke,
+                           double *residual, int32_t *reset)
+{
+    CHECK_CTX(c);
+    PDX_TRY(need_K(c));
+    PDX_TRY(read_sc(c));
+    for (int k = 0; k < c->ncase; ++k) {
+        c->h_sc[k].prev_ke = prev_ke ? prev_ke[k] : 0.0;
+        c->h_sc[k].inc = inc ? inc[k] : 1;
+        c->h_sc[k].converged = 0;
+        c->h_sc[k].tol = -1.0;
+        c->h_sc[k].reset_pending = 0;
+    }
+    PDX_TRY(write_sc(c));
+    PDX_TRY(pdx_launch_kd_move(c, c->ucur, false, dt, false));
+    c->ucur ^= 1;
+    if (c->form == PDX_FORM_NOSB) PDX_TRY(pdx_launch_fpass(c, c->ucur, false, false, false));
+    PDX_TRY(pdx_launch_force(c, c->ucur, 2, false, dt, visc));
+    PDX_TRY(read_sc(c));
+    bool any_reset = false;
+    for (int k = 0; k < c->ncase; ++k) {
+        if (prev_ke) prev_ke[k] = c->h_sc[k].prev_ke;
+        if (inc) inc[k] = c->h_sc[k].inc;
+        if (ke) ke[k] = c->h_sc[k].ke;
+        if (residual) residual[k] = c->h_sc[k].res;
+        if (reset) reset[k] = c->h_sc[k].reset_pending;
+        any_reset |= c->h_sc[k].reset_pending != 0;
+    }
+    if (any_reset) {
+        // velocities of the cases that reset go to zero now (nosb_solver.py:239-243)
+        if (c->ncase == 1) {
+            PDX_TRY(zero_v_kernel(c));
+        } else {
+            for (int k = 0; k < c->ncase; ++k)
+                if (c->h_sc[k].reset_pending)
+                    PDX_HIP(hipMemsetAsync(c->d.v + c->case_p0[k], 0,
+                                           sizeof(double4) * (c->case_p0[k + 1] - c->case_p0[k]), c->stream));
+        }
+        for (int k = 0; k < c->ncase; ++k) c->h_sc[k].reset_pending = 0;
+        PDX_TRY(write_sc(c));
+    }
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    c->have_A = true;
+    return PDX_OK;
+}
+
+// one iteration of the chosen method on the context stream
+static int launch_iter(pdx_ctx *c, int method, double dt, double visc, bool solve_mode)
+{
+    if (method == PDX_DR_ADAPTIVE) {
+        if (c->form == PDX_FORM_NOSB) PDX_TRY(pdx_launch_fpass(c, c->ucur, false, false, solve_mode));
+        PDX_TRY(pdx_launch_force(c, c->ucur, 1, solve_mode, 0.0, 0.0));
+        c->ucur ^= 1;
+    } else {
+        PDX_TRY(pdx_launch_kd_move(c, c->ucur, solve_mode, dt, true));
+        c->ucur ^= 1;
+        if (c->form == PDX_FORM_NOSB) PDX_TRY(pdx_launch_fpass(c, c->ucur, false, false, solve_mode));
+        PDX_TRY(pdx_launch_force(c, c->ucur, 2, solve_mode, dt, visc));
+    }
+    return PDX_OK;
+}
+
+extern "C" int pdx_dr_solve(pdx_ctx *c, const pdx_dr_params *p, pdx_dr_result *res)
+{
+    CHECK_CTX(c);
+    if (!p || !res) return fail(PDX_EINVAL, "params/result NULL");
+    PDX_TRY(need_K(c));
+    const int method = p->method;
+    if (method != PDX_DR_ADAPTIVE && method != PDX_DR_KINETIC) return fail(PDX_EINVAL, "unknown DR method");
+    if (p->max_iterations < 0) return fail(PDX_EINVAL, "max_iterations < 0");
+    if (method == PDX_DR_KINETIC && !(p->dt > 0.0)) return fail(PDX_EINVAL, "kinetic DR needs dt > 0");
+    if (method == PDX_DR_ADAPTIVE && !c->have_mass) PDX_TRY(pdx_adr_mass(c, 1.0));
+    int chunk = p->chunk > 0 ? p->chunk : 64;
+    if (chunk & 1) chunk += 1;
+    // velocities to zero (nosb_solver.py:282); adaptive DR also clears its previous force
+    PDX_HIP(hipMemsetAsync(c->d.v, 0, sizeof(double4) * c->ni, c->stream));
+    if (method == PDX_DR_ADAPTIVE) PDX_HIP(hipMemsetAsync(c->d.fp, 0, sizeof(double4) * c->ni, c->stream));
+    // reference residual of the start state unless given (nosb_solver.py:287-293)
+    std::vector<double> ref(c->ncase, p->ref);
+    if (!(p->ref > 0.0)) {
+        if (c->form == PDX_FORM_NOSB) PDX_TRY(pdx_launch_fpass(c, c->ucur, false, false, false));
+        PDX_TRY(pdx_launch_force(c, c->ucur, 0, false, 0.0, 0.0));
+        PDX_TRY(read_sc(c));
+        for (int k = 0; k < c->ncase; ++k) ref[k] = c->h_sc[k].res < 1e-20 ? 1.0 : c->h_sc[k].res;
+    }
+    PDX_TRY(read_sc(c));
+    for (int k = 0; k < c->ncase; ++k) {
+        DScal &s = c->h_sc[k];
+        s.ref = ref[k];
+        s.tol = p->tol;
+        s.res = 0.0;
+        s.rel = 1.0;
+        s.c = 0.0;
+        s.cmax = p->cmax > 0.0 ? p->cmax : 1.9;
+        s.ke = 0.0;
+        s.prev_ke = 0.0;
+        s.iter = 0;
+        s.conv_iter = 0;
+        s.resets = 0;
+        s.converged = 0;
+        s.sol_buf = -1;
+        s.first = 1;
+        s.inc = 1;
+        s.reset_pending = 0;
+        s.counter = 0;
+    }
+    PDX_TRY(write_sc(c));
+    const int ucur0 = c->ucur;
+    auto t0 = std::chrono::steady_clock::now();
+    int64_t launched = 0;
+    bool done = p->max_iterations == 0;
+    bool captured_here = false;
+    while (!done) {
+        const int64_t left = p->max_iterations - launched;
+        if (p->use_graph && left >= chunk) {
+            const int64_t key = ((int64_t)chunk << 3) | ((int64_t)method << 1) | c->ucur;
+            // kinetic DR bakes dt and the viscous factor into the graph: capture once per solve
+            if (!c->gexec || c->gkey != key || (method == PDX_DR_KINETIC && !captured_here)) {
+                invalidate_graph(c);
+                captured_here = true;
+                hipGraph_t g;
+                const int u0 = c->ucur;
+                PDX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                int r = PDX_OK;
+                for (int t = 0; t < chunk && r == PDX_OK; ++t) r = launch_iter(c, method, p->dt, p->viscous_damping, true);
+                hipError_t ec = hipStreamEndCapture(c->stream, &g);
+                c->ucur = u0;
+                if (r != PDX_OK) return r;
+                PDX_HIP(ec);
+                PDX_HIP(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+                hipGraphDestroy(g);
+                c->gkey = key;
+            }
+            PDX_HIP(hipGraphLaunch(c->gexec, c->stream));
+            // chunk is even: the buffer parity is unchanged
+            launched += chunk;
+        } else {
+            const int64_t m = std::min<int64_t>(left, chunk);
+            for (int64_t t = 0; t < m; ++t) PDX_TRY(launch_iter(c, method, p->dt, p->viscous_damping, true));
+            launched += m;
+        }
+        PDX_TRY(read_sc(c));
+        bool all = true;
+        for (int k = 0; k < c->ncase; ++k) all = all && c->h_sc[k].converged;
+        done = all || launched >= p->max_iterations;
+    }
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    (void)ucur0;
+    // converged cases: bring the solution into the current buffer
+    for (int k = 0; k < c->ncase; ++k) {
+        const DScal &s = c->h_sc[k];
+        if (s.converged && s.sol_buf >= 0 && s.sol_buf != c->ucur) {
+            const size_t o = c->case_p0[k], nn = c->case_p0[k + 1] - c->case_p0[k];
+            PDX_HIP(hipMemcpyAsync(c->d.u[c->ucur] + o, c->d.u[s.sol_buf] + o, sizeof(double4) * nn,
+                                   hipMemcpyDeviceToDevice, c->stream));
+        }
+    }
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < c->ncase; ++k) {
+        const DScal &s = c->h_sc[k];
+        pdx_dr_result &o = res[k];
+        memset(&o, 0, sizeof(o));
+        o.converged = s.converged;
+        o.iterations = s.converged ? s.conv_iter : s.iter;
+        o.residual = s.res;
+        o.relative_residual = s.rel;
+        o.reference_residual = s.ref;
+        o.damping = s.c;
+        o.resets = s.resets;
+        o.elapsed_s = el;
+    }
+    c->have_A = false;
+    return PDX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// measurement
+// ---------------------------------------------------------------------------
+extern "C" int pdx_time_kernels(pdx_ctx *c, int32_t iters, double *ms)
+{
+    CHECK_CTX(c);
+    PDX_TRY(need_K(c));
+    if (iters <= 0) return fail(PDX_EINVAL, "iters must be positive");
+    if (!c->have_mass) PDX_TRY(pdx_adr_mass(c, 1.0));
+    PDX_TRY(read_sc(c));
+    for (int k = 0; k < c->ncase; ++k) {
+        c->h_sc[k].converged = 0;
+        c->h_sc[k].tol = -1.0;
+        c->h_sc[k].counter = 0;
+        if (!(c->h_sc[k].ref > 0)) c->h_sc[k].ref = 1.0;
+    }
+    PDX_TRY(write_sc(c));
+    std::vector<hipEvent_t> ev(3 * iters);
+    for (auto &e : ev) PDX_HIP(hipEventCreate(&e));
+    for (int t = 0; t < iters; ++t) {
+        PDX_HIP(hipEventRecord(ev[3 * t], c->stream));
+        if (c->form == PDX_FORM_NOSB) PDX_TRY(pdx_launch_fpass(c, c->ucur, false, false, false));
+        PDX_HIP(hipEventRecord(ev[3 * t + 1], c->stream));
+        PDX_TRY(pdx_launch_force(c, c->ucur, 1, false, 0.0, 0.0));
+        PDX_HIP(hipEventRecord(ev[3 * t + 2], c->stream));
+        c->ucur ^= 1;
+    }
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    double a = 0.0, b = 0.0;
+    for (int t = 0; t < iters; ++t) {
+        float x = 0.f, y = 0.f;
+        hipEventElapsedTime(&x, ev[3 * t], ev[3 * t + 1]);
+        hipEventElapsedTime(&y, ev[3 * t + 1], ev[3 * t + 2]);
+        a += x;
+        b += y;
+    }
+    float tot = 0.f;
+    hipEventElapsedTime(&tot, ev[0], ev[3 * iters - 1]);
+    for (auto &e : ev) hipEventDestroy(e);
+    ms[0] = a / iters;
+    ms[1] = b / iters;
+    ms[2] = tot;
+    c->have_A = false;
+    return PDX_OK;
+}
+
+extern "C" int pdx_stats(pdx_ctx *c, int64_t *out4)
+{
+    CHECK_CTX(c);
+    int64_t intact = 0;
+    if (c->have_bonds) {
+        std::vector<uint32_t> bj;
+        PDX_TRY(download(c, bj, c->d.bj, (size_t)std::max<int64_t>(c->slots, 1)));
+        for (int64_t s = 0; s < c->slots; ++s)
+            if (bj[s] != 0xffffffffu && !(bj[s] & PDX_BROKEN)) intact++;
+    }
+    out4[0] = c->n;
+    out4[1] = c->nb;
+    out4[2] = intact;
+    out4[3] = c->slots;
+    return PDX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// generic field access
+// ---------------------------------------------------------------------------
+static void pack_sym(const double *full, float *p6)
+{
+    p6[0] = (float)full[0]; p6[1] = (float)full[4]; p6[2] = (float)full[8];
+    p6[3] = (float)full[1]; p6[4] = (float)full[5]; p6[5] = (float)full[2];
+}
+
+static void unpack_sym(const float *p6, double *full)
+{
+    full[0] = p6[0]; full[4] = p6[1]; full[8] = p6[2];
+    full[1] = full[3] = p6[3];
+    full[5] = full[7] = p6[4];
+    full[2] = full[6] = p6[5];
+}
+
+extern "C" int pdx_get(pdx_ctx *c, int32_t field, void *out)
+{
+    CHECK_CTX(c);
+    if (!out) return fail(PDX_EINVAL, "out is NULL");
+    if (!c->have_particles) return fail(PDX_ESTATE, "set particles first");
+    HostMirror &m = mirror(c);
+    double *o = (double *)out;
+    const int D = c->dim;
+    switch (field) {
+    case PDX_FIELD_X:
+        memcpy(o, c->hX.data(), sizeof(double) * c->hX.size());
+        return PDX_OK;
+    case PDX_FIELD_x: {
+        PDX_TRY(get_vec(c, c->d.u[c->ucur], o));
+        for (size_t k = 0; k < c->hX.size(); ++k) o[k] = c->hX[k] + o[k];
+        return PDX_OK;
+    }
+    case PDX_FIELD_u: return get_vec(c, c->d.u[c->ucur], o);
+    case PDX_FIELD_v: return get_vec(c, c->d.v, o);
+    case PDX_FIELD_a: return get_vec(c, c->d.fp, o);
+    case PDX_FIELD_f: return get_vec(c, c->d.f, o);
+    case PDX_FIELD_VOLUME: memcpy(o, m.vol.data(), sizeof(double) * c->n); return PDX_OK;
+    case PDX_FIELD_MASS: memcpy(o, m.mass.data(), sizeof(double) * c->n); return PDX_OK;
+    case PDX_FIELD_DENSITY: memcpy(o, m.rho.data(), sizeof(double) * c->n); return PDX_OK;
+    case PDX_FIELD_BULK: memcpy(o, m.bulk.data(), sizeof(double) * c->n); return PDX_OK;
+    case PDX_FIELD_SHEAR: memcpy(o, m.shear.data(), sizeof(double) * c->n); return PDX_OK;
+    case PDX_FIELD_CSTAB: memcpy(o, m.cst.data(), sizeof(double) * c->n); return PDX_OK;
+    case PDX_FIELD_FEXT: memcpy(o, m.fext.data(), sizeof(double) * m.fext.size()); return PDX_OK;
+    case PDX_FIELD_FIXED: memcpy(out, m.fixed.data(), c->n); return PDX_OK;
+    case PDX_FIELD_K: return get_tensor(c, c->d.K64, o);
+    case PDX_FIELD_KINV: return get_tensor(c, c->d.Kinv64, o);
+    case PDX_FIELD_F: return get_tensor(c, c->d.Fd, o);
+    case PDX_FIELD_P: return get_tensor(c, c->d.Pd, o);
+    case PDX_FIELD_DAMAGE: return get_scalar(c, c->d.damage, o);
+    case PDX_FIELD_DRMASS: return get_scalar(c, c->d.lam, o);
+    case PDX_FIELD_NNB: {
+        int32_t *oi = (int32_t *)out;
+        for (int r = 0; r < c->n; ++r) oi[r] = m.nnb[c->inv[r]];
+        return PDX_OK;
+    }
+    case PDX_FIELD_SING: {
+        std::vector<KRec> kr;
+        PDX_TRY(download(c, kr, c->d.KR, c->ni));
+        uint8_t *ob = (uint8_t *)out;
+        for (int r = 0; r < c->n; ++r) ob[r] = (kr[c->inv[r]].flags & PDX_FL_SING) ? 1 : 0;
+        return PDX_OK;
+    }
+    case PDX_FIELD_EP:
+    case PDX_FIELD_EPBAR:
+    case PDX_FIELD_BACK: {
+        std::vector<J2State> s;
+        PDX_TRY(download(c, s, c->d.j2s, c->ni));
+        for (int r = 0; r < c->n; ++r) {
+            const J2State &t = s[c->inv[r]];
+            if (field == PDX_FIELD_EP) unpack_sym(t.ep, o + (size_t)r * 9);
+            else if (field == PDX_FIELD_BACK) unpack_sym(t.beta, o + (size_t)r * 9);
+            else o[r] = t.epbar;
+        }
+        return PDX_OK;
+    }
+    default: return fail(PDX_EINVAL, "unknown field id");
+    }
+    (void)D;
+}
+
+extern "C" int pdx_set(pdx_ctx *c, int32_t field, const void *in)
+{
+    CHECK_CTX(c);
+    if (!in) return fail(PDX_EINVAL, "in is NULL");
+    if (!c->have_particles) return fail(PDX_ESTATE, "set particles first");
+    HostMirror &m = mirror(c);
+    const double *x = (const double *)in;
+    const int D = c->dim;
+    switch (field) {
+    case PDX_FIELD_x: {
+        std::vector<double> u(c->hX.size());
+        for (size_t k = 0; k < u.size(); ++k) u[k] = x[k] - c->hX[k];
+        c->have_A = false;
+        return put_vec(c, c->d.u[c->ucur], u.data());
+    }
+    case PDX_FIELD_u: c->have_A = false; return put_vec(c, c->d.u[c->ucur], x);
+    case PDX_FIELD_v: return put_vec(c, c->d.v, x);
+    case PDX_FIELD_a: return put_vec(c, c->d.fp, x);
+    case PDX_FIELD_f: return put_vec(c, c->d.f, x);
+    case PDX_FIELD_VOLUME:
+    case PDX_FIELD_MASS:
+    case PDX_FIELD_DENSITY: {
+        for (int r = 0; r < c->n; ++r) {
+            if (field == PDX_FIELD_VOLUME) m.vol[r] = x[r];
+            else if (field == PDX_FIELD_MASS) m.mass[r] = x[r];
+            else m.rho[r] = x[r];
+        }
+        if (field == PDX_FIELD_VOLUME) {
+            c->have_K = c->have_mass = false;
+            invalidate_graph(c);
+            return push_prec(c);
+        }
+        return put_scalar(c, c->d.mass, m.mass.data());
+    }
+    case PDX_FIELD_BULK:
+    case PDX_FIELD_SHEAR: {
+        std::vector<double> b(m.bulk), s(m.shear);
+        if (field == PDX_FIELD_BULK) b.assign(x, x + c->n);
+        else s.assign(x, x + c->n);
+        // J2 parameters are kept on the device; only bulk/shear are rewritten
+        std::vector<float> w((size_t)c->ni * 2, 0.f);
+        for (int i = 0; i < c->ni; ++i) {
+            int r = c->perm[i];
+            if (r < 0) continue;
+            w[(size_t)i * 2] = (float)b[r];
+            w[(size_t)i * 2 + 1] = (float)s[r];
+        }
+        m.bulk = b;
+        m.shear = s;
+        c->have_mass = false;
+        return kr_put(c, w, KR_OFF_BULK, 2);
+    }
+    case PDX_FIELD_CSTAB: return pdx_set_stabilization(c, c->G_s, x, 0.0);
+    case PDX_FIELD_FEXT: return pdx_set_bc(c, m.fixed.data(), x);
+    case PDX_FIELD_FIXED: {
+        std::vector<double> fe(m.fext);
+        return pdx_set_bc(c, (const uint8_t *)in, fe.data());
+    }
+    case PDX_FIELD_DAMAGE: return put_scalar(c, c->d.damage, x);
+    case PDX_FIELD_DRMASS: c->have_mass = true; return put_scalar(c, c->d.lam, x);
+    case PDX_FIELD_EP:
+    case PDX_FIELD_EPBAR:
+    case PDX_FIELD_BACK: {
+        std::vector<J2State> s;
+        PDX_TRY(download(c, s, c->d.j2s, c->ni));
+        for (int r = 0; r < c->n; ++r) {
+            J2State &t = s[c->inv[r]];
+            if (field == PDX_FIELD_EP) pack_sym(x + (size_t)r * 9, t.ep);
+            else if (field == PDX_FIELD_BACK) pack_sym(x + (size_t)r * 9, t.beta);
+            else t.epbar = (float)x[r];
+        }
+        return upload(c, c->d.j2s, s);
+    }
+    default: return fail(PDX_EINVAL, "field is read-only or unknown");
+    }
+    (void)D;
+}
