@@ -88,7 +88,7 @@ typedef struct {
     int32_t method;       /* PDX_DR_ADAPTIVE or PDX_DR_KINETIC */
     int32_t use_graph;    /* 1: replay iteration chunks as a captured hipGraph */
     int32_t chunk;        /* iterations per host convergence check (even; 0 = default) */
-    int32_t reserved;
+    int32_t resume;       /* 1: continue from the current velocity / damping state */
     int64_t max_iterations;
     double tol;           /* stop when residual / ref < tol (strict, nosb_solver.py:318) */
     double ref;           /* > 0: fixed reference residual; <= 0: residual of the start state (nosb_solver.py:288-293) */
@@ -182,6 +182,12 @@ int pdx_kd_step(pdx_ctx *ctx, double dt, double viscous_damping, double *prev_ke
 /* NOSBSolver.solve, nosb_solver.py:262-332, iterations on the device; results per case
  * (res has ncase entries) */
 int pdx_dr_solve(pdx_ctx *ctx, const pdx_dr_params *params, pdx_dr_result *res);
+
+/* ---- field access -------------------------------------------------------------- */
+/* field.to_numpy() / field.from_numpy() of the reference's Taichi fields
+ * (particles.py:37-62, 221-231); layout per field id above, caller order */
+int pdx_get(pdx_ctx *ctx, int32_t field, void *out);
+int pdx_set(pdx_ctx *ctx, int32_t field, const void *in);
 
 /* ---- measurement ------------------------------------------------------------ */
 /* times `iters` adaptive-DR iterations with HIP events around every launch on the

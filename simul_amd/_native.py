@@ -15,7 +15,7 @@ import numpy as np
 from .validation import FEAValidationError
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_lib", "libpdx.so")
+LIB_PATH = os.environ.get("SIMUL_AMD_LIB") or os.path.join(_HERE, "_lib", "libpdx.so")
 
 OK, EINVAL, ESTATE, EDEVICE, ENOMEM, EUNSUP = 0, 1, 2, 3, 4, 5
 
@@ -46,7 +46,7 @@ FIELD_KIND = {
 
 class DRParams(C.Structure):
     _fields_ = [("method", C.c_int32), ("use_graph", C.c_int32), ("chunk", C.c_int32),
-                ("reserved", C.c_int32), ("max_iterations", C.c_int64), ("tol", C.c_double),
+                ("resume", C.c_int32), ("max_iterations", C.c_int64), ("tol", C.c_double),
                 ("ref", C.c_double), ("dt", C.c_double), ("viscous_damping", C.c_double),
                 ("cmax", C.c_double)]
 

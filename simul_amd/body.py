@@ -199,10 +199,10 @@ class PDBody:
                 "prev_ke": pk, "ke_increasing": inc.astype(bool)}
 
     def solve(self, method="adaptive", tol=1e-6, max_iterations=100000, ref=0.0, dt=0.0,
-              viscous_damping=0.0, cmax=1.9, use_graph=True, chunk=64):
+              viscous_damping=0.0, cmax=1.9, use_graph=True, chunk=64, resume=False):
         p = N.DRParams()
         p.method = {"adaptive": N.DR_ADAPTIVE, "kinetic": N.DR_KINETIC}[method]
-        p.use_graph, p.chunk = int(bool(use_graph)), int(chunk)
+        p.use_graph, p.chunk, p.resume = int(bool(use_graph)), int(chunk), int(bool(resume))
         p.max_iterations, p.tol, p.ref = int(max_iterations), float(tol), float(ref)
         p.dt, p.viscous_damping, p.cmax = float(dt), float(viscous_damping), float(cmax)
         res = (N.DRResult * self.ncase)()

@@ -9,6 +9,7 @@
 #include <numeric>
 
 #include "pdx_internal.h"
+#include "pdx_layout.h"
 
 static thread_local std::string g_err;
 void pdx_set_error(const std::string &msg) { g_err = msg; }
@@ -28,31 +29,6 @@ static int fail(int code, const std::string &msg)
 
 #define CHECK_CTX(c) \
     if (!(c)) return fail(PDX_EINVAL, "null context")
-
-// ---------------------------------------------------------------------------
-// Morton keys (21 bits per axis in 3-D, 31 in 2-D)
-// ---------------------------------------------------------------------------
-static uint64_t spread3(uint64_t x)
-{
-    x &= 0x1fffff;
-    x = (x | x << 32) & 0x1f00000000ffffULL;
-    x = (x | x << 16) & 0x1f0000ff0000ffULL;
-    x = (x | x << 8) & 0x100f00f00f00f00fULL;
-    x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
-    x = (x | x << 2) & 0x1249249249249249ULL;
-    return x;
-}
-
-static uint64_t spread2(uint64_t x)
-{
-    x &= 0x7fffffff;
-    x = (x | x << 16) & 0x0000ffff0000ffffULL;
-    x = (x | x << 8) & 0x00ff00ff00ff00ffULL;
-    x = (x | x << 4) & 0x0f0f0f0f0f0f0f0fULL;
-    x = (x | x << 2) & 0x3333333333333333ULL;
-    x = (x | x << 1) & 0x5555555555555555ULL;
-    return x;
-}
 
 // ---------------------------------------------------------------------------
 // allocation helpers
@@ -155,16 +131,8 @@ static int create_common(int32_t ncase, const int64_t *off, int32_t dim, pdx_ctx
         return fail(PDX_EDEVICE, "hipHostMalloc failed");
     }
     // internal layout: each case padded to a multiple of PDX_BLOCK
-    c->case_p0.resize(ncase + 1);
-    int64_t p = 0;
-    for (int k = 0; k < ncase; ++k) {
-        c->case_p0[k] = (int32_t)p;
-        int64_t nk = off[k + 1] - off[k];
-        p += ((nk + PDX_BLOCK - 1) / PDX_BLOCK) * PDX_BLOCK;
-        if (nk == 0) p += PDX_BLOCK;  // empty case keeps one (inactive) block
-    }
-    c->case_p0[ncase] = (int32_t)p;
-    c->ni = (int32_t)p;
+    c->case_p0 = pdx_layout::case_starts(c->case_off);
+    c->ni = c->case_p0[ncase];
     int nb0 = c->ni / PDX_BLOCK;
     c->nblk = ((nb0 + PDX_NXCD - 1) / PDX_NXCD) * PDX_NXCD;
     c->perm.assign(c->ni, -1);
@@ -202,13 +170,8 @@ static int create_common(int32_t ncase, const int64_t *off, int32_t dim, pdx_ctx
         return r;
     }
     // block -> case tables; padding blocks of the grid join the last case
-    std::vector<int32_t> cob(c->nblk), cb0(ncase + 1);
-    for (int k = 0; k < ncase; ++k) {
-        cb0[k] = c->case_p0[k] / PDX_BLOCK;
-        for (int b = c->case_p0[k] / PDX_BLOCK; b < c->case_p0[k + 1] / PDX_BLOCK; ++b) cob[b] = k;
-    }
-    for (int b = nb0; b < c->nblk; ++b) cob[b] = ncase - 1;
-    cb0[ncase] = c->nblk;
+    std::vector<int32_t> cob, cb0;
+    pdx_layout::block_cases(c->case_p0, c->nblk, cob, cb0);
     hipMemcpy(d.case_of_blk, cob.data(), sizeof(int32_t) * cob.size(), hipMemcpyHostToDevice);
     hipMemcpy(d.case_blk0, cb0.data(), sizeof(int32_t) * cb0.size(), hipMemcpyHostToDevice);
     std::vector<DScal> sc(ncase);
@@ -308,22 +271,22 @@ static int push_flags(pdx_ctx *c)
     return kr_put(c, w, KR_OFF_FLAGS, 1);
 }
 
-// caller-order n*dim doubles -> internal double4
-static int put_vec(pdx_ctx *c, double4 *dst, const double *src)
+// caller-order n*dim doubles -> internal pdx_d4
+static int put_vec(pdx_ctx *c, pdx_d4 *dst, const double *src)
 {
-    std::vector<double4> h(c->ni, make_double4(0, 0, 0, 0));
+    std::vector<pdx_d4> h(c->ni, d4(0, 0, 0));
     for (int i = 0; i < c->ni; ++i) {
         int r = c->perm[i];
         if (r < 0) continue;
         const double *s = src + (size_t)r * c->dim;
-        h[i] = make_double4(s[0], s[1], c->dim == 3 ? s[2] : 0.0, 0.0);
+        h[i] = d4(s[0], s[1], c->dim == 3 ? s[2] : 0.0);
     }
     return upload(c, dst, h);
 }
 
-static int get_vec(pdx_ctx *c, const double4 *src, double *dst)
+static int get_vec(pdx_ctx *c, const pdx_d4 *src, double *dst)
 {
-    std::vector<double4> h;
+    std::vector<pdx_d4> h;
     PDX_TRY(download(c, h, src, c->ni));
     for (int i = 0; i < c->ni; ++i) {
         int r = c->perm[i];
@@ -403,35 +366,7 @@ extern "C" int pdx_set_particles(pdx_ctx *c, const double *X, const double *vol,
             if (!isfinite(X[(size_t)r * D + q])) return fail(PDX_EINVAL, "positions must be finite");
     }
     c->hX.assign(X, X + (size_t)c->n * D);
-    // Morton order per case
-    for (int k = 0; k < c->ncase; ++k) {
-        const int64_t r0 = c->case_off[k], r1 = c->case_off[k + 1], nk = r1 - r0;
-        if (nk == 0) continue;
-        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-        for (int64_t r = r0; r < r1; ++r)
-            for (int q = 0; q < D; ++q) {
-                lo[q] = std::min(lo[q], X[r * D + q]);
-                hi[q] = std::max(hi[q], X[r * D + q]);
-            }
-        double vol_box = 1.0;
-        for (int q = 0; q < D; ++q) vol_box *= std::max(hi[q] - lo[q], 1e-300);
-        double hq = pow(vol_box / (double)nk, 1.0 / D);
-        for (int q = 0; q < D; ++q) hq = std::max(hq, (hi[q] - lo[q]) / (D == 3 ? 2097151.0 : 2147483647.0));
-        if (!(hq > 0.0)) hq = 1.0;
-        std::vector<std::pair<uint64_t, int32_t>> key(nk);
-        for (int64_t r = r0; r < r1; ++r) {
-            uint64_t qv[3] = {0, 0, 0};
-            for (int q = 0; q < D; ++q) qv[q] = (uint64_t)floor((X[r * D + q] - lo[q]) / hq);
-            uint64_t kk = D == 3 ? (spread3(qv[0]) << 2 | spread3(qv[1]) << 1 | spread3(qv[2]))
-                                 : (spread2(qv[0]) << 1 | spread2(qv[1]));
-            key[r - r0] = {kk, (int32_t)r};
-        }
-        std::sort(key.begin(), key.end());
-        for (int64_t t = 0; t < nk; ++t) {
-            c->perm[c->case_p0[k] + t] = key[t].second;
-            c->inv[key[t].second] = (int32_t)(c->case_p0[k] + t);
-        }
-    }
+    pdx_layout::morton_order(D, X, c->case_off, c->case_p0, c->perm, c->inv);
     HostMirror &m = mirror(c);
     for (int r = 0; r < c->n; ++r) {
         m.vol[r] = vol[r];
@@ -446,11 +381,11 @@ extern "C" int pdx_set_particles(pdx_ctx *c, const double *X, const double *vol,
     PDX_TRY(upload(c, c->d.case_of_p, cop));
     PDX_TRY(put_scalar(c, c->d.mass, m.mass.data()));
     // state back to the reference initial values (particles.py:159-173)
-    PDX_HIP(hipMemset(c->d.u[0], 0, sizeof(double4) * c->ni));
-    PDX_HIP(hipMemset(c->d.u[1], 0, sizeof(double4) * c->ni));
-    PDX_HIP(hipMemset(c->d.v, 0, sizeof(double4) * c->ni));
-    PDX_HIP(hipMemset(c->d.fp, 0, sizeof(double4) * c->ni));
-    PDX_HIP(hipMemset(c->d.f, 0, sizeof(double4) * c->ni));
+    PDX_HIP(hipMemset(c->d.u[0], 0, sizeof(pdx_d4) * c->ni));
+    PDX_HIP(hipMemset(c->d.u[1], 0, sizeof(pdx_d4) * c->ni));
+    PDX_HIP(hipMemset(c->d.v, 0, sizeof(pdx_d4) * c->ni));
+    PDX_HIP(hipMemset(c->d.fp, 0, sizeof(pdx_d4) * c->ni));
+    PDX_HIP(hipMemset(c->d.f, 0, sizeof(pdx_d4) * c->ni));
     PDX_HIP(hipMemset(c->d.damage, 0, sizeof(double) * c->ni));
     PDX_HIP(hipMemset(c->d.KR, 0, sizeof(KRec) * c->ni));
     c->ucur = 0;
@@ -550,64 +485,11 @@ extern "C" int pdx_set_bc(pdx_ctx *c, const uint8_t *fixed, const double *fext)
 // ---------------------------------------------------------------------------
 static int build_grid(pdx_ctx *c, double horizon)
 {
-    const int D = c->dim;
-    const double h = horizon * 1.01;  // cell edge >= horizon (neighbor.py:47)
-    std::vector<GridP> gp(c->ncase);
+    std::vector<GridP> gp;
     std::vector<int32_t> cstart, cidx;
-    cidx.reserve(c->n);
-    int64_t cell_off = 0;
-    for (int k = 0; k < c->ncase; ++k) {
-        GridP g;
-        memset(&g, 0, sizeof(g));
-        g.h = h;
-        g.nc[0] = g.nc[1] = g.nc[2] = 1;
-        const int i0 = c->case_p0[k], i1 = c->case_p0[k + 1];
-        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-        bool any = false;
-        for (int i = i0; i < i1; ++i) {
-            int r = c->perm[i];
-            if (r < 0) continue;
-            any = true;
-            for (int q = 0; q < D; ++q) {
-                lo[q] = std::min(lo[q], c->hX[(size_t)r * D + q]);
-                hi[q] = std::max(hi[q], c->hX[(size_t)r * D + q]);
-            }
-        }
-        if (!any) for (int q = 0; q < 3; ++q) lo[q] = hi[q] = 0.0;
-        int64_t ncell = 1;
-        for (int q = 0; q < D; ++q) {
-            g.lo[q] = lo[q];
-            double span = (hi[q] - lo[q]) / h;
-            if (span > 4.0e6) return fail(PDX_EINVAL, "domain too large for the neighbour grid");
-            g.nc[q] = (int32_t)floor(span) + 1;
-            ncell *= g.nc[q];
-        }
-        if (ncell > (int64_t)1 << 28) return fail(PDX_EINVAL, "neighbour grid too large (sparse domain)");
-        g.cell_off = cell_off;
-        std::vector<int32_t> cnt(ncell + 1, 0);
-        std::vector<int64_t> cell_of(i1 - i0, -1);
-        for (int i = i0; i < i1; ++i) {
-            int r = c->perm[i];
-            if (r < 0) continue;
-            int64_t cc[3] = {0, 0, 0};
-            for (int q = 0; q < D; ++q) {
-                int64_t t = (int64_t)floor((c->hX[(size_t)r * D + q] - g.lo[q]) / g.h);
-                cc[q] = std::max<int64_t>(0, std::min<int64_t>(t, g.nc[q] - 1));
-            }
-            int64_t cell = (cc[0] * g.nc[1] + cc[1]) * g.nc[2] + cc[2];
-            cell_of[i - i0] = cell;
-            cnt[cell + 1]++;
-        }
-        const int32_t base = (int32_t)cidx.size();
-        for (int64_t t = 0; t < ncell; ++t) cnt[t + 1] += cnt[t];
-        std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
-        cidx.resize(base + cnt[ncell]);
-        for (int i = i0; i < i1; ++i)
-            if (cell_of[i - i0] >= 0) cidx[base + fill[cell_of[i - i0]]++] = i;
-        for (int64_t t = 0; t <= ncell; ++t) cstart.push_back(base + cnt[t]);
-        cell_off += ncell + 1;
-        gp[k] = g;
-    }
+    std::string err;
+    if (!pdx_layout::cell_grids(c->dim, c->hX.data(), c->case_p0, c->perm, horizon, gp, cstart, cidx, err))
+        return fail(PDX_EINVAL, err);
     PDX_TRY(dalloc(&c->d.cell_start, cstart.size()));
     PDX_TRY(dalloc(&c->d.cell_idx, std::max<size_t>(cidx.size(), 1)));
     PDX_TRY(upload(c, c->d.cell_start, cstart));
@@ -644,17 +526,8 @@ extern "C" int pdx_build_bonds(pdx_ctx *c, double horizon, int64_t *n_bonds)
     std::vector<int32_t> cnt;
     PDX_TRY(download(c, cnt, c->d.counts, c->ni));
     const int nsl = c->ni / PDX_SLICE;
-    c->h_slptr.assign(nsl + 1, 0);
     int64_t tot = 0;
-    for (int s = 0; s < nsl; ++s) {
-        int w = 0;
-        for (int t = 0; t < PDX_SLICE; ++t) {
-            w = std::max(w, cnt[s * PDX_SLICE + t]);
-            tot += cnt[s * PDX_SLICE + t];
-        }
-        c->h_slptr[s + 1] = c->h_slptr[s] + (int64_t)w * PDX_SLICE;
-    }
-    c->slots = c->h_slptr[nsl];
+    c->slots = pdx_layout::sell_pointers(cnt, c->h_slptr, tot);
     c->nb = tot;
     PDX_TRY(dalloc(&c->d.sl_ptr, (size_t)nsl + 1));
     PDX_TRY(upload(c, c->d.sl_ptr, c->h_slptr));
@@ -856,16 +729,16 @@ extern "C" int pdx_cold_residual(pdx_ctx *c, double *out)
 {
     CHECK_CTX(c);
     PDX_TRY(need_K(c));
-    double4 *save = nullptr;
-    PDX_HIP(hipMalloc((void **)&save, sizeof(double4) * c->ni));
-    PDX_HIP(hipMemcpyAsync(save, c->d.u[c->ucur], sizeof(double4) * c->ni, hipMemcpyDeviceToDevice, c->stream));
+    pdx_d4 *save = nullptr;
+    PDX_HIP(hipMalloc((void **)&save, sizeof(pdx_d4) * c->ni));
+    PDX_HIP(hipMemcpyAsync(save, c->d.u[c->ucur], sizeof(pdx_d4) * c->ni, hipMemcpyDeviceToDevice, c->stream));
     int r = pdx_launch_zero_free(c, c->ucur);
     if (r == PDX_OK && c->form == PDX_FORM_NOSB) r = pdx_launch_fpass(c, c->ucur, false, false, false);
     if (r == PDX_OK) r = pdx_launch_force(c, c->ucur, 0, false, 0.0, 0.0);
     if (r == PDX_OK) r = read_sc(c);
     if (r == PDX_OK)
         for (int k = 0; k < c->ncase; ++k) out[k] = c->h_sc[k].res;
-    hipMemcpyAsync(c->d.u[c->ucur], save, sizeof(double4) * c->ni, hipMemcpyDeviceToDevice, c->stream);
+    hipMemcpyAsync(c->d.u[c->ucur], save, sizeof(pdx_d4) * c->ni, hipMemcpyDeviceToDevice, c->stream);
     hipStreamSynchronize(c->stream);
     hipFree(save);
     c->have_A = false;
@@ -955,7 +828,7 @@ extern "C" int pdx_adr_mass(pdx_ctx *c, double beta)
 
 static int zero_v_kernel(pdx_ctx *c)
 {
-    PDX_HIP(hipMemsetAsync(c->d.v, 0, sizeof(double4) * c->ni, c->stream));
+    PDX_HIP(hipMemsetAsync(c->d.v, 0, sizeof(pdx_d4) * c->ni, c->stream));
     return PDX_OK;
 }
 
@@ -995,7 +868,7 @@ extern "C" int pdx_kd_step(pdx_ctx *c, double dt, double visc, double *prev_ke, 
             for (int k = 0; k < c->ncase; ++k)
                 if (c->h_sc[k].reset_pending)
                     PDX_HIP(hipMemsetAsync(c->d.v + c->case_p0[k], 0,
-                                           sizeof(double4) * (c->case_p0[k + 1] - c->case_p0[k]), c->stream));
+                                           sizeof(pdx_d4) * (c->case_p0[k + 1] - c->case_p0[k]), c->stream));
         }
         for (int k = 0; k < c->ncase; ++k) c->h_sc[k].reset_pending = 0;
         PDX_TRY(write_sc(c));
@@ -1034,8 +907,10 @@ extern "C" int pdx_dr_solve(pdx_ctx *c, const pdx_dr_params *p, pdx_dr_result *r
     int chunk = p->chunk > 0 ? p->chunk : 64;
     if (chunk & 1) chunk += 1;
     // velocities to zero (nosb_solver.py:282); adaptive DR also clears its previous force
-    PDX_HIP(hipMemsetAsync(c->d.v, 0, sizeof(double4) * c->ni, c->stream));
-    if (method == PDX_DR_ADAPTIVE) PDX_HIP(hipMemsetAsync(c->d.fp, 0, sizeof(double4) * c->ni, c->stream));
+    if (!p->resume) {
+        PDX_HIP(hipMemsetAsync(c->d.v, 0, sizeof(pdx_d4) * c->ni, c->stream));
+        if (method == PDX_DR_ADAPTIVE) PDX_HIP(hipMemsetAsync(c->d.fp, 0, sizeof(pdx_d4) * c->ni, c->stream));
+    }
     // reference residual of the start state unless given (nosb_solver.py:287-293)
     std::vector<double> ref(c->ncase, p->ref);
     if (!(p->ref > 0.0)) {
@@ -1051,19 +926,21 @@ extern "C" int pdx_dr_solve(pdx_ctx *c, const pdx_dr_params *p, pdx_dr_result *r
         s.tol = p->tol;
         s.res = 0.0;
         s.rel = 1.0;
-        s.c = 0.0;
         s.cmax = p->cmax > 0.0 ? p->cmax : 1.9;
-        s.ke = 0.0;
-        s.prev_ke = 0.0;
         s.iter = 0;
         s.conv_iter = 0;
-        s.resets = 0;
         s.converged = 0;
         s.sol_buf = -1;
-        s.first = 1;
-        s.inc = 1;
-        s.reset_pending = 0;
         s.counter = 0;
+        if (!p->resume) {
+            s.c = 0.0;
+            s.ke = 0.0;
+            s.prev_ke = 0.0;
+            s.resets = 0;
+            s.first = 1;
+            s.inc = 1;
+            s.reset_pending = 0;
+        }
     }
     PDX_TRY(write_sc(c));
     const int ucur0 = c->ucur;
@@ -1113,7 +990,7 @@ extern "C" int pdx_dr_solve(pdx_ctx *c, const pdx_dr_params *p, pdx_dr_result *r
         const DScal &s = c->h_sc[k];
         if (s.converged && s.sol_buf >= 0 && s.sol_buf != c->ucur) {
             const size_t o = c->case_p0[k], nn = c->case_p0[k + 1] - c->case_p0[k];
-            PDX_HIP(hipMemcpyAsync(c->d.u[c->ucur] + o, c->d.u[s.sol_buf] + o, sizeof(double4) * nn,
+            PDX_HIP(hipMemcpyAsync(c->d.u[c->ucur] + o, c->d.u[s.sol_buf] + o, sizeof(pdx_d4) * nn,
                                    hipMemcpyDeviceToDevice, c->stream));
         }
     }

@@ -1,0 +1,950 @@
+// pdx_device.h -- device records and the per-particle bodies of every libpdx
+// kernel, written as host+device functions.  The HIP kernels in
+// pdx_kernels.hip are thin wrappers over these functions; the same functions
+// compile with a host C++ compiler for the bounds-checked CPU build of the
+// tests (no HIP types are used here).
+//
+// Particle data lives in Morton order, one case after another, each case
+// padded to a multiple of PDX_BLOCK particles so that every block of every
+// launch belongs to exactly one case.  Padding particles have no bonds and are
+// flagged inactive.
+//
+// Bonds are stored SELL-64 (sliced ELL, slice = 64 consecutive particles =
+// one wavefront): entry k of particle i sits at sl_ptr[i/64] + 64*k + i%64,
+// so the k-th bond of the 64 lanes of a wave is one coalesced 256-byte read.
+// An entry is the neighbour's internal index with bit 31 = broken.
+//
+// Reference arithmetic: backend/fea/peridynamics/core/nosb.py:59-94 (K),
+// :96-126 (F), :211-227 (P), :229-275 (force); neighbor.py:154-206 (sets);
+// bonds.py:116-137,161-180; damage.py:94-142; nosb_solver.py:94-198 (DR).
+#pragma once
+#include <math.h>
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define PDX_HD __host__ __device__ __forceinline__
+#else
+#define PDX_HD inline
+#endif
+
+#define PDX_BLOCK 256
+#define PDX_SLICE 64
+#define PDX_BROKEN 0x80000000u
+#define PDX_NXCD 8
+
+// flags word of KRec
+#define PDX_FL_FIXMASK 0x7u
+#define PDX_FL_SING 0x8u
+#define PDX_FL_LOADED 0x10u
+#define PDX_FL_ACTIVE 0x20u
+
+struct alignas(32) pdx_d4 {
+    double x, y, z, w;
+};
+
+PDX_HD pdx_d4 d4(double x, double y, double z) { return pdx_d4{x, y, z, 0.0}; }
+
+// gather record of the reference geometry: 32 bytes
+struct alignas(32) PRec {
+    double X[3];
+    float V;  // particle volume
+    float c;  // stabilisation micromodulus c_i
+};
+
+// per-particle constants of the hot loop: 64 bytes
+struct alignas(16) KRec {
+    float Kinv[9];  // K^-1 in FP32 (K itself when singular)
+    float m[3];     // sum_j w_ij V_j xi_ij over intact bonds
+    float bulk, shear;
+    uint32_t flags;
+    uint32_t nnb;   // bonds in the row (intact + broken)
+};
+
+// correspondence tensor A = P K^-1 of the current iterate: 48 bytes
+struct alignas(16) ARec {
+    float A[9];
+    float pad[3];
+};
+
+struct alignas(16) J2Par {
+    float sy0, hiso, hkin, pad;
+};
+
+// committed J2 state (symmetric tensors as xx,yy,zz,xy,yz,xz): 64 bytes
+struct alignas(16) J2State {
+    float ep[6];
+    float epbar;
+    float beta[6];
+    float pad[3];
+};
+
+// per-case device scalars of the iteration
+struct DScal {
+    double ref, tol, res, rel;
+    double c, cmax;
+    double ke, prev_ke;
+    double num, den;
+    int64_t iter, conv_iter, resets, pad64;
+    int32_t converged, sol_buf, first, inc;
+    int32_t reset_pending, pad32;
+    uint32_t counter, pad_u;
+};
+
+// per-case neighbour-search grid
+struct GridP {
+    double lo[3];
+    double h;
+    int32_t nc[3];
+    int32_t pad;
+    int64_t cell_off;  // offset of this case's cells in cell_start
+};
+
+// device pointers handed to every kernel (by value)
+struct Dev {
+    int32_t ni;        // internal particle count (padded)
+    int32_t nblk;      // launch blocks (multiple of PDX_NXCD)
+    int32_t ncase;
+    int32_t dim;
+    PRec *P;
+    KRec *KR;
+    ARec *AR;
+    J2Par *j2p;
+    J2State *j2s;
+    pdx_d4 *u[2];
+    pdx_d4 *v;
+    pdx_d4 *fp;        // previous force (adaptive DR) / acceleration (kinetic DR)
+    pdx_d4 *f;         // force output of force-only passes
+    pdx_d4 *fext;
+    double *mass;      // physical mass rho V (kinetic DR)
+    double *lam;       // adaptive-DR diagonal mass
+    double *V64;       // particle volume in FP64 (setup kernels)
+    double *K64, *Kinv64;
+    float *Fd, *Pd;    // diagnostic F and P (9 per particle)
+    double *damage;
+    int32_t *init_nb;  // bonds at build time
+    int64_t *sl_ptr;
+    uint32_t *bj;
+    int32_t *case_of_blk;
+    int32_t *case_blk0;  // first block of each case (ncase+1)
+    DScal *sc;
+    double *part;      // per-block partial sums (4 per block)
+    // neighbour search
+    GridP *grid;
+    int32_t *case_of_p;  // case of each internal particle
+    int32_t *cell_start; // per cell start into cell_idx (+1 sentinel per case grid)
+    int32_t *cell_idx;
+    int32_t *counts;
+    double h, inv_h, h2;  // horizon, 1/horizon, horizon^2
+    double G_s;
+    int32_t model, form;
+};
+
+PDX_HD int64_t slot0(const Dev &d, int i) { return d.sl_ptr[i >> 6] + (i & 63); }
+
+// ---------------------------------------------------------------------------
+// small tensor algebra
+// ---------------------------------------------------------------------------
+template <int D>
+PDX_HD void matmul_f(const float *A, const float *B, float *C)
+{
+    for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) {
+            float s = 0.f;
+            for (int k = 0; k < D; ++k) s = fmaf(A[r * D + k], B[k * D + c], s);
+            C[r * D + c] = s;
+        }
+}
+
+template <int D>
+PDX_HD float lame_f(float Kb, float mu)
+{
+    return D == 3 ? Kb - 2.0f * mu / 3.0f : Kb - mu;
+}
+
+// linear elastic: eps = sym(H), P = lam tr(eps) I + 2 mu eps   (nosb.py:218-227)
+template <int D>
+PDX_HD void stress_le(const float *H, float Kb, float mu, float *P)
+{
+    const float lam = lame_f<D>(Kb, mu);
+    float tr = 0.f;
+    for (int q = 0; q < D; ++q) tr += H[q * D + q];
+    for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) {
+            const float e = 0.5f * (H[r * D + c] + H[c * D + r]);
+            P[r * D + c] = 2.0f * mu * e + (r == c ? lam * tr : 0.0f);
+        }
+}
+
+// Neo-Hookean P = mu (F - F^-T) + lam ln(J_s) F^-T with F = I + H
+// (neo_hookean.py:117-127 sigma, :176-180 P = J sigma F^-T), evaluated without
+// cancellation: J - 1 from the invariants of H, F^-T = I + M^T with
+// M = F^-1 - I = -F^-1 H.
+template <int D>
+PDX_HD void stress_nh(const float *H, float Kb, float mu, float *P)
+{
+    const float lam = lame_f<D>(Kb, mu);
+    float Jm1, adj[D * D];
+    if constexpr (D == 3) {
+        const float tr = H[0] + H[4] + H[8];
+        float trH2 = 0.f;
+        for (int r = 0; r < D; ++r)
+            for (int c = 0; c < D; ++c) trH2 = fmaf(H[r * D + c], H[c * D + r], trH2);
+        const float *h = H;
+        const float detH = h[0] * (h[4] * h[8] - h[5] * h[7]) - h[1] * (h[3] * h[8] - h[5] * h[6]) +
+                           h[2] * (h[3] * h[7] - h[4] * h[6]);
+        Jm1 = tr + 0.5f * (tr * tr - trH2) + detH;
+        const float F0 = 1.f + h[0], F4 = 1.f + h[4], F8 = 1.f + h[8];
+        const float F1 = h[1], F2 = h[2], F3 = h[3], F5 = h[5], F6 = h[6], F7 = h[7];
+        float *a = adj;
+        a[0] = F4 * F8 - F5 * F7; a[1] = F2 * F7 - F1 * F8; a[2] = F1 * F5 - F2 * F4;
+        a[3] = F5 * F6 - F3 * F8; a[4] = F0 * F8 - F2 * F6; a[5] = F2 * F3 - F0 * F5;
+        a[6] = F3 * F7 - F4 * F6; a[7] = F1 * F6 - F0 * F7; a[8] = F0 * F4 - F1 * F3;
+    } else {
+        Jm1 = H[0] + H[3] + (H[0] * H[3] - H[1] * H[2]);
+        adj[0] = 1.f + H[3];
+        adj[1] = -H[1];
+        adj[2] = -H[2];
+        adj[3] = 1.f + H[0];
+    }
+    const float J = 1.f + Jm1;
+    const float lnJ = J > 1e-8f ? log1pf(Jm1) : logf(1e-8f);
+    const float iJ = 1.0f / J;
+    float M[D * D];  // M = -F^-1 H
+    for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) {
+            float s = 0.f;
+            for (int k = 0; k < D; ++k) s = fmaf(adj[r * D + k], H[k * D + c], s);
+            M[r * D + c] = -s * iJ;
+        }
+    for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) {
+            const float mt = M[c * D + r];
+            P[r * D + c] = mu * (H[r * D + c] - mt) + lam * lnJ * ((r == c ? 1.f : 0.f) + mt);
+        }
+}
+
+// J2 small-strain radial return with linear isotropic + kinematic hardening,
+// FP64 (j2_plasticity.py:202-249; h_kin = 0 is the reference).  Symmetric
+// tensors packed xx,yy,zz,xy,yz,xz; off-diagonal entries count twice in norms.
+PDX_HD void stress_j2(const float *H, float Kb_, float mu_, const J2Par &pp, const J2State &st, float *P,
+                      J2State *out)
+{
+    const double Kb = Kb_, mu = mu_;
+    const double lam = Kb - 2.0 * mu / 3.0;
+    double eps[6];
+    eps[0] = H[0]; eps[1] = H[4]; eps[2] = H[8];
+    eps[3] = 0.5 * ((double)H[1] + (double)H[3]);
+    eps[4] = 0.5 * ((double)H[5] + (double)H[7]);
+    eps[5] = 0.5 * ((double)H[2] + (double)H[6]);
+    double ee[6];
+    for (int q = 0; q < 6; ++q) ee[q] = eps[q] - (double)st.ep[q];
+    const double tre = ee[0] + ee[1] + ee[2];
+    double sig[6];
+    for (int q = 0; q < 6; ++q) sig[q] = 2.0 * mu * ee[q] + (q < 3 ? lam * tre : 0.0);
+    const double pr = (sig[0] + sig[1] + sig[2]) / 3.0;
+    double eta[6], nsq = 0.0;
+    for (int q = 0; q < 6; ++q) {
+        eta[q] = sig[q] - (q < 3 ? pr : 0.0) - (double)st.beta[q];
+        nsq += (q < 3 ? 1.0 : 2.0) * eta[q] * eta[q];
+    }
+    const double nrm = sqrt(nsq > 1e-30 ? nsq : 1e-30);
+    const double qv = sqrt(1.5) * nrm;
+    const double epbar = st.epbar;
+    const double fy = qv - ((double)pp.sy0 + (double)pp.hiso * epbar);
+    double sfin[6];
+    if (fy <= 0.0) {
+        for (int q = 0; q < 6; ++q) sfin[q] = sig[q];
+        if (out) *out = st;
+    } else {
+        const double dg = fy / (3.0 * mu + (double)pp.hiso + (double)pp.hkin);
+        const double fac = 1.0 - 3.0 * mu * dg / qv;
+        J2State ns = st;
+        for (int q = 0; q < 6; ++q) {
+            const double nf = eta[q] / nrm;
+            ns.ep[q] = (float)((double)st.ep[q] + sqrt(1.5) * dg * nf);
+            const double bn = (double)st.beta[q] + sqrt(2.0 / 3.0) * (double)pp.hkin * dg * nf;
+            ns.beta[q] = (float)bn;
+            sfin[q] = bn + fac * eta[q] + (q < 3 ? pr : 0.0);
+        }
+        ns.epbar = (float)(epbar + dg);
+        if (out) *out = ns;
+    }
+    P[0] = (float)sfin[0]; P[4] = (float)sfin[1]; P[8] = (float)sfin[2];
+    P[1] = P[3] = (float)sfin[3];
+    P[5] = P[7] = (float)sfin[4];
+    P[2] = P[6] = (float)sfin[5];
+}
+
+// influence weight w = max(0, 1 - |xi|/delta) (bonds.py:131-137) times V_j
+PDX_HD float bond_weight(float L, float invh, float Vj)
+{
+    return (L > 0.f ? fmaxf(0.f, 1.f - L * invh) : 1.f) * Vj;
+}
+
+// ---------------------------------------------------------------------------
+// F-pass: H = [sum_j w V_j du (x) xi] K^-1, P(H), A = P K^-1
+// ---------------------------------------------------------------------------
+template <int D, int MODEL, bool DIAG, bool COMMIT>
+PDX_HD void fpass_one(const Dev &d, int i, int ubuf)
+{
+    const KRec kr = d.KR[i];
+    if (!(kr.flags & PDX_FL_ACTIVE)) return;
+    const pdx_d4 *__restrict__ U = d.u[ubuf];
+    const PRec pi = d.P[i];
+    const pdx_d4 ui = U[i];
+    const float invh = (float)d.inv_h;
+    // H accumulated in FP64: a 100+-term FP32 sum carries ~1e-6 relative error
+    // into every A_j and from there into the residual
+    double Hd[D * D];
+    for (int q = 0; q < D * D; ++q) Hd[q] = 0.0;
+    const int64_t base = slot0(d, i);
+    const int nnb = (int)kr.nnb;
+    for (int k = 0; k < nnb; ++k) {
+        const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
+        if (e & PDX_BROKEN) continue;
+        const int j = (int)e;
+        const PRec pj = d.P[j];
+        const pdx_d4 uj = U[j];
+        float xi[3], du[3];
+        xi[0] = (float)(pj.X[0] - pi.X[0]);
+        xi[1] = (float)(pj.X[1] - pi.X[1]);
+        xi[2] = (float)(pj.X[2] - pi.X[2]);
+        du[0] = (float)(uj.x - ui.x);
+        du[1] = (float)(uj.y - ui.y);
+        du[2] = (float)(uj.z - ui.z);
+        float L2 = 0.f;
+        for (int r = 0; r < D; ++r) L2 = fmaf(xi[r], xi[r], L2);
+        const float w = bond_weight(sqrtf(L2), invh, pj.V);
+        double xd[3];
+        for (int c = 0; c < D; ++c) xd[c] = (double)xi[c];
+        for (int r = 0; r < D; ++r) {
+            const double wd = (double)(w * du[r]);
+            for (int c = 0; c < D; ++c) Hd[r * D + c] = fma(wd, xd[c], Hd[r * D + c]);
+        }
+    }
+    float Hs[D * D], Km[D * D], H[D * D];
+    for (int q = 0; q < D * D; ++q) Hs[q] = (float)Hd[q];
+    for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) Km[r * D + c] = kr.Kinv[r * 3 + c];
+    const bool sing = (kr.flags & PDX_FL_SING) != 0;
+    if (sing) {
+        // K^-1 fell back to I, so F = N = K + sum w du (x) xi  (nosb.py:92-94, 126)
+        for (int q = 0; q < D * D; ++q) H[q] = Hs[q] + Km[q] - ((q % (D + 1)) == 0 ? 1.f : 0.f);
+    } else {
+        matmul_f<D>(Hs, Km, H);
+    }
+    float P[9];
+    if constexpr (MODEL == 1) {
+        stress_nh<D>(H, kr.bulk, kr.shear, P);
+    } else if constexpr (MODEL == 2 && D == 3) {
+        J2State ns;
+        stress_j2(H, kr.bulk, kr.shear, d.j2p[i], d.j2s[i], P, COMMIT ? &ns : nullptr);
+        if (COMMIT) d.j2s[i] = ns;
+    } else {
+        stress_le<D>(H, kr.bulk, kr.shear, P);
+    }
+    ARec ar;
+    if (sing) {
+        for (int q = 0; q < D * D; ++q) ar.A[q] = P[q];
+    } else {
+        matmul_f<D>(P, Km, ar.A);
+    }
+    for (int q = D * D; q < 9; ++q) ar.A[q] = 0.f;
+    ar.pad[0] = ar.pad[1] = ar.pad[2] = 0.f;
+    d.AR[i] = ar;
+    if (DIAG) {
+        for (int q = 0; q < D * D; ++q) {
+            d.Fd[(int64_t)i * 9 + q] = H[q] + ((q % (D + 1)) == 0 ? 1.f : 0.f);
+            d.Pd[(int64_t)i * 9 + q] = P[q];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// force pass for one particle: f = A_i m_i + sum_j w V_j [A_j xi + g eta]
+// + external load, then the DR update of UPD:
+//   UPD 0: write f
+//   UPD 1: adaptive DR (damping lagged one step), u_out = u_in + V
+//   UPD 2: kinetic DR velocity update (nosb_solver.py:164-182)
+// red[0] += |f|^2 over free DOFs; UPD 1: red[1], red[2] = Rayleigh-quotient
+// numerator and denominator; UPD 2: red[1] = kinetic energy.
+// ---------------------------------------------------------------------------
+template <int D, int FORM, int UPD>
+PDX_HD void force_one(const Dev &d, int i, int ubuf, int cs, double dt, double visc, double *red)
+{
+    const KRec kr = d.KR[i];
+    if (!(kr.flags & PDX_FL_ACTIVE)) return;
+    const pdx_d4 *__restrict__ U = d.u[ubuf];
+    const PRec pi = d.P[i];
+    const pdx_d4 ui = U[i];
+    const float invh = (float)d.inv_h;
+    const float Gs = (float)d.G_s;
+    const float ci = pi.c;
+    double fa[3] = {0.0, 0.0, 0.0};
+    const int64_t base = slot0(d, i);
+    const int nnb = (int)kr.nnb;
+    for (int k = 0; k < nnb; ++k) {
+        const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
+        if (e & PDX_BROKEN) continue;
+        const int j = (int)e;
+        const PRec pj = d.P[j];
+        const pdx_d4 uj = U[j];
+        float xi[3], du[3];
+        xi[0] = (float)(pj.X[0] - pi.X[0]);
+        xi[1] = (float)(pj.X[1] - pi.X[1]);
+        xi[2] = (float)(pj.X[2] - pi.X[2]);
+        du[0] = (float)(uj.x - ui.x);
+        du[1] = (float)(uj.y - ui.y);
+        du[2] = (float)(uj.z - ui.z);
+        float L2 = 0.f, xd = 0.f, dd = 0.f;
+        for (int r = 0; r < D; ++r) {
+            L2 = fmaf(xi[r], xi[r], L2);
+            xd = fmaf(xi[r], du[r], xd);
+            dd = fmaf(du[r], du[r], dd);
+        }
+        const float L = sqrtf(L2);
+        const float w = bond_weight(L, invh, pj.V);
+        float eta[3];
+        for (int r = 0; r < D; ++r) eta[r] = xi[r] + du[r];
+        // |eta|^2 = |xi|^2 + (2 xi.du + du.du); the stretch uses the
+        // cancellation-free |eta| - |xi| = (2 xi.du + du.du) / (|eta| + |xi|)
+        const float dn = fmaf(2.f, xd, dd);
+        const float el = sqrtf(fmaxf(L2 + dn, 0.f));
+        float tb[3] = {0.f, 0.f, 0.f};
+        if (FORM == 0) {
+            const ARec aj = d.AR[j];
+            for (int r = 0; r < D; ++r) {
+                float t = 0.f;
+                for (int c = 0; c < D; ++c) t = fmaf(aj.A[r * D + c], xi[c], t);
+                tb[r] = t;
+            }
+            if (el > 1e-10f && L > 1e-10f) {
+                const float s = dn / ((el + L) * L);
+                const float g = Gs * 0.5f * (ci + pj.c) * s / el;
+                for (int r = 0; r < D; ++r) tb[r] = fmaf(g, eta[r], tb[r]);
+            }
+        } else if (el > 1e-10f) {
+            const float s = dn / ((el + L) * L);
+            const float g = 0.5f * (ci + pj.c) * s / el;
+            for (int r = 0; r < D; ++r) tb[r] = g * eta[r];
+        }
+        for (int r = 0; r < D; ++r) fa[r] += (double)(w * tb[r]);
+    }
+    if (FORM == 0) {
+        const ARec ai = d.AR[i];
+        for (int r = 0; r < D; ++r) {
+            float t = 0.f;
+            for (int c = 0; c < D; ++c) t = fmaf(ai.A[r * D + c], kr.m[c], t);
+            fa[r] += (double)t;
+        }
+    }
+    if (kr.flags & PDX_FL_LOADED) {
+        const pdx_d4 fe = d.fext[i];
+        fa[0] += fe.x;
+        fa[1] += fe.y;
+        if (D == 3) fa[2] += fe.z;
+    }
+    const uint32_t fm = kr.flags & PDX_FL_FIXMASK;
+    for (int r = 0; r < D; ++r)
+        if (!((fm >> r) & 1u)) red[0] += fa[r] * fa[r];
+    if (UPD == 0) {
+        d.f[i] = d4(fa[0], fa[1], D == 3 ? fa[2] : 0.0);
+    } else if (UPD == 1) {
+        const DScal &S = d.sc[cs];
+        const double c = S.c;
+        const int first = S.first;
+        const double lam = d.lam[i];
+        const pdx_d4 vo4 = d.v[i], fo4 = d.fp[i];
+        const double vo[3] = {vo4.x, vo4.y, vo4.z}, fo[3] = {fo4.x, fo4.y, fo4.z};
+        const double uo[3] = {ui.x, ui.y, ui.z};
+        double vn[3] = {0.0, 0.0, 0.0}, un[3] = {uo[0], uo[1], uo[2]}, fn[3] = {0.0, 0.0, 0.0};
+        for (int r = 0; r < D; ++r) {
+            if ((fm >> r) & 1u) continue;
+            const double F = fa[r];
+            if (first) {
+                vn[r] = F / (2.0 * lam);
+            } else {
+                if (vo[r] != 0.0) {
+                    const double kk = -(F - fo[r]) / (lam * vo[r]);
+                    red[1] += uo[r] * kk * uo[r];
+                }
+                red[2] += uo[r] * uo[r];
+                vn[r] = ((2.0 - c) * vo[r] + 2.0 * F / lam) / (2.0 + c);
+            }
+            fn[r] = F;
+            un[r] = uo[r] + vn[r];
+        }
+        d.v[i] = d4(vn[0], vn[1], vn[2]);
+        d.fp[i] = d4(fn[0], fn[1], fn[2]);
+        d.u[ubuf ^ 1][i] = d4(un[0], un[1], un[2]);
+    } else {
+        const double m = d.mass[i];
+        const pdx_d4 vo4 = d.v[i], ao4 = d.fp[i];
+        double vv[3] = {vo4.x, vo4.y, vo4.z}, aa[3] = {ao4.x, ao4.y, ao4.z};
+        const double damp = 1.0 - visc;
+        for (int r = 0; r < D; ++r) {
+            if (!((fm >> r) & 1u) && m > 1e-20) {
+                const double an = fa[r] / m;
+                vv[r] += 0.5 * (aa[r] + an) * dt;
+                vv[r] *= damp;
+                aa[r] = an;
+                red[1] += 0.5 * m * vv[r] * vv[r];
+            } else {
+                vv[r] = 0.0;
+                aa[r] = 0.0;
+            }
+        }
+        d.v[i] = d4(vv[0], vv[1], vv[2]);
+        d.fp[i] = d4(aa[0], aa[1], aa[2]);
+        d.f[i] = d4(fa[0], fa[1], D == 3 ? fa[2] : 0.0);
+    }
+}
+
+// per-case scalar update done once per force pass by the case's last block
+template <int UPD>
+PDX_HD void case_update(DScal &S, const double *tot, int ubuf)
+{
+    const double res = sqrt(tot[0]);
+    S.res = res;
+    S.rel = res / S.ref;
+    if (UPD == 1) {
+        if (S.rel < S.tol && !S.converged) {
+            S.converged = 1;
+            S.conv_iter = S.iter;
+            S.sol_buf = ubuf;
+        }
+        S.num = tot[1];
+        S.den = tot[2];
+        if (!S.first) {
+            const double cn = (tot[2] > 0.0 && tot[1] > 0.0) ? 2.0 * sqrt(tot[1] / tot[2]) : 0.0;
+            S.c = cn < S.cmax ? cn : S.cmax;
+        }
+        S.first = 0;
+        S.iter += 1;
+    } else if (UPD == 2) {
+        double ke = tot[1];
+        if (ke < S.prev_ke && S.inc) {
+            S.reset_pending = 1;
+            ke = 0.0;
+            S.inc = 0;
+            S.resets += 1;
+        } else {
+            S.reset_pending = 0;
+            if (ke > S.prev_ke) S.inc = 1;
+        }
+        S.prev_ke = ke;
+        S.ke = ke;
+        S.iter += 1;
+        if (S.rel < S.tol && !S.converged) {
+            S.converged = 1;
+            S.conv_iter = S.iter;
+            S.sol_buf = ubuf;
+        }
+    }
+}
+
+// kinetic-damping position update x += v dt + a dt^2/2 on free DOFs
+// (nosb_solver.py:156-162), after a pending velocity reset (:239-243)
+template <int D>
+PDX_HD void kd_move_one(const Dev &d, int i, int ubuf_in, double dt, bool reset)
+{
+    const KRec kr = d.KR[i];
+    const pdx_d4 u = d.u[ubuf_in][i];
+    pdx_d4 v = d.v[i];
+    if (reset) {
+        v = d4(0.0, 0.0, 0.0);
+        d.v[i] = v;
+    }
+    const pdx_d4 a = d.fp[i];
+    const uint32_t fm = kr.flags & PDX_FL_FIXMASK;
+    pdx_d4 o = u;
+    if (kr.flags & PDX_FL_ACTIVE) {
+        if (!(fm & 1u)) o.x = u.x + v.x * dt + 0.5 * a.x * dt * dt;
+        if (!(fm & 2u)) o.y = u.y + v.y * dt + 0.5 * a.y * dt * dt;
+        if (D == 3 && !(fm & 4u)) o.z = u.z + v.z * dt + 0.5 * a.z * dt * dt;
+    }
+    d.u[ubuf_in ^ 1][i] = o;
+}
+
+// ---------------------------------------------------------------------------
+// setup (FP64)
+// ---------------------------------------------------------------------------
+// neighbour scan over the 27 (9) cells around i; distance summed as
+// (dx*dx + dy*dy) + dz*dz without contraction, the reference's set test
+// (neighbor.py:179-183, 199-203); FILL writes the row sorted by index
+template <int D, bool FILL>
+PDX_HD void neighbors_one(const Dev &d, int i)
+{
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    const int cs = d.case_of_p[i];
+    if (cs < 0) {
+        if (!FILL) d.counts[i] = 0;
+        return;
+    }
+    const GridP g = d.grid[cs];
+    const PRec pi = d.P[i];
+    int c[3] = {0, 0, 0};
+    for (int q = 0; q < D; ++q) {
+        int t = (int)floor((pi.X[q] - g.lo[q]) / g.h);
+        c[q] = t < 0 ? 0 : (t >= g.nc[q] ? g.nc[q] - 1 : t);
+    }
+    const int rz = D == 3 ? 1 : 0;
+    int cnt = 0;
+    int64_t base = 0;
+    if (FILL) base = slot0(d, i);
+    for (int a = c[0] - 1; a <= c[0] + 1; ++a) {
+        if (a < 0 || a >= g.nc[0]) continue;
+        for (int b = c[1] - 1; b <= c[1] + 1; ++b) {
+            if (b < 0 || b >= g.nc[1]) continue;
+            for (int e = c[2] - rz; e <= c[2] + rz; ++e) {
+                if (e < 0 || e >= g.nc[2]) continue;
+                const int64_t cell = g.cell_off + ((int64_t)a * g.nc[1] + b) * g.nc[2] + e;
+                const int k0 = d.cell_start[cell], k1 = d.cell_start[cell + 1];
+                for (int k = k0; k < k1; ++k) {
+                    const int j = d.cell_idx[k];
+                    if (j == i) continue;
+                    const PRec pj = d.P[j];
+                    double s = 0.0;
+                    for (int q = 0; q < D; ++q) {
+                        const double t = pj.X[q] - pi.X[q];
+                        s = s + t * t;
+                    }
+                    if (s < d.h2) {
+                        if (FILL) d.bj[base + (int64_t)cnt * PDX_SLICE] = (uint32_t)j;
+                        cnt++;
+                    }
+                }
+            }
+        }
+    }
+    if (!FILL) {
+        d.counts[i] = cnt;
+        return;
+    }
+    for (int p = 1; p < cnt; ++p) {
+        const uint32_t key = d.bj[base + (int64_t)p * PDX_SLICE];
+        int q = p - 1;
+        while (q >= 0 && d.bj[base + (int64_t)q * PDX_SLICE] > key) {
+            d.bj[base + (int64_t)(q + 1) * PDX_SLICE] = d.bj[base + (int64_t)q * PDX_SLICE];
+            --q;
+        }
+        d.bj[base + (int64_t)(q + 1) * PDX_SLICE] = key;
+    }
+    d.init_nb[i] = cnt;
+}
+
+PDX_HD double det3(const double *M)
+{
+    return M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) +
+           M[2] * (M[3] * M[7] - M[4] * M[6]);
+}
+
+// shape tensor K = sum_intact w xi (x) xi V_j in FP64 with its inverse
+// (nosb.py:59-94); mode 0: |det| > tol, mode 1: |det| > tol (tr K/dim)^dim;
+// also m_i = sum_intact w V_j xi for the force pass
+template <int D>
+PDX_HD void shape_one(const Dev &d, int i, int mode, double tol)
+{
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    KRec kr = d.KR[i];
+    if (!(kr.flags & PDX_FL_ACTIVE)) return;
+    const PRec pi = d.P[i];
+    double K[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double m[3] = {0, 0, 0};
+    const int64_t base = slot0(d, i);
+    for (int k = 0; k < (int)kr.nnb; ++k) {
+        const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
+        if (e & PDX_BROKEN) continue;
+        const PRec pj = d.P[(int)e];
+        double xi[3] = {0, 0, 0}, s = 0.0;
+        for (int q = 0; q < D; ++q) {
+            xi[q] = pj.X[q] - pi.X[q];
+            s = s + xi[q] * xi[q];
+        }
+        const double L = sqrt(s);
+        double w = 1.0;
+        if (L > 0) {
+            w = 1.0 - L / d.h;
+            w = w > 0.0 ? w : 0.0;
+        }
+        const double Vj = d.V64[(int)e];
+        for (int r = 0; r < D; ++r) {
+            for (int c = 0; c < D; ++c) K[r * D + c] += w * xi[r] * xi[c] * Vj;
+            m[r] += w * Vj * xi[r];
+        }
+    }
+    double tr = 0.0;
+    for (int q = 0; q < D; ++q) tr += K[q * D + q];
+    tr /= D;
+    const double det = D == 3 ? det3(K) : K[0] * K[3] - K[1] * K[2];
+    const double thr = mode == 1 ? tol * (D == 3 ? tr * tr * tr : tr * tr) : tol;
+    const bool sing = !(fabs(det) > thr && det != 0.0);
+    double Ki[9];
+    if (!sing) {
+        if (D == 3) {
+            Ki[0] = (K[4] * K[8] - K[5] * K[7]) / det;
+            Ki[1] = (K[2] * K[7] - K[1] * K[8]) / det;
+            Ki[2] = (K[1] * K[5] - K[2] * K[4]) / det;
+            Ki[3] = (K[5] * K[6] - K[3] * K[8]) / det;
+            Ki[4] = (K[0] * K[8] - K[2] * K[6]) / det;
+            Ki[5] = (K[2] * K[3] - K[0] * K[5]) / det;
+            Ki[6] = (K[3] * K[7] - K[4] * K[6]) / det;
+            Ki[7] = (K[1] * K[6] - K[0] * K[7]) / det;
+            Ki[8] = (K[0] * K[4] - K[1] * K[3]) / det;
+        } else {
+            Ki[0] = K[3] / det; Ki[1] = -K[1] / det; Ki[2] = -K[2] / det; Ki[3] = K[0] / det;
+        }
+    } else {
+        for (int q = 0; q < D * D; ++q) Ki[q] = (q % (D + 1)) == 0 ? 1.0 : 0.0;
+    }
+    for (int q = 0; q < D * D; ++q) {
+        d.K64[(int64_t)i * 9 + q] = K[q];
+        d.Kinv64[(int64_t)i * 9 + q] = Ki[q];
+    }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            kr.Kinv[r * 3 + c] = (r < D && c < D) ? (float)(sing ? K[r * D + c] : Ki[r * D + c]) : 0.f;
+    for (int q = 0; q < 3; ++q) kr.m[q] = (float)m[q];
+    kr.flags = sing ? (kr.flags | PDX_FL_SING) : (kr.flags & ~PDX_FL_SING);
+    d.KR[i] = kr;
+}
+
+// adaptive-DR mass (a Gershgorin bound of the force-density stiffness):
+//   g_ik = w V_k Kinv_i xi_ik, G_i = sum_k g_ik, S_i = sum_k |g_ik| + |G_i|,
+//   C_i = max(3 K_i, 2 mu_i), k_corr_i = sqrt3 (C_i S_i |G_i| + sum_j w V_j C_j S_j |Kinv_j xi_ji|),
+//   k_stab_i = 2 sqrt3 G_s sum_j c_ij w V_j / |xi|,  lam_i = beta/4 (k_corr_i + k_stab_i)
+// pass A writes C_i S_i and |G_i|
+template <int D>
+PDX_HD void adr_mass_a_one(const Dev &d, int i, double *cs_out, double *g_out)
+{
+    const KRec kr = d.KR[i];
+    cs_out[i] = 0.0;
+    g_out[i] = 0.0;
+    if (!(kr.flags & PDX_FL_ACTIVE) || d.form != 0) return;
+    const PRec pi = d.P[i];
+    const double *Ki = d.Kinv64 + (int64_t)i * 9;
+    double G[3] = {0, 0, 0}, sabs = 0.0;
+    const int64_t base = slot0(d, i);
+    for (int k = 0; k < (int)kr.nnb; ++k) {
+        const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
+        if (e & PDX_BROKEN) continue;
+        const PRec pj = d.P[(int)e];
+        double xi[3] = {0, 0, 0}, s = 0.0;
+        for (int q = 0; q < D; ++q) {
+            xi[q] = pj.X[q] - pi.X[q];
+            s += xi[q] * xi[q];
+        }
+        const double L = sqrt(s);
+        const double w = L > 0 ? fmax(0.0, 1.0 - L / d.h) : 1.0;
+        double gn = 0.0;
+        for (int r = 0; r < D; ++r) {
+            double t = 0.0;
+            for (int c = 0; c < D; ++c) t += Ki[r * D + c] * xi[c];
+            t *= w * d.V64[(int)e];
+            G[r] += t;
+            gn += t * t;
+        }
+        sabs += sqrt(gn);
+    }
+    double gl = 0.0;
+    for (int r = 0; r < D; ++r) gl += G[r] * G[r];
+    gl = sqrt(gl);
+    const double C = fmax(3.0 * (double)kr.bulk, 2.0 * (double)kr.shear);
+    cs_out[i] = C * (sabs + gl);
+    g_out[i] = gl;
+}
+
+template <int D>
+PDX_HD void adr_mass_b_one(const Dev &d, int i, const double *cs_in, const double *g_in, double beta)
+{
+    const KRec kr = d.KR[i];
+    if (!(kr.flags & PDX_FL_ACTIVE)) {
+        d.lam[i] = 1.0;
+        return;
+    }
+    const PRec pi = d.P[i];
+    const bool nosb = d.form == 0;
+    double kc = nosb ? cs_in[i] * g_in[i] : 0.0, ks = 0.0;
+    const int64_t base = slot0(d, i);
+    for (int k = 0; k < (int)kr.nnb; ++k) {
+        const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
+        if (e & PDX_BROKEN) continue;
+        const int j = (int)e;
+        const PRec pj = d.P[j];
+        double xi[3] = {0, 0, 0}, s = 0.0;
+        for (int q = 0; q < D; ++q) {
+            xi[q] = pj.X[q] - pi.X[q];
+            s += xi[q] * xi[q];
+        }
+        const double L = sqrt(s);
+        const double w = L > 0 ? fmax(0.0, 1.0 - L / d.h) : 1.0;
+        if (nosb) {
+            const double *Kj = d.Kinv64 + (int64_t)j * 9;
+            double gn = 0.0;
+            for (int r = 0; r < D; ++r) {
+                double t = 0.0;
+                for (int c = 0; c < D; ++c) t -= Kj[r * D + c] * xi[c];
+                gn += t * t;
+            }
+            kc += w * d.V64[j] * cs_in[j] * sqrt(gn);
+        }
+        if (L > 1e-10) ks += 0.5 * ((double)pi.c + (double)pj.c) * w * d.V64[j] / L;
+    }
+    const double gs = nosb ? d.G_s : 1.0;
+    const double r3 = 1.7320508075688772;
+    double lam = 0.25 * beta * (r3 * kc + 2.0 * r3 * gs * ks);
+    if (!(lam > 0.0)) lam = 1.0;
+    d.lam[i] = lam;
+}
+
+// dt estimate per particle (nosb_solver.py:121-143): lam_i = k_eff / m_i
+template <int D>
+PDX_HD void dt_est_one(const Dev &d, int i, double modulus, double *out)
+{
+    const KRec kr = d.KR[i];
+    out[i] = 0.0;
+    if (!(kr.flags & PDX_FL_ACTIVE)) return;
+    const double mi = d.mass[i];
+    if (mi < 1e-30) return;
+    const PRec pi = d.P[i];
+    const double *Ki = d.Kinv64 + (int64_t)i * 9;
+    double ds[3] = {0, 0, 0}, dsq = 0.0;
+    const int64_t base = slot0(d, i);
+    for (int k = 0; k < (int)kr.nnb; ++k) {
+        const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
+        if (e & PDX_BROKEN) continue;
+        const PRec pj = d.P[(int)e];
+        double xi[3] = {0, 0, 0}, s = 0.0;
+        for (int q = 0; q < D; ++q) {
+            xi[q] = pj.X[q] - pi.X[q];
+            s += xi[q] * xi[q];
+        }
+        const double L = sqrt(s);
+        const double w = L > 0 ? fmax(0.0, 1.0 - L / d.h) : 1.0;
+        for (int r = 0; r < D; ++r) {
+            double t = 0.0;
+            for (int c = 0; c < D; ++c) t += Ki[r * D + c] * xi[c];
+            t *= w;
+            ds[r] += t;
+            dsq += t * t;
+        }
+    }
+    double s2 = 0.0;
+    for (int r = 0; r < D; ++r) s2 += ds[r] * ds[r];
+    out[i] = modulus * d.V64[i] * (s2 + dsq) / mi;
+}
+
+// critical-stretch bond breaking with the FP64 stretch of bonds.py:161-180:
+// x = X + u, eta = x_j - x_i, s = (|eta| - |xi|)/|xi| if |xi| > 1e-10, broken if
+// s > s_c strictly (damage.py:94-116); returns the bonds broken now
+template <int D>
+PDX_HD unsigned damage_one(const Dev &d, int i, int ubuf, double s_c)
+{
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    const KRec kr = d.KR[i];
+    if (!(kr.flags & PDX_FL_ACTIVE)) return 0;
+    const pdx_d4 *U = d.u[ubuf];
+    const PRec pi = d.P[i];
+    const pdx_d4 ui = U[i];
+    const double xi0[3] = {pi.X[0] + ui.x, pi.X[1] + ui.y, pi.X[2] + ui.z};
+    const int64_t base = slot0(d, i);
+    unsigned nb = 0;
+    for (int k = 0; k < (int)kr.nnb; ++k) {
+        const int64_t a = base + (int64_t)k * PDX_SLICE;
+        const uint32_t e = d.bj[a];
+        if (e & PDX_BROKEN) continue;
+        const int j = (int)e;
+        const PRec pj = d.P[j];
+        const pdx_d4 uj = U[j];
+        const double xj[3] = {pj.X[0] + uj.x, pj.X[1] + uj.y, pj.X[2] + uj.z};
+        double se = 0.0, sl = 0.0;
+        for (int q = 0; q < D; ++q) {
+            const double t = xj[q] - xi0[q];
+            se = se + t * t;
+            const double r = pj.X[q] - pi.X[q];
+            sl = sl + r * r;
+        }
+        const double el = sqrt(se), L = sqrt(sl);
+        double st = 0.0;
+        if (L > 1e-10) st = (el - L) / L;
+        if (st > s_c) {
+            d.bj[a] = e | PDX_BROKEN;
+            nb++;
+        }
+    }
+    return nb;
+}
+
+// phi_i = broken / initial bonds (damage.py:118-142)
+PDX_HD void local_damage_one(const Dev &d, int i)
+{
+    const KRec kr = d.KR[i];
+    const int init = d.init_nb[i];
+    if (!(kr.flags & PDX_FL_ACTIVE) || init <= 0) {
+        d.damage[i] = 0.0;
+        return;
+    }
+    const int64_t base = slot0(d, i);
+    int nbk = 0;
+    for (int k = 0; k < (int)kr.nnb; ++k) nbk += (d.bj[base + (int64_t)k * PDX_SLICE] & PDX_BROKEN) ? 1 : 0;
+    d.damage[i] = (double)nbk / (double)init;
+}
+
+// per-bond reference data in internal CSR order (bonds.py:116-137)
+template <int D>
+PDX_HD void bond_data_one(const Dev &d, int i, double *xi_o, double *len_o, double *w_o, const int64_t *rowi)
+{
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    const KRec kr = d.KR[i];
+    if (!(kr.flags & PDX_FL_ACTIVE)) return;
+    const PRec pi = d.P[i];
+    const int64_t base = slot0(d, i);
+    for (int k = 0; k < (int)kr.nnb; ++k) {
+        const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
+        const PRec pj = d.P[(int)(e & ~PDX_BROKEN)];
+        const int64_t b = rowi[i] + k;
+        double s = 0.0;
+        for (int q = 0; q < D; ++q) {
+            const double t = pj.X[q] - pi.X[q];
+            if (xi_o) xi_o[b * D + q] = t;
+            s = s + t * t;
+        }
+        const double L = sqrt(s);
+        if (len_o) len_o[b] = L;
+        if (w_o) w_o[b] = L > 0 ? fmax(0.0, 1.0 - L / d.h) : 1.0;
+    }
+}
+
+// |f|^2 over free DOFs from the stored force
+template <int D>
+PDX_HD double resid_one(const Dev &d, int i)
+{
+    const KRec kr = d.KR[i];
+    if (!(kr.flags & PDX_FL_ACTIVE)) return 0.0;
+    const pdx_d4 f = d.f[i];
+    const double fv[3] = {f.x, f.y, f.z};
+    const uint32_t fm = kr.flags & PDX_FL_FIXMASK;
+    double s = 0.0;
+    for (int r = 0; r < D; ++r)
+        if (!((fm >> r) & 1u)) s += fv[r] * fv[r];
+    return s;
+}
+
+// free DOFs of u set to zero (cold-start state)
+PDX_HD void zero_free_one(const Dev &d, int i, int ubuf)
+{
+    const uint32_t fm = d.KR[i].flags & PDX_FL_FIXMASK;
+    pdx_d4 u = d.u[ubuf][i];
+    if (!(fm & 1u)) u.x = 0.0;
+    if (!(fm & 2u)) u.y = 0.0;
+    if (!(fm & 4u)) u.z = 0.0;
+    d.u[ubuf][i] = u;
+}

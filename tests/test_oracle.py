@@ -1,0 +1,328 @@
+"""Pins the FP64 oracle (oracle/pd_oracle.c) before it is trusted as the checker.
+
+Each function of the oracle is checked against the reference's own known-answer
+tests (file:line cited), closed forms, invariants, or an independent
+pure-Python brute force on tiny inputs -- never against a retyping of its own
+formula.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.optimize as so
+
+from helpers import lattice
+from oracle import pdoracle as po
+from simul_amd import workloads as W
+
+
+def grid2(nx, ny, dx=1.0):
+    g = [np.arange(k, dtype=float) * dx for k in (nx, ny)]
+    return np.stack(np.meshgrid(*g, indexing="ij"), -1).reshape(-1, 2)
+
+
+# -- a1 neighbour sets ------------------------------------------------------
+def test_neighbor_kats_3d():
+    # backend/fea/peridynamics/tests/test_3d.py:67-138
+    X = lattice(3, 3, 3)
+    c = np.diff(po.neighbors(X, 1.1)[0])
+    assert c[13] == 6 and c[0] == 3
+    c = np.diff(po.neighbors(X, 1.5)[0])
+    assert c[13] == 18 and c[0] == 6
+
+
+def test_neighbor_kats_2d():
+    # backend/fea/peridynamics/tests/test_particles.py:84-146
+    c = np.diff(po.neighbors(grid2(3, 3), 1.5)[0])
+    assert c[4] == 8 and all(c[k] == 3 for k in (0, 2, 6, 8))
+    assert np.all(np.diff(po.neighbors(grid2(2, 2), 1.1)[0]) == 2)
+
+
+def test_config1_bond_count_closed_form():
+    # 20^3 cube, horizon 3.015 dx: interior stencil = lattice points with
+    # 0 < |v|^2 < 3.015^2 -> 122; total over the cube by direct enumeration
+    r = np.arange(-3, 4)
+    V = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)
+    d2 = (V ** 2).sum(1)
+    stencil = V[(d2 > 0) & (d2 < 3.015 ** 2)]
+    assert len(stencil) == 122
+    n = 20
+    per_axis = lambda off: np.array([n - abs(o) for o in off])
+    total = int(np.sum(per_axis(stencil[:, 0]) * per_axis(stencil[:, 1]) * per_axis(stencil[:, 2])))
+    row, _ = po.neighbors(W.cube().X, 3.015)
+    cnt = np.diff(row)
+    assert row[-1] == total == 816728
+    assert cnt.min() == 28 and cnt.max() == 122
+
+
+def test_cell_list_equals_brute_force_on_jittered_points():
+    X = lattice(7, 6, 5, jitter=0.3, seed=3)
+    row, col = po.neighbors(X, 2.2)
+    brute = po.neighbor_counts_brute(X, 2.2)
+    assert np.array_equal(np.diff(row), brute)
+    # independent pure-Python set check on a subset
+    for i in range(0, len(X), 17):
+        s = {j for j in range(len(X)) if j != i and float(((X[j] - X[i]) ** 2).sum()) < 2.2 ** 2}
+        assert s == set(col[row[i]:row[i + 1]].tolist())
+
+
+# -- a2 bond data ------------------------------------------------------------
+def test_bond_lengths_kat():
+    # test_3d.py:179-213: horizon 1.1 on a unit lattice -> every |xi| = spacing
+    X = lattice(2, 2, 2)
+    p = po.Problem(X, 1.0, 1.1, 1.0, 1.0)
+    assert np.allclose(p.xlen[: p.nb], 1.0, atol=1e-12)
+    assert np.allclose(p.omega[: p.nb], 1.0 - 1.0 / 1.1)
+
+
+# -- a3 shape tensor ------------------------------------------------------------
+def test_shape_tensor_symmetric_spd_and_interior_closed_form():
+    # test_3d.py:219-291 (symmetric everywhere, SPD at the centre)
+    X = lattice(9, 9, 9)
+    p = po.Problem(X, 1.0, 3.015, 1.0, 1.0)
+    p.shape_tensor()
+    assert np.allclose(p.K, np.transpose(p.K, (0, 2, 1)), atol=1e-12)
+    c = 4 * 81 + 4 * 9 + 4
+    assert np.all(np.linalg.eigvalsh(p.K[c]) > 0)
+    # full neighbourhood: K = k I with k = sum over the stencil of w v_x^2
+    r = np.arange(-3, 4)
+    V = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3).astype(float)
+    L = np.linalg.norm(V, axis=1)
+    m = (L > 0) & (L < 3.015)
+    k = np.sum((1 - L[m] / 3.015) * V[m, 0] ** 2)
+    assert np.allclose(p.K[c], k * np.eye(3), rtol=1e-13, atol=1e-12)
+    assert np.allclose(p.Kinv[c] @ p.K[c], np.eye(3), atol=1e-12)
+
+
+def test_singular_tests():
+    # one isolated particle: K = 0 -> identity inverse under both tests
+    X = np.vstack([lattice(4, 4, 4), [[40.0, 40.0, 40.0]]])
+    p = po.Problem(X, 1e-9, 3.015, 1.0, 1.0)  # tiny volumes: det K ~ 1e-40
+    p.shape_tensor("absolute", 1e-12)
+    assert p.sing[:-1].all()  # the unit-dependent reference test rejects them
+    p.shape_tensor("relative", 1e-12)
+    assert not p.sing[:-1].any() and p.sing[-1] == 1
+    assert np.allclose(p.Kinv[-1], np.eye(3))
+
+
+# -- a4 deformation gradient ---------------------------------------------------
+@pytest.mark.parametrize("jitter", [0.0, 0.2])
+def test_patch_test_affine(jitter):
+    X = lattice(6, 6, 6, jitter=jitter, seed=1)
+    p = po.Problem(X, 1.0, 3.015, 1.0, 1.0)
+    p.shape_tensor()
+    p.deformation_gradient()
+    assert np.allclose(p.F, np.eye(3), atol=1e-12)  # test_3d.py:293-332 (every particle here)
+    G = np.random.default_rng(0).uniform(-0.01, 0.01, (3, 3))
+    p.set_u(X @ G.T)
+    p.deformation_gradient()
+    assert np.allclose(p.F, np.eye(3) + G, atol=1e-12)
+
+
+# -- a5 stress ----------------------------------------------------------------------
+def _one_particle(model, F, E=200e3, nu=0.3, **kw):
+    Kb, mu = W.bulk_shear(E, nu)
+    X = np.zeros((1, 3))
+    p = po.Problem(X, 1.0, 1.0, Kb, mu, model=model, **kw)
+    p.F[0] = F
+    p.Kinv[0] = np.eye(3)
+    p.stress()
+    return p
+
+
+def test_linear_elastic_closed_form_uniaxial():
+    # sigma = lam tr(eps) I + 2 mu eps; uniaxial strain e: s11 = (lam + 2mu) e, s22 = lam e
+    e = 1e-3
+    Kb, mu = W.bulk_shear(1000.0, 0.25)
+    lam = Kb - 2 * mu / 3
+    p = _one_particle("linear_elastic", np.diag([1 + e, 1, 1]), E=1000.0, nu=0.25)
+    assert np.allclose(p.P[0], np.diag([(lam + 2 * mu) * e, lam * e, lam * e]), rtol=1e-12)
+
+
+def test_neo_hookean_rotation_and_tangent():
+    th = 0.7
+    R = np.array([[math.cos(th), -math.sin(th), 0], [math.sin(th), math.cos(th), 0], [0, 0, 1]])
+    assert np.abs(_one_particle("neo_hookean", R).P[0]).max() < 1e-9  # rigid rotation: no stress
+    # initial tangent = linear elastic C (backend/fea/fem/tests/test_fem.py:83-102)
+    H = np.random.default_rng(2).uniform(-1, 1, (3, 3))
+    h = 1e-7
+    Pn = _one_particle("neo_hookean", np.eye(3) + h * H, E=1000.0).P[0] / h
+    Pl = _one_particle("linear_elastic", np.eye(3) + h * H, E=1000.0).P[0] / h
+    assert np.allclose(Pn, Pl, rtol=1e-5, atol=1e-5 * np.abs(Pl).max())
+
+
+def test_neo_hookean_volumetric_closed_form():
+    # F = s I: B - I = (s^2 - 1) I, J = s^3; P = J sigma F^-T = (mu (s^2-1) + 3 lam ln s) / s I
+    s = 0.97
+    E, nu = 1000.0, 0.3
+    Kb, mu = W.bulk_shear(E, nu)
+    lam = Kb - 2 * mu / 3
+    P = _one_particle("neo_hookean", s * np.eye(3), E=E, nu=nu).P[0]
+    assert np.allclose(P, (mu * (s * s - 1) + 3 * lam * math.log(s)) / s * np.eye(3), rtol=1e-12)
+
+
+def _j2(F, sy, H, E=200e3, nu=0.3, hk=0.0, state=None):
+    n = 1
+    p = _one_particle("j2_plasticity", F, E=E, nu=nu, sy0=np.full(n, sy), h_iso=np.full(n, H),
+                      h_kin=np.full(n, hk))
+    return p
+
+
+def _mises(s):
+    d = s - np.trace(s) / 3 * np.eye(3)
+    return math.sqrt(1.5 * np.sum(d * d))
+
+
+def test_j2_kats():
+    # backend/fea/fem/tests/test_j2_plasticity.py:149-288
+    F = np.array([[1.001, 0.0002, 0], [0.0001, 0.999, 0], [0, 0, 1.0]])
+    pe = _j2(F, 1e12, 0.0)
+    pl = _one_particle("linear_elastic", F)
+    assert np.allclose(pe.P[0], pl.P[0], rtol=1e-10)         # elastic == linear
+    assert pe.epbar_t[0] == 0.0
+    Fy = np.diag([1.01, 1.0, 1.0])
+    pp = _j2(Fy, 250.0, 0.0)
+    assert pp.epbar_t[0] > 1e-12                               # yields
+    assert _mises(pp.P[0]) <= 250.0 * (1 + 1e-12)              # stress cap (perfect plasticity)
+    assert abs(_mises(pp.P[0]) - 250.0) < 1e-9                 # consistency: on the yield surface
+    H = 1e3
+    ph = _j2(Fy, 250.0, H)
+    assert abs(_mises(ph.P[0]) - (250.0 + H * ph.epbar_t[0])) < 1e-9  # hardened surface
+    assert ph.epbar_t[0] < pp.epbar_t[0]
+
+
+def test_j2_uniaxial_onset_closed_form():
+    # uniaxial strain e11: von Mises of the trial stress = 2 mu e (deviator of diag(e,0,0))
+    E, nu, sy = 200e3, 0.3, 250.0
+    _, mu = W.bulk_shear(E, nu)
+    e_y = sy / (2 * mu)
+    assert _j2(np.diag([1 + 0.999 * e_y, 1, 1]), sy, 0.0).epbar_t[0] == 0.0
+    assert _j2(np.diag([1 + 1.001 * e_y, 1, 1]), sy, 0.0).epbar_t[0] > 0.0
+
+
+# -- a6 forces ---------------------------------------------------------------------
+def _brute_force_f(X, u, h, Kb, mu, Gs, c):
+    """Independent pure-Python restatement of nosb.py:59-275 for tiny inputs."""
+    n = len(X)
+    x = X + u
+    nb = [[j for j in range(n) if j != i and np.sum((X[j] - X[i]) ** 2) < h * h] for i in range(n)]
+    K, Ki, P = [], [], []
+    for i in range(n):
+        k = np.zeros((3, 3))
+        for j in nb[i]:
+            xi = X[j] - X[i]
+            k += (1 - np.linalg.norm(xi) / h) * np.outer(xi, xi)
+        K.append(k)
+        Ki.append(np.linalg.inv(k))
+    for i in range(n):
+        N = np.zeros((3, 3))
+        for j in nb[i]:
+            xi = X[j] - X[i]
+            N += (1 - np.linalg.norm(xi) / h) * np.outer(x[j] - x[i], xi)
+        F = N @ Ki[i]
+        eps = 0.5 * (F + F.T) - np.eye(3)
+        P.append((Kb - 2 * mu / 3) * np.trace(eps) * np.eye(3) + 2 * mu * eps)
+    f = np.zeros((n, 3))
+    for i in range(n):
+        for j in nb[i]:
+            xi = X[j] - X[i]
+            w = 1 - np.linalg.norm(xi) / h
+            f[i] += w * (P[i] @ Ki[i] + P[j] @ Ki[j]) @ xi
+            eta = x[j] - x[i]
+            s = (np.linalg.norm(eta) - np.linalg.norm(xi)) / np.linalg.norm(xi)
+            f[i] += Gs * c * s * w * eta / np.linalg.norm(eta)
+    return f
+
+
+def test_force_matches_pure_python_brute_force():
+    X = lattice(3, 3, 3, jitter=0.1, seed=4)
+    u = np.random.default_rng(5).uniform(-0.01, 0.01, X.shape)
+    Kb, mu = 100.0, 40.0
+    c = 18 * Kb / (math.pi * 1.8 ** 4)
+    p = po.Problem(X, 1.0, 1.8, Kb, mu, G_s=0.1, cstab=c)
+    p.shape_tensor()
+    p.set_u(u)
+    p.compute_forces()
+    assert np.allclose(p.f, _brute_force_f(X, u, 1.8, Kb, mu, 0.1, c), rtol=1e-10, atol=1e-10)
+
+
+def test_force_invariants():
+    X = lattice(7, 7, 7, jitter=0.15, seed=6)
+    p = po.Problem(X, 1.0, 3.015, 1000.0, 500.0, model="neo_hookean")
+    p.shape_tensor()
+    p.set_u(np.random.default_rng(1).uniform(-0.02, 0.02, X.shape))
+    p.compute_forces()
+    assert np.abs(p.f.sum(0)).max() < 1e-9 * np.abs(p.f).sum()    # linear momentum
+    # affine deformation: F uniform, P uniform -> interior force vanishes (LE, G_s = 0)
+    q = po.Problem(lattice(9, 9, 9), 1.0, 3.015, 1000.0, 500.0, G_s=0.0)
+    q.shape_tensor()
+    q.set_u(q.X @ np.diag([1e-3, -2e-4, 5e-4]).T)
+    q.compute_forces()
+    c = 4 * 81 + 4 * 9 + 4
+    assert np.abs(q.f[c]).max() < 1e-10
+
+
+def test_bond_based_force_closed_form():
+    # two particles, PMB (quasi_static.py:119-143): f = c s w V eta/|eta|
+    X = np.array([[0.0, 0, 0], [1.0, 0, 0]])
+    p = po.Problem(X, 2.0, 1.5, 1.0, 1.0, formulation="bond", cstab=7.0)
+    p.set_u(np.array([[0, 0, 0], [0.1, 0, 0]]))
+    p.compute_forces()
+    expect = 7.0 * 0.1 * (1 - 1 / 1.5) * 2.0
+    assert np.allclose(p.f, [[expect, 0, 0], [-expect, 0, 0]], rtol=1e-14)
+
+
+# -- a7 dynamic relaxation ------------------------------------------------------------
+def test_kinetic_dr_single_dof_closed_form():
+    # two particles, one fixed; kinetic DR of nosb_solver.py:156-256 converges to
+    # the equilibrium of the PMB spring: s = fext / (c w V) (+ geometric exactness)
+    X = np.array([[0.0, 0, 0], [1.0, 0, 0]])
+    fixed = np.array([7, 6], np.uint8)  # particle 1 free in x only
+    fext = np.array([[0, 0, 0], [0.01, 0, 0]])
+    p = po.Problem(X, 1.0, 1.5, 1.0, 1.0, formulation="bond", cstab=3.0, fixed=fixed, fext=fext, density=1.0)
+    r = p.kd_solve(0.1, 0.0, max_it=200000, tol=1e-10)
+    assert r["converged"]
+    assert abs(p.u[1, 0] - 0.01 / (3.0 * (1 - 1 / 1.5))) < 1e-9
+
+
+def test_adaptive_dr_reaches_newton_equilibrium():
+    spec = W.cube(n_side=8)
+    p = po.Problem(spec.X, spec.vol, spec.horizon, spec.bulk, spec.shear, model="neo_hookean", fixed=spec.fixed)
+    p.shape_tensor()
+    p.set_u(spec.u0)
+    p.compute_forces()
+    cold = p.residual_norm()
+    r = p.adr_solve(p.adr_mass(1.0), max_it=50000, tol=1e-11, ref=cold)
+    assert r["converged"]
+    u_adr = p.u.copy()
+    free = p.free_mask().reshape(-1)
+
+    def R(w):
+        uu = spec.u0.reshape(-1).copy()
+        uu[free] += w
+        p.set_u(uu.reshape(-1, 3))
+        p.compute_forces()
+        return p.f.reshape(-1)[free].copy()
+
+    w = so.newton_krylov(R, u_adr.reshape(-1)[free] * 0.0, f_tol=1e-11 * cold, method="lgmres")
+    un = spec.u0.reshape(-1).copy()
+    un[free] += w
+    assert np.linalg.norm(u_adr.reshape(-1) - un) / np.linalg.norm(un) < 1e-8
+
+
+# -- a9 damage ------------------------------------------------------------------------
+def test_damage_closed_form_uniform_stretch():
+    # u = e x1: a bond along direction n stretches sqrt((1+e)^2 n1^2 + 1 - n1^2) - 1
+    X = lattice(6, 6, 6)
+    e = 0.05
+    p = po.Problem(X, 1.0, 3.015, 1.0, 1.0)
+    p.set_u(X * np.array([e, 0, 0]))
+    s_c = 0.03
+    nbk = p.update_damage(s_c)
+    i, j = p.bond_pairs()
+    n1 = (X[j, 0] - X[i, 0]) / np.linalg.norm(X[j] - X[i], axis=1)
+    s = np.sqrt((n1 * (1 + e)) ** 2 + 1 - n1 ** 2) - 1
+    assert nbk == int((s > s_c).sum())
+    assert np.array_equal(p.broken[: p.nb].astype(bool), s > s_c)
+    dmg = p.local_damage()
+    assert np.allclose(dmg, np.bincount(i, weights=(s > s_c), minlength=len(X)) / np.diff(p.row))
