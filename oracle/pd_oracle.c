@@ -748,8 +748,8 @@ void orc_adr_mass(const orc_pd *p, double beta, double *lam)
  *   c_next = 2 sqrt(sum u k u / sum u u),  k = -(F^n - F^{n-1}) / (lam V^{n-1/2})
  *   u^{n+1} = u^n + V^{n+1/2}
  * out[0]=residual out[1]=rel out[2]=ref out[3]=last damping */
-int64_t orc_adr_solve(orc_pd *p, const double *lam, int64_t max_it, double tol, double ref_in,
-                      double cmax, double *out)
+int64_t orc_adr_solve_mode(orc_pd *p, const double *lam, int64_t max_it, double tol, double ref_in,
+                           double cmax, int32_t mode, double *out)
 {
     int dim = p->dim;
     int64_t n = p->n;
@@ -775,11 +775,16 @@ int64_t orc_adr_solve(orc_pd *p, const double *lam, int64_t max_it, double tol, 
                 if (it == 0) {
                     vnew = Fk / (2.0 * lam[i]);
                 } else {
-                    if (vold != 0.0) {
-                        double kk = -(Fk - p->a[k]) / (lam[i] * vold);
-                        num += u * kk * u;
+                    if (mode == 0) {
+                        if (vold != 0.0) {
+                            double kk = -(Fk - p->a[k]) / (lam[i] * vold);
+                            num += u * kk * u;
+                        }
+                        den += u * u;
+                    } else {
+                        num -= vold * (Fk - p->a[k]);
+                        den += lam[i] * vold * vold;
                     }
-                    den += u * u;
                     vnew = ((2.0 - c) * vold + 2.0 * Fk / lam[i]) / (2.0 + c);
                 }
                 p->v[k] = vnew;
@@ -798,6 +803,12 @@ int64_t orc_adr_solve(orc_pd *p, const double *lam, int64_t max_it, double tol, 
     }
     if (out) { out[0] = res; out[1] = rel; out[2] = ref; out[3] = c; }
     return conv ? it : -it;
+}
+
+int64_t orc_adr_solve(orc_pd *p, const double *lam, int64_t max_it, double tol, double ref_in,
+                      double cmax, double *out)
+{
+    return orc_adr_solve_mode(p, lam, max_it, tol, ref_in, cmax, 0, out);
 }
 
 /* ------------------------------------------------------------------ */

@@ -81,6 +81,8 @@ def lib():
         L.orc_kd_solve.argtypes = [P, C.c_double, C.c_double, C.c_int64, C.c_double, C.c_double, _dp]
         L.orc_adr_mass.restype = None
         L.orc_adr_mass.argtypes = [P, C.c_double, _dp]
+        L.orc_adr_solve_mode.restype = C.c_int64
+        L.orc_adr_solve_mode.argtypes = [P, _dp, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int32, _dp]
         L.orc_adr_solve.restype = C.c_int64
         L.orc_adr_solve.argtypes = [P, _dp, C.c_int64, C.c_double, C.c_double, C.c_double, _dp]
         L.orc_update_damage.restype = C.c_int64
@@ -230,11 +232,12 @@ class Problem:
         lib().orc_adr_mass(C.byref(self._s), beta, _ptr(lam, _dp))
         return lam
 
-    def adr_solve(self, lam, max_it=100000, tol=1e-6, ref=0.0, cmax=1.9):
+    def adr_solve(self, lam, max_it=100000, tol=1e-6, ref=0.0, cmax=1.9, damping="velocity"):
         lam = np.ascontiguousarray(lam, np.float64)
         out = np.zeros(4)
-        it = lib().orc_adr_solve(C.byref(self._s), _ptr(lam, _dp), int(max_it), tol, ref, cmax,
-                                 _ptr(out, _dp))
+        mode = {"local": 0, "velocity": 1}[damping]
+        it = lib().orc_adr_solve_mode(C.byref(self._s), _ptr(lam, _dp), int(max_it), tol, ref, cmax, mode,
+                                      _ptr(out, _dp))
         return {"converged": it >= 0 and out[1] < tol, "iterations": abs(int(it)), "residual": out[0],
                 "relative_residual": out[1], "reference_residual": out[2], "damping": out[3]}
 

@@ -33,18 +33,23 @@ static int fail(int code, const std::string &msg)
 // ---------------------------------------------------------------------------
 // allocation helpers
 // ---------------------------------------------------------------------------
+// (re)allocates a zeroed device array.  Every memory operation of a context is
+// ordered on the context's own stream: that stream is created non-blocking, so
+// a null-stream hipMemset/hipMemcpy would not be ordered against its kernels.
 template <typename T>
-static int dalloc(T **p, size_t count)
+static int dalloc(hipStream_t s, T **p, size_t count)
 {
     if (*p) {
+        hipStreamSynchronize(s);
         hipFree(*p);
         *p = nullptr;
     }
     if (count == 0) count = 1;
     hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
     if (e != hipSuccess) return fail(PDX_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    e = hipMemset(*p, 0, sizeof(T) * count);
-    if (e != hipSuccess) return fail(PDX_EDEVICE, std::string("hipMemset: ") + hipGetErrorString(e));
+    e = hipMemsetAsync(*p, 0, sizeof(T) * count, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(PDX_EDEVICE, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
     return PDX_OK;
 }
 
@@ -154,16 +159,16 @@ static int create_common(int32_t ncase, const int64_t *off, int32_t dim, pdx_ctx
     d.dim = dim;
     int r = PDX_OK;
     size_t ni = c->ni;
-    if ((r = dalloc(&d.P, ni)) || (r = dalloc(&d.KR, ni)) || (r = dalloc(&d.AR, ni)) ||
-        (r = dalloc(&d.j2p, ni)) || (r = dalloc(&d.j2s, ni)) || (r = dalloc(&d.u[0], ni)) ||
-        (r = dalloc(&d.u[1], ni)) || (r = dalloc(&d.v, ni)) || (r = dalloc(&d.fp, ni)) ||
-        (r = dalloc(&d.f, ni)) || (r = dalloc(&d.fext, ni)) || (r = dalloc(&d.mass, ni)) ||
-        (r = dalloc(&d.lam, ni)) || (r = dalloc(&d.V64, ni)) || (r = dalloc(&d.K64, ni * 9)) ||
-        (r = dalloc(&d.Kinv64, ni * 9)) || (r = dalloc(&d.Fd, ni * 9)) || (r = dalloc(&d.Pd, ni * 9)) ||
-        (r = dalloc(&d.damage, ni)) || (r = dalloc(&d.init_nb, ni)) || (r = dalloc(&d.case_of_p, ni)) ||
-        (r = dalloc(&d.counts, ni)) || (r = dalloc(&d.case_of_blk, (size_t)c->nblk)) ||
-        (r = dalloc(&d.case_blk0, (size_t)ncase + 1)) || (r = dalloc(&d.sc, (size_t)ncase)) ||
-        (r = dalloc(&d.part, (size_t)c->nblk * 4)) || (r = dalloc(&d.grid, (size_t)ncase))) {
+    if ((r = dalloc(c->stream, &d.P, ni)) || (r = dalloc(c->stream, &d.KR, ni)) || (r = dalloc(c->stream, &d.AR, ni)) ||
+        (r = dalloc(c->stream, &d.j2p, ni)) || (r = dalloc(c->stream, &d.j2s, ni)) || (r = dalloc(c->stream, &d.u[0], ni)) ||
+        (r = dalloc(c->stream, &d.u[1], ni)) || (r = dalloc(c->stream, &d.v, ni)) || (r = dalloc(c->stream, &d.fp, ni)) ||
+        (r = dalloc(c->stream, &d.f, ni)) || (r = dalloc(c->stream, &d.fext, ni)) || (r = dalloc(c->stream, &d.mass, ni)) ||
+        (r = dalloc(c->stream, &d.lam, ni)) || (r = dalloc(c->stream, &d.V64, ni)) || (r = dalloc(c->stream, &d.K64, ni * 9)) ||
+        (r = dalloc(c->stream, &d.Kinv64, ni * 9)) || (r = dalloc(c->stream, &d.Fd, ni * 9)) || (r = dalloc(c->stream, &d.Pd, ni * 9)) ||
+        (r = dalloc(c->stream, &d.damage, ni)) || (r = dalloc(c->stream, &d.init_nb, ni)) || (r = dalloc(c->stream, &d.case_of_p, ni)) ||
+        (r = dalloc(c->stream, &d.counts, ni)) || (r = dalloc(c->stream, &d.case_of_blk, (size_t)c->nblk)) ||
+        (r = dalloc(c->stream, &d.case_blk0, (size_t)ncase + 1)) || (r = dalloc(c->stream, &d.sc, (size_t)ncase)) ||
+        (r = dalloc(c->stream, &d.part, (size_t)c->nblk * 4)) || (r = dalloc(c->stream, &d.grid, (size_t)ncase))) {
         free_all(c);
         hipStreamDestroy(c->stream);
         delete c;
@@ -172,12 +177,13 @@ static int create_common(int32_t ncase, const int64_t *off, int32_t dim, pdx_ctx
     // block -> case tables; padding blocks of the grid join the last case
     std::vector<int32_t> cob, cb0;
     pdx_layout::block_cases(c->case_p0, c->nblk, cob, cb0);
-    hipMemcpy(d.case_of_blk, cob.data(), sizeof(int32_t) * cob.size(), hipMemcpyHostToDevice);
-    hipMemcpy(d.case_blk0, cb0.data(), sizeof(int32_t) * cb0.size(), hipMemcpyHostToDevice);
+    hipMemcpyAsync(d.case_of_blk, cob.data(), sizeof(int32_t) * cob.size(), hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(d.case_blk0, cb0.data(), sizeof(int32_t) * cb0.size(), hipMemcpyHostToDevice, c->stream);
     std::vector<DScal> sc(ncase);
     memset(sc.data(), 0, sizeof(DScal) * ncase);
     for (auto &s : sc) { s.ref = 1.0; s.cmax = 1.9; s.first = 1; s.inc = 1; }
-    hipMemcpy(d.sc, sc.data(), sizeof(DScal) * ncase, hipMemcpyHostToDevice);
+    hipMemcpyAsync(d.sc, sc.data(), sizeof(DScal) * ncase, hipMemcpyHostToDevice, c->stream);
+    hipStreamSynchronize(c->stream);
     *out = c;
     return PDX_OK;
 }
@@ -381,13 +387,13 @@ extern "C" int pdx_set_particles(pdx_ctx *c, const double *X, const double *vol,
     PDX_TRY(upload(c, c->d.case_of_p, cop));
     PDX_TRY(put_scalar(c, c->d.mass, m.mass.data()));
     // state back to the reference initial values (particles.py:159-173)
-    PDX_HIP(hipMemset(c->d.u[0], 0, sizeof(pdx_d4) * c->ni));
-    PDX_HIP(hipMemset(c->d.u[1], 0, sizeof(pdx_d4) * c->ni));
-    PDX_HIP(hipMemset(c->d.v, 0, sizeof(pdx_d4) * c->ni));
-    PDX_HIP(hipMemset(c->d.fp, 0, sizeof(pdx_d4) * c->ni));
-    PDX_HIP(hipMemset(c->d.f, 0, sizeof(pdx_d4) * c->ni));
-    PDX_HIP(hipMemset(c->d.damage, 0, sizeof(double) * c->ni));
-    PDX_HIP(hipMemset(c->d.KR, 0, sizeof(KRec) * c->ni));
+    PDX_HIP(hipMemsetAsync(c->d.u[0], 0, sizeof(pdx_d4) * c->ni, c->stream));
+    PDX_HIP(hipMemsetAsync(c->d.u[1], 0, sizeof(pdx_d4) * c->ni, c->stream));
+    PDX_HIP(hipMemsetAsync(c->d.v, 0, sizeof(pdx_d4) * c->ni, c->stream));
+    PDX_HIP(hipMemsetAsync(c->d.fp, 0, sizeof(pdx_d4) * c->ni, c->stream));
+    PDX_HIP(hipMemsetAsync(c->d.f, 0, sizeof(pdx_d4) * c->ni, c->stream));
+    PDX_HIP(hipMemsetAsync(c->d.damage, 0, sizeof(double) * c->ni, c->stream));
+    PDX_HIP(hipMemsetAsync(c->d.KR, 0, sizeof(KRec) * c->ni, c->stream));
     c->ucur = 0;
     c->have_particles = true;
     c->have_bonds = c->have_K = c->have_mass = c->have_A = false;
@@ -490,8 +496,8 @@ static int build_grid(pdx_ctx *c, double horizon)
     std::string err;
     if (!pdx_layout::cell_grids(c->dim, c->hX.data(), c->case_p0, c->perm, horizon, gp, cstart, cidx, err))
         return fail(PDX_EINVAL, err);
-    PDX_TRY(dalloc(&c->d.cell_start, cstart.size()));
-    PDX_TRY(dalloc(&c->d.cell_idx, std::max<size_t>(cidx.size(), 1)));
+    PDX_TRY(dalloc(c->stream, &c->d.cell_start, cstart.size()));
+    PDX_TRY(dalloc(c->stream, &c->d.cell_idx, std::max<size_t>(cidx.size(), 1)));
     PDX_TRY(upload(c, c->d.cell_start, cstart));
     if (!cidx.empty()) PDX_TRY(upload(c, c->d.cell_idx, cidx));
     PDX_TRY(upload(c, c->d.grid, gp));
@@ -529,12 +535,18 @@ extern "C" int pdx_build_bonds(pdx_ctx *c, double horizon, int64_t *n_bonds)
     int64_t tot = 0;
     c->slots = pdx_layout::sell_pointers(cnt, c->h_slptr, tot);
     c->nb = tot;
-    PDX_TRY(dalloc(&c->d.sl_ptr, (size_t)nsl + 1));
+    PDX_TRY(dalloc(c->stream, &c->d.sl_ptr, (size_t)nsl + 1));
     PDX_TRY(upload(c, c->d.sl_ptr, c->h_slptr));
-    PDX_TRY(dalloc(&c->d.bj, (size_t)std::max<int64_t>(c->slots, 1)));
-    PDX_HIP(hipMemset(c->d.bj, 0xff, sizeof(uint32_t) * (size_t)std::max<int64_t>(c->slots, 1)));
-    PDX_HIP(hipMemset(c->d.init_nb, 0, sizeof(int32_t) * c->ni));
+    PDX_TRY(dalloc(c->stream, &c->d.bj, (size_t)std::max<int64_t>(c->slots, 1)));
+    PDX_HIP(hipMemsetAsync(c->d.bj, 0xff, sizeof(uint32_t) * (size_t)std::max<int64_t>(c->slots, 1), c->stream));
+    PDX_HIP(hipMemsetAsync(c->d.init_nb, 0, sizeof(int32_t) * c->ni, c->stream));
     PDX_TRY(pdx_launch_fill(c));
+    std::vector<int32_t> filled;
+    PDX_TRY(download(c, filled, c->d.init_nb, c->ni));
+    for (int i = 0; i < c->ni; ++i)
+        if (filled[i] != cnt[i])
+            return fail(PDX_EDEVICE, "neighbour fill wrote " + std::to_string(filled[i]) + " bonds for particle " +
+                                         std::to_string(i) + ", count pass found " + std::to_string(cnt[i]));
     HostMirror &m = mirror(c);
     m.nnb = cnt;
     std::vector<float> w(c->ni);
@@ -564,29 +576,9 @@ static int row_map(pdx_ctx *c, RowMap &rm)
     HostMirror &m = mirror(c);
     std::vector<uint32_t> bj;
     PDX_TRY(download(c, bj, c->d.bj, (size_t)std::max<int64_t>(c->slots, 1)));
-    std::vector<int64_t> rowi(c->ni + 1, 0);
-    for (int i = 0; i < c->ni; ++i) rowi[i + 1] = rowi[i] + m.nnb[i];
-    rm.row.assign(c->n + 1, 0);
-    for (int r = 0; r < c->n; ++r) rm.row[r + 1] = rm.row[r] + m.nnb[c->inv[r]];
-    rm.col.resize(rm.row[c->n]);
-    rm.slot.resize(rm.row[c->n]);
-    rm.csr.resize(rm.row[c->n]);
-    std::vector<std::pair<int32_t, int32_t>> tmp;
-    for (int r = 0; r < c->n; ++r) {
-        const int i = c->inv[r];
-        const int64_t base = c->h_slptr[i / PDX_SLICE] + (i % PDX_SLICE);
-        tmp.clear();
-        for (int k = 0; k < m.nnb[i]; ++k) {
-            uint32_t e = bj[base + (int64_t)k * PDX_SLICE] & ~PDX_BROKEN;
-            tmp.push_back({c->perm[e], k});
-        }
-        std::sort(tmp.begin(), tmp.end());
-        for (size_t t = 0; t < tmp.size(); ++t) {
-            rm.col[rm.row[r] + t] = tmp[t].first;
-            rm.slot[rm.row[r] + t] = base + (int64_t)tmp[t].second * PDX_SLICE;
-            rm.csr[rm.row[r] + t] = rowi[i] + tmp[t].second;
-        }
-    }
+    if (!pdx_layout::bond_rows(c->n, c->ni, c->perm, c->inv, m.nnb, c->h_slptr, bj.data(), rm.row, rm.col, rm.slot,
+                               rm.csr))
+        return fail(PDX_EDEVICE, "bond list holds an entry that is not a particle");
     return PDX_OK;
 }
 

@@ -155,23 +155,23 @@ PDX_HD void matmul_f(const float *A, const float *B, float *C)
         }
 }
 
-template <int D>
-PDX_HD float lame_f(float Kb, float mu)
+template <int D, typename T>
+PDX_HD T lame(T Kb, T mu)
 {
-    return D == 3 ? Kb - 2.0f * mu / 3.0f : Kb - mu;
+    return D == 3 ? Kb - T(2) * mu / T(3) : Kb - mu;
 }
 
 // linear elastic: eps = sym(H), P = lam tr(eps) I + 2 mu eps   (nosb.py:218-227)
-template <int D>
-PDX_HD void stress_le(const float *H, float Kb, float mu, float *P)
+template <int D, typename T>
+PDX_HD void stress_le(const T *H, T Kb, T mu, T *P)
 {
-    const float lam = lame_f<D>(Kb, mu);
-    float tr = 0.f;
+    const T lam = lame<D>(Kb, mu);
+    T tr = 0;
     for (int q = 0; q < D; ++q) tr += H[q * D + q];
     for (int r = 0; r < D; ++r)
         for (int c = 0; c < D; ++c) {
-            const float e = 0.5f * (H[r * D + c] + H[c * D + r]);
-            P[r * D + c] = 2.0f * mu * e + (r == c ? lam * tr : 0.0f);
+            const T e = T(0.5) * (H[r * D + c] + H[c * D + r]);
+            P[r * D + c] = T(2) * mu * e + (r == c ? lam * tr : T(0));
         }
 }
 
@@ -179,63 +179,63 @@ PDX_HD void stress_le(const float *H, float Kb, float mu, float *P)
 // (neo_hookean.py:117-127 sigma, :176-180 P = J sigma F^-T), evaluated without
 // cancellation: J - 1 from the invariants of H, F^-T = I + M^T with
 // M = F^-1 - I = -F^-1 H.
-template <int D>
-PDX_HD void stress_nh(const float *H, float Kb, float mu, float *P)
+template <int D, typename T>
+PDX_HD void stress_nh(const T *H, T Kb, T mu, T *P)
 {
-    const float lam = lame_f<D>(Kb, mu);
-    float Jm1, adj[D * D];
+    const T lam = lame<D>(Kb, mu);
+    T Jm1, adj[D * D];
     if constexpr (D == 3) {
-        const float tr = H[0] + H[4] + H[8];
-        float trH2 = 0.f;
+        const T tr = H[0] + H[4] + H[8];
+        T trH2 = 0;
         for (int r = 0; r < D; ++r)
-            for (int c = 0; c < D; ++c) trH2 = fmaf(H[r * D + c], H[c * D + r], trH2);
-        const float *h = H;
-        const float detH = h[0] * (h[4] * h[8] - h[5] * h[7]) - h[1] * (h[3] * h[8] - h[5] * h[6]) +
-                           h[2] * (h[3] * h[7] - h[4] * h[6]);
-        Jm1 = tr + 0.5f * (tr * tr - trH2) + detH;
-        const float F0 = 1.f + h[0], F4 = 1.f + h[4], F8 = 1.f + h[8];
-        const float F1 = h[1], F2 = h[2], F3 = h[3], F5 = h[5], F6 = h[6], F7 = h[7];
-        float *a = adj;
+            for (int c = 0; c < D; ++c) trH2 += H[r * D + c] * H[c * D + r];
+        const T *h = H;
+        const T detH = h[0] * (h[4] * h[8] - h[5] * h[7]) - h[1] * (h[3] * h[8] - h[5] * h[6]) +
+                       h[2] * (h[3] * h[7] - h[4] * h[6]);
+        Jm1 = tr + T(0.5) * (tr * tr - trH2) + detH;
+        const T F0 = T(1) + h[0], F4 = T(1) + h[4], F8 = T(1) + h[8];
+        const T F1 = h[1], F2 = h[2], F3 = h[3], F5 = h[5], F6 = h[6], F7 = h[7];
+        T *a = adj;
         a[0] = F4 * F8 - F5 * F7; a[1] = F2 * F7 - F1 * F8; a[2] = F1 * F5 - F2 * F4;
         a[3] = F5 * F6 - F3 * F8; a[4] = F0 * F8 - F2 * F6; a[5] = F2 * F3 - F0 * F5;
         a[6] = F3 * F7 - F4 * F6; a[7] = F1 * F6 - F0 * F7; a[8] = F0 * F4 - F1 * F3;
     } else {
         Jm1 = H[0] + H[3] + (H[0] * H[3] - H[1] * H[2]);
-        adj[0] = 1.f + H[3];
+        adj[0] = T(1) + H[3];
         adj[1] = -H[1];
         adj[2] = -H[2];
-        adj[3] = 1.f + H[0];
+        adj[3] = T(1) + H[0];
     }
-    const float J = 1.f + Jm1;
-    const float lnJ = J > 1e-8f ? log1pf(Jm1) : logf(1e-8f);
-    const float iJ = 1.0f / J;
-    float M[D * D];  // M = -F^-1 H
+    const T J = T(1) + Jm1;
+    const T lnJ = J > T(1e-8) ? log1p(Jm1) : log(T(1e-8));
+    const T iJ = T(1) / J;
+    T M[D * D];  // M = -F^-1 H
     for (int r = 0; r < D; ++r)
         for (int c = 0; c < D; ++c) {
-            float s = 0.f;
-            for (int k = 0; k < D; ++k) s = fmaf(adj[r * D + k], H[k * D + c], s);
-            M[r * D + c] = -s * iJ;
+            T acc = 0;
+            for (int k = 0; k < D; ++k) acc += adj[r * D + k] * H[k * D + c];
+            M[r * D + c] = -acc * iJ;
         }
     for (int r = 0; r < D; ++r)
         for (int c = 0; c < D; ++c) {
-            const float mt = M[c * D + r];
-            P[r * D + c] = mu * (H[r * D + c] - mt) + lam * lnJ * ((r == c ? 1.f : 0.f) + mt);
+            const T mt = M[c * D + r];
+            P[r * D + c] = mu * (H[r * D + c] - mt) + lam * lnJ * ((r == c ? T(1) : T(0)) + mt);
         }
 }
 
 // J2 small-strain radial return with linear isotropic + kinematic hardening,
 // FP64 (j2_plasticity.py:202-249; h_kin = 0 is the reference).  Symmetric
 // tensors packed xx,yy,zz,xy,yz,xz; off-diagonal entries count twice in norms.
-PDX_HD void stress_j2(const float *H, float Kb_, float mu_, const J2Par &pp, const J2State &st, float *P,
+PDX_HD void stress_j2(const double *H, double Kb_, double mu_, const J2Par &pp, const J2State &st, double *P,
                       J2State *out)
 {
     const double Kb = Kb_, mu = mu_;
     const double lam = Kb - 2.0 * mu / 3.0;
     double eps[6];
     eps[0] = H[0]; eps[1] = H[4]; eps[2] = H[8];
-    eps[3] = 0.5 * ((double)H[1] + (double)H[3]);
-    eps[4] = 0.5 * ((double)H[5] + (double)H[7]);
-    eps[5] = 0.5 * ((double)H[2] + (double)H[6]);
+    eps[3] = 0.5 * (H[1] + H[3]);
+    eps[4] = 0.5 * (H[5] + H[7]);
+    eps[5] = 0.5 * (H[2] + H[6]);
     double ee[6];
     for (int q = 0; q < 6; ++q) ee[q] = eps[q] - (double)st.ep[q];
     const double tre = ee[0] + ee[1] + ee[2];
@@ -269,10 +269,10 @@ PDX_HD void stress_j2(const float *H, float Kb_, float mu_, const J2Par &pp, con
         ns.epbar = (float)(epbar + dg);
         if (out) *out = ns;
     }
-    P[0] = (float)sfin[0]; P[4] = (float)sfin[1]; P[8] = (float)sfin[2];
-    P[1] = P[3] = (float)sfin[3];
-    P[5] = P[7] = (float)sfin[4];
-    P[2] = P[6] = (float)sfin[5];
+    P[0] = sfin[0]; P[4] = sfin[1]; P[8] = sfin[2];
+    P[1] = P[3] = sfin[3];
+    P[5] = P[7] = sfin[4];
+    P[2] = P[6] = sfin[5];
 }
 
 // influence weight w = max(0, 1 - |xi|/delta) (bonds.py:131-137) times V_j
@@ -301,7 +301,9 @@ PDX_HD void fpass_one(const Dev &d, int i, int ubuf)
     const int nnb = (int)kr.nnb;
     for (int k = 0; k < nnb; ++k) {
         const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
-        if (e & PDX_BROKEN) continue;
+        // broken bonds (bit 31) are skipped; an index outside the particles
+        // never becomes an address
+        if (e >= (uint32_t)d.ni) continue;
         const int j = (int)e;
         const PRec pj = d.P[j];
         const pdx_d4 uj = U[j];
@@ -322,40 +324,51 @@ PDX_HD void fpass_one(const Dev &d, int i, int ubuf)
             for (int c = 0; c < D; ++c) Hd[r * D + c] = fma(wd, xd[c], Hd[r * D + c]);
         }
     }
-    float Hs[D * D], Km[D * D], H[D * D];
-    for (int q = 0; q < D * D; ++q) Hs[q] = (float)Hd[q];
-    for (int r = 0; r < D; ++r)
-        for (int c = 0; c < D; ++c) Km[r * D + c] = kr.Kinv[r * 3 + c];
+    // per-particle tensor algebra in FP64 with the FP64 inverse; A is rounded
+    // to FP32 only when stored
+    const double *Ki = d.Kinv64 + (int64_t)i * 9;
     const bool sing = (kr.flags & PDX_FL_SING) != 0;
+    double H[D * D];
     if (sing) {
         // K^-1 fell back to I, so F = N = K + sum w du (x) xi  (nosb.py:92-94, 126)
-        for (int q = 0; q < D * D; ++q) H[q] = Hs[q] + Km[q] - ((q % (D + 1)) == 0 ? 1.f : 0.f);
+        const double *Kf = d.K64 + (int64_t)i * 9;
+        for (int q = 0; q < D * D; ++q) H[q] = Hd[q] + Kf[q] - ((q % (D + 1)) == 0 ? 1.0 : 0.0);
     } else {
-        matmul_f<D>(Hs, Km, H);
+        for (int r = 0; r < D; ++r)
+            for (int c = 0; c < D; ++c) {
+                double acc = 0.0;
+                for (int k = 0; k < D; ++k) acc += Hd[r * D + k] * Ki[k * D + c];
+                H[r * D + c] = acc;
+            }
     }
-    float P[9];
+    double P[9];
     if constexpr (MODEL == 1) {
-        stress_nh<D>(H, kr.bulk, kr.shear, P);
+        stress_nh<D, double>(H, (double)kr.bulk, (double)kr.shear, P);
     } else if constexpr (MODEL == 2 && D == 3) {
         J2State ns;
         stress_j2(H, kr.bulk, kr.shear, d.j2p[i], d.j2s[i], P, COMMIT ? &ns : nullptr);
         if (COMMIT) d.j2s[i] = ns;
     } else {
-        stress_le<D>(H, kr.bulk, kr.shear, P);
+        stress_le<D, double>(H, (double)kr.bulk, (double)kr.shear, P);
     }
     ARec ar;
-    if (sing) {
-        for (int q = 0; q < D * D; ++q) ar.A[q] = P[q];
-    } else {
-        matmul_f<D>(P, Km, ar.A);
-    }
+    for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) {
+            double acc = 0.0;
+            if (sing) {
+                acc = P[r * D + c];
+            } else {
+                for (int k = 0; k < D; ++k) acc += P[r * D + k] * Ki[k * D + c];
+            }
+            ar.A[r * D + c] = (float)acc;
+        }
     for (int q = D * D; q < 9; ++q) ar.A[q] = 0.f;
     ar.pad[0] = ar.pad[1] = ar.pad[2] = 0.f;
     d.AR[i] = ar;
     if (DIAG) {
         for (int q = 0; q < D * D; ++q) {
-            d.Fd[(int64_t)i * 9 + q] = H[q] + ((q % (D + 1)) == 0 ? 1.f : 0.f);
-            d.Pd[(int64_t)i * 9 + q] = P[q];
+            d.Fd[(int64_t)i * 9 + q] = (float)(H[q] + ((q % (D + 1)) == 0 ? 1.0 : 0.0));
+            d.Pd[(int64_t)i * 9 + q] = (float)P[q];
         }
     }
 }
@@ -364,7 +377,9 @@ PDX_HD void fpass_one(const Dev &d, int i, int ubuf)
 // force pass for one particle: f = A_i m_i + sum_j w V_j [A_j xi + g eta]
 // + external load, then the DR update of UPD:
 //   UPD 0: write f
-//   UPD 1: adaptive DR (damping lagged one step), u_out = u_in + V
+//   UPD 1: adaptive DR (damping lagged one step), u_out = u_in + V; the damping
+//          is 2 w with w^2 = -sum V.dF / sum lam V^2, the Rayleigh quotient of
+//          the mass-scaled stiffness along the last increment
 //   UPD 2: kinetic DR velocity update (nosb_solver.py:164-182)
 // red[0] += |f|^2 over free DOFs; UPD 1: red[1], red[2] = Rayleigh-quotient
 // numerator and denominator; UPD 2: red[1] = kinetic energy.
@@ -381,11 +396,18 @@ PDX_HD void force_one(const Dev &d, int i, int ubuf, int cs, double dt, double v
     const float Gs = (float)d.G_s;
     const float ci = pi.c;
     double fa[3] = {0.0, 0.0, 0.0};
+    // correspondence term in difference form: sum_j w (A_i + A_j) xi =
+    // sum_j w (A_j - A_i) xi + 2 A_i m_i, so each FP32 bond product is rounded
+    // relative to |A_j - A_i| rather than |A|
+    ARec ai;
+    if (FORM == 0) ai = d.AR[i];
     const int64_t base = slot0(d, i);
     const int nnb = (int)kr.nnb;
     for (int k = 0; k < nnb; ++k) {
         const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
-        if (e & PDX_BROKEN) continue;
+        // broken bonds (bit 31) are skipped; an index outside the particles
+        // never becomes an address
+        if (e >= (uint32_t)d.ni) continue;
         const int j = (int)e;
         const PRec pj = d.P[j];
         const pdx_d4 uj = U[j];
@@ -415,7 +437,7 @@ PDX_HD void force_one(const Dev &d, int i, int ubuf, int cs, double dt, double v
             const ARec aj = d.AR[j];
             for (int r = 0; r < D; ++r) {
                 float t = 0.f;
-                for (int c = 0; c < D; ++c) t = fmaf(aj.A[r * D + c], xi[c], t);
+                for (int c = 0; c < D; ++c) t = fmaf(aj.A[r * D + c] - ai.A[r * D + c], xi[c], t);
                 tb[r] = t;
             }
             if (el > 1e-10f && L > 1e-10f) {
@@ -431,11 +453,10 @@ PDX_HD void force_one(const Dev &d, int i, int ubuf, int cs, double dt, double v
         for (int r = 0; r < D; ++r) fa[r] += (double)(w * tb[r]);
     }
     if (FORM == 0) {
-        const ARec ai = d.AR[i];
         for (int r = 0; r < D; ++r) {
-            float t = 0.f;
-            for (int c = 0; c < D; ++c) t = fmaf(ai.A[r * D + c], kr.m[c], t);
-            fa[r] += (double)t;
+            double t = 0.0;
+            for (int c = 0; c < D; ++c) t += (double)ai.A[r * D + c] * (double)kr.m[c];
+            fa[r] += 2.0 * t;
         }
     }
     if (kr.flags & PDX_FL_LOADED) {
@@ -464,11 +485,8 @@ PDX_HD void force_one(const Dev &d, int i, int ubuf, int cs, double dt, double v
             if (first) {
                 vn[r] = F / (2.0 * lam);
             } else {
-                if (vo[r] != 0.0) {
-                    const double kk = -(F - fo[r]) / (lam * vo[r]);
-                    red[1] += uo[r] * kk * uo[r];
-                }
-                red[2] += uo[r] * uo[r];
+                red[1] -= vo[r] * (F - fo[r]);
+                red[2] += lam * vo[r] * vo[r];
                 vn[r] = ((2.0 - c) * vo[r] + 2.0 * F / lam) / (2.0 + c);
             }
             fn[r] = F;

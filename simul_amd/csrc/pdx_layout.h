@@ -193,4 +193,40 @@ inline void block_cases(const std::vector<int32_t> &p0, int nblk, std::vector<in
     }
 }
 
+// bonds in caller order: for every caller row the neighbours' caller indices
+// sorted ascending, the bj slot of each entry and its position in internal
+// CSR order.  Returns false when a row holds an entry that is not a particle
+// (a SELL padding word inside a row).
+inline bool bond_rows(int n, int ni, const std::vector<int32_t> &perm, const std::vector<int32_t> &inv,
+                      const std::vector<int32_t> &nnb, const std::vector<int64_t> &slptr, const uint32_t *bj,
+                      std::vector<int64_t> &row, std::vector<int32_t> &col, std::vector<int64_t> &slot,
+                      std::vector<int64_t> &csr)
+{
+    std::vector<int64_t> rowi(ni + 1, 0);
+    for (int i = 0; i < ni; ++i) rowi[i + 1] = rowi[i] + nnb[i];
+    row.assign(n + 1, 0);
+    for (int r = 0; r < n; ++r) row[r + 1] = row[r] + nnb[inv[r]];
+    col.resize(row[n]);
+    slot.resize(row[n]);
+    csr.resize(row[n]);
+    std::vector<std::pair<int32_t, int32_t>> tmp;
+    for (int r = 0; r < n; ++r) {
+        const int i = inv[r];
+        const int64_t base = slptr[i / PDX_SLICE] + (i % PDX_SLICE);
+        tmp.clear();
+        for (int k = 0; k < nnb[i]; ++k) {
+            const uint32_t e = bj[base + (int64_t)k * PDX_SLICE] & ~PDX_BROKEN;
+            if ((int64_t)e >= ni || perm[e] < 0) return false;
+            tmp.push_back({perm[e], k});
+        }
+        std::sort(tmp.begin(), tmp.end());
+        for (size_t t = 0; t < tmp.size(); ++t) {
+            col[row[r] + t] = tmp[t].first;
+            slot[row[r] + t] = base + (int64_t)tmp[t].second * PDX_SLICE;
+            csr[row[r] + t] = rowi[i] + tmp[t].second;
+        }
+    }
+    return true;
+}
+
 }  // namespace pdx_layout

@@ -264,6 +264,18 @@ int main(int argc, char **argv)
             return 4;
         }
     }
+    {
+        std::vector<int64_t> row, slot, csr;
+        std::vector<int32_t> col;
+        if (!pdx_layout::bond_rows(n, ni, R.perm, R.inv, cnt, slptr, d.bj, row, col, slot, csr)) {
+            fprintf(stderr, "bond_rows: entry outside the particles\n");
+            return 5;
+        }
+        if (row[n] != R.nb) {
+            fprintf(stderr, "bond_rows: %lld bonds, expected %lld\n", (long long)row[n], (long long)R.nb);
+            return 6;
+        }
+    }
     double chk = 0.0;
     if (D == 3) {
         if (R.form == 1) chk = sweep<3, 0, 1>(R, iters);

@@ -223,3 +223,16 @@ def test_batch_matches_single_cases(gpu):
         rs = single.solve("adaptive", tol=1e-6)
         assert rb[k].iterations == rs.iterations
         assert np.array_equal(ub[off[k]:off[k + 1]], single.u)
+
+
+def test_large_bond_build_is_intact(gpu):
+    """Regression: every stored bond of the 365,056-particle vertebra survives the
+    build (no stream-unordered clear overwrites rows) and the sets match the oracle."""
+    spec = W.vertebra(0.4)
+    body = PDBody(spec.X, spec.vol, spec.horizon, spec.bulk, spec.shear, build=False)
+    nb = body.build_bonds(spec.horizon)
+    st = body.stats()
+    assert st["n_bonds"] == nb == st["n_intact"] == 42517012
+    row, col, br = body.bonds()
+    orow, ocol = po.neighbors(spec.X, spec.horizon)
+    assert np.array_equal(row, orow) and np.array_equal(col, ocol) and not br.any()
