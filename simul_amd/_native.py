@@ -94,6 +94,8 @@ SIGNATURES = {
     "pdx_dr_solve": [_vp, C.POINTER(DRParams), C.POINTER(DRResult)],
     "pdx_time_kernels": [_vp, C.c_int32, _dp],
     "pdx_stats": [_vp, _i64p],
+    "pdx_autotune": [_vp, C.c_int32, _i32p],
+    "pdx_set_group": [_vp, C.c_int32],
     "pdx_get": [_vp, C.c_int32, _vp],
     "pdx_set": [_vp, C.c_int32, _vp],
 }

@@ -216,6 +216,14 @@ class PDBody:
         self.ctx.call("pdx_time_kernels", int(iters), N.ptr(ms, N._dp))
         return {"fpass_ms": ms[0], "force_ms": ms[1], "total_ms": ms[2]}
 
+    def autotune(self, iters=10):
+        g = C.c_int32()
+        self.ctx.call("pdx_autotune", int(iters), C.byref(g))
+        return g.value
+
+    def set_group(self, g):
+        self.ctx.call("pdx_set_group", int(g))
+
     def stats(self):
         o = np.zeros(4, np.int64)
         self.ctx.call("pdx_stats", N.ptr(o, N._i64p))

@@ -1050,6 +1050,59 @@ extern "C" int pdx_time_kernels(pdx_ctx *c, int32_t iters, double *ms)
     return PDX_OK;
 }
 
+extern "C" int pdx_set_group(pdx_ctx *c, int32_t g)
+{
+    CHECK_CTX(c);
+    if (g != 1 && g != 2 && g != 4) return fail(PDX_EINVAL, "group must be 1, 2 or 4");
+    c->tune_g = g;
+    invalidate_graph(c);
+    return PDX_OK;
+}
+
+extern "C" int pdx_autotune(pdx_ctx *c, int32_t iters, int32_t *group_out)
+{
+    CHECK_CTX(c);
+    PDX_TRY(need_K(c));
+    if (iters <= 0) return fail(PDX_EINVAL, "iters must be positive");
+    if (!c->have_mass) PDX_TRY(pdx_adr_mass(c, 1.0));
+    // keep the solver state: the timing iterations run on a copy
+    const size_t nv = sizeof(pdx_d4) * c->ni;
+    pdx_d4 *save = nullptr;
+    PDX_HIP(hipMalloc((void **)&save, 3 * nv));
+    PDX_HIP(hipMemcpyAsync(save, c->d.u[c->ucur], nv, hipMemcpyDeviceToDevice, c->stream));
+    PDX_HIP(hipMemcpyAsync(save + c->ni, c->d.v, nv, hipMemcpyDeviceToDevice, c->stream));
+    PDX_HIP(hipMemcpyAsync(save + 2 * c->ni, c->d.fp, nv, hipMemcpyDeviceToDevice, c->stream));
+    PDX_TRY(read_sc(c));
+    std::vector<DScal> sc0(c->h_sc, c->h_sc + c->ncase);
+    const int u0 = c->ucur;
+    double best = 1e300;
+    int best_g = c->tune_g;
+    for (int g : {1, 2, 4}) {
+        c->tune_g = g;
+        double ms[3];
+        int r = pdx_time_kernels(c, iters, ms);
+        if (r != PDX_OK) {
+            hipFree(save);
+            return r;
+        }
+        if (ms[0] + ms[1] < best) {
+            best = ms[0] + ms[1];
+            best_g = g;
+        }
+    }
+    c->tune_g = best_g;
+    c->ucur = u0;
+    PDX_HIP(hipMemcpyAsync(c->d.u[c->ucur], save, nv, hipMemcpyDeviceToDevice, c->stream));
+    PDX_HIP(hipMemcpyAsync(c->d.v, save + c->ni, nv, hipMemcpyDeviceToDevice, c->stream));
+    PDX_HIP(hipMemcpyAsync(c->d.fp, save + 2 * c->ni, nv, hipMemcpyDeviceToDevice, c->stream));
+    std::copy(sc0.begin(), sc0.end(), c->h_sc);
+    PDX_TRY(write_sc(c));
+    hipFree(save);
+    invalidate_graph(c);
+    if (group_out) *group_out = best_g;
+    return PDX_OK;
+}
+
 extern "C" int pdx_stats(pdx_ctx *c, int64_t *out4)
 {
     CHECK_CTX(c);

@@ -284,7 +284,7 @@ PDX_HD float bond_weight(float L, float invh, float Vj)
 // ---------------------------------------------------------------------------
 // F-pass: H = [sum_j w V_j du (x) xi] K^-1, P(H), A = P K^-1
 // ---------------------------------------------------------------------------
-template <int D, int MODEL, bool DIAG, bool COMMIT>
+template <int D, int MODEL, bool DIAG, bool COMMIT, int G>
 PDX_HD void fpass_one(const Dev &d, int i, int ubuf)
 {
     const KRec kr = d.KR[i];
@@ -297,32 +297,44 @@ PDX_HD void fpass_one(const Dev &d, int i, int ubuf)
     // into every A_j and from there into the residual
     double Hd[D * D];
     for (int q = 0; q < D * D; ++q) Hd[q] = 0.0;
-    const int64_t base = slot0(d, i);
+    // bond loop in groups of G: the entries of the next group are loaded while
+    // the current group's neighbour records are gathered
+    const uint32_t *__restrict__ bp = d.bj + slot0(d, i);
     const int nnb = (int)kr.nnb;
-    for (int k = 0; k < nnb; ++k) {
-        const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
-        // broken bonds (bit 31) are skipped; an index outside the particles
-        // never becomes an address
-        if (e >= (uint32_t)d.ni) continue;
-        const int j = (int)e;
-        const PRec pj = d.P[j];
-        const pdx_d4 uj = U[j];
-        float xi[3], du[3];
-        xi[0] = (float)(pj.X[0] - pi.X[0]);
-        xi[1] = (float)(pj.X[1] - pi.X[1]);
-        xi[2] = (float)(pj.X[2] - pi.X[2]);
-        du[0] = (float)(uj.x - ui.x);
-        du[1] = (float)(uj.y - ui.y);
-        du[2] = (float)(uj.z - ui.z);
-        float L2 = 0.f;
-        for (int r = 0; r < D; ++r) L2 = fmaf(xi[r], xi[r], L2);
-        const float w = bond_weight(sqrtf(L2), invh, pj.V);
-        double xd[3];
-        for (int c = 0; c < D; ++c) xd[c] = (double)xi[c];
-        for (int r = 0; r < D; ++r) {
-            const double wd = (double)(w * du[r]);
-            for (int c = 0; c < D; ++c) Hd[r * D + c] = fma(wd, xd[c], Hd[r * D + c]);
+    uint32_t ec[G], en[G];
+    for (int q = 0; q < G; ++q) ec[q] = q < nnb ? bp[(int64_t)q * PDX_SLICE] : PDX_BROKEN;
+    for (int k = 0; k < nnb; k += G) {
+        for (int q = 0; q < G; ++q) en[q] = (k + G + q) < nnb ? bp[(int64_t)(k + G + q) * PDX_SLICE] : PDX_BROKEN;
+        PRec pj[G];
+        pdx_d4 uj[G];
+        bool live[G];
+        for (int q = 0; q < G; ++q) {
+            // broken bonds (bit 31) and anything that is not a particle index
+            // gather the particle's own record and get weight 0
+            live[q] = ec[q] < (uint32_t)d.ni;
+            const int j = live[q] ? (int)ec[q] : i;
+            pj[q] = d.P[j];
+            uj[q] = U[j];
         }
+        for (int q = 0; q < G; ++q) {
+            float xi[3], du[3];
+            xi[0] = (float)(pj[q].X[0] - pi.X[0]);
+            xi[1] = (float)(pj[q].X[1] - pi.X[1]);
+            xi[2] = (float)(pj[q].X[2] - pi.X[2]);
+            du[0] = (float)(uj[q].x - ui.x);
+            du[1] = (float)(uj[q].y - ui.y);
+            du[2] = (float)(uj[q].z - ui.z);
+            float L2 = 0.f;
+            for (int r = 0; r < D; ++r) L2 = fmaf(xi[r], xi[r], L2);
+            const float w = live[q] ? bond_weight(sqrtf(L2), invh, pj[q].V) : 0.f;
+            double xd[3];
+            for (int c = 0; c < D; ++c) xd[c] = (double)xi[c];
+            for (int r = 0; r < D; ++r) {
+                const double wd = (double)(w * du[r]);
+                for (int c = 0; c < D; ++c) Hd[r * D + c] = fma(wd, xd[c], Hd[r * D + c]);
+            }
+        }
+        for (int q = 0; q < G; ++q) ec[q] = en[q];
     }
     // per-particle tensor algebra in FP64 with the FP64 inverse; A is rounded
     // to FP32 only when stored
@@ -384,7 +396,7 @@ PDX_HD void fpass_one(const Dev &d, int i, int ubuf)
 // red[0] += |f|^2 over free DOFs; UPD 1: red[1], red[2] = Rayleigh-quotient
 // numerator and denominator; UPD 2: red[1] = kinetic energy.
 // ---------------------------------------------------------------------------
-template <int D, int FORM, int UPD>
+template <int D, int FORM, int UPD, int G>
 PDX_HD void force_one(const Dev &d, int i, int ubuf, int cs, double dt, double visc, double *red)
 {
     const KRec kr = d.KR[i];
@@ -401,56 +413,63 @@ PDX_HD void force_one(const Dev &d, int i, int ubuf, int cs, double dt, double v
     // relative to |A_j - A_i| rather than |A|
     ARec ai;
     if (FORM == 0) ai = d.AR[i];
-    const int64_t base = slot0(d, i);
+    const uint32_t *__restrict__ bp = d.bj + slot0(d, i);
     const int nnb = (int)kr.nnb;
-    for (int k = 0; k < nnb; ++k) {
-        const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
-        // broken bonds (bit 31) are skipped; an index outside the particles
-        // never becomes an address
-        if (e >= (uint32_t)d.ni) continue;
-        const int j = (int)e;
-        const PRec pj = d.P[j];
-        const pdx_d4 uj = U[j];
-        float xi[3], du[3];
-        xi[0] = (float)(pj.X[0] - pi.X[0]);
-        xi[1] = (float)(pj.X[1] - pi.X[1]);
-        xi[2] = (float)(pj.X[2] - pi.X[2]);
-        du[0] = (float)(uj.x - ui.x);
-        du[1] = (float)(uj.y - ui.y);
-        du[2] = (float)(uj.z - ui.z);
-        float L2 = 0.f, xd = 0.f, dd = 0.f;
-        for (int r = 0; r < D; ++r) {
-            L2 = fmaf(xi[r], xi[r], L2);
-            xd = fmaf(xi[r], du[r], xd);
-            dd = fmaf(du[r], du[r], dd);
+    uint32_t ec[G], en[G];
+    for (int q = 0; q < G; ++q) ec[q] = q < nnb ? bp[(int64_t)q * PDX_SLICE] : PDX_BROKEN;
+    for (int k = 0; k < nnb; k += G) {
+        for (int q = 0; q < G; ++q) en[q] = (k + G + q) < nnb ? bp[(int64_t)(k + G + q) * PDX_SLICE] : PDX_BROKEN;
+        PRec pj[G];
+        pdx_d4 uj[G];
+        ARec aj[G];
+        bool live[G];
+        for (int q = 0; q < G; ++q) {
+            live[q] = ec[q] < (uint32_t)d.ni;
+            const int j = live[q] ? (int)ec[q] : i;
+            pj[q] = d.P[j];
+            uj[q] = U[j];
+            if (FORM == 0) aj[q] = d.AR[j];
         }
-        const float L = sqrtf(L2);
-        const float w = bond_weight(L, invh, pj.V);
-        float eta[3];
-        for (int r = 0; r < D; ++r) eta[r] = xi[r] + du[r];
-        // |eta|^2 = |xi|^2 + (2 xi.du + du.du); the stretch uses the
-        // cancellation-free |eta| - |xi| = (2 xi.du + du.du) / (|eta| + |xi|)
-        const float dn = fmaf(2.f, xd, dd);
-        const float el = sqrtf(fmaxf(L2 + dn, 0.f));
-        float tb[3] = {0.f, 0.f, 0.f};
-        if (FORM == 0) {
-            const ARec aj = d.AR[j];
+        for (int q = 0; q < G; ++q) {
+            float xi[3], du[3];
+            xi[0] = (float)(pj[q].X[0] - pi.X[0]);
+            xi[1] = (float)(pj[q].X[1] - pi.X[1]);
+            xi[2] = (float)(pj[q].X[2] - pi.X[2]);
+            du[0] = (float)(uj[q].x - ui.x);
+            du[1] = (float)(uj[q].y - ui.y);
+            du[2] = (float)(uj[q].z - ui.z);
+            float L2 = 0.f, xd = 0.f, dd = 0.f;
             for (int r = 0; r < D; ++r) {
-                float t = 0.f;
-                for (int c = 0; c < D; ++c) t = fmaf(aj.A[r * D + c] - ai.A[r * D + c], xi[c], t);
-                tb[r] = t;
+                L2 = fmaf(xi[r], xi[r], L2);
+                xd = fmaf(xi[r], du[r], xd);
+                dd = fmaf(du[r], du[r], dd);
             }
-            if (el > 1e-10f && L > 1e-10f) {
-                const float s = dn / ((el + L) * L);
-                const float g = Gs * 0.5f * (ci + pj.c) * s / el;
-                for (int r = 0; r < D; ++r) tb[r] = fmaf(g, eta[r], tb[r]);
+            const float L = sqrtf(L2);
+            const float w = bond_weight(L, invh, pj[q].V);
+            float eta[3];
+            for (int r = 0; r < D; ++r) eta[r] = xi[r] + du[r];
+            // |eta|^2 = |xi|^2 + (2 xi.du + du.du); the stretch uses the
+            // cancellation-free |eta| - |xi| = (2 xi.du + du.du) / (|eta| + |xi|)
+            const float dn = fmaf(2.f, xd, dd);
+            const float el = sqrtf(fmaxf(L2 + dn, 0.f));
+            float tb[3] = {0.f, 0.f, 0.f};
+            float g = 0.f;
+            if (FORM == 0) {
+                for (int r = 0; r < D; ++r) {
+                    float t = 0.f;
+                    for (int c = 0; c < D; ++c) t = fmaf(aj[q].A[r * D + c] - ai.A[r * D + c], xi[c], t);
+                    tb[r] = t;
+                }
+                if (el > 1e-10f && L > 1e-10f) g = Gs * 0.5f * (ci + pj[q].c) * (dn / ((el + L) * L)) / el;
+            } else if (el > 1e-10f) {
+                g = 0.5f * (ci + pj[q].c) * (dn / ((el + L) * L)) / el;
             }
-        } else if (el > 1e-10f) {
-            const float s = dn / ((el + L) * L);
-            const float g = 0.5f * (ci + pj.c) * s / el;
-            for (int r = 0; r < D; ++r) tb[r] = g * eta[r];
+            for (int r = 0; r < D; ++r) {
+                const float contrib = w * fmaf(g, eta[r], tb[r]);
+                fa[r] += live[q] ? (double)contrib : 0.0;
+            }
         }
-        for (int r = 0; r < D; ++r) fa[r] += (double)(w * tb[r]);
+        for (int q = 0; q < G; ++q) ec[q] = en[q];
     }
     if (FORM == 0) {
         for (int r = 0; r < D; ++r) {
@@ -587,6 +606,20 @@ PDX_HD void kd_move_one(const Dev &d, int i, int ubuf_in, double dt, bool reset)
 // ---------------------------------------------------------------------------
 // setup (FP64)
 // ---------------------------------------------------------------------------
+// sort key of a bond: quantised (z, y, x) offset in 10 bits each, then the index
+template <int D>
+PDX_HD uint64_t stencil_key(const Dev &d, const PRec &pi, uint32_t j, double hk)
+{
+    const PRec pj = d.P[j];
+    uint64_t key = 0;
+    for (int q = D - 1; q >= 0; --q) {
+        long long t = llround((pj.X[q] - pi.X[q]) * hk) + 512;
+        t = t < 0 ? 0 : (t > 1023 ? 1023 : t);
+        key = (key << 10) | (uint64_t)t;
+    }
+    return (key << 31) | (uint64_t)j;
+}
+
 // neighbour scan over the 27 (9) cells around i; distance summed as
 // (dx*dx + dy*dy) + dz*dz without contraction, the reference's set test
 // (neighbor.py:179-183, 199-203); FILL writes the row sorted by index
@@ -641,14 +674,22 @@ PDX_HD void neighbors_one(const Dev &d, int i)
         d.counts[i] = cnt;
         return;
     }
+    // rows in stencil order: by the bond vector quantised to horizon/256
+    // (z, then y, then x), ties by index -- on a lattice every interior
+    // particle lists its bonds in the same order, so the k-th gathers of a
+    // wave land on one shifted Morton brick
+    const double hk = 256.0 / sqrt(d.h2);
     for (int p = 1; p < cnt; ++p) {
-        const uint32_t key = d.bj[base + (int64_t)p * PDX_SLICE];
+        const uint32_t ej = d.bj[base + (int64_t)p * PDX_SLICE];
+        const uint64_t key = stencil_key<D>(d, pi, ej, hk);
         int q = p - 1;
-        while (q >= 0 && d.bj[base + (int64_t)q * PDX_SLICE] > key) {
-            d.bj[base + (int64_t)(q + 1) * PDX_SLICE] = d.bj[base + (int64_t)q * PDX_SLICE];
+        while (q >= 0) {
+            const uint32_t eq = d.bj[base + (int64_t)q * PDX_SLICE];
+            if (stencil_key<D>(d, pi, eq, hk) <= key) break;
+            d.bj[base + (int64_t)(q + 1) * PDX_SLICE] = eq;
             --q;
         }
-        d.bj[base + (int64_t)(q + 1) * PDX_SLICE] = key;
+        d.bj[base + (int64_t)(q + 1) * PDX_SLICE] = ej;
     }
     d.init_nb[i] = cnt;
 }

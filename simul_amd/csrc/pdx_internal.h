@@ -27,6 +27,7 @@ struct pdx_ctx {
     double horizon = 0.0, G_s = 0.0;
     int64_t nb = 0, slots = 0;
     int ucur = 0;
+    int tune_g = 2;  // bonds per group in the hot loops (1, 2 or 4)
     Dev d{};
     // host mirror of slice pointers
     std::vector<int64_t> h_slptr;

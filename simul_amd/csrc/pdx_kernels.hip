@@ -85,17 +85,17 @@ __device__ bool reduce_case(const Dev &d, int lb, int cs, double (&val)[NV], dou
 // ---------------------------------------------------------------------------
 // hot kernels
 // ---------------------------------------------------------------------------
-template <int D, int MODEL, bool DIAG, bool COMMIT>
+template <int D, int MODEL, bool DIAG, bool COMMIT, int G>
 __global__ __launch_bounds__(PDX_BLOCK) void k_fpass(Dev d, int ubuf, int solve_mode)
 {
     const int lb = logical_block();
     if (solve_mode && d.sc[d.case_of_blk[lb]].converged) return;
     const int i = lb * PDX_BLOCK + threadIdx.x;
     if (i >= d.ni) return;
-    fpass_one<D, MODEL, DIAG, COMMIT>(d, i, ubuf);
+    fpass_one<D, MODEL, DIAG, COMMIT, G>(d, i, ubuf);
 }
 
-template <int D, int FORM, int UPD>
+template <int D, int FORM, int UPD, int G>
 __global__ __launch_bounds__(PDX_BLOCK) void k_force(Dev d, int ubuf, int solve_mode, double dt, double visc)
 {
     const int lb = logical_block();
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_force(Dev d, int ubuf, int solve_
     if (solve_mode && d.sc[cs].converged) return;
     const int i = lb * PDX_BLOCK + threadIdx.x;
     double red[3] = {0.0, 0.0, 0.0};
-    if (i < d.ni) force_one<D, FORM, UPD>(d, i, ubuf, cs, dt, visc, red);
+    if (i < d.ni) force_one<D, FORM, UPD, G>(d, i, ubuf, cs, dt, visc, red);
     double tot[3];
     if (!reduce_case<3>(d, lb, cs, red, tot)) return;
     if (threadIdx.x == 0) case_update<UPD>(d.sc[cs], tot, ubuf);
@@ -218,16 +218,28 @@ int pdx_launch_shape(pdx_ctx *c, int mode, double tol)
     return PDX_OK;
 }
 
-template <int D, int MODEL>
-static void launch_fpass_t(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm)
+// the hot variants come in bond-group sizes 1, 2, 4 (chosen per context by
+// pdx_autotune); diagnostic and commit passes use group 1
+template <int D, int MODEL, int G>
+static void launch_fpass_g(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm)
 {
     dim3 g(c->nblk), b(PDX_BLOCK);
     if (commit)
-        hipLaunchKernelGGL((k_fpass<D, MODEL, false, true>), g, b, 0, c->stream, c->d, ubuf, sm);
+        hipLaunchKernelGGL((k_fpass<D, MODEL, false, true, 1>), g, b, 0, c->stream, c->d, ubuf, sm);
     else if (diag)
-        hipLaunchKernelGGL((k_fpass<D, MODEL, true, false>), g, b, 0, c->stream, c->d, ubuf, sm);
+        hipLaunchKernelGGL((k_fpass<D, MODEL, true, false, 1>), g, b, 0, c->stream, c->d, ubuf, sm);
     else
-        hipLaunchKernelGGL((k_fpass<D, MODEL, false, false>), g, b, 0, c->stream, c->d, ubuf, sm);
+        hipLaunchKernelGGL((k_fpass<D, MODEL, false, false, G>), g, b, 0, c->stream, c->d, ubuf, sm);
+}
+
+template <int D, int MODEL>
+static void launch_fpass_t(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm)
+{
+    switch (c->tune_g) {
+    case 4: launch_fpass_g<D, MODEL, 4>(c, ubuf, diag, commit, sm); break;
+    case 2: launch_fpass_g<D, MODEL, 2>(c, ubuf, diag, commit, sm); break;
+    default: launch_fpass_g<D, MODEL, 1>(c, ubuf, diag, commit, sm); break;
+    }
 }
 
 int pdx_launch_fpass(pdx_ctx *c, int ubuf, bool diag, bool commit, bool solve_mode)
@@ -249,16 +261,26 @@ int pdx_launch_fpass(pdx_ctx *c, int ubuf, bool diag, bool commit, bool solve_mo
     return PDX_OK;
 }
 
-template <int D, int FORM>
-static void launch_force_t(pdx_ctx *c, int ubuf, int upd, int sm, double dt, double visc)
+template <int D, int FORM, int G>
+static void launch_force_g(pdx_ctx *c, int ubuf, int upd, int sm, double dt, double visc)
 {
     dim3 g(c->nblk), b(PDX_BLOCK);
     if (upd == 1)
-        hipLaunchKernelGGL((k_force<D, FORM, 1>), g, b, 0, c->stream, c->d, ubuf, sm, dt, visc);
+        hipLaunchKernelGGL((k_force<D, FORM, 1, G>), g, b, 0, c->stream, c->d, ubuf, sm, dt, visc);
     else if (upd == 2)
-        hipLaunchKernelGGL((k_force<D, FORM, 2>), g, b, 0, c->stream, c->d, ubuf, sm, dt, visc);
+        hipLaunchKernelGGL((k_force<D, FORM, 2, G>), g, b, 0, c->stream, c->d, ubuf, sm, dt, visc);
     else
-        hipLaunchKernelGGL((k_force<D, FORM, 0>), g, b, 0, c->stream, c->d, ubuf, sm, dt, visc);
+        hipLaunchKernelGGL((k_force<D, FORM, 0, 1>), g, b, 0, c->stream, c->d, ubuf, sm, dt, visc);
+}
+
+template <int D, int FORM>
+static void launch_force_t(pdx_ctx *c, int ubuf, int upd, int sm, double dt, double visc)
+{
+    switch (c->tune_g) {
+    case 4: launch_force_g<D, FORM, 4>(c, ubuf, upd, sm, dt, visc); break;
+    case 2: launch_force_g<D, FORM, 2>(c, ubuf, upd, sm, dt, visc); break;
+    default: launch_force_g<D, FORM, 1>(c, ubuf, upd, sm, dt, visc); break;
+    }
 }
 
 int pdx_launch_force(pdx_ctx *c, int ubuf, int upd, bool solve_mode, double dt, double visc)

@@ -45,7 +45,7 @@ struct Run {
     int64_t slots = 0, nb = 0;
 };
 
-template <int D, int MODEL, int FORM>
+template <int D, int MODEL, int FORM, int G>
 static double sweep(Run &R, int iters)
 {
     Dev &d = R.d;
@@ -53,11 +53,11 @@ static double sweep(Run &R, int iters)
     double check = 0.0;
     for (int i = 0; i < ni; ++i) shape_one<D>(d, i, 1, 1e-12);
     for (int i = 0; i < ni; ++i) zero_free_one(d, i, 1);
-    for (int i = 0; i < ni; ++i) fpass_one<D, MODEL, true, false>(d, i, 0);
+    for (int i = 0; i < ni; ++i) fpass_one<D, MODEL, true, false, 1>(d, i, 0);
     std::vector<double> red(3 * ni, 0.0);
     for (int i = 0; i < ni; ++i) {
         const int cs = d.case_of_blk[i / PDX_BLOCK];
-        force_one<D, FORM, 0>(d, i, 0, cs, 0.0, 0.0, &red[3 * i]);
+        force_one<D, FORM, 0, 1>(d, i, 0, cs, 0.0, 0.0, &red[3 * i]);
     }
     for (int i = 0; i < ni; ++i) check += red[3 * i] + resid_one<D>(d, i);
     // adaptive-DR mass, then a few adaptive iterations with the case update
@@ -68,12 +68,12 @@ static double sweep(Run &R, int iters)
     free(g_buf);
     int ub = 0;
     for (int it = 0; it < iters; ++it) {
-        for (int i = 0; i < ni; ++i) fpass_one<D, MODEL, false, false>(d, i, ub);
+        for (int i = 0; i < ni; ++i) fpass_one<D, MODEL, false, false, G>(d, i, ub);
         std::vector<double> tot(3 * R.ncase, 0.0);
         for (int i = 0; i < ni; ++i) {
             const int cs = d.case_of_blk[i / PDX_BLOCK];
             double r[3] = {0, 0, 0};
-            force_one<D, FORM, 1>(d, i, ub, cs, 0.0, 0.0, r);
+            force_one<D, FORM, 1, G>(d, i, ub, cs, 0.0, 0.0, r);
             for (int q = 0; q < 3; ++q) tot[3 * cs + q] += r[q];
         }
         for (int k = 0; k < R.ncase; ++k) case_update<1>(d.sc[k], &tot[3 * k], ub);
@@ -82,14 +82,14 @@ static double sweep(Run &R, int iters)
     // kinetic-damping step
     for (int i = 0; i < ni; ++i) kd_move_one<D>(d, i, ub, 1e-3, (i & 1) != 0);
     ub ^= 1;
-    for (int i = 0; i < ni; ++i) fpass_one<D, MODEL, false, false>(d, i, ub);
+    for (int i = 0; i < ni; ++i) fpass_one<D, MODEL, false, false, G>(d, i, ub);
     for (int i = 0; i < ni; ++i) {
         double r[3] = {0, 0, 0};
-        force_one<D, FORM, 2>(d, i, ub, d.case_of_blk[i / PDX_BLOCK], 1e-3, 0.0, r);
+        force_one<D, FORM, 2, G>(d, i, ub, d.case_of_blk[i / PDX_BLOCK], 1e-3, 0.0, r);
         check += r[0];
     }
     if (MODEL == 2)
-        for (int i = 0; i < ni; ++i) fpass_one<D, MODEL, false, true>(d, i, ub);
+        for (int i = 0; i < ni; ++i) fpass_one<D, MODEL, false, true, 1>(d, i, ub);
     // setup and damage bodies
     double *dtb = alloc<double>(ni);
     for (int i = 0; i < ni; ++i) dt_est_one<D>(d, i, 1.0, dtb);
@@ -114,10 +114,11 @@ static double sweep(Run &R, int iters)
 int main(int argc, char **argv)
 {
     if (argc < 2) {
-        fprintf(stderr, "usage: %s input.bin [iters]\n", argv[0]);
+        fprintf(stderr, "usage: %s input.bin [iters] [group]\n", argv[0]);
         return 2;
     }
     const int iters = argc > 2 ? atoi(argv[2]) : 3;
+    const int gsel = argc > 3 ? atoi(argv[3]) : 0;  // bond-group size to sweep (0 = all)
     FILE *f = fopen(argv[1], "rb");
     if (!f) return 2;
     int32_t hdr[5];
@@ -277,15 +278,18 @@ int main(int argc, char **argv)
         }
     }
     double chk = 0.0;
-    if (D == 3) {
-        if (R.form == 1) chk = sweep<3, 0, 1>(R, iters);
-        else if (R.model == 1) chk = sweep<3, 1, 0>(R, iters);
-        else if (R.model == 2) chk = sweep<3, 2, 0>(R, iters);
-        else chk = sweep<3, 0, 0>(R, iters);
-    } else {
-        if (R.form == 1) chk = sweep<2, 0, 1>(R, iters);
-        else if (R.model == 1) chk = sweep<2, 1, 0>(R, iters);
-        else chk = sweep<2, 0, 0>(R, iters);
+    for (int G : {1, 2, 4}) {
+        if (gsel && G != gsel) continue;
+        if (D == 3) {
+            if (R.form == 1) chk = G == 1 ? sweep<3, 0, 1, 1>(R, iters) : G == 2 ? sweep<3, 0, 1, 2>(R, iters) : sweep<3, 0, 1, 4>(R, iters);
+            else if (R.model == 1) chk = G == 1 ? sweep<3, 1, 0, 1>(R, iters) : G == 2 ? sweep<3, 1, 0, 2>(R, iters) : sweep<3, 1, 0, 4>(R, iters);
+            else if (R.model == 2) chk = G == 1 ? sweep<3, 2, 0, 1>(R, iters) : G == 2 ? sweep<3, 2, 0, 2>(R, iters) : sweep<3, 2, 0, 4>(R, iters);
+            else chk = G == 1 ? sweep<3, 0, 0, 1>(R, iters) : G == 2 ? sweep<3, 0, 0, 2>(R, iters) : sweep<3, 0, 0, 4>(R, iters);
+        } else {
+            if (R.form == 1) chk = G == 1 ? sweep<2, 0, 1, 1>(R, iters) : G == 2 ? sweep<2, 0, 1, 2>(R, iters) : sweep<2, 0, 1, 4>(R, iters);
+            else if (R.model == 1) chk = G == 1 ? sweep<2, 1, 0, 1>(R, iters) : G == 2 ? sweep<2, 1, 0, 2>(R, iters) : sweep<2, 1, 0, 4>(R, iters);
+            else chk = G == 1 ? sweep<2, 0, 0, 1>(R, iters) : G == 2 ? sweep<2, 0, 0, 2>(R, iters) : sweep<2, 0, 0, 4>(R, iters);
+        }
     }
     printf("ok n=%d ni=%d bonds=%lld slots=%lld check=%.6e\n", n, ni, (long long)R.nb, (long long)R.slots, chk);
     return 0;
