@@ -193,11 +193,16 @@ int pdx_set(pdx_ctx *ctx, int32_t field, const void *in);
 /* times `iters` adaptive-DR iterations with HIP events around every launch on the
  * context stream; ms[0] = mean F-pass, ms[1] = mean force pass, ms[2] = total */
 int pdx_time_kernels(pdx_ctx *ctx, int32_t iters, double *ms);
+/* hot-pass variants: per-lane neighbour gathers with bond groups 1, 2, 4, or
+ * wave-cooperative gathers (four lanes fetch one 64-byte neighbour record, staged
+ * through LDS) with bond groups 1, 2 as PDX_VARIANT_COOP | group.  All variants give
+ * the same bits. */
+#define PDX_VARIANT_COOP 16
 /* launch tuning (no reference counterpart): times `iters` adaptive-DR iterations of
- * every bond-group size of the hot kernels on this context's own data, keeps the
- * fastest and leaves the solver state as it was; group_out = chosen size */
+ * every hot-pass variant on this context's own data, keeps the fastest and leaves
+ * the solver state as it was; group_out = chosen variant */
 int pdx_autotune(pdx_ctx *ctx, int32_t iters, int32_t *group_out);
-/* sets the bond-group size (1, 2 or 4) directly */
+/* sets the hot-pass variant (1, 2, 4, PDX_VARIANT_COOP | 1, PDX_VARIANT_COOP | 2) */
 int pdx_set_group(pdx_ctx *ctx, int32_t group);
 /* n_particles, n_bonds (intact+broken), n_intact, sell_slots (padded bond slots) */
 int pdx_stats(pdx_ctx *ctx, int64_t *out4);

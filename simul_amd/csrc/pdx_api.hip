@@ -63,8 +63,8 @@ static void dfree(T *&p)
 static void free_all(pdx_ctx *c)
 {
     Dev &d = c->d;
-    dfree(d.P); dfree(d.KR); dfree(d.AR); dfree(d.j2p); dfree(d.j2s);
-    dfree(d.u[0]); dfree(d.u[1]); dfree(d.v); dfree(d.fp); dfree(d.f); dfree(d.fext);
+    dfree(d.xu[0]); dfree(d.xu[1]); dfree(d.KR); dfree(d.AR); dfree(d.j2p); dfree(d.j2s);
+    dfree(d.v); dfree(d.fp); dfree(d.f); dfree(d.fext);
     dfree(d.mass); dfree(d.lam); dfree(d.V64); dfree(d.K64); dfree(d.Kinv64);
     dfree(d.Fd); dfree(d.Pd); dfree(d.damage); dfree(d.init_nb);
     dfree(d.sl_ptr); dfree(d.bj); dfree(d.case_of_blk); dfree(d.case_blk0);
@@ -159,9 +159,9 @@ static int create_common(int32_t ncase, const int64_t *off, int32_t dim, pdx_ctx
     d.dim = dim;
     int r = PDX_OK;
     size_t ni = c->ni;
-    if ((r = dalloc(c->stream, &d.P, ni)) || (r = dalloc(c->stream, &d.KR, ni)) || (r = dalloc(c->stream, &d.AR, ni)) ||
-        (r = dalloc(c->stream, &d.j2p, ni)) || (r = dalloc(c->stream, &d.j2s, ni)) || (r = dalloc(c->stream, &d.u[0], ni)) ||
-        (r = dalloc(c->stream, &d.u[1], ni)) || (r = dalloc(c->stream, &d.v, ni)) || (r = dalloc(c->stream, &d.fp, ni)) ||
+    if ((r = dalloc(c->stream, &d.xu[0], ni)) || (r = dalloc(c->stream, &d.KR, ni)) || (r = dalloc(c->stream, &d.AR, ni)) ||
+        (r = dalloc(c->stream, &d.j2p, ni)) || (r = dalloc(c->stream, &d.j2s, ni)) ||
+        (r = dalloc(c->stream, &d.xu[1], ni)) || (r = dalloc(c->stream, &d.v, ni)) || (r = dalloc(c->stream, &d.fp, ni)) ||
         (r = dalloc(c->stream, &d.f, ni)) || (r = dalloc(c->stream, &d.fext, ni)) || (r = dalloc(c->stream, &d.mass, ni)) ||
         (r = dalloc(c->stream, &d.lam, ni)) || (r = dalloc(c->stream, &d.V64, ni)) || (r = dalloc(c->stream, &d.K64, ni * 9)) ||
         (r = dalloc(c->stream, &d.Kinv64, ni * 9)) || (r = dalloc(c->stream, &d.Fd, ni * 9)) || (r = dalloc(c->stream, &d.Pd, ni * 9)) ||
@@ -305,6 +305,40 @@ static int get_vec(pdx_ctx *c, const pdx_d4 *src, double *dst)
     return PDX_OK;
 }
 
+// displacement part of a gather buffer <-> caller-order n*dim doubles; the
+// strided copies touch only the u words of the 64-byte records
+static int put_u(pdx_ctx *c, int buf, const double *src)
+{
+    std::vector<pdx_d4> h(c->ni, d4(0, 0, 0));
+    for (int i = 0; i < c->ni; ++i) {
+        int r = c->perm[i];
+        if (r < 0) continue;
+        const double *s = src + (size_t)r * c->dim;
+        h[i] = d4(s[0], s[1], c->dim == 3 ? s[2] : 0.0);
+    }
+    PDX_HIP(hipMemcpy2DAsync(c->d.xu[buf]->u, sizeof(XRec), h.data(), sizeof(pdx_d4), sizeof(pdx_d4), c->ni,
+                             hipMemcpyHostToDevice, c->stream));
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    return PDX_OK;
+}
+
+static int get_u(pdx_ctx *c, int buf, double *dst)
+{
+    std::vector<pdx_d4> h(c->ni);
+    PDX_HIP(hipMemcpy2DAsync(h.data(), sizeof(pdx_d4), c->d.xu[buf]->u, sizeof(XRec), sizeof(pdx_d4), c->ni,
+                             hipMemcpyDeviceToHost, c->stream));
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->ni; ++i) {
+        int r = c->perm[i];
+        if (r < 0) continue;
+        double *o = dst + (size_t)r * c->dim;
+        o[0] = h[i].x;
+        o[1] = h[i].y;
+        if (c->dim == 3) o[2] = h[i].z;
+    }
+    return PDX_OK;
+}
+
 static int put_scalar(pdx_ctx *c, double *dst, const double *src)
 {
     std::vector<double> h(c->ni, 0.0);
@@ -340,14 +374,15 @@ static int get_tensor(pdx_ctx *c, const T *src, double *dst)
 // ---------------------------------------------------------------------------
 // inputs
 // ---------------------------------------------------------------------------
-// reference geometry records (X, V, c) and the FP64 volumes, internal order
+// reference geometry (X, V, c) of both gather buffers and the FP64 volumes,
+// internal order; the displacement words stay as they are
 static int push_prec(pdx_ctx *c)
 {
     HostMirror &m = mirror(c);
     const int D = c->dim;
-    std::vector<PRec> hp(c->ni);
+    std::vector<XRec> hp(c->ni);
     for (int i = 0; i < c->ni; ++i) {
-        PRec p;
+        XRec p;
         memset(&p, 0, sizeof(p));
         int r = c->perm[i];
         if (r >= 0) {
@@ -357,7 +392,13 @@ static int push_prec(pdx_ctx *c)
         }
         hp[i] = p;
     }
-    PDX_TRY(upload(c, c->d.P, hp));
+    XRec *tmp = nullptr;
+    PDX_HIP(hipMallocAsync((void **)&tmp, sizeof(XRec) * hp.size(), c->stream));
+    PDX_HIP(hipMemcpyAsync(tmp, hp.data(), sizeof(XRec) * hp.size(), hipMemcpyHostToDevice, c->stream));
+    int r = pdx_launch_xu_geom(c, tmp);
+    hipFreeAsync(tmp, c->stream);
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    if (r != PDX_OK) return r;
     return put_scalar(c, c->d.V64, m.vol.data());
 }
 
@@ -383,12 +424,12 @@ extern "C" int pdx_set_particles(pdx_ctx *c, const double *X, const double *vol,
     for (int k = 0; k < c->ncase; ++k)
         for (int i = c->case_p0[k]; i < c->case_p0[k + 1]; ++i)
             if (c->perm[i] >= 0) cop[i] = k;
+    // state back to the reference initial values (particles.py:159-173)
+    PDX_HIP(hipMemsetAsync(c->d.xu[0], 0, sizeof(XRec) * c->ni, c->stream));
+    PDX_HIP(hipMemsetAsync(c->d.xu[1], 0, sizeof(XRec) * c->ni, c->stream));
     PDX_TRY(push_prec(c));
     PDX_TRY(upload(c, c->d.case_of_p, cop));
     PDX_TRY(put_scalar(c, c->d.mass, m.mass.data()));
-    // state back to the reference initial values (particles.py:159-173)
-    PDX_HIP(hipMemsetAsync(c->d.u[0], 0, sizeof(pdx_d4) * c->ni, c->stream));
-    PDX_HIP(hipMemsetAsync(c->d.u[1], 0, sizeof(pdx_d4) * c->ni, c->stream));
     PDX_HIP(hipMemsetAsync(c->d.v, 0, sizeof(pdx_d4) * c->ni, c->stream));
     PDX_HIP(hipMemsetAsync(c->d.fp, 0, sizeof(pdx_d4) * c->ni, c->stream));
     PDX_HIP(hipMemsetAsync(c->d.f, 0, sizeof(pdx_d4) * c->ni, c->stream));
@@ -721,16 +762,16 @@ extern "C" int pdx_cold_residual(pdx_ctx *c, double *out)
 {
     CHECK_CTX(c);
     PDX_TRY(need_K(c));
-    pdx_d4 *save = nullptr;
-    PDX_HIP(hipMalloc((void **)&save, sizeof(pdx_d4) * c->ni));
-    PDX_HIP(hipMemcpyAsync(save, c->d.u[c->ucur], sizeof(pdx_d4) * c->ni, hipMemcpyDeviceToDevice, c->stream));
+    XRec *save = nullptr;
+    PDX_HIP(hipMalloc((void **)&save, sizeof(XRec) * c->ni));
+    PDX_HIP(hipMemcpyAsync(save, c->d.xu[c->ucur], sizeof(XRec) * c->ni, hipMemcpyDeviceToDevice, c->stream));
     int r = pdx_launch_zero_free(c, c->ucur);
     if (r == PDX_OK && c->form == PDX_FORM_NOSB) r = pdx_launch_fpass(c, c->ucur, false, false, false);
     if (r == PDX_OK) r = pdx_launch_force(c, c->ucur, 0, false, 0.0, 0.0);
     if (r == PDX_OK) r = read_sc(c);
     if (r == PDX_OK)
         for (int k = 0; k < c->ncase; ++k) out[k] = c->h_sc[k].res;
-    hipMemcpyAsync(c->d.u[c->ucur], save, sizeof(pdx_d4) * c->ni, hipMemcpyDeviceToDevice, c->stream);
+    hipMemcpyAsync(c->d.xu[c->ucur], save, sizeof(XRec) * c->ni, hipMemcpyDeviceToDevice, c->stream);
     hipStreamSynchronize(c->stream);
     hipFree(save);
     c->have_A = false;
@@ -982,7 +1023,7 @@ extern "C" int pdx_dr_solve(pdx_ctx *c, const pdx_dr_params *p, pdx_dr_result *r
         const DScal &s = c->h_sc[k];
         if (s.converged && s.sol_buf >= 0 && s.sol_buf != c->ucur) {
             const size_t o = c->case_p0[k], nn = c->case_p0[k + 1] - c->case_p0[k];
-            PDX_HIP(hipMemcpyAsync(c->d.u[c->ucur] + o, c->d.u[s.sol_buf] + o, sizeof(pdx_d4) * nn,
+            PDX_HIP(hipMemcpyAsync(c->d.xu[c->ucur] + o, c->d.xu[s.sol_buf] + o, sizeof(XRec) * nn,
                                    hipMemcpyDeviceToDevice, c->stream));
         }
     }
@@ -1053,7 +1094,8 @@ extern "C" int pdx_time_kernels(pdx_ctx *c, int32_t iters, double *ms)
 extern "C" int pdx_set_group(pdx_ctx *c, int32_t g)
 {
     CHECK_CTX(c);
-    if (g != 1 && g != 2 && g != 4) return fail(PDX_EINVAL, "group must be 1, 2 or 4");
+    if (g != 1 && g != 2 && g != 4 && g != (PDX_VARIANT_COOP | 1) && g != (PDX_VARIANT_COOP | 2))
+        return fail(PDX_EINVAL, "variant must be 1, 2, 4, PDX_VARIANT_COOP|1 or PDX_VARIANT_COOP|2");
     c->tune_g = g;
     invalidate_graph(c);
     return PDX_OK;
@@ -1066,18 +1108,19 @@ extern "C" int pdx_autotune(pdx_ctx *c, int32_t iters, int32_t *group_out)
     if (iters <= 0) return fail(PDX_EINVAL, "iters must be positive");
     if (!c->have_mass) PDX_TRY(pdx_adr_mass(c, 1.0));
     // keep the solver state: the timing iterations run on a copy
-    const size_t nv = sizeof(pdx_d4) * c->ni;
+    const size_t nv = sizeof(pdx_d4) * c->ni, nx = sizeof(XRec) * c->ni;
     pdx_d4 *save = nullptr;
-    PDX_HIP(hipMalloc((void **)&save, 3 * nv));
-    PDX_HIP(hipMemcpyAsync(save, c->d.u[c->ucur], nv, hipMemcpyDeviceToDevice, c->stream));
-    PDX_HIP(hipMemcpyAsync(save + c->ni, c->d.v, nv, hipMemcpyDeviceToDevice, c->stream));
-    PDX_HIP(hipMemcpyAsync(save + 2 * c->ni, c->d.fp, nv, hipMemcpyDeviceToDevice, c->stream));
+    PDX_HIP(hipMalloc((void **)&save, 2 * nv + nx));
+    XRec *save_x = reinterpret_cast<XRec *>(save + 2 * c->ni);
+    PDX_HIP(hipMemcpyAsync(save_x, c->d.xu[c->ucur], nx, hipMemcpyDeviceToDevice, c->stream));
+    PDX_HIP(hipMemcpyAsync(save, c->d.v, nv, hipMemcpyDeviceToDevice, c->stream));
+    PDX_HIP(hipMemcpyAsync(save + c->ni, c->d.fp, nv, hipMemcpyDeviceToDevice, c->stream));
     PDX_TRY(read_sc(c));
     std::vector<DScal> sc0(c->h_sc, c->h_sc + c->ncase);
     const int u0 = c->ucur;
     double best = 1e300;
     int best_g = c->tune_g;
-    for (int g : {1, 2, 4}) {
+    for (int g : {1, 2, 4, PDX_VARIANT_COOP | 1, PDX_VARIANT_COOP | 2}) {
         c->tune_g = g;
         double ms[3];
         int r = pdx_time_kernels(c, iters, ms);
@@ -1092,9 +1135,9 @@ extern "C" int pdx_autotune(pdx_ctx *c, int32_t iters, int32_t *group_out)
     }
     c->tune_g = best_g;
     c->ucur = u0;
-    PDX_HIP(hipMemcpyAsync(c->d.u[c->ucur], save, nv, hipMemcpyDeviceToDevice, c->stream));
-    PDX_HIP(hipMemcpyAsync(c->d.v, save + c->ni, nv, hipMemcpyDeviceToDevice, c->stream));
-    PDX_HIP(hipMemcpyAsync(c->d.fp, save + 2 * c->ni, nv, hipMemcpyDeviceToDevice, c->stream));
+    PDX_HIP(hipMemcpyAsync(c->d.xu[c->ucur], save_x, nx, hipMemcpyDeviceToDevice, c->stream));
+    PDX_HIP(hipMemcpyAsync(c->d.v, save, nv, hipMemcpyDeviceToDevice, c->stream));
+    PDX_HIP(hipMemcpyAsync(c->d.fp, save + c->ni, nv, hipMemcpyDeviceToDevice, c->stream));
     std::copy(sc0.begin(), sc0.end(), c->h_sc);
     PDX_TRY(write_sc(c));
     hipFree(save);
@@ -1150,11 +1193,11 @@ extern "C" int pdx_get(pdx_ctx *c, int32_t field, void *out)
         memcpy(o, c->hX.data(), sizeof(double) * c->hX.size());
         return PDX_OK;
     case PDX_FIELD_x: {
-        PDX_TRY(get_vec(c, c->d.u[c->ucur], o));
+        PDX_TRY(get_u(c, c->ucur, o));
         for (size_t k = 0; k < c->hX.size(); ++k) o[k] = c->hX[k] + o[k];
         return PDX_OK;
     }
-    case PDX_FIELD_u: return get_vec(c, c->d.u[c->ucur], o);
+    case PDX_FIELD_u: return get_u(c, c->ucur, o);
     case PDX_FIELD_v: return get_vec(c, c->d.v, o);
     case PDX_FIELD_a: return get_vec(c, c->d.fp, o);
     case PDX_FIELD_f: return get_vec(c, c->d.f, o);
@@ -1215,9 +1258,9 @@ extern "C" int pdx_set(pdx_ctx *c, int32_t field, const void *in)
         std::vector<double> u(c->hX.size());
         for (size_t k = 0; k < u.size(); ++k) u[k] = x[k] - c->hX[k];
         c->have_A = false;
-        return put_vec(c, c->d.u[c->ucur], u.data());
+        return put_u(c, c->ucur, u.data());
     }
-    case PDX_FIELD_u: c->have_A = false; return put_vec(c, c->d.u[c->ucur], x);
+    case PDX_FIELD_u: c->have_A = false; return put_u(c, c->ucur, x);
     case PDX_FIELD_v: return put_vec(c, c->d.v, x);
     case PDX_FIELD_a: return put_vec(c, c->d.fp, x);
     case PDX_FIELD_f: return put_vec(c, c->d.f, x);

@@ -44,11 +44,17 @@ struct alignas(32) pdx_d4 {
 
 PDX_HD pdx_d4 d4(double x, double y, double z) { return pdx_d4{x, y, z, 0.0}; }
 
-// gather record of the reference geometry: 32 bytes
-struct alignas(32) PRec {
+// gather record of a particle, 64 bytes: reference position, volume,
+// stabilisation micromodulus and the displacement iterate.  The record is
+// double-buffered on u (ping-pong between passes); X, V and c are identical in
+// both buffers.  One 64-byte record per neighbour lets four lanes fetch it
+// with one 16-byte load each (wave-cooperative gathers).
+struct alignas(64) XRec {
     double X[3];
     float V;  // particle volume
     float c;  // stabilisation micromodulus c_i
+    double u[3];
+    double w;  // unused, zero
 };
 
 // per-particle constants of the hot loop: 64 bytes
@@ -105,12 +111,11 @@ struct Dev {
     int32_t nblk;      // launch blocks (multiple of PDX_NXCD)
     int32_t ncase;
     int32_t dim;
-    PRec *P;
+    XRec *xu[2];       // ping-pong gather records (u differs, X/V/c shared)
     KRec *KR;
     ARec *AR;
     J2Par *j2p;
     J2State *j2s;
-    pdx_d4 *u[2];
     pdx_d4 *v;
     pdx_d4 *fp;        // previous force (adaptive DR) / acceleration (kinetic DR)
     pdx_d4 *f;         // force output of force-only passes
@@ -284,17 +289,43 @@ PDX_HD float bond_weight(float L, float invh, float Vj)
 // ---------------------------------------------------------------------------
 // F-pass: H = [sum_j w V_j du (x) xi] K^-1, P(H), A = P K^-1
 // ---------------------------------------------------------------------------
+// F-pass term of one bond: Hd += w V_j du (x) xi (FP32 product, FP64 sum).
+// H is accumulated in FP64: a 100+-term FP32 sum carries ~1e-6 relative error
+// into every A_j and from there into the residual.
+template <int D>
+PDX_HD void fpass_bond(const XRec &pi, const XRec &pj, float invh, bool live, double *Hd)
+{
+    float xi[3], du[3];
+    for (int r = 0; r < 3; ++r) {
+        xi[r] = (float)(pj.X[r] - pi.X[r]);
+        du[r] = (float)(pj.u[r] - pi.u[r]);
+    }
+    float L2 = 0.f;
+    for (int r = 0; r < D; ++r) L2 = fmaf(xi[r], xi[r], L2);
+    const float w = live ? bond_weight(sqrtf(L2), invh, pj.V) : 0.f;
+    for (int r = 0; r < D; ++r) {
+        const double wd = (double)(w * du[r]);
+        for (int c = 0; c < D; ++c) Hd[r * D + c] = fma(wd, (double)xi[c], Hd[r * D + c]);
+    }
+}
+
+// entry of bond slot k of a row, or "no bond" past the row's end
+PDX_HD uint32_t bond_entry(const uint32_t *bp, int k, int nnb)
+{
+    return k < nnb ? bp[(int64_t)k * PDX_SLICE] : PDX_BROKEN;
+}
+
+template <int D, int MODEL, bool DIAG, bool COMMIT>
+PDX_HD void fpass_finish(const Dev &d, int i, const KRec &kr, const double *Hd);
+
 template <int D, int MODEL, bool DIAG, bool COMMIT, int G>
 PDX_HD void fpass_one(const Dev &d, int i, int ubuf)
 {
     const KRec kr = d.KR[i];
     if (!(kr.flags & PDX_FL_ACTIVE)) return;
-    const pdx_d4 *__restrict__ U = d.u[ubuf];
-    const PRec pi = d.P[i];
-    const pdx_d4 ui = U[i];
+    const XRec *__restrict__ X = d.xu[ubuf];
+    const XRec pi = X[i];
     const float invh = (float)d.inv_h;
-    // H accumulated in FP64: a 100+-term FP32 sum carries ~1e-6 relative error
-    // into every A_j and from there into the residual
     double Hd[D * D];
     for (int q = 0; q < D * D; ++q) Hd[q] = 0.0;
     // bond loop in groups of G: the entries of the next group are loaded while
@@ -302,42 +333,28 @@ PDX_HD void fpass_one(const Dev &d, int i, int ubuf)
     const uint32_t *__restrict__ bp = d.bj + slot0(d, i);
     const int nnb = (int)kr.nnb;
     uint32_t ec[G], en[G];
-    for (int q = 0; q < G; ++q) ec[q] = q < nnb ? bp[(int64_t)q * PDX_SLICE] : PDX_BROKEN;
+    for (int q = 0; q < G; ++q) ec[q] = bond_entry(bp, q, nnb);
     for (int k = 0; k < nnb; k += G) {
-        for (int q = 0; q < G; ++q) en[q] = (k + G + q) < nnb ? bp[(int64_t)(k + G + q) * PDX_SLICE] : PDX_BROKEN;
-        PRec pj[G];
-        pdx_d4 uj[G];
+        for (int q = 0; q < G; ++q) en[q] = bond_entry(bp, k + G + q, nnb);
+        XRec pj[G];
         bool live[G];
         for (int q = 0; q < G; ++q) {
             // broken bonds (bit 31) and anything that is not a particle index
             // gather the particle's own record and get weight 0
             live[q] = ec[q] < (uint32_t)d.ni;
-            const int j = live[q] ? (int)ec[q] : i;
-            pj[q] = d.P[j];
-            uj[q] = U[j];
+            pj[q] = X[live[q] ? (int)ec[q] : i];
         }
-        for (int q = 0; q < G; ++q) {
-            float xi[3], du[3];
-            xi[0] = (float)(pj[q].X[0] - pi.X[0]);
-            xi[1] = (float)(pj[q].X[1] - pi.X[1]);
-            xi[2] = (float)(pj[q].X[2] - pi.X[2]);
-            du[0] = (float)(uj[q].x - ui.x);
-            du[1] = (float)(uj[q].y - ui.y);
-            du[2] = (float)(uj[q].z - ui.z);
-            float L2 = 0.f;
-            for (int r = 0; r < D; ++r) L2 = fmaf(xi[r], xi[r], L2);
-            const float w = live[q] ? bond_weight(sqrtf(L2), invh, pj[q].V) : 0.f;
-            double xd[3];
-            for (int c = 0; c < D; ++c) xd[c] = (double)xi[c];
-            for (int r = 0; r < D; ++r) {
-                const double wd = (double)(w * du[r]);
-                for (int c = 0; c < D; ++c) Hd[r * D + c] = fma(wd, xd[c], Hd[r * D + c]);
-            }
-        }
+        for (int q = 0; q < G; ++q) fpass_bond<D>(pi, pj[q], invh, live[q], Hd);
         for (int q = 0; q < G; ++q) ec[q] = en[q];
     }
-    // per-particle tensor algebra in FP64 with the FP64 inverse; A is rounded
-    // to FP32 only when stored
+    fpass_finish<D, MODEL, DIAG, COMMIT>(d, i, kr, Hd);
+}
+
+// per-particle tensor algebra of the F-pass in FP64 with the FP64 inverse:
+// H = Hd K^-1, P(H), A = P K^-1; A is rounded to FP32 only when stored
+template <int D, int MODEL, bool DIAG, bool COMMIT>
+PDX_HD void fpass_finish(const Dev &d, int i, const KRec &kr, const double *Hd)
+{
     const double *Ki = d.Kinv64 + (int64_t)i * 9;
     const bool sing = (kr.flags & PDX_FL_SING) != 0;
     double H[D * D];
@@ -396,81 +413,94 @@ PDX_HD void fpass_one(const Dev &d, int i, int ubuf)
 // red[0] += |f|^2 over free DOFs; UPD 1: red[1], red[2] = Rayleigh-quotient
 // numerator and denominator; UPD 2: red[1] = kinetic energy.
 // ---------------------------------------------------------------------------
+// force term of one bond, w V_j [(A_j - A_i) xi + g eta] (FP32), summed into
+// fa in FP64.  The correspondence term is in difference form:
+// sum_j w (A_i + A_j) xi = sum_j w (A_j - A_i) xi + 2 A_i m_i, so each FP32
+// bond product is rounded relative to |A_j - A_i| rather than |A|.
+template <int D, int FORM>
+PDX_HD void force_bond(const XRec &pi, const XRec &pj, const float *Ai, const float *Aj, float invh, float Gs,
+                       bool live, double *fa)
+{
+    float xi[3], du[3];
+    for (int r = 0; r < 3; ++r) {
+        xi[r] = (float)(pj.X[r] - pi.X[r]);
+        du[r] = (float)(pj.u[r] - pi.u[r]);
+    }
+    float L2 = 0.f, xd = 0.f, dd = 0.f;
+    for (int r = 0; r < D; ++r) {
+        L2 = fmaf(xi[r], xi[r], L2);
+        xd = fmaf(xi[r], du[r], xd);
+        dd = fmaf(du[r], du[r], dd);
+    }
+    const float L = sqrtf(L2);
+    const float w = bond_weight(L, invh, pj.V);
+    float eta[3];
+    for (int r = 0; r < D; ++r) eta[r] = xi[r] + du[r];
+    // |eta|^2 = |xi|^2 + (2 xi.du + du.du); the stretch uses the
+    // cancellation-free |eta| - |xi| = (2 xi.du + du.du) / (|eta| + |xi|)
+    const float dn = fmaf(2.f, xd, dd);
+    const float el = sqrtf(fmaxf(L2 + dn, 0.f));
+    float tb[3] = {0.f, 0.f, 0.f};
+    float g = 0.f;
+    if (FORM == 0) {
+        for (int r = 0; r < D; ++r) {
+            float t = 0.f;
+            for (int c = 0; c < D; ++c) t = fmaf(Aj[r * D + c] - Ai[r * D + c], xi[c], t);
+            tb[r] = t;
+        }
+        if (el > 1e-10f && L > 1e-10f) g = Gs * 0.5f * (pi.c + pj.c) * (dn / ((el + L) * L)) / el;
+    } else if (el > 1e-10f) {
+        g = 0.5f * (pi.c + pj.c) * (dn / ((el + L) * L)) / el;
+    }
+    for (int r = 0; r < D; ++r) {
+        const float contrib = w * fmaf(g, eta[r], tb[r]);
+        fa[r] += live ? (double)contrib : 0.0;
+    }
+}
+
+template <int D, int FORM, int UPD>
+PDX_HD void force_finish(const Dev &d, int i, const KRec &kr, const XRec &pi, const ARec &ai, int ubuf, int cs,
+                         double dt, double visc, double *fa, double *red);
+
 template <int D, int FORM, int UPD, int G>
 PDX_HD void force_one(const Dev &d, int i, int ubuf, int cs, double dt, double visc, double *red)
 {
     const KRec kr = d.KR[i];
     if (!(kr.flags & PDX_FL_ACTIVE)) return;
-    const pdx_d4 *__restrict__ U = d.u[ubuf];
-    const PRec pi = d.P[i];
-    const pdx_d4 ui = U[i];
+    const XRec *__restrict__ X = d.xu[ubuf];
+    const XRec pi = X[i];
     const float invh = (float)d.inv_h;
     const float Gs = (float)d.G_s;
-    const float ci = pi.c;
     double fa[3] = {0.0, 0.0, 0.0};
-    // correspondence term in difference form: sum_j w (A_i + A_j) xi =
-    // sum_j w (A_j - A_i) xi + 2 A_i m_i, so each FP32 bond product is rounded
-    // relative to |A_j - A_i| rather than |A|
-    ARec ai;
+    ARec ai{};
     if (FORM == 0) ai = d.AR[i];
     const uint32_t *__restrict__ bp = d.bj + slot0(d, i);
     const int nnb = (int)kr.nnb;
     uint32_t ec[G], en[G];
-    for (int q = 0; q < G; ++q) ec[q] = q < nnb ? bp[(int64_t)q * PDX_SLICE] : PDX_BROKEN;
+    for (int q = 0; q < G; ++q) ec[q] = bond_entry(bp, q, nnb);
     for (int k = 0; k < nnb; k += G) {
-        for (int q = 0; q < G; ++q) en[q] = (k + G + q) < nnb ? bp[(int64_t)(k + G + q) * PDX_SLICE] : PDX_BROKEN;
-        PRec pj[G];
-        pdx_d4 uj[G];
-        ARec aj[G];
+        for (int q = 0; q < G; ++q) en[q] = bond_entry(bp, k + G + q, nnb);
+        XRec pj[G];
+        ARec aj[G]{};
         bool live[G];
         for (int q = 0; q < G; ++q) {
             live[q] = ec[q] < (uint32_t)d.ni;
             const int j = live[q] ? (int)ec[q] : i;
-            pj[q] = d.P[j];
-            uj[q] = U[j];
+            pj[q] = X[j];
             if (FORM == 0) aj[q] = d.AR[j];
         }
-        for (int q = 0; q < G; ++q) {
-            float xi[3], du[3];
-            xi[0] = (float)(pj[q].X[0] - pi.X[0]);
-            xi[1] = (float)(pj[q].X[1] - pi.X[1]);
-            xi[2] = (float)(pj[q].X[2] - pi.X[2]);
-            du[0] = (float)(uj[q].x - ui.x);
-            du[1] = (float)(uj[q].y - ui.y);
-            du[2] = (float)(uj[q].z - ui.z);
-            float L2 = 0.f, xd = 0.f, dd = 0.f;
-            for (int r = 0; r < D; ++r) {
-                L2 = fmaf(xi[r], xi[r], L2);
-                xd = fmaf(xi[r], du[r], xd);
-                dd = fmaf(du[r], du[r], dd);
-            }
-            const float L = sqrtf(L2);
-            const float w = bond_weight(L, invh, pj[q].V);
-            float eta[3];
-            for (int r = 0; r < D; ++r) eta[r] = xi[r] + du[r];
-            // |eta|^2 = |xi|^2 + (2 xi.du + du.du); the stretch uses the
-            // cancellation-free |eta| - |xi| = (2 xi.du + du.du) / (|eta| + |xi|)
-            const float dn = fmaf(2.f, xd, dd);
-            const float el = sqrtf(fmaxf(L2 + dn, 0.f));
-            float tb[3] = {0.f, 0.f, 0.f};
-            float g = 0.f;
-            if (FORM == 0) {
-                for (int r = 0; r < D; ++r) {
-                    float t = 0.f;
-                    for (int c = 0; c < D; ++c) t = fmaf(aj[q].A[r * D + c] - ai.A[r * D + c], xi[c], t);
-                    tb[r] = t;
-                }
-                if (el > 1e-10f && L > 1e-10f) g = Gs * 0.5f * (ci + pj[q].c) * (dn / ((el + L) * L)) / el;
-            } else if (el > 1e-10f) {
-                g = 0.5f * (ci + pj[q].c) * (dn / ((el + L) * L)) / el;
-            }
-            for (int r = 0; r < D; ++r) {
-                const float contrib = w * fmaf(g, eta[r], tb[r]);
-                fa[r] += live[q] ? (double)contrib : 0.0;
-            }
-        }
+        for (int q = 0; q < G; ++q) force_bond<D, FORM>(pi, pj[q], ai.A, aj[q].A, invh, Gs, live[q], fa);
         for (int q = 0; q < G; ++q) ec[q] = en[q];
     }
+    force_finish<D, FORM, UPD>(d, i, kr, pi, ai, ubuf, cs, dt, visc, fa, red);
+}
+
+// per-particle end of the force pass: + 2 A_i m_i, + external load, |f|^2
+// over free DOFs into red[0], then the update of UPD
+template <int D, int FORM, int UPD>
+PDX_HD void force_finish(const Dev &d, int i, const KRec &kr, const XRec &pi, const ARec &ai, int ubuf, int cs,
+                         double dt, double visc, double *fa, double *red)
+{
     if (FORM == 0) {
         for (int r = 0; r < D; ++r) {
             double t = 0.0;
@@ -496,7 +526,7 @@ PDX_HD void force_one(const Dev &d, int i, int ubuf, int cs, double dt, double v
         const double lam = d.lam[i];
         const pdx_d4 vo4 = d.v[i], fo4 = d.fp[i];
         const double vo[3] = {vo4.x, vo4.y, vo4.z}, fo[3] = {fo4.x, fo4.y, fo4.z};
-        const double uo[3] = {ui.x, ui.y, ui.z};
+        const double uo[3] = {pi.u[0], pi.u[1], pi.u[2]};
         double vn[3] = {0.0, 0.0, 0.0}, un[3] = {uo[0], uo[1], uo[2]}, fn[3] = {0.0, 0.0, 0.0};
         for (int r = 0; r < D; ++r) {
             if ((fm >> r) & 1u) continue;
@@ -513,7 +543,10 @@ PDX_HD void force_one(const Dev &d, int i, int ubuf, int cs, double dt, double v
         }
         d.v[i] = d4(vn[0], vn[1], vn[2]);
         d.fp[i] = d4(fn[0], fn[1], fn[2]);
-        d.u[ubuf ^ 1][i] = d4(un[0], un[1], un[2]);
+        double *uw = d.xu[ubuf ^ 1][i].u;
+        uw[0] = un[0];
+        uw[1] = un[1];
+        uw[2] = un[2];
     } else {
         const double m = d.mass[i];
         const pdx_d4 vo4 = d.v[i], ao4 = d.fp[i];
@@ -586,7 +619,8 @@ template <int D>
 PDX_HD void kd_move_one(const Dev &d, int i, int ubuf_in, double dt, bool reset)
 {
     const KRec kr = d.KR[i];
-    const pdx_d4 u = d.u[ubuf_in][i];
+    const double *ui = d.xu[ubuf_in][i].u;
+    const pdx_d4 u = d4(ui[0], ui[1], ui[2]);
     pdx_d4 v = d.v[i];
     if (reset) {
         v = d4(0.0, 0.0, 0.0);
@@ -600,7 +634,10 @@ PDX_HD void kd_move_one(const Dev &d, int i, int ubuf_in, double dt, bool reset)
         if (!(fm & 2u)) o.y = u.y + v.y * dt + 0.5 * a.y * dt * dt;
         if (D == 3 && !(fm & 4u)) o.z = u.z + v.z * dt + 0.5 * a.z * dt * dt;
     }
-    d.u[ubuf_in ^ 1][i] = o;
+    double *uo = d.xu[ubuf_in ^ 1][i].u;
+    uo[0] = o.x;
+    uo[1] = o.y;
+    uo[2] = o.z;
 }
 
 // ---------------------------------------------------------------------------
@@ -608,9 +645,9 @@ PDX_HD void kd_move_one(const Dev &d, int i, int ubuf_in, double dt, bool reset)
 // ---------------------------------------------------------------------------
 // sort key of a bond: quantised (z, y, x) offset in 10 bits each, then the index
 template <int D>
-PDX_HD uint64_t stencil_key(const Dev &d, const PRec &pi, uint32_t j, double hk)
+PDX_HD uint64_t stencil_key(const Dev &d, const XRec &pi, uint32_t j, double hk)
 {
-    const PRec pj = d.P[j];
+    const XRec pj = d.xu[0][j];
     uint64_t key = 0;
     for (int q = D - 1; q >= 0; --q) {
         long long t = llround((pj.X[q] - pi.X[q]) * hk) + 512;
@@ -635,7 +672,7 @@ PDX_HD void neighbors_one(const Dev &d, int i)
         return;
     }
     const GridP g = d.grid[cs];
-    const PRec pi = d.P[i];
+    const XRec pi = d.xu[0][i];
     int c[3] = {0, 0, 0};
     for (int q = 0; q < D; ++q) {
         int t = (int)floor((pi.X[q] - g.lo[q]) / g.h);
@@ -656,7 +693,7 @@ PDX_HD void neighbors_one(const Dev &d, int i)
                 for (int k = k0; k < k1; ++k) {
                     const int j = d.cell_idx[k];
                     if (j == i) continue;
-                    const PRec pj = d.P[j];
+                    const XRec pj = d.xu[0][j];
                     double s = 0.0;
                     for (int q = 0; q < D; ++q) {
                         const double t = pj.X[q] - pi.X[q];
@@ -711,14 +748,14 @@ PDX_HD void shape_one(const Dev &d, int i, int mode, double tol)
 #endif
     KRec kr = d.KR[i];
     if (!(kr.flags & PDX_FL_ACTIVE)) return;
-    const PRec pi = d.P[i];
+    const XRec pi = d.xu[0][i];
     double K[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     double m[3] = {0, 0, 0};
     const int64_t base = slot0(d, i);
     for (int k = 0; k < (int)kr.nnb; ++k) {
         const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
         if (e & PDX_BROKEN) continue;
-        const PRec pj = d.P[(int)e];
+        const XRec pj = d.xu[0][(int)e];
         double xi[3] = {0, 0, 0}, s = 0.0;
         for (int q = 0; q < D; ++q) {
             xi[q] = pj.X[q] - pi.X[q];
@@ -784,14 +821,14 @@ PDX_HD void adr_mass_a_one(const Dev &d, int i, double *cs_out, double *g_out)
     cs_out[i] = 0.0;
     g_out[i] = 0.0;
     if (!(kr.flags & PDX_FL_ACTIVE) || d.form != 0) return;
-    const PRec pi = d.P[i];
+    const XRec pi = d.xu[0][i];
     const double *Ki = d.Kinv64 + (int64_t)i * 9;
     double G[3] = {0, 0, 0}, sabs = 0.0;
     const int64_t base = slot0(d, i);
     for (int k = 0; k < (int)kr.nnb; ++k) {
         const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
         if (e & PDX_BROKEN) continue;
-        const PRec pj = d.P[(int)e];
+        const XRec pj = d.xu[0][(int)e];
         double xi[3] = {0, 0, 0}, s = 0.0;
         for (int q = 0; q < D; ++q) {
             xi[q] = pj.X[q] - pi.X[q];
@@ -825,7 +862,7 @@ PDX_HD void adr_mass_b_one(const Dev &d, int i, const double *cs_in, const doubl
         d.lam[i] = 1.0;
         return;
     }
-    const PRec pi = d.P[i];
+    const XRec pi = d.xu[0][i];
     const bool nosb = d.form == 0;
     double kc = nosb ? cs_in[i] * g_in[i] : 0.0, ks = 0.0;
     const int64_t base = slot0(d, i);
@@ -833,7 +870,7 @@ PDX_HD void adr_mass_b_one(const Dev &d, int i, const double *cs_in, const doubl
         const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
         if (e & PDX_BROKEN) continue;
         const int j = (int)e;
-        const PRec pj = d.P[j];
+        const XRec pj = d.xu[0][j];
         double xi[3] = {0, 0, 0}, s = 0.0;
         for (int q = 0; q < D; ++q) {
             xi[q] = pj.X[q] - pi.X[q];
@@ -869,14 +906,14 @@ PDX_HD void dt_est_one(const Dev &d, int i, double modulus, double *out)
     if (!(kr.flags & PDX_FL_ACTIVE)) return;
     const double mi = d.mass[i];
     if (mi < 1e-30) return;
-    const PRec pi = d.P[i];
+    const XRec pi = d.xu[0][i];
     const double *Ki = d.Kinv64 + (int64_t)i * 9;
     double ds[3] = {0, 0, 0}, dsq = 0.0;
     const int64_t base = slot0(d, i);
     for (int k = 0; k < (int)kr.nnb; ++k) {
         const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
         if (e & PDX_BROKEN) continue;
-        const PRec pj = d.P[(int)e];
+        const XRec pj = d.xu[0][(int)e];
         double xi[3] = {0, 0, 0}, s = 0.0;
         for (int q = 0; q < D; ++q) {
             xi[q] = pj.X[q] - pi.X[q];
@@ -908,10 +945,9 @@ PDX_HD unsigned damage_one(const Dev &d, int i, int ubuf, double s_c)
 #endif
     const KRec kr = d.KR[i];
     if (!(kr.flags & PDX_FL_ACTIVE)) return 0;
-    const pdx_d4 *U = d.u[ubuf];
-    const PRec pi = d.P[i];
-    const pdx_d4 ui = U[i];
-    const double xi0[3] = {pi.X[0] + ui.x, pi.X[1] + ui.y, pi.X[2] + ui.z};
+    const XRec *X = d.xu[ubuf];
+    const XRec pi = X[i];
+    const double xi0[3] = {pi.X[0] + pi.u[0], pi.X[1] + pi.u[1], pi.X[2] + pi.u[2]};
     const int64_t base = slot0(d, i);
     unsigned nb = 0;
     for (int k = 0; k < (int)kr.nnb; ++k) {
@@ -919,9 +955,8 @@ PDX_HD unsigned damage_one(const Dev &d, int i, int ubuf, double s_c)
         const uint32_t e = d.bj[a];
         if (e & PDX_BROKEN) continue;
         const int j = (int)e;
-        const PRec pj = d.P[j];
-        const pdx_d4 uj = U[j];
-        const double xj[3] = {pj.X[0] + uj.x, pj.X[1] + uj.y, pj.X[2] + uj.z};
+        const XRec pj = X[j];
+        const double xj[3] = {pj.X[0] + pj.u[0], pj.X[1] + pj.u[1], pj.X[2] + pj.u[2]};
         double se = 0.0, sl = 0.0;
         for (int q = 0; q < D; ++q) {
             const double t = xj[q] - xi0[q];
@@ -964,11 +999,11 @@ PDX_HD void bond_data_one(const Dev &d, int i, double *xi_o, double *len_o, doub
 #endif
     const KRec kr = d.KR[i];
     if (!(kr.flags & PDX_FL_ACTIVE)) return;
-    const PRec pi = d.P[i];
+    const XRec pi = d.xu[0][i];
     const int64_t base = slot0(d, i);
     for (int k = 0; k < (int)kr.nnb; ++k) {
         const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
-        const PRec pj = d.P[(int)(e & ~PDX_BROKEN)];
+        const XRec pj = d.xu[0][(int)(e & ~PDX_BROKEN)];
         const int64_t b = rowi[i] + k;
         double s = 0.0;
         for (int q = 0; q < D; ++q) {
@@ -1001,9 +1036,7 @@ PDX_HD double resid_one(const Dev &d, int i)
 PDX_HD void zero_free_one(const Dev &d, int i, int ubuf)
 {
     const uint32_t fm = d.KR[i].flags & PDX_FL_FIXMASK;
-    pdx_d4 u = d.u[ubuf][i];
-    if (!(fm & 1u)) u.x = 0.0;
-    if (!(fm & 2u)) u.y = 0.0;
-    if (!(fm & 4u)) u.z = 0.0;
-    d.u[ubuf][i] = u;
+    double *u = d.xu[ubuf][i].u;
+    for (int r = 0; r < 3; ++r)
+        if (!((fm >> r) & 1u)) u[r] = 0.0;
 }

@@ -27,7 +27,7 @@ struct pdx_ctx {
     double horizon = 0.0, G_s = 0.0;
     int64_t nb = 0, slots = 0;
     int ucur = 0;
-    int tune_g = 2;  // bonds per group in the hot loops (1, 2 or 4)
+    int tune_g = 2;  // hot-pass variant: bond group 1, 2, 4 or PDX_VARIANT_COOP | 1, 2
     Dev d{};
     // host mirror of slice pointers
     std::vector<int64_t> h_slptr;
@@ -60,6 +60,7 @@ int pdx_launch_kd_move(pdx_ctx *c, int ubuf_in, bool solve_mode, double dt, bool
 int pdx_launch_resid(pdx_ctx *c);
 int pdx_launch_zero_free(pdx_ctx *c, int ubuf);
 int pdx_launch_kr_put(pdx_ctx *c, const float *d_src, int off, int cnt);
+int pdx_launch_xu_geom(pdx_ctx *c, const XRec *d_src);
 int pdx_launch_adr_mass(pdx_ctx *c, double beta);
 int pdx_launch_dt_est(pdx_ctx *c, double modulus, double *d_out);
 int pdx_launch_damage(pdx_ctx *c, double s_c, unsigned long long *d_count);

@@ -109,6 +109,233 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_force(Dev d, int ubuf, int solve_
     if (threadIdx.x == 0) case_update<UPD>(d.sc[cs], tot, ubuf);
 }
 
+// ---------------------------------------------------------------------------
+// wave-cooperative hot kernels
+// ---------------------------------------------------------------------------
+// One wave = one SELL slice = 64 particles; the wave walks the slice's bond
+// slots k = 0 .. width-1 in lockstep.  At each slot the 64 neighbour records
+// are fetched cooperatively: lane L loads the 16-byte quarter (L & 3) of the
+// 64-byte XRec of bond 16h + (L >> 2), h = 0..3, so one load instruction
+// covers 16 whole records instead of 64 scattered 16-byte pieces (the gathers
+// of the per-lane kernels are bound by L1 tag lookups, about one per lane and
+// instruction).  The quarters pass through a per-wave LDS tile and every lane
+// reads back its own bond's record.  The arithmetic is fpass_bond /
+// force_bond in the same bond order as the per-lane kernels, so both give the
+// same bits.
+#define PDX_WAVES (PDX_BLOCK / 64)
+
+// 16-byte quarter of a record as a native vector (register-resident arrays)
+typedef float pdx_q4 __attribute__((ext_vector_type(4)));
+
+// LDS slot (16-byte units) of quarter q of record b: rotating the quarter by
+// b >> 2 spreads 16 consecutive lanes' reads of one quarter over all 64 banks
+// (four 64-byte records fill one 256-byte bank row)
+__device__ __forceinline__ int xq_slot(int b, int q) { return b * 4 + ((q + (b >> 2)) & 3); }
+
+// orders a wave's LDS writes before its reads of other lanes' data
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// XRec / ARec from the 16-byte quarters read back from LDS, in registers
+struct PdxD2 {
+    double a, b;
+};
+
+__device__ __forceinline__ XRec xrec_of(const pdx_q4 (&q)[4])
+{
+    const PdxD2 x01 = __builtin_bit_cast(PdxD2, q[0]);
+    const PdxD2 u01 = __builtin_bit_cast(PdxD2, q[2]);
+    const PdxD2 u2w = __builtin_bit_cast(PdxD2, q[3]);
+    XRec r;
+    r.X[0] = x01.a;
+    r.X[1] = x01.b;
+    r.X[2] = __builtin_bit_cast(double, q[1].xy);
+    r.V = q[1].z;
+    r.c = q[1].w;
+    r.u[0] = u01.a;
+    r.u[1] = u01.b;
+    r.u[2] = u2w.a;
+    r.w = u2w.b;
+    return r;
+}
+
+__device__ __forceinline__ void arec_of(const pdx_q4 (&q)[3], float (&A)[9])
+{
+    A[0] = q[0].x; A[1] = q[0].y; A[2] = q[0].z; A[3] = q[0].w;
+    A[4] = q[1].x; A[5] = q[1].y; A[6] = q[1].z; A[7] = q[1].w;
+    A[8] = q[2].x;
+}
+
+// bond slot q of a wave's row walk: entry, liveness and the index to gather
+// (own record for a dead slot)
+struct WSlot {
+    bool live;
+    int j;
+};
+
+__device__ __forceinline__ WSlot wslot(const Dev &d, uint32_t e, int k, int nnb, bool act, int i)
+{
+    WSlot s;
+    s.live = act && k < nnb && e < (uint32_t)d.ni;
+    s.j = s.live ? (int)e : i;
+    return s;
+}
+
+template <int D, int MODEL, int G>
+__global__ __launch_bounds__(PDX_BLOCK) void k_fpass_w(Dev d, int ubuf, int solve_mode)
+{
+    __shared__ pdx_q4 tile[PDX_WAVES][64 * 4];
+    const int lb = logical_block();
+    if (solve_mode && d.sc[d.case_of_blk[lb]].converged) return;
+    if (lb * PDX_BLOCK >= d.ni) return;  // grid padding blocks (whole block)
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = lb * PDX_BLOCK + threadIdx.x;
+    const KRec kr = d.KR[i];
+    const bool act = (kr.flags & PDX_FL_ACTIVE) != 0;
+    const int nnb = (int)kr.nnb;
+    const XRec *__restrict__ X = d.xu[ubuf];
+    const pdx_q4 *__restrict__ Xq = reinterpret_cast<const pdx_q4 *>(X);
+    const XRec pi = X[i];
+    const float invh = (float)d.inv_h;
+    double Hd[D * D];
+#pragma unroll
+    for (int q = 0; q < D * D; ++q) Hd[q] = 0.0;
+    const int64_t s0 = d.sl_ptr[i >> 6];
+    const int width = (int)((d.sl_ptr[(i >> 6) + 1] - s0) / PDX_SLICE);  // same in all lanes
+    const uint32_t *__restrict__ bp = d.bj + s0 + lane;
+    pdx_q4 *tw = tile[wv];
+    uint32_t ec[G], en[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) ec[q] = q < width ? bp[(int64_t)q * PDX_SLICE] : PDX_BROKEN;
+    for (int k = 0; k < width; k += G) {
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+            en[q] = (k + G + q) < width ? bp[(int64_t)(k + G + q) * PDX_SLICE] : PDX_BROKEN;
+        // all source indices first, then all loads, so the group's gathers
+        // are in flight together
+        WSlot sl[G];
+        int jb[G][4];
+        pdx_q4 g[G][4];
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            sl[q] = wslot(d, ec[q], k + q, nnb, act, i);
+#pragma unroll
+            for (int h = 0; h < 4; ++h) jb[q][h] = __shfl(sl[q].j, 16 * h + (lane >> 2), 64);
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+#pragma unroll
+            for (int h = 0; h < 4; ++h) g[q][h] = Xq[(int64_t)jb[q][h] * 4 + (lane & 3)];
+        // keep the scheduler from sinking the loads between the LDS writes
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+#pragma unroll
+            for (int h = 0; h < 4; ++h) tw[xq_slot(16 * h + (lane >> 2), lane & 3)] = g[q][h];
+            wave_lds_sync();
+            pdx_q4 rq[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) rq[p] = tw[xq_slot(lane, p)];
+            wave_lds_sync();
+            fpass_bond<D>(pi, xrec_of(rq), invh, sl[q].live, Hd);
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q) ec[q] = en[q];
+    }
+    if (act) fpass_finish<D, MODEL, false, false>(d, i, kr, Hd);
+}
+
+template <int D, int FORM, int UPD, int G>
+__global__ __launch_bounds__(PDX_BLOCK) void k_force_w(Dev d, int ubuf, int solve_mode, double dt, double visc)
+{
+    __shared__ pdx_q4 tile[PDX_WAVES][64 * 4];
+    __shared__ pdx_q4 atile[PDX_WAVES][FORM == 0 ? 64 * 4 : 1];
+    const int lb = logical_block();
+    const int cs = d.case_of_blk[lb];
+    if (solve_mode && d.sc[cs].converged) return;
+    double red[3] = {0.0, 0.0, 0.0};
+    if (lb * PDX_BLOCK < d.ni) {  // grid padding blocks only join the reduction
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const int i = lb * PDX_BLOCK + threadIdx.x;
+        const KRec kr = d.KR[i];
+        const bool act = (kr.flags & PDX_FL_ACTIVE) != 0;
+        const int nnb = (int)kr.nnb;
+        const XRec *__restrict__ X = d.xu[ubuf];
+        const pdx_q4 *__restrict__ Xq = reinterpret_cast<const pdx_q4 *>(X);
+        const pdx_q4 *__restrict__ Aq = reinterpret_cast<const pdx_q4 *>(d.AR);
+        const XRec pi = X[i];
+        const float invh = (float)d.inv_h;
+        const float Gs = (float)d.G_s;
+        ARec ai{};
+        if (FORM == 0) ai = d.AR[i];
+        double fa[3] = {0.0, 0.0, 0.0};
+        const int64_t s0 = d.sl_ptr[i >> 6];
+        const int width = (int)((d.sl_ptr[(i >> 6) + 1] - s0) / PDX_SLICE);
+        const uint32_t *__restrict__ bp = d.bj + s0 + lane;
+        pdx_q4 *tw = tile[wv];
+        pdx_q4 *ta = atile[wv];
+        const int qa = lane & 3;  // quarter this lane fetches
+        // A records have 3 quarters; lanes of quarter 3 fetch quarter 2 again
+        // (same cache line, keeps the loads unconditional)
+        const int qa3 = qa < 3 ? qa : 2;
+        uint32_t ec[G], en[G];
+#pragma unroll
+        for (int q = 0; q < G; ++q) ec[q] = q < width ? bp[(int64_t)q * PDX_SLICE] : PDX_BROKEN;
+        for (int k = 0; k < width; k += G) {
+#pragma unroll
+            for (int q = 0; q < G; ++q)
+                en[q] = (k + G + q) < width ? bp[(int64_t)(k + G + q) * PDX_SLICE] : PDX_BROKEN;
+            WSlot sl[G];
+            int jb[G][4];
+            pdx_q4 g[G][4], ga[G][4];
+#pragma unroll
+            for (int q = 0; q < G; ++q) {
+                sl[q] = wslot(d, ec[q], k + q, nnb, act, i);
+#pragma unroll
+                for (int h = 0; h < 4; ++h) jb[q][h] = __shfl(sl[q].j, 16 * h + (lane >> 2), 64);
+            }
+#pragma unroll
+            for (int q = 0; q < G; ++q)
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    g[q][h] = Xq[(int64_t)jb[q][h] * 4 + qa];
+                    if (FORM == 0) ga[q][h] = Aq[(int64_t)jb[q][h] * 3 + qa3];
+                }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < G; ++q) {
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    const int b = 16 * h + (lane >> 2);
+                    tw[xq_slot(b, qa)] = g[q][h];
+                    if (FORM == 0) ta[xq_slot(b, qa)] = ga[q][h];
+                }
+                wave_lds_sync();
+                pdx_q4 rq[4], ra[3];
+                float aj[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int p = 0; p < 4; ++p) rq[p] = tw[xq_slot(lane, p)];
+                if (FORM == 0)
+#pragma unroll
+                    for (int p = 0; p < 3; ++p) ra[p] = ta[xq_slot(lane, p)];
+                wave_lds_sync();
+                if (FORM == 0) arec_of(ra, aj);
+                force_bond<D, FORM>(pi, xrec_of(rq), ai.A, aj, invh, Gs, sl[q].live, fa);
+            }
+#pragma unroll
+            for (int q = 0; q < G; ++q) ec[q] = en[q];
+        }
+        if (act) force_finish<D, FORM, UPD>(d, i, kr, pi, ai, ubuf, cs, dt, visc, fa, red);
+    }
+    double tot[3];
+    if (!reduce_case<3>(d, lb, cs, red, tot)) return;
+    if (threadIdx.x == 0) case_update<UPD>(d.sc[cs], tot, ubuf);
+}
+
 template <int D>
 __global__ __launch_bounds__(PDX_BLOCK) void k_kd_move(Dev d, int ubuf_in, int solve_mode, double dt, int apply_reset)
 {
@@ -161,6 +388,21 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_damage(Dev d, int ubuf, double s_
     if (i >= d.ni) return;
     const unsigned nb = damage_one<D>(d, i, ubuf, s_c);
     if (nb) atomicAdd(count, (unsigned long long)nb);
+}
+
+// X, V and c of both gather buffers from a staged copy (u left as it is)
+__global__ __launch_bounds__(PDX_BLOCK) void k_xu_geom(Dev d, const XRec *src)
+{
+    const int i = blockIdx.x * PDX_BLOCK + threadIdx.x;
+    if (i >= d.ni) return;
+    const XRec s = src[i];
+    for (int b = 0; b < 2; ++b) {
+        XRec &r = d.xu[b][i];
+        for (int q = 0; q < 3; ++q) r.X[q] = s.X[q];
+        r.V = s.V;
+        r.c = s.c;
+        r.w = 0.0;
+    }
 }
 
 // scatter cnt 32-bit words per particle into KRec word offset off
@@ -218,24 +460,32 @@ int pdx_launch_shape(pdx_ctx *c, int mode, double tol)
     return PDX_OK;
 }
 
-// the hot variants come in bond-group sizes 1, 2, 4 (chosen per context by
-// pdx_autotune); diagnostic and commit passes use group 1
+// the hot passes come in variants chosen per context by pdx_autotune:
+// per-lane gathers with bond groups 1, 2, 4 and wave-cooperative gathers with
+// groups 1, 2 (PDX_VARIANT_COOP | group); diagnostic and commit passes use
+// the per-lane kernel with group 1
 template <int D, int MODEL, int G>
 static void launch_fpass_g(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm)
 {
     dim3 g(c->nblk), b(PDX_BLOCK);
-    if (commit)
+    if (commit) {
         hipLaunchKernelGGL((k_fpass<D, MODEL, false, true, 1>), g, b, 0, c->stream, c->d, ubuf, sm);
-    else if (diag)
+    } else if (diag) {
         hipLaunchKernelGGL((k_fpass<D, MODEL, true, false, 1>), g, b, 0, c->stream, c->d, ubuf, sm);
-    else
-        hipLaunchKernelGGL((k_fpass<D, MODEL, false, false, G>), g, b, 0, c->stream, c->d, ubuf, sm);
+    } else {
+        if constexpr (G > PDX_VARIANT_COOP)
+            hipLaunchKernelGGL((k_fpass_w<D, MODEL, G - PDX_VARIANT_COOP>), g, b, 0, c->stream, c->d, ubuf, sm);
+        else
+            hipLaunchKernelGGL((k_fpass<D, MODEL, false, false, G>), g, b, 0, c->stream, c->d, ubuf, sm);
+    }
 }
 
 template <int D, int MODEL>
 static void launch_fpass_t(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm)
 {
     switch (c->tune_g) {
+    case PDX_VARIANT_COOP | 2: launch_fpass_g<D, MODEL, PDX_VARIANT_COOP | 2>(c, ubuf, diag, commit, sm); break;
+    case PDX_VARIANT_COOP | 1: launch_fpass_g<D, MODEL, PDX_VARIANT_COOP | 1>(c, ubuf, diag, commit, sm); break;
     case 4: launch_fpass_g<D, MODEL, 4>(c, ubuf, diag, commit, sm); break;
     case 2: launch_fpass_g<D, MODEL, 2>(c, ubuf, diag, commit, sm); break;
     default: launch_fpass_g<D, MODEL, 1>(c, ubuf, diag, commit, sm); break;
@@ -265,18 +515,30 @@ template <int D, int FORM, int G>
 static void launch_force_g(pdx_ctx *c, int ubuf, int upd, int sm, double dt, double visc)
 {
     dim3 g(c->nblk), b(PDX_BLOCK);
-    if (upd == 1)
-        hipLaunchKernelGGL((k_force<D, FORM, 1, G>), g, b, 0, c->stream, c->d, ubuf, sm, dt, visc);
-    else if (upd == 2)
-        hipLaunchKernelGGL((k_force<D, FORM, 2, G>), g, b, 0, c->stream, c->d, ubuf, sm, dt, visc);
-    else
-        hipLaunchKernelGGL((k_force<D, FORM, 0, 1>), g, b, 0, c->stream, c->d, ubuf, sm, dt, visc);
+    if constexpr (G > PDX_VARIANT_COOP) {
+        constexpr int GW = G - PDX_VARIANT_COOP;
+        if (upd == 1)
+            hipLaunchKernelGGL((k_force_w<D, FORM, 1, GW>), g, b, 0, c->stream, c->d, ubuf, sm, dt, visc);
+        else if (upd == 2)
+            hipLaunchKernelGGL((k_force_w<D, FORM, 2, GW>), g, b, 0, c->stream, c->d, ubuf, sm, dt, visc);
+        else
+            hipLaunchKernelGGL((k_force_w<D, FORM, 0, GW>), g, b, 0, c->stream, c->d, ubuf, sm, dt, visc);
+    } else {
+        if (upd == 1)
+            hipLaunchKernelGGL((k_force<D, FORM, 1, G>), g, b, 0, c->stream, c->d, ubuf, sm, dt, visc);
+        else if (upd == 2)
+            hipLaunchKernelGGL((k_force<D, FORM, 2, G>), g, b, 0, c->stream, c->d, ubuf, sm, dt, visc);
+        else
+            hipLaunchKernelGGL((k_force<D, FORM, 0, 1>), g, b, 0, c->stream, c->d, ubuf, sm, dt, visc);
+    }
 }
 
 template <int D, int FORM>
 static void launch_force_t(pdx_ctx *c, int ubuf, int upd, int sm, double dt, double visc)
 {
     switch (c->tune_g) {
+    case PDX_VARIANT_COOP | 2: launch_force_g<D, FORM, PDX_VARIANT_COOP | 2>(c, ubuf, upd, sm, dt, visc); break;
+    case PDX_VARIANT_COOP | 1: launch_force_g<D, FORM, PDX_VARIANT_COOP | 1>(c, ubuf, upd, sm, dt, visc); break;
     case 4: launch_force_g<D, FORM, 4>(c, ubuf, upd, sm, dt, visc); break;
     case 2: launch_force_g<D, FORM, 2>(c, ubuf, upd, sm, dt, visc); break;
     default: launch_force_g<D, FORM, 1>(c, ubuf, upd, sm, dt, visc); break;
@@ -379,6 +641,13 @@ int pdx_launch_resid(pdx_ctx *c)
 int pdx_launch_zero_free(pdx_ctx *c, int ubuf)
 {
     hipLaunchKernelGGL(k_zero_free, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, ubuf);
+    PDX_LAUNCH_CHECK();
+    return PDX_OK;
+}
+
+int pdx_launch_xu_geom(pdx_ctx *c, const XRec *d_src)
+{
+    hipLaunchKernelGGL(k_xu_geom, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_src);
     PDX_LAUNCH_CHECK();
     return PDX_OK;
 }

@@ -161,13 +161,12 @@ int main(int argc, char **argv)
     d.nblk = nblk;
     d.ncase = R.ncase;
     d.dim = D;
-    d.P = alloc<PRec>(ni);
+    d.xu[0] = alloc<XRec>(ni);
+    d.xu[1] = alloc<XRec>(ni);
     d.KR = alloc<KRec>(ni);
     d.AR = alloc<ARec>(ni);
     d.j2p = alloc<J2Par>(ni);
     d.j2s = alloc<J2State>(ni);
-    d.u[0] = alloc<pdx_d4>(ni);
-    d.u[1] = alloc<pdx_d4>(ni);
     d.v = alloc<pdx_d4>(ni);
     d.fp = alloc<pdx_d4>(ni);
     d.f = alloc<pdx_d4>(ni);
@@ -209,11 +208,13 @@ int main(int argc, char **argv)
     for (int i = 0; i < ni; ++i) {
         const int r = R.perm[i];
         if (r < 0) continue;
-        PRec p{};
+        XRec p{};
         for (int q = 0; q < D; ++q) p.X[q] = X[(size_t)r * D + q];
         p.V = (float)vol[r];
         p.c = (float)cst[r];
-        d.P[i] = p;
+        d.xu[1][i] = p;
+        for (int q = 0; q < D; ++q) p.u[q] = u0[(size_t)r * D + q];
+        d.xu[0][i] = p;
         KRec &k = d.KR[i];
         k.bulk = (float)bulk[r];
         k.shear = (float)shear[r];
@@ -222,7 +223,6 @@ int main(int argc, char **argv)
         for (int q = 0; q < D; ++q) loaded |= fext[(size_t)r * D + q] != 0.0;
         if (loaded) k.flags |= PDX_FL_LOADED;
         d.fext[i] = d4(fext[(size_t)r * D], fext[(size_t)r * D + 1], D == 3 ? fext[(size_t)r * D + 2] : 0.0);
-        d.u[0][i] = d4(u0[(size_t)r * D], u0[(size_t)r * D + 1], D == 3 ? u0[(size_t)r * D + 2] : 0.0);
         d.mass[i] = mass[r];
         d.V64[i] = vol[r];
         d.j2p[i] = J2Par{(float)sy[r], (float)hiso[r], (float)hkin[r], 0.f};
