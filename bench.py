@@ -60,6 +60,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--kernel-iters", type=int, default=50)
+    ap.add_argument("--variant", type=int, default=0,
+                    help="hot-pass variant (1, 2, 4, 17, 18); 0 = autotune")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 256 MiB device copy (scales rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -75,6 +79,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a GPU"
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        copy_calibration(torch)
+        return
 
     t0 = time.perf_counter()
     spec = W.vertebra(dx=args.dx, model=args.model)
@@ -89,9 +96,13 @@ def main():
 
     cold = body.cold_residual()
     body.u = spec.u0
-    group = body.autotune(10)
+    if args.variant:
+        group = args.variant
+        body.set_group(group)
+    else:
+        group = body.autotune(10)
     vlabel, kf_name, kg_name = variant_names(group)
-    log(f"autotune: {vlabel}")
+    log(f"hot-pass variant: {vlabel}")
     # warmup + timed steps: fixed-count iterations that never stop early (tol = 0)
     body.u = spec.u0
     body.solve("adaptive", tol=0.0, max_iterations=args.warmup, ref=cold, chunk=64)
@@ -158,6 +169,19 @@ def main():
         "time_to_solve": tts, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_calibration(torch):
+    """One elementwise kernel reading 256 MiB and writing 256 MiB: the
+    rocprof FETCH_SIZE / WRITE_SIZE readings of this launch give the factor
+    between the counters and bytes actually moved."""
+    n = (256 << 20) // 4
+    x = torch.ones(n, device="cuda", dtype=torch.float32)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    torch.add(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    log(f"copy calibration: read {4 * n} B, wrote {4 * n} B in one elementwise kernel")
 
 
 def cpu_baseline(spec, seconds):

@@ -305,8 +305,9 @@ static int get_vec(pdx_ctx *c, const pdx_d4 *src, double *dst)
     return PDX_OK;
 }
 
-// displacement part of a gather buffer <-> caller-order n*dim doubles; the
-// strided copies touch only the u words of the 64-byte records
+// displacement part of a gather buffer <-> caller-order n*dim doubles.  The
+// host side moves one contiguous array; the strided part (only the u words of
+// the 64-byte records) is a device-to-device 2-D copy.
 static int put_u(pdx_ctx *c, int buf, const double *src)
 {
     std::vector<pdx_d4> h(c->ni, d4(0, 0, 0));
@@ -316,8 +317,14 @@ static int put_u(pdx_ctx *c, int buf, const double *src)
         const double *s = src + (size_t)r * c->dim;
         h[i] = d4(s[0], s[1], c->dim == 3 ? s[2] : 0.0);
     }
-    PDX_HIP(hipMemcpy2DAsync(c->d.xu[buf]->u, sizeof(XRec), h.data(), sizeof(pdx_d4), sizeof(pdx_d4), c->ni,
-                             hipMemcpyHostToDevice, c->stream));
+    pdx_d4 *tmp = nullptr;
+    PDX_HIP(hipMallocAsync((void **)&tmp, sizeof(pdx_d4) * c->ni, c->stream));
+    hipError_t e = hipMemcpyAsync(tmp, h.data(), sizeof(pdx_d4) * c->ni, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(c->d.xu[buf]->u, sizeof(XRec), tmp, sizeof(pdx_d4), sizeof(pdx_d4), c->ni,
+                             hipMemcpyDeviceToDevice, c->stream);
+    hipFreeAsync(tmp, c->stream);
+    PDX_HIP(e);
     PDX_HIP(hipStreamSynchronize(c->stream));
     return PDX_OK;
 }
@@ -325,8 +332,14 @@ static int put_u(pdx_ctx *c, int buf, const double *src)
 static int get_u(pdx_ctx *c, int buf, double *dst)
 {
     std::vector<pdx_d4> h(c->ni);
-    PDX_HIP(hipMemcpy2DAsync(h.data(), sizeof(pdx_d4), c->d.xu[buf]->u, sizeof(XRec), sizeof(pdx_d4), c->ni,
-                             hipMemcpyDeviceToHost, c->stream));
+    pdx_d4 *tmp = nullptr;
+    PDX_HIP(hipMallocAsync((void **)&tmp, sizeof(pdx_d4) * c->ni, c->stream));
+    hipError_t e = hipMemcpy2DAsync(tmp, sizeof(pdx_d4), c->d.xu[buf]->u, sizeof(XRec), sizeof(pdx_d4), c->ni,
+                                    hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h.data(), tmp, sizeof(pdx_d4) * c->ni, hipMemcpyDeviceToHost, c->stream);
+    hipFreeAsync(tmp, c->stream);
+    PDX_HIP(e);
     PDX_HIP(hipStreamSynchronize(c->stream));
     for (int i = 0; i < c->ni; ++i) {
         int r = c->perm[i];
@@ -1211,7 +1224,13 @@ extern "C" int pdx_get(pdx_ctx *c, int32_t field, void *out)
     case PDX_FIELD_FIXED: memcpy(out, m.fixed.data(), c->n); return PDX_OK;
     case PDX_FIELD_K: return get_tensor(c, c->d.K64, o);
     case PDX_FIELD_KINV: return get_tensor(c, c->d.Kinv64, o);
-    case PDX_FIELD_F: return get_tensor(c, c->d.Fd, o);
+    case PDX_FIELD_F: {
+        // stored as the displacement gradient F - I; the identity is added in FP64
+        PDX_TRY(get_tensor(c, c->d.Fd, o));
+        for (int r = 0; r < c->n; ++r)
+            for (int q = 0; q < D; ++q) o[(size_t)r * D * D + q * (D + 1)] += 1.0;
+        return PDX_OK;
+    }
     case PDX_FIELD_P: return get_tensor(c, c->d.Pd, o);
     case PDX_FIELD_DAMAGE: return get_scalar(c, c->d.damage, o);
     case PDX_FIELD_DRMASS: return get_scalar(c, c->d.lam, o);

@@ -124,7 +124,7 @@ struct Dev {
     double *lam;       // adaptive-DR diagonal mass
     double *V64;       // particle volume in FP64 (setup kernels)
     double *K64, *Kinv64;
-    float *Fd, *Pd;    // diagnostic F and P (9 per particle)
+    float *Fd, *Pd;    // diagnostic F - I and P (9 per particle)
     double *damage;
     int32_t *init_nb;  // bonds at build time
     int64_t *sl_ptr;
@@ -396,7 +396,7 @@ PDX_HD void fpass_finish(const Dev &d, int i, const KRec &kr, const double *Hd)
     d.AR[i] = ar;
     if (DIAG) {
         for (int q = 0; q < D * D; ++q) {
-            d.Fd[(int64_t)i * 9 + q] = (float)(H[q] + ((q % (D + 1)) == 0 ? 1.0 : 0.0));
+            d.Fd[(int64_t)i * 9 + q] = (float)H[q];  // F - I: FP32 keeps the gradient's own precision
             d.Pd[(int64_t)i * 9 + q] = (float)P[q];
         }
     }
