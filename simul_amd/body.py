@@ -90,6 +90,10 @@ class PDBody:
         fe = None if fext is None else _f64(fext, (self.n, self.dim))
         self.ctx.call("pdx_set_bc", N.ptr(fx, N._u8p), N.ptr(fe, N._dp))
 
+    def set_formulation(self, formulation):
+        self.ctx.call("pdx_set_formulation", FORMS[formulation])
+        self.formulation = formulation
+
     def build_bonds(self, horizon):
         nb = C.c_int64()
         self.ctx.call("pdx_build_bonds", float(horizon), C.byref(nb))

@@ -1,0 +1,1 @@
+"""Mirror of backend/fea/peridynamics/solver (NOSB kinetic-damping DR, bond-based quasi-static)."""
