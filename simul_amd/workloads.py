@@ -8,6 +8,8 @@
   (median 300 MPa, sigma_log 0.4) from a seeded generator; nu = 0.3.
   Bottom endplate fixed, top endplate loaded by a uniform pressure applied as
   the force density p / dx on the top voxel layer.
+* vertebra_j2: the same geometry with J2 plasticity under a prescribed 2 %
+  endplate compression in 20 increments, bonds breaking at stretch 0.02.
 * batch: several random box / cylinder / ellipsoid specimens at dx = 0.5 mm.
 
 The lateral distance to the ellipse uses the first-order estimate
@@ -120,6 +122,25 @@ def vertebra(dx=0.4, a=20.0, b=15.0, height=25.0, shell=1.2, E_cort=12000.0, E_c
     return Specimen(f"vertebra_dx{dx}", X, dx, HORIZON_FACTOR * dx, E, nu, RHO_BONE, fixed,
                     np.zeros((n, 3)), fext, model, np.where(cortical, 0, 1).astype(np.int32),
                     {"pressure": pressure, "seed": seed, "top": top, "bottom": bot})
+
+
+def vertebra_j2(dx=0.25, yield_cort=80.0, yield_canc=4.0, hardening=0.05, strain=-0.02, n_increments=20,
+                critical_stretch=0.02, seed=3000, **kw):
+    """The vertebral body with J2 plasticity (linear isotropic hardening H =
+    hardening x E, yield 80 MPa cortical / 4 MPa cancellous), both endplates
+    held by rigid platens (all DOFs prescribed): bottom fixed, top moved to
+    u_z = strain x height in ``n_increments`` load increments, bonds breaking
+    at ``critical_stretch``."""
+    spec = vertebra(dx=dx, pressure=0.0, seed=seed, model="j2_plasticity", **kw)
+    top, bot = spec.extra["top"], spec.extra["bottom"]
+    spec.fixed[top | bot] = 7
+    height = kw.get("height", 25.0)
+    cort = spec.material_id == 0
+    spec.name = f"vertebra_j2_dx{dx}"
+    spec.extra.update({"yield": np.where(cort, yield_cort, yield_canc), "h_iso": hardening * spec.E,
+                       "top_idx": np.where(top)[0], "u_final": np.array([0.0, 0.0, strain * height]),
+                       "n_increments": n_increments, "critical_stretch": critical_stretch})
+    return spec
 
 
 def random_specimen(seed, n_target, dx=0.5, shell=1.2, E_cort=12000.0, E_canc_median=300.0,

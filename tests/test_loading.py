@@ -1,0 +1,120 @@
+"""Incremental loading with J2 plasticity and bond breaking:
+the HIP driver (simul_amd.loading) against the same increments run by the
+oracle, and the full-size config-3 specimen (dx 0.25 mm, 1,507,600 particles)
+through size-independent checks: neighbour sets and broken flags bit-exact
+with the oracle on the same displacement field, J2 state of that field within
+the J2 tolerance.
+"""
+import numpy as np
+import pytest
+
+from helpers import rel_l2
+from oracle import pdoracle as po
+from simul_amd import workloads as W
+from simul_amd.body import from_specimen
+from simul_amd.loading import IncrementalLoading
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(spec, nthreads=None):
+    p = po.Problem(spec.X, spec.vol, spec.horizon, spec.bulk, spec.shear, model="j2_plasticity",
+                   sy0=spec.extra["yield"], h_iso=spec.extra["h_iso"], fixed=spec.fixed, G_s=0.1,
+                   cstab=W.stab_modulus(spec.bulk, spec.horizon), mass=spec.vol * spec.rho, nthreads=nthreads)
+    p.shape_tensor("relative", 1e-12)
+    return p
+
+
+def _oracle_increments(p, idx, u_final, n_inc, tol):
+    its = []
+    for k in range(1, n_inc + 1):
+        u = p.u
+        u[idx] = (k / n_inc) * u_final
+        p.set_u(u)
+        r = p.adr_solve(p.adr_mass(1.0), max_it=100000, tol=tol)
+        assert r["converged"]
+        p.deformation_gradient()
+        p.stress()
+        p.commit()
+        its.append(r["iterations"])
+    return its
+
+
+def test_incremental_j2_matches_oracle(gpu):
+    spec = W.vertebra_j2(dx=1.25, n_increments=4)
+    idx, uf = spec.extra["top_idx"], spec.extra["u_final"]
+    body = from_specimen(spec)
+    drv = IncrementalLoading(body, idx, uf, n_increments=4, critical_stretch=None, tol=1e-6)
+    res = drv.run()
+    assert all(r.converged for r in res)
+    prob = _oracle(spec)
+    its = _oracle_increments(prob, idx, uf, 4, 1e-6)
+    for r, io in zip(res, its):
+        assert abs(r.iterations - io) <= max(10, 0.15 * io), (r.iterations, io)
+    assert np.allclose(body.u[idx], uf)
+    assert rel_l2(body.u, prob.u) < 1e-4, rel_l2(body.u, prob.u)
+    assert prob.epbar.max() > 0  # the cancellous core yields
+    assert rel_l2(body.get("epbar"), prob.epbar) < 1e-3, rel_l2(body.get("epbar"), prob.epbar)
+
+
+def test_incremental_breaking_invariants(gpu):
+    """Breaking inside the driver: counts only grow, every broken flag is one the
+    criterion gives for the state it was taken at, damage = broken / initial."""
+    spec = W.vertebra_j2(dx=1.25, n_increments=3)
+    idx, uf = spec.extra["top_idx"], spec.extra["u_final"] * 3.0
+    body = from_specimen(spec)
+    drv = IncrementalLoading(body, idx, uf, n_increments=3, critical_stretch=0.01, tol=1e-5)
+    res = drv.run()
+    totals = [r.broken_total for r in res]
+    assert totals == sorted(totals) and totals[-1] > 0
+    assert sum(r.broken_now for r in res) == totals[-1]
+    row, col, br = body.bonds()
+    init = np.diff(row)
+    nbr = np.add.reduceat(br.astype(np.int64), row[:-1]) * (init > 0)
+    assert np.allclose(body.local_damage(), np.where(init > 0, nbr / np.maximum(init, 1), 0.0))
+    # breaking keeps the bond family symmetric: internal forces still sum to zero
+    f = body.compute_forces()
+    assert np.isfinite(f).all()
+    assert np.abs(f.sum(0)).max() < 1e-5 * np.abs(f).sum(0).max()
+
+
+def test_config3_full_size_flags_and_j2(gpu):
+    """dx 0.25 mm: 1,507,600 particles.  300 adaptive-DR iterations towards the
+    full 2 % compression, then: neighbour CSR equal to the oracle's, broken flags of the
+    GPU field equal to the oracle's on the same field (two thresholds), J2 trial
+    state of that field within the J2 tolerance."""
+    spec = W.vertebra_j2(dx=0.25)
+    assert spec.n == 1507600
+    body = from_specimen(spec, build=False)
+    nb = body.build_bonds(spec.horizon)
+    body.shape_tensor("relative", 1e-12)
+    idx, uf = spec.extra["top_idx"], spec.extra["u_final"]
+    u = body.u
+    u[idx] = uf  # the whole 2 % at once: the endplate region yields
+    body.u = u
+    body.solve("adaptive", tol=0.0, max_iterations=300)
+    uu = body.u
+    row, col, br = body.bonds()
+    orow, ocol = po.neighbors(spec.X, spec.horizon)
+    assert np.array_equal(row, orow) and np.array_equal(col, ocol) and not br.any() and nb == len(ocol)
+    prob = po.Problem(spec.X, spec.vol, spec.horizon, spec.bulk, spec.shear, model="j2_plasticity",
+                      sy0=spec.extra["yield"], h_iso=spec.extra["h_iso"], fixed=spec.fixed, row=orow, col=ocol)
+    del orow, ocol
+    prob.set_u(uu)
+    for s_c in (spec.extra["critical_stretch"], 0.002):
+        nb_g = body.break_bonds(s_c)
+        nb_o = prob.update_damage(s_c)
+        assert nb_g == nb_o
+        _, _, br = body.bonds()
+        assert np.array_equal(br, prob.broken[: prob.nb].astype(bool))
+    assert nb_g > 0
+    body.shape_tensor("relative", 1e-12)
+    prob.shape_tensor("relative", 1e-12)
+    body.compute_F_stress()
+    prob.deformation_gradient()
+    prob.stress()
+    assert rel_l2(body.get("P"), prob.P) < 2e-5
+    body.commit_state()
+    prob.commit()
+    assert prob.epbar.max() > 0
+    assert rel_l2(body.get("epbar"), prob.epbar) < 1e-3
