@@ -95,6 +95,7 @@ typedef struct {
     double dt;            /* kinetic DR time step (nosb_solver.py:81-83) */
     double viscous_damping;/* kinetic DR viscous factor (nosb_solver.py:168-174) */
     double cmax;          /* adaptive DR damping cap (0 = default 1.9) */
+    const double *refs;   /* per-case reference residuals (ncase values) or NULL for `ref` */
 } pdx_dr_params;
 
 typedef struct {

@@ -48,7 +48,7 @@ class DRParams(C.Structure):
     _fields_ = [("method", C.c_int32), ("use_graph", C.c_int32), ("chunk", C.c_int32),
                 ("resume", C.c_int32), ("max_iterations", C.c_int64), ("tol", C.c_double),
                 ("ref", C.c_double), ("dt", C.c_double), ("viscous_damping", C.c_double),
-                ("cmax", C.c_double)]
+                ("cmax", C.c_double), ("refs", C.POINTER(C.c_double))]
 
 
 class DRResult(C.Structure):

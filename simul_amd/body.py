@@ -203,12 +203,60 @@ class PDBody:
                 "prev_ke": pk, "ke_increasing": inc.astype(bool)}
 
     def solve(self, method="adaptive", tol=1e-6, max_iterations=100000, ref=0.0, dt=0.0,
-              viscous_damping=0.0, cmax=1.9, use_graph=True, chunk=64, resume=False):
+              viscous_damping=0.0, cmax=1.9, use_graph=True, chunk=64, resume=False, refs=None,
+              progress=None, slice_iterations=None):
+        """Dynamic relaxation to |f_free| / ref < tol per case.
+
+        ref > 0 fixes the reference residual of every case, refs (ncase) of
+        each case; otherwise each case's residual at the start state is used.
+        With ``slice_iterations`` the iterations run in slices of that many
+        (resumed on the device, so the trajectory is the same) and
+        ``progress(done, result)`` is called after each slice with the slice's
+        result (a list for a batch); the returned iteration counts are totals
+        over the slices."""
+        if slice_iterations is None:
+            return self._solve_once(method, tol, max_iterations, ref, dt, viscous_damping, cmax, use_graph,
+                                    chunk, resume, refs)
+        if refs is None:
+            if ref > 0.0:
+                refs = np.full(self.ncase, float(ref))
+            else:
+                self.compute_forces()
+                refs = np.atleast_1d(self.residual_norm()).astype(np.float64)
+                refs = np.where(refs < 1e-20, 1.0, refs)
+        refs = np.ascontiguousarray(np.broadcast_to(np.asarray(refs, np.float64), (self.ncase,)))
+        total = np.zeros(self.ncase, np.int64)
+        done = np.zeros(self.ncase, bool)
+        last = None
+        launched = 0
+        sl = max(2 * int(chunk), int(slice_iterations) // (2 * int(chunk)) * (2 * int(chunk)))
+        while launched < max_iterations and not done.all():
+            n = int(min(sl, max_iterations - launched))
+            out = self._solve_once(method, tol, n, 0.0, dt, viscous_damping, cmax, use_graph, chunk,
+                                   resume or launched > 0, refs)
+            out = out if isinstance(out, list) else [out]
+            for k, r in enumerate(out):
+                if not done[k]:
+                    total[k] += r.iterations
+                    done[k] = r.converged
+            launched += n
+            last = out
+            if progress is not None:
+                progress(launched, last if self.ncase > 1 else last[0])
+        res = [DRResult(bool(done[k]), int(total[k]), r.residual, r.relative_residual, r.reference_residual,
+                        r.damping, r.resets, r.elapsed_s) for k, r in enumerate(last)]
+        return res if self.ncase > 1 else res[0]
+
+    def _solve_once(self, method, tol, max_iterations, ref, dt, viscous_damping, cmax, use_graph, chunk,
+                    resume, refs):
         p = N.DRParams()
         p.method = {"adaptive": N.DR_ADAPTIVE, "kinetic": N.DR_KINETIC}[method]
         p.use_graph, p.chunk, p.resume = int(bool(use_graph)), int(chunk), int(bool(resume))
         p.max_iterations, p.tol, p.ref = int(max_iterations), float(tol), float(ref)
         p.dt, p.viscous_damping, p.cmax = float(dt), float(viscous_damping), float(cmax)
+        if refs is not None:
+            refs = np.ascontiguousarray(np.broadcast_to(np.asarray(refs, np.float64), (self.ncase,)))
+            p.refs = N.ptr(refs, N._dp)
         res = (N.DRResult * self.ncase)()
         self.ctx.call("pdx_dr_solve", C.byref(p), res)
         out = [DRResult(bool(r.converged), int(r.iterations), r.residual, r.relative_residual,

@@ -959,11 +959,20 @@ extern "C" int pdx_dr_solve(pdx_ctx *c, const pdx_dr_params *p, pdx_dr_result *r
     }
     // reference residual of the start state unless given (nosb_solver.py:287-293)
     std::vector<double> ref(c->ncase, p->ref);
-    if (!(p->ref > 0.0)) {
+    bool need_ref = !(p->ref > 0.0);
+    if (p->refs) {
+        need_ref = false;
+        for (int k = 0; k < c->ncase; ++k) {
+            ref[k] = p->refs[k];
+            if (!(ref[k] > 0.0)) need_ref = true;
+        }
+    }
+    if (need_ref) {
         if (c->form == PDX_FORM_NOSB) PDX_TRY(pdx_launch_fpass(c, c->ucur, false, false, false));
         PDX_TRY(pdx_launch_force(c, c->ucur, 0, false, 0.0, 0.0));
         PDX_TRY(read_sc(c));
-        for (int k = 0; k < c->ncase; ++k) ref[k] = c->h_sc[k].res < 1e-20 ? 1.0 : c->h_sc[k].res;
+        for (int k = 0; k < c->ncase; ++k)
+            if (!(ref[k] > 0.0)) ref[k] = c->h_sc[k].res < 1e-20 ? 1.0 : c->h_sc[k].res;
     }
     PDX_TRY(read_sc(c));
     for (int k = 0; k < c->ncase; ++k) {

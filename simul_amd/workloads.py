@@ -10,7 +10,7 @@
   the force density p / dx on the top voxel layer.
 * vertebra_j2: the same geometry with J2 plasticity under a prescribed 2 %
   endplate compression in 20 increments, bonds breaking at stretch 0.02.
-* batch: several random box / cylinder / ellipsoid specimens at dx = 0.5 mm.
+* batch: several random box / cylinder / truncated-ellipsoid specimens at dx = 0.5 mm.
 
 The lateral distance to the ellipse uses the first-order estimate
 (1 - r) / |grad r| with r = sqrt(x^2/a^2 + y^2/b^2).
@@ -151,7 +151,7 @@ def random_specimen(seed, n_target, dx=0.5, shell=1.2, E_cort=12000.0, E_canc_me
     rng = np.random.default_rng(seed)
     shape = ["box", "cylinder", "ellipsoid"][int(rng.integers(3))]
     asp = rng.uniform(0.7, 1.4, size=3)
-    frac = {"box": 1.0, "cylinder": math.pi / 4, "ellipsoid": math.pi / 6}[shape]
+    frac = {"box": 1.0, "cylinder": math.pi / 4, "ellipsoid": 0.8 * math.pi / 4 * (1 - 0.64 / 3)}[shape]
     s = (n_target * dx ** 3 / (frac * np.prod(asp))) ** (1.0 / 3.0)
     L = asp * s
     cnt = np.maximum(np.round(L / dx).astype(int), 3)
@@ -161,7 +161,11 @@ def random_specimen(seed, n_target, dx=0.5, shell=1.2, E_cort=12000.0, E_canc_me
     if shape == "cylinder":
         X = X[(X[:, 0] / hx) ** 2 + (X[:, 1] / hy) ** 2 <= 1.0]
     elif shape == "ellipsoid":
-        X = X[(X[:, 0] / hx) ** 2 + (X[:, 1] / hy) ** 2 + (X[:, 2] / hz) ** 2 <= 1.0]
+        # truncated at |z| <= 0.8 hz so that the specimen has flat endplates to
+        # hold and to load (an ellipsoid supported and loaded at its tips has
+        # near-rigid modes)
+        X = X[((X[:, 0] / hx) ** 2 + (X[:, 1] / hy) ** 2 + (X[:, 2] / hz) ** 2 <= 1.0)
+              & (np.abs(X[:, 2]) <= 0.8 * hz)]
     n = len(X)
     if shape == "box":
         d = np.min(np.stack([hx - np.abs(X[:, 0]), hy - np.abs(X[:, 1]), hz - np.abs(X[:, 2])]), 0)
@@ -169,13 +173,12 @@ def random_specimen(seed, n_target, dx=0.5, shell=1.2, E_cort=12000.0, E_canc_me
         d = np.minimum(_ellipse_lateral_distance(X[:, 0], X[:, 1], hx, hy), hz - np.abs(X[:, 2]))
     else:
         r = np.sqrt((X[:, 0] / hx) ** 2 + (X[:, 1] / hy) ** 2 + (X[:, 2] / hz) ** 2)
-        d = (1.0 - r) * min(hx, hy, hz)
+        d = np.minimum((1.0 - r) * min(hx, hy, hz), 0.8 * hz - np.abs(X[:, 2]))
     cortical = d <= shell
     E = np.where(cortical, E_cort, np.exp(np.log(E_canc_median) + sigma_log * rng.standard_normal(n)))
     z = X[:, 2]
     fixed = np.zeros(n, np.uint8)
     bot = z < z.min() + 0.5 * dx
-    # ellipsoid: the loaded "endplate" is the top dx layer
     top = z > z.max() - 0.5 * dx
     fixed[bot] = 7
     p = rng.uniform(*p_range)
@@ -193,3 +196,18 @@ def batch(n_cases=64, seed=4000, n_lo=10_000, n_hi=200_000, dx=0.5, model="neo_h
     rng = np.random.default_rng(seed)
     sizes = rng.integers(n_lo, n_hi + 1, size=n_cases)
     return [random_specimen(seed + 1 + k, int(sizes[k]), dx=dx, model=model) for k in range(n_cases)]
+
+
+def pack(specs):
+    """Several specimens of one spacing packed caller-order into one batched
+    input: the concatenated specimen and the case offsets (ncase + 1)."""
+    dx = specs[0].dx
+    if any(s.dx != dx for s in specs):
+        raise ValueError("packed specimens must share the particle spacing")
+    off = np.zeros(len(specs) + 1, np.int64)
+    off[1:] = np.cumsum([s.n for s in specs])
+    cat = lambda name: np.concatenate([getattr(s, name) for s in specs])
+    packed = Specimen(f"batch{len(specs)}_dx{dx}", cat("X"), dx, specs[0].horizon, cat("E"), specs[0].nu,
+                      specs[0].rho, cat("fixed"), cat("u0"), cat("fext"), specs[0].model, cat("material_id"),
+                      {"cases": [s.name for s in specs]})
+    return packed, off

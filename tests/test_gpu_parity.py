@@ -283,3 +283,27 @@ def test_hot_pass_variants_bitwise_identical(gpu, formulation):
             assert it == base[2] == [25, 25]
             assert np.array_equal(f, base[0]), (method, g)
             assert np.array_equal(uu, base[1]), (method, g)
+
+
+def test_sliced_solve_matches_single(gpu):
+    """Resumed slices of the device DR loop follow the same trajectory: same
+    iteration counts and the same bits as one uninterrupted solve."""
+    spec, body, _ = _cube_pair(10)
+    r1 = body.solve("adaptive", tol=1e-6)
+    u1 = body.u
+    body.u = spec.u0
+    seen = []
+    r2 = body.solve("adaptive", tol=1e-6, slice_iterations=128, progress=lambda d, rs: seen.append(d))
+    assert r2.converged and r2.iterations == r1.iterations and len(seen) > 1
+    assert np.array_equal(body.u, u1)
+    specs = [W.cube(n_side=k) for k in (7, 9)]
+    off = np.cumsum([0] + [s.n for s in specs])
+    bulk, shear = W.bulk_shear(15000.0, 0.3)
+    mk = lambda: PDBody(np.concatenate([s.X for s in specs]), 1.0, 3.015, bulk, shear,
+                        fixed=np.concatenate([s.fixed for s in specs]), u0=np.concatenate([s.u0 for s in specs]),
+                        case_offsets=off)
+    b1, b2 = mk(), mk()
+    ra = b1.solve("adaptive", tol=1e-6)
+    rb = b2.solve("adaptive", tol=1e-6, slice_iterations=128, progress=lambda d, rs: None)
+    assert [r.iterations for r in ra] == [r.iterations for r in rb]
+    assert np.array_equal(b1.u, b2.u)
