@@ -171,6 +171,11 @@ int pdx_local_damage(pdx_ctx *ctx);
  * converged load increment; the reference commits per call, j2_plasticity.py:242-243) */
 int pdx_commit_state(pdx_ctx *ctx);
 
+/* weighted neighbour mean over the intact bonds of a per-particle feature vector
+ * (in/out n*width doubles, width 1..64): out_i = sum_j w_ij V_j in_j / sum_j w_ij V_j.
+ * No reference counterpart: the message-passing step of the learned warm start. */
+int pdx_neighbor_mean(pdx_ctx *ctx, int32_t width, const double *in, double *out);
+
 /* ---- dynamic relaxation ---------------------------------------------------- */
 /* NOSBSolver._estimate_stable_dt, nosb_solver.py:94-154 */
 int pdx_estimate_dt(pdx_ctx *ctx, double bulk, double shear, double rho_mean, double *dt);

@@ -96,6 +96,14 @@ SIGNATURES = {
     "pdx_stats": [_vp, _i64p],
     "pdx_autotune": [_vp, C.c_int32, _i32p],
     "pdx_set_group": [_vp, C.c_int32],
+    "pdx_neighbor_mean": [_vp, C.c_int32, _dp, _dp],
+    "pdx_nn_linear_fwd": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, C.c_int, _vp],
+    "pdx_nn_linear_dgrad": [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp],
+    "pdx_nn_linear_wgrad": [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp],
+    "pdx_nn_mse": [_vp, C.c_int, _vp, _vp, C.c_int, C.c_float, _vp, _vp, _vp],
+    "pdx_nn_adam": [_vp, _vp, _vp, _vp, _vp, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int64,
+                    C.c_float, _vp],
+    "pdx_nn_to_bf16": [_vp, C.c_int64, _vp, _vp],
     "pdx_get": [_vp, C.c_int32, _vp],
     "pdx_set": [_vp, C.c_int32, _vp],
 }

@@ -176,6 +176,15 @@ class PDBody:
         self.ctx.call("pdx_break_bonds", float(s_c), C.byref(n))
         return n.value
 
+    def neighbor_mean(self, values):
+        """Weighted mean of a per-particle vector (n x width, width <= 64) over
+        each particle's intact bonds (pdx_neighbor_mean)."""
+        v = np.ascontiguousarray(values, np.float64)
+        w = 1 if v.ndim == 1 else v.shape[1]
+        out = np.zeros_like(v)
+        self.ctx.call("pdx_neighbor_mean", int(w), N.ptr(v, N._dp), N.ptr(out, N._dp))
+        return out
+
     def local_damage(self):
         self.ctx.call("pdx_local_damage")
         return self.ctx.get("damage")

@@ -810,6 +810,34 @@ extern "C" int pdx_break_bonds(pdx_ctx *c, double s_c, int64_t *n_new)
     return r;
 }
 
+extern "C" int pdx_neighbor_mean(pdx_ctx *c, int32_t width, const double *in, double *out)
+{
+    CHECK_CTX(c);
+    if (!in || !out) return fail(PDX_EINVAL, "in/out NULL");
+    if (width < 1 || width > 64) return fail(PDX_EINVAL, "width must be 1..64");
+    if (!c->have_bonds) return fail(PDX_ESTATE, "bonds not built");
+    std::vector<double> h((size_t)c->ni * width, 0.0);
+    for (int i = 0; i < c->ni; ++i) {
+        const int r = c->perm[i];
+        if (r >= 0) memcpy(&h[(size_t)i * width], in + (size_t)r * width, sizeof(double) * width);
+    }
+    double *dv = nullptr;
+    PDX_HIP(hipMallocAsync((void **)&dv, sizeof(double) * h.size() * 2, c->stream));
+    hipError_t e = hipMemcpyAsync(dv, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream);
+    int r = e == hipSuccess ? pdx_launch_nbr_mean(c, dv, dv + h.size(), width) : PDX_EDEVICE;
+    if (r == PDX_OK)
+        e = hipMemcpyAsync(h.data(), dv + h.size(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream);
+    hipFreeAsync(dv, c->stream);
+    PDX_HIP(e);
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    if (r != PDX_OK) return r;
+    for (int i = 0; i < c->ni; ++i) {
+        const int rr = c->perm[i];
+        if (rr >= 0) memcpy(out + (size_t)rr * width, &h[(size_t)i * width], sizeof(double) * width);
+    }
+    return PDX_OK;
+}
+
 extern "C" int pdx_local_damage(pdx_ctx *c)
 {
     CHECK_CTX(c);

@@ -1017,6 +1017,35 @@ PDX_HD void bond_data_one(const Dev &d, int i, double *xi_o, double *len_o, doub
     }
 }
 
+// weighted neighbour mean of a per-particle feature vector (width <= 64) over
+// the intact bonds: out_i = sum_j w_ij V_j in_j / sum_j w_ij V_j (FP64), the
+// message-passing step of the learned warm start; a particle without intact
+// bonds keeps its own value
+PDX_HD void nbr_mean_one(const Dev &d, int i, const double *in, double *out, int width)
+{
+    const KRec kr = d.KR[i];
+    const XRec pi = d.xu[0][i];
+    double acc[64];
+    for (int q = 0; q < width; ++q) acc[q] = 0.0;
+    double ws = 0.0;
+    const int64_t base = slot0(d, i);
+    if (kr.flags & PDX_FL_ACTIVE) {
+        for (int k = 0; k < (int)kr.nnb; ++k) {
+            const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
+            if (e >= (uint32_t)d.ni) continue;  // broken bond or no bond
+            const XRec pj = d.xu[0][(int)e];
+            double s = 0.0;
+            for (int q = 0; q < 3; ++q) s += (pj.X[q] - pi.X[q]) * (pj.X[q] - pi.X[q]);
+            const double L = sqrt(s);
+            const double w = (L > 0 ? fmax(0.0, 1.0 - L / d.h) : 1.0) * d.V64[(int)e];
+            for (int q = 0; q < width; ++q) acc[q] += w * in[(int64_t)e * width + q];
+            ws += w;
+        }
+    }
+    for (int q = 0; q < width; ++q)
+        out[(int64_t)i * width + q] = ws > 0.0 ? acc[q] / ws : in[(int64_t)i * width + q];
+}
+
 // |f|^2 over free DOFs from the stored force
 template <int D>
 PDX_HD double resid_one(const Dev &d, int i)

@@ -380,6 +380,7 @@ template <int D> PDX_PER_PARTICLE(k_bond_data(Dev d, double *xi, double *len, do
                                   (bond_data_one<D>(d, i, xi, len, w, rowi)))
 PDX_PER_PARTICLE(k_local_damage(Dev d), (local_damage_one(d, i)))
 PDX_PER_PARTICLE(k_zero_free(Dev d, int ubuf), (zero_free_one(d, i, ubuf)))
+PDX_PER_PARTICLE(k_nbr_mean(Dev d, const double *in, double *out, int width), (nbr_mean_one(d, i, in, out, width)))
 
 template <int D>
 __global__ __launch_bounds__(PDX_BLOCK) void k_damage(Dev d, int ubuf, double s_c, unsigned long long *count)
@@ -641,6 +642,13 @@ int pdx_launch_resid(pdx_ctx *c)
 int pdx_launch_zero_free(pdx_ctx *c, int ubuf)
 {
     hipLaunchKernelGGL(k_zero_free, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, ubuf);
+    PDX_LAUNCH_CHECK();
+    return PDX_OK;
+}
+
+int pdx_launch_nbr_mean(pdx_ctx *c, const double *d_in, double *d_out, int width)
+{
+    hipLaunchKernelGGL(k_nbr_mean, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_in, d_out, width);
     PDX_LAUNCH_CHECK();
     return PDX_OK;
 }

@@ -1,0 +1,189 @@
+"""Training data, training and evaluation of the learned warm start.
+
+    python -m simul_amd.learned.train --train 1024 --heldout 64 --dx 0.8 \
+        --out simul_amd/learned/weights_vertebra.npz --report gpurun_out/warmstart_report.json
+
+Data: random vertebral bodies (simul_amd.workloads.random_vertebra: size,
+cancellous field, pressure magnitude and direction within 20 deg), solved by
+the plain cold-start adaptive DR to 1e-6 in batched contexts of ``group``
+cases.  The network learns u / s (s = p H / 1000 MPa) on the free particles.
+Evaluation: the held-out cases solved cold and warm-started (same batched
+context layout, same criterion: residual relative to each case's cold-start
+residual), iterations and wall time including feature building and inference.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _group_body(specs):
+    from ..body import from_specimen
+    from ..workloads import pack
+    packed, off = pack(specs)
+    body = from_specimen(packed, case_offsets=off)
+    body.set_group(16 | 2)
+    return packed, off, body
+
+
+def generate(seeds, dx, group, tol, max_iterations):
+    """Features, targets and weights of the cases ``seeds`` (host float32)."""
+    from ..workloads import random_vertebra
+    from .features import feature_matrix, target_scale
+    Fs, Ts, Ws, stats = [], [], [], []
+    for g0 in range(0, len(seeds), group):
+        t0 = time.perf_counter()
+        specs = [random_vertebra(int(s), dx=dx) for s in seeds[g0:g0 + group]]
+        packed, off, body = _group_body(specs)
+        rs = body.solve("adaptive", tol=tol, max_iterations=max_iterations, slice_iterations=2048,
+                        progress=lambda d, r: log(f"    {d} iterations: {sum(x.converged for x in r)}/{len(r)} converged"))
+        u = body.u
+        F = feature_matrix(body, packed, off)
+        s = target_scale(packed, off)
+        Fs.append(F)
+        Ts.append((u / s[:, None]).astype(np.float32))
+        Ws.append((packed.fixed == 0).astype(np.float32))
+        its = [r.iterations for r in rs]
+        stats.append({"cases": len(specs), "particles": packed.n, "converged": int(sum(r.converged for r in rs)),
+                      "iterations_max": int(max(its)), "seconds": time.perf_counter() - t0})
+        log(f"  data group {g0 // group}: {len(specs)} cases, {packed.n} particles, iterations "
+            f"{min(its)}..{max(its)}, {stats[-1]['seconds']:.1f}s")
+        del body
+    return np.concatenate(Fs), np.concatenate(Ts), np.concatenate(Ws), stats
+
+
+def train(net, F, T, Wt, epochs, batch_rows, lr0, lr_min, seed=0):
+    import torch
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(len(F))
+    Fd = torch.from_numpy(F[order]).cuda()
+    Xbf = net.to_bf16(Fd)
+    del Fd
+    Td = torch.from_numpy(np.ascontiguousarray(T[order])).cuda()
+    Wd = torch.from_numpy(np.ascontiguousarray(Wt[order])).cuda()
+    n = len(F)
+    nb = max(1, n // batch_rows)
+    total = epochs * nb
+    hist = []
+    step = 0
+    t0 = time.perf_counter()
+    for ep in range(epochs):
+        ep_loss = 0.0
+        for bi in range(nb):
+            a, b = bi * batch_rows, min(n, (bi + 1) * batch_rows)
+            lr = lr_min + 0.5 * (lr0 - lr_min) * (1 + np.cos(np.pi * step / total))
+            wsum = float(Wt[order[a:b]].sum())
+            loss = net.train_step(Xbf[a:b], Td[a:b], Wd[a:b], max(wsum, 1.0), lr)
+            ep_loss += loss
+            step += 1
+        hist.append(ep_loss / nb)
+        log(f"  epoch {ep + 1}/{epochs}: mean loss {hist[-1]:.3e} ({time.perf_counter() - t0:.1f}s)")
+    return hist
+
+
+def evaluate(net, seeds, dx, group, tol, max_iterations):
+    """Cold vs warm-started batched solves of the held-out cases."""
+    import torch
+    from ..workloads import random_vertebra
+    from .warmstart import predict_u
+    out = []
+    for g0 in range(0, len(seeds), group):
+        specs = [random_vertebra(int(s), dx=dx) for s in seeds[g0:g0 + group]]
+        packed, off, body = _group_body(specs)
+        cold = np.atleast_1d(body.cold_residual())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rc = body.solve("adaptive", tol=tol, max_iterations=max_iterations, refs=cold, slice_iterations=2048,
+                        progress=lambda d, r: None)
+        torch.cuda.synchronize()
+        t_cold = time.perf_counter() - t0
+        u_cold = body.u
+        body.u = packed.u0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        body.u = predict_u(net, body, packed, off)
+        u_pred = body.u
+        torch.cuda.synchronize()
+        t_inf = time.perf_counter() - t0
+        rw = body.solve("adaptive", tol=tol, max_iterations=max_iterations, refs=cold, slice_iterations=2048,
+                        progress=lambda d, r: None)
+        torch.cuda.synchronize()
+        t_warm = time.perf_counter() - t0
+        u_warm = body.u
+        err_pred = [float(np.linalg.norm(u_pred[a:b] - u_cold[a:b]) / np.linalg.norm(u_cold[a:b]))
+                    for a, b in zip(off[:-1], off[1:])]
+        err_final = [float(np.linalg.norm(u_warm[a:b] - u_cold[a:b]) / np.linalg.norm(u_cold[a:b]))
+                     for a, b in zip(off[:-1], off[1:])]
+        ic, iw = [r.iterations for r in rc], [r.iterations for r in rw]
+        out.append({"cases": len(specs), "cold_s": t_cold, "warm_s": t_warm, "inference_s": t_inf,
+                    "cold_iterations": ic, "warm_iterations": iw,
+                    "all_converged": bool(all(r.converged for r in rc) and all(r.converged for r in rw)),
+                    "prediction_rel_l2": err_pred, "final_rel_l2_vs_cold": err_final})
+        log(f"  held-out group {g0 // group}: cold {t_cold:.2f}s ({max(ic)} it max, {np.mean(ic):.0f} mean), "
+            f"warm {t_warm:.2f}s incl. {t_inf:.2f}s inference ({max(iw)} it max, {np.mean(iw):.0f} mean), "
+            f"prediction error {np.median(err_pred):.3f} median")
+        del body
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=1024)
+    ap.add_argument("--heldout", type=int, default=64)
+    ap.add_argument("--dx", type=float, default=0.8)
+    ap.add_argument("--group", type=int, default=64)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--max-iterations", type=int, default=60000)
+    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--batch-rows", type=int, default=65536)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr-min", type=float, default=2e-5)
+    ap.add_argument("--seed", type=int, default=5000)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--report", default=None)
+    args = ap.parse_args(argv)
+
+    import torch
+    assert torch.cuda.is_available(), "training runs on the GPU"
+    from .net import WarmStartNet
+    t_all = time.perf_counter()
+    train_seeds = np.arange(args.seed, args.seed + args.train)
+    held_seeds = np.arange(args.seed + 100000, args.seed + 100000 + args.heldout)
+    log(f"generating {args.train} training cases at dx {args.dx}")
+    F, T, Wt, gstats = generate(train_seeds, args.dx, args.group, args.tol, args.max_iterations)
+    log(f"training set: {len(F)} particles, {int(Wt.sum())} free")
+    net = WarmStartNet(seed=args.seed)
+    hist = train(net, F, T, Wt, args.epochs, args.batch_rows, args.lr, args.lr_min)
+    del F, T, Wt
+    if args.out:
+        net.save(args.out, dx=args.dx, train_cases=args.train, epochs=args.epochs, final_loss=hist[-1])
+        log(f"weights -> {args.out}")
+    log(f"evaluating on {args.heldout} held-out cases")
+    ev = evaluate(net, held_seeds, args.dx, args.group, args.tol, args.max_iterations)
+    cold = sum(e["cold_s"] for e in ev)
+    warm = sum(e["warm_s"] for e in ev)
+    ic = [i for e in ev for i in e["cold_iterations"]]
+    iw = [i for e in ev for i in e["warm_iterations"]]
+    rep = {"train_cases": args.train, "heldout_cases": args.heldout, "dx": args.dx, "epochs": args.epochs,
+           "loss_history": hist, "data_groups": gstats, "heldout": ev,
+           "summary": {"cold_s": cold, "warm_s": warm, "speedup_wall": cold / max(warm, 1e-9),
+                       "cold_iterations_mean": float(np.mean(ic)), "warm_iterations_mean": float(np.mean(iw)),
+                       "speedup_iterations_mean": float(np.mean(ic) / max(np.mean(iw), 1e-9)),
+                       "total_s": time.perf_counter() - t_all}}
+    log(json.dumps(rep["summary"]))
+    if args.report:
+        with open(args.report, "w") as fh:
+            json.dump(rep, fh)
+    return rep
+
+
+if __name__ == "__main__":
+    main()

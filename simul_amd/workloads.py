@@ -92,7 +92,7 @@ def _ellipse_lateral_distance(x, y, a, b):
 
 
 def vertebra(dx=0.4, a=20.0, b=15.0, height=25.0, shell=1.2, E_cort=12000.0, E_canc_median=300.0,
-             sigma_log=0.4, nu=0.3, pressure=2.0, seed=2000, model="neo_hookean"):
+             sigma_log=0.4, nu=0.3, pressure=2.0, seed=2000, model="neo_hookean", direction=None):
     nx, ny, nz = int(round(2 * a / dx)), int(round(2 * b / dx)), int(math.floor(height / dx + 1e-9))
     xs = -a + (np.arange(nx) + 0.5) * dx
     ys = -b + (np.arange(ny) + 0.5) * dx
@@ -118,7 +118,8 @@ def vertebra(dx=0.4, a=20.0, b=15.0, height=25.0, shell=1.2, E_cort=12000.0, E_c
     top = X[:, 2] > zs[-1] - 0.5 * dx
     fixed[bot] = 7
     fext = np.zeros((n, 3))
-    fext[top, 2] = -pressure / dx
+    d = np.array([0.0, 0.0, -1.0]) if direction is None else np.asarray(direction, np.float64)
+    fext[top] = d / np.linalg.norm(d) * pressure / dx
     return Specimen(f"vertebra_dx{dx}", X, dx, HORIZON_FACTOR * dx, E, nu, RHO_BONE, fixed,
                     np.zeros((n, 3)), fext, model, np.where(cortical, 0, 1).astype(np.int32),
                     {"pressure": pressure, "seed": seed, "top": top, "bottom": bot})
@@ -211,3 +212,22 @@ def pack(specs):
                       specs[0].rho, cat("fixed"), cat("u0"), cat("fext"), specs[0].model, cat("material_id"),
                       {"cases": [s.name for s in specs]})
     return packed, off
+
+
+def random_vertebra(seed, dx=0.8, p_range=(0.5, 3.0), cone_deg=20.0, model="neo_hookean"):
+    """A vertebral body of random size (semi-axes 14-24 mm x 0.6-0.9 of that,
+    height 18-30 mm), random cancellous field and a pressure of random magnitude
+    and direction within cone_deg of the axis: the training / held-out family of
+    the learned warm start."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(14.0, 24.0)
+    b = a * rng.uniform(0.6, 0.9)
+    h = rng.uniform(18.0, 30.0)
+    p = rng.uniform(*p_range)
+    th = math.radians(cone_deg) * math.sqrt(rng.uniform())
+    ph = rng.uniform(0, 2 * math.pi)
+    d = -np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+    spec = vertebra(dx=dx, a=a, b=b, height=h, pressure=p, seed=int(rng.integers(1 << 31)), model=model,
+                    direction=d)
+    spec.name = f"rvert{seed}_dx{dx}"
+    return spec

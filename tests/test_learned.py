@@ -1,0 +1,174 @@
+"""The learned warm start: the BF16 MFMA kernels of include/pdx_nn.h against
+CPU references, and the warm-started solve against the cold start.
+
+- single GEMMs (forward, data gradient, weight gradient) on BF16-representable
+  inputs against float64 products of the same values: FP32 accumulation, so
+  1e-5 relative (the output layout of the 32x32 MFMA tile is what a wrong
+  index would break);
+- the whole network's loss and parameter gradients against torch CPU float64
+  autograd of the same BF16 weights and inputs, rounded to BF16 where the
+  kernels store BF16: 5e-3 relative per tensor (rounding-boundary flips of
+  FP32 versus FP64 accumulation), loss to 1e-5;
+- training lowers the loss of a smooth regression by 10x;
+- DR started from any prediction reaches the cold-start solution (1e-5).
+"""
+import numpy as np
+import pytest
+
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(x):
+    """Round to BF16 (nearest even), returned as float32 values."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+def bits(x):
+    return (np.ascontiguousarray(bf16(x)).view(np.uint32) >> 16).astype(np.uint16).view(np.int16)
+
+
+def _dev(a, dtype=None):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a if dtype is None else a.astype(dtype))).cuda()
+
+
+@pytest.fixture(scope="module")
+def nn(gpu):
+    from simul_amd import _native as N
+    return N.load()
+
+
+def _p(t):
+    import ctypes as C
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+@pytest.mark.parametrize("M,K,N", [(300, 64, 256), (131, 256, 32), (1000, 256, 256)])
+def test_gemm_forward_and_gradients(nn, M, K, N):
+    import torch
+    rng = np.random.default_rng(M + K + N)
+    X, W, b = bf16(rng.standard_normal((M, K))), bf16(rng.standard_normal((K, N)) / 8), rng.standard_normal(N).astype(np.float32)
+    Xd, Wd, bd = _dev(bits(X)), _dev(bits(W)), _dev(b)
+    Yf = torch.empty((M, N), dtype=torch.float32, device="cuda")
+    assert nn.pdx_nn_linear_fwd(_p(Xd), _p(Wd), _p(bd), M, K, N, 0, None, _p(Yf), N, None) == 0
+    ref = X.astype(np.float64) @ W.astype(np.float64) + b
+    assert rel_l2(Yf.cpu().numpy(), ref) < 1e-5
+    Yb = torch.empty((M, N), dtype=torch.int16, device="cuda")
+    assert nn.pdx_nn_linear_fwd(_p(Xd), _p(Wd), _p(bd), M, K, N, 1, _p(Yb), None, N, None) == 0
+    yb = (Yb.cpu().numpy().view(np.uint16).astype(np.uint32) << 16).view(np.float32)
+    r = np.maximum(ref, 0)
+    assert np.all(np.abs(yb - r) <= 2.0 ** -8 * np.abs(r) + 1e-30)  # one BF16 rounding of the FP32 result
+    # data gradient with the ReLU mask of the input activations
+    dY = bf16(rng.standard_normal((M, N)))
+    act = bf16(rng.standard_normal((M, K)))
+    dX = torch.empty((M, K), dtype=torch.int16, device="cuda")
+    assert nn.pdx_nn_linear_dgrad(_p(_dev(bits(dY))), _p(Wd), M, K, N, N, _p(_dev(bits(act))), _p(dX), None) == 0
+    dx = (dX.cpu().numpy().view(np.uint16).astype(np.uint32) << 16).view(np.float32)
+    refx = (dY.astype(np.float64) @ W.astype(np.float64).T) * (act > 0)
+    assert rel_l2(dx, refx) < 5e-3  # BF16 output
+    # weight and bias gradients (FP32 outputs), reduced in row slices of 64
+    ns = (M + 63) // 64
+    part = torch.empty((ns * K * N,), dtype=torch.float32, device="cuda")
+    dW = torch.empty((K, N), dtype=torch.float32, device="cuda")
+    db = torch.empty((N,), dtype=torch.float32, device="cuda")
+    assert nn.pdx_nn_linear_wgrad(_p(Xd), _p(_dev(bits(dY))), M, K, N, N, 64, _p(part), _p(dW), _p(db), None) == 0
+    assert rel_l2(dW.cpu().numpy(), X.astype(np.float64).T @ dY.astype(np.float64)) < 1e-5
+    assert rel_l2(db.cpu().numpy(), dY.astype(np.float64).sum(0)) < 1e-5
+
+
+def _bf16_t(x):
+    import torch
+    return torch.from_numpy(bf16(x.detach().numpy().astype(np.float32)).astype(np.float64))
+
+
+def test_network_gradients_match_cpu_autograd(gpu):
+    """CPU float64 autograd of the same network, rounding to BF16 exactly where
+    the kernels store BF16: hidden activations (forward), the loss gradient and
+    every back-propagated activation gradient (backward)."""
+    import torch
+    from simul_amd.learned.net import WarmStartNet
+
+    class RoundAct(torch.autograd.Function):      # BF16 activation: rounded value and rounded gradient
+        @staticmethod
+        def forward(ctx, x):
+            return _bf16_t(x)
+
+        @staticmethod
+        def backward(ctx, g):
+            return _bf16_t(g)
+
+    class RoundGrad(torch.autograd.Function):     # FP32 output whose gradient is stored in BF16
+        @staticmethod
+        def forward(ctx, x):
+            return x.clone()
+
+        @staticmethod
+        def backward(ctx, g):
+            return _bf16_t(g)
+
+    rng = np.random.default_rng(3)
+    net = WarmStartNet(seed=7)
+    # BF16-representable master weights so both sides use the same values
+    for W in net.W:
+        W.copy_(torch.from_numpy(bf16(W.cpu().numpy())).cuda())
+    for b in net.b:
+        b.copy_(torch.from_numpy(rng.standard_normal(b.shape[0]).astype(np.float32) * 0.1).cuda())
+    net.refresh_bf16()
+    M = 300
+    X = bf16(rng.standard_normal((M, 64)))
+    T = rng.standard_normal((M, 3)).astype(np.float32)
+    w = (rng.uniform(size=M) > 0.2).astype(np.float32)
+    loss, gW, gb = net.gradients(_dev(bits(X)), _dev(T), _dev(w), float(w.sum()))
+    Ws = [torch.tensor(W.cpu().numpy(), dtype=torch.float64, requires_grad=True) for W in net.W]
+    bs = [torch.tensor(b.cpu().numpy(), dtype=torch.float64, requires_grad=True) for b in net.b]
+    h = torch.tensor(X, dtype=torch.float64)
+    for li in range(len(Ws)):
+        z = h @ Ws[li] + bs[li]
+        h = RoundAct.apply(torch.relu(z)) if li < len(Ws) - 1 else RoundGrad.apply(z)
+    d = h[:, :3] - torch.tensor(T, dtype=torch.float64)
+    L = (torch.tensor(w, dtype=torch.float64)[:, None] * d * d).sum() / float(w.sum())
+    L.backward()
+    assert abs(loss - L.item()) < 1e-5 * L.item()
+    for li in range(len(Ws)):
+        assert rel_l2(gW[li].cpu().numpy(), Ws[li].grad.numpy()) < 5e-3, li
+        assert rel_l2(gb[li].cpu().numpy(), bs[li].grad.numpy()) < 5e-3, li
+
+
+def test_training_lowers_the_loss(gpu):
+    import torch
+    from simul_amd.learned.net import WarmStartNet
+    rng = np.random.default_rng(11)
+    M = 16384
+    X = np.zeros((M, 64), np.float32)
+    X[:, :3] = rng.uniform(-1, 1, (M, 3))
+    X[:, 15] = 1.0
+    T = np.stack([np.sin(2 * X[:, 0]), X[:, 1] * X[:, 2], np.cos(X[:, 2])], 1).astype(np.float32)
+    net = WarmStartNet(seed=1)
+    Xb = net.to_bf16(_dev(X))
+    Td, wd = _dev(T), _dev(np.ones(M, np.float32))
+    first = net.train_step(Xb, Td, wd, float(M), 2e-3)
+    for it in range(300):
+        last = net.train_step(Xb, Td, wd, float(M), 2e-3)
+    assert last < 0.1 * first, (first, last)
+
+
+def test_warm_start_reaches_the_cold_solution(gpu):
+    """Whatever the network predicts, DR from it converges to the cold-start
+    equilibrium under the same criterion (an untrained network here)."""
+    from simul_amd import workloads as W
+    from simul_amd.body import from_specimen
+    from simul_amd.learned import WarmStartNet, warm_solve
+    spec = W.random_vertebra(77, dx=1.6)
+    body = from_specimen(spec)
+    cold = body.cold_residual()
+    rc = body.solve("adaptive", tol=1e-6, ref=cold)
+    uc = body.u
+    body.u = spec.u0
+    net = WarmStartNet(seed=3)
+    rw, t = warm_solve(body, spec, net, tol=1e-6, ref=cold)
+    assert rc.converged and rw.converged and t["inference_s"] > 0
+    assert rel_l2(body.u, uc) < 1e-5
