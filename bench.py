@@ -158,10 +158,26 @@ def main():
         wall = time.perf_counter() - t0
         tts = {"cold_start_s": round(wall, 4), "iterations": rs.iterations, "converged": rs.converged,
                "relative_residual": rs.relative_residual, "residual_normalisation": "cold-start residual",
-               "warm_start_s": None, "warm_start": "learned warm start not built yet",
-               "setup_s": round(setup_s, 3)}
+               "warm_start_s": None, "warm_start": "no trained weights", "setup_s": round(setup_s, 3)}
         log(f"time-to-solve: {wall:.3f}s, {rs.iterations} iterations, converged={rs.converged}, "
             f"rel={rs.relative_residual:.2e}")
+        u_cold = body.u
+        wfile = os.path.join(ROOT, "simul_amd", "learned", "weights_vertebra.npz")
+        if os.path.exists(wfile):
+            from simul_amd.learned import WarmStartNet, warm_solve
+            net = WarmStartNet.load(wfile)
+            body.u = spec.u0
+            rw, tw = warm_solve(body, spec, net, tol=args.tol, ref=cold, max_iterations=args.max_solve_iters,
+                                chunk=128)
+            du = float(np.linalg.norm(body.u - u_cold) / np.linalg.norm(u_cold))
+            tts.update({"warm_start_s": round(tw["total_s"], 4), "warm_iterations": rw.iterations,
+                        "warm_converged": rw.converged, "warm_inference_s": round(tw["inference_s"], 4),
+                        "warm_speedup": round(wall / tw["total_s"], 3),
+                        "warm_vs_cold_u_rel_l2": du,
+                        "warm_start": "learned per-particle network (BF16 MFMA kernels), features, inference and "
+                                      "DR included in warm_start_s"})
+            log(f"warm start: {tw['total_s']:.3f}s ({tw['inference_s']:.3f}s features+inference), "
+                f"{rw.iterations} iterations, converged={rw.converged}, |u_warm - u_cold|/|u_cold| = {du:.2e}")
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -1,16 +1,19 @@
 """Per-particle input features of the warm-start network.
 
-Raw features (16 per particle), all dimensionless:
-  0-2   position: (x - cx) / R, (y - cy) / R, (z - z_min) / H  (R = max lateral half-width)
+Raw features (20 per particle), all dimensionless:
+  0-2   position: (x - cx) / a, (y - cy) / b, (z - z_min) / H  (a, b lateral half-widths)
   3     log10(E / 1000 MPa)
   4     1 for the cortical phase, else 0
   5-7   fixed-DOF mask bits (x, y, z)
   8     1 on the loaded particles
   9-11  unit load direction (the specimen's resultant external force)
   12    log10(resultant pressure / 1 MPa)
-  13    H / 25 mm,  14  R / 20 mm,  15  1
+  13-15 H / 25 mm, a / 20 mm, b / 20 mm
+  16    specimen mean of log10(E / 1000 MPa)
+  17    specimen cortical fraction
+  18    1,   19  0
 Two message-passing hops over the bonds (weighted neighbour means, libpdx
-pdx_neighbor_mean) of the raw vector give 16 + 16 more; 16 zeros pad the
+pdx_neighbor_mean) of the raw vector give 20 + 20 more; 4 zeros pad the
 input to 64 columns (the GEMM K granularity).
 """
 from __future__ import annotations
@@ -18,7 +21,7 @@ from __future__ import annotations
 import numpy as np
 
 FEAT = 64
-RAW = 16
+RAW = 20
 
 
 def _per_case(spec, off):
@@ -30,32 +33,38 @@ def _per_case(spec, off):
 def raw_features(spec, case_offsets=None) -> np.ndarray:
     X, n = spec.X, spec.n
     F = np.zeros((n, RAW))
-    for a, b in _per_case(spec, case_offsets):
-        Xc = X[a:b]
+    logE = np.log10(spec.E / 1000.0)
+    cort = (spec.material_id == 0).astype(np.float64) if spec.material_id is not None else np.zeros(n)
+    for a0, b0 in _per_case(spec, case_offsets):
+        Xc = X[a0:b0]
         lo, hi = Xc.min(0), Xc.max(0)
         c = 0.5 * (lo + hi)
-        R = max(0.5 * (hi[0] - lo[0]), 0.5 * (hi[1] - lo[1]), 1e-9)
+        ax = max(0.5 * (hi[0] - lo[0]), 1e-9)
+        by = max(0.5 * (hi[1] - lo[1]), 1e-9)
         H = max(hi[2] - lo[2], 1e-9)
-        F[a:b, 0] = (Xc[:, 0] - c[0]) / R
-        F[a:b, 1] = (Xc[:, 1] - c[1]) / R
-        F[a:b, 2] = (Xc[:, 2] - lo[2]) / H
-        fe = spec.fext[a:b]
+        F[a0:b0, 0] = (Xc[:, 0] - c[0]) / ax
+        F[a0:b0, 1] = (Xc[:, 1] - c[1]) / by
+        F[a0:b0, 2] = (Xc[:, 2] - lo[2]) / H
+        fe = spec.fext[a0:b0]
         tot = fe.sum(0)
         loaded = np.abs(fe).sum(1) > 0
         nrm = np.linalg.norm(tot)
-        F[a:b, 8] = loaded
+        F[a0:b0, 8] = loaded
         if nrm > 0:
-            F[a:b, 9:12] = tot / nrm
-            # resultant pressure: |sum f| * V / loaded area (area = n_loaded dx^2)
+            F[a0:b0, 9:12] = tot / nrm
+            # resultant pressure: |sum f| V / loaded area (area = n_loaded dx^2)
             p = nrm * spec.dx ** 3 / max(loaded.sum() * spec.dx ** 2, 1e-30)
-            F[a:b, 12] = np.log10(max(p, 1e-12))
-        F[a:b, 13] = H / 25.0
-        F[a:b, 14] = R / 20.0
-    F[:, 3] = np.log10(spec.E / 1000.0)
-    F[:, 4] = (spec.material_id == 0) if spec.material_id is not None else 0.0
+            F[a0:b0, 12] = np.log10(max(p, 1e-12))
+        F[a0:b0, 13] = H / 25.0
+        F[a0:b0, 14] = ax / 20.0
+        F[a0:b0, 15] = by / 20.0
+        F[a0:b0, 16] = logE[a0:b0].mean()
+        F[a0:b0, 17] = cort[a0:b0].mean()
+    F[:, 3] = logE
+    F[:, 4] = cort
     for q in range(3):
         F[:, 5 + q] = (spec.fixed >> q) & 1
-    F[:, 15] = 1.0
+    F[:, 18] = 1.0
     return F
 
 

@@ -35,7 +35,7 @@ def _group_body(specs):
 
 
 def generate(seeds, dx, group, tol, max_iterations):
-    """Features, targets and weights of the cases ``seeds`` (host float32)."""
+    """Features, targets and weights of the cases ``seeds`` at spacing dx (host float32)."""
     from ..workloads import random_vertebra
     from .features import feature_matrix, target_scale
     Fs, Ts, Ws, stats = [], [], [], []
@@ -138,7 +138,8 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=1024)
     ap.add_argument("--heldout", type=int, default=64)
-    ap.add_argument("--dx", type=float, default=0.8)
+    ap.add_argument("--dx", type=float, nargs="+", default=[0.8, 0.6],
+                    help="training spacings; the cases are split evenly over them")
     ap.add_argument("--group", type=int, default=64)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iterations", type=int, default=60000)
@@ -157,22 +158,28 @@ def main(argv=None):
     t_all = time.perf_counter()
     train_seeds = np.arange(args.seed, args.seed + args.train)
     held_seeds = np.arange(args.seed + 100000, args.seed + 100000 + args.heldout)
-    log(f"generating {args.train} training cases at dx {args.dx}")
-    F, T, Wt, gstats = generate(train_seeds, args.dx, args.group, args.tol, args.max_iterations)
+    parts = np.array_split(train_seeds, len(args.dx))
+    Fs, Ts, Ws, gstats = [], [], [], []
+    for dx, seeds in zip(args.dx, parts):
+        log(f"generating {len(seeds)} training cases at dx {dx}")
+        F_, T_, W_, g_ = generate(seeds, dx, args.group, args.tol, args.max_iterations)
+        Fs.append(F_), Ts.append(T_), Ws.append(W_), gstats.extend(g_)
+    F, T, Wt = np.concatenate(Fs), np.concatenate(Ts), np.concatenate(Ws)
+    del Fs, Ts, Ws
     log(f"training set: {len(F)} particles, {int(Wt.sum())} free")
     net = WarmStartNet(seed=args.seed)
     hist = train(net, F, T, Wt, args.epochs, args.batch_rows, args.lr, args.lr_min)
     del F, T, Wt
     if args.out:
-        net.save(args.out, dx=args.dx, train_cases=args.train, epochs=args.epochs, final_loss=hist[-1])
+        net.save(args.out, dx=np.array(args.dx), train_cases=args.train, epochs=args.epochs, final_loss=hist[-1])
         log(f"weights -> {args.out}")
     log(f"evaluating on {args.heldout} held-out cases")
-    ev = evaluate(net, held_seeds, args.dx, args.group, args.tol, args.max_iterations)
+    ev = evaluate(net, held_seeds, args.dx[0], args.group, args.tol, args.max_iterations)
     cold = sum(e["cold_s"] for e in ev)
     warm = sum(e["warm_s"] for e in ev)
     ic = [i for e in ev for i in e["cold_iterations"]]
     iw = [i for e in ev for i in e["warm_iterations"]]
-    rep = {"train_cases": args.train, "heldout_cases": args.heldout, "dx": args.dx, "epochs": args.epochs,
+    rep = {"train_cases": args.train, "heldout_cases": args.heldout, "dx": list(args.dx), "epochs": args.epochs,
            "loss_history": hist, "data_groups": gstats, "heldout": ev,
            "summary": {"cold_s": cold, "warm_s": warm, "speedup_wall": cold / max(warm, 1e-9),
                        "cold_iterations_mean": float(np.mean(ic)), "warm_iterations_mean": float(np.mean(iw)),
