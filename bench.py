@@ -33,14 +33,27 @@ sys.path.insert(0, ROOT)
 
 METRIC = "PD bond-updates/s; time-to-solve at rel. residual 1e-6 (plain vs warm-start)"
 HBM_PEAK_GBS = 8000.0
-VARIANT_COOP = 16  # PDX_VARIANT_COOP of include/pdx.h
+VARIANT_COOP, VARIANT_HALO = 16, 32  # PDX_VARIANT_COOP / PDX_VARIANT_HALO of include/pdx.h
+VARIANT_SPLIT2, VARIANT_SPLIT4 = 64, 128
 
 
 def variant_names(v):
     """(label, F-pass kernel, force kernel) of a hot-pass variant code."""
+    if v & VARIANT_HALO:
+        split = 4 if v & VARIANT_SPLIT4 else (2 if v & VARIANT_SPLIT2 else 1)
+        return f"LDS-staged halo, bond group {v & 15}, rows split over {split} waves", "k_fpass_h", "k_force_h"
     if v & VARIANT_COOP:
         return f"cooperative gathers, bond group {v & ~VARIANT_COOP}", "k_fpass_w", "k_force_w"
     return f"per-lane gathers, bond group {v}", "k_fpass", "k_force"
+
+
+def pass_variants(body):
+    """(label, F-pass kernel, force kernel) of the variants a body runs."""
+    fv, gv = body.variants()
+    if fv == gv:
+        return variant_names(fv)
+    return (f"F-pass: {variant_names(fv)[0]}; force: {variant_names(gv)[0]}", variant_names(fv)[1],
+            variant_names(gv)[2])
 
 
 def log(*a):
@@ -61,7 +74,7 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--kernel-iters", type=int, default=50)
     ap.add_argument("--variant", type=int, default=0,
-                    help="hot-pass variant (1, 2, 4, 17, 18); 0 = autotune")
+                    help="hot-pass variant (1, 2, 4, 17, 18, 33, 34, 97, 98, 161, 162); 0 = autotune")
     ap.add_argument("--workload", default="vertebra", choices=["vertebra", "vertebra_j2", "batch"],
                     help="vertebra: the headline (dx 0.4, Neo-Hookean, pressure); vertebra_j2: J2, 20 "
                          "prescribed increments with breaking (dx 0.25); batch: 64 random cases in one context")
@@ -106,11 +119,10 @@ def main():
     cold = body.cold_residual()
     body.u = spec.u0
     if args.variant:
-        group = args.variant
-        body.set_group(group)
+        body.set_group(args.variant)
     else:
-        group = body.autotune(10)
-    vlabel, kf_name, kg_name = variant_names(group)
+        body.autotune(10)
+    vlabel, kf_name, kg_name = pass_variants(body)
     log(f"hot-pass variant: {vlabel}")
     # warmup + timed steps: fixed-count iterations that never stop early (tol = 0)
     body.u = spec.u0
@@ -221,8 +233,10 @@ def run_loading(args, torch):
     setup = time.perf_counter() - t0
     nb0 = body.stats()["n_intact"]
     log(f"{spec.name}: {spec.n} particles, {nb0} bonds, setup {setup:.2f}s")
-    group = args.variant or body.autotune(5)
-    body.set_group(group)
+    if args.variant:
+        body.set_group(args.variant)
+    else:
+        body.autotune(5)
     drv = IncrementalLoading(body, spec.extra["top_idx"], spec.extra["u_final"], spec.extra["n_increments"],
                              critical_stretch=spec.extra["critical_stretch"], tol=args.tol,
                              max_iterations=args.max_solve_iters,
@@ -241,7 +255,7 @@ def run_loading(args, torch):
     upd = nb0 * iters  # intact bonds at the start; broken bonds are a small fraction
     _line(METRIC, upd / wall, "bond-updates/s", iters, 0, 1e3 * wall / max(iters, 1),
           {"workload": spec.name, "n_particles": spec.n, "n_bonds": nb0, "increments": len(res),
-           "critical_stretch": spec.extra["critical_stretch"], "variant": variant_names(group)[0]},
+           "critical_stretch": spec.extra["critical_stretch"], "variant": pass_variants(body)[0]},
           time_to_solve={"loading_s": round(wall, 3), "setup_s": round(setup, 3),
                          "all_converged": all(r.converged for r in res),
                          "increments": [{"k": r.increment, "iterations": r.iterations,
@@ -269,8 +283,10 @@ def run_batch(args, torch):
     log(f"{packed.name}: {len(specs)} cases, {packed.n} particles, {nb} bonds, setup {setup:.2f}s")
     # ref = 0: each case is normalised by its own residual at the start state,
     # which for u0 is the cold-start residual
-    group = args.variant or body.autotune(5)
-    body.set_group(group)
+    if args.variant:
+        body.set_group(args.variant)
+    else:
+        body.autotune(5)
     body.u = packed.u0
     body.solve("adaptive", tol=0.0, max_iterations=args.warmup, ref=0.0, chunk=64)
     torch.cuda.synchronize()
@@ -301,7 +317,7 @@ def run_batch(args, torch):
             f"{tts['converged']}/{len(rs)} converged")
     _line(METRIC, nb * steps / dt, "bond-updates/s", steps, args.warmup, 1e3 * dt / steps,
           {"workload": packed.name, "cases": len(specs), "n_particles": packed.n, "n_bonds": nb,
-           "variant": variant_names(group)[0]},
+           "variant": pass_variants(body)[0]},
           time_to_solve=tts, roofline=None, cpu_baseline=None)
 
 

@@ -201,15 +201,31 @@ int pdx_set(pdx_ctx *ctx, int32_t field, const void *in);
 int pdx_time_kernels(pdx_ctx *ctx, int32_t iters, double *ms);
 /* hot-pass variants: per-lane neighbour gathers with bond groups 1, 2, 4, or
  * wave-cooperative gathers (four lanes fetch one 64-byte neighbour record, staged
- * through LDS) with bond groups 1, 2 as PDX_VARIANT_COOP | group.  All variants give
- * the same bits. */
+ * through LDS) with bond groups 1, 2 as PDX_VARIANT_COOP | group.  These variants
+ * give the same bits.  PDX_VARIANT_HALO | group (1, 2): each block stages the records
+ * of every particle its rows reach into LDS once, positions relative to a per-block
+ * anchor in FP32, and reads its bonds from LDS; results agree with the others to FP32
+ * rounding of the bond vectors (about 1e-7 relative).  A pass whose halo does not fit
+ * the device's LDS runs the PDX_VARIANT_COOP | 2 kernel instead. */
 #define PDX_VARIANT_COOP 16
+#define PDX_VARIANT_HALO 32
+/* with PDX_VARIANT_HALO: each row's bond slots split over 2 or 4 waves that share the
+ * block's halo (partial sums added in a fixed order) */
+#define PDX_VARIANT_SPLIT2 64
+#define PDX_VARIANT_SPLIT4 128
 /* launch tuning (no reference counterpart): times `iters` adaptive-DR iterations of
- * every hot-pass variant on this context's own data, keeps the fastest and leaves
- * the solver state as it was; group_out = chosen variant */
+ * every hot-pass variant on this context's own data, keeps the fastest for each pass
+ * and leaves the solver state as it was; group_out = the force pass's variant */
 int pdx_autotune(pdx_ctx *ctx, int32_t iters, int32_t *group_out);
-/* sets the hot-pass variant (1, 2, 4, PDX_VARIANT_COOP | 1, PDX_VARIANT_COOP | 2) */
+/* sets the hot-pass variant (1, 2, 4, PDX_VARIANT_COOP | 1 or 2, PDX_VARIANT_HALO | 1 or 2
+ * optionally | PDX_VARIANT_SPLIT2 or PDX_VARIANT_SPLIT4; the halo variants need the bonds
+ * built) */
 int pdx_set_group(pdx_ctx *ctx, int32_t group);
+/* launch tuning (no reference counterpart): the F-pass and the force pass are separate
+ * launches and may use different variants; pdx_autotune keeps the fastest of each and
+ * reports the force pass's in group_out */
+int pdx_set_variants(pdx_ctx *ctx, int32_t fpass_variant, int32_t force_variant);
+int pdx_get_variants(pdx_ctx *ctx, int32_t *fpass_variant, int32_t *force_variant);
 /* n_particles, n_bonds (intact+broken), n_intact, sell_slots (padded bond slots) */
 int pdx_stats(pdx_ctx *ctx, int64_t *out4);
 

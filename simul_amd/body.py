@@ -285,6 +285,16 @@ class PDBody:
     def set_group(self, g):
         self.ctx.call("pdx_set_group", int(g))
 
+    def set_variants(self, fpass, force):
+        """Hot-pass variants of the F-pass and the force pass (pdx_set_variants)."""
+        self.ctx.call("pdx_set_variants", int(fpass), int(force))
+
+    def variants(self):
+        """(F-pass variant, force-pass variant) in use."""
+        f, g = C.c_int32(), C.c_int32()
+        self.ctx.call("pdx_get_variants", C.byref(f), C.byref(g))
+        return f.value, g.value
+
     def stats(self):
         o = np.zeros(4, np.int64)
         self.ctx.call("pdx_stats", N.ptr(o, N._i64p))

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <chrono>
 #include <numeric>
+#include <thread>
 
 #include "pdx_internal.h"
 #include "pdx_layout.h"
@@ -68,6 +69,7 @@ static void free_all(pdx_ctx *c)
     dfree(d.mass); dfree(d.lam); dfree(d.V64); dfree(d.K64); dfree(d.Kinv64);
     dfree(d.Fd); dfree(d.Pd); dfree(d.damage); dfree(d.init_nb);
     dfree(d.sl_ptr); dfree(d.bj); dfree(d.case_of_blk); dfree(d.case_blk0);
+    dfree(d.hl_ptr); dfree(d.hl); dfree(d.bl); dfree(d.anchor);
     dfree(d.sc); dfree(d.part); dfree(d.grid); dfree(d.case_of_p);
     dfree(d.cell_start); dfree(d.cell_idx); dfree(d.counts);
     if (c->gexec) hipGraphExecDestroy(c->gexec);
@@ -124,6 +126,13 @@ static int create_common(int32_t ncase, const int64_t *off, int32_t dim, pdx_ctx
     c->ncase = ncase;
     c->case_off.assign(off, off + ncase + 1);
     hipGetDevice(&c->device);
+    {
+        // dynamic LDS of the halo passes: the per-block limit less the
+        // static LDS of the block reduction (and a margin)
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess) lds = 0;
+        c->lds_dyn_max = std::max(0, lds - 1024);
+    }
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -576,6 +585,95 @@ extern "C" int pdx_count_neighbors(pdx_ctx *c, double horizon, int32_t *counts)
     return PDX_OK;
 }
 
+// halo tables of the LDS-staged hot passes (Dev::hl_ptr, hl, bl, anchor):
+// per block, its own PDX_BLOCK particles, then every other particle its rows
+// reach in index order; each bond slot's position in that list; and the
+// centre of the block's own particles as the FP64 anchor.  Blocks are built
+// in parallel host threads.  A halo above PDX_HALO_LIMIT entries leaves the
+// tables unbuilt (the halo variants then run the cooperative kernels).
+static int build_halo(pdx_ctx *c)
+{
+    Dev &d = c->d;
+    dfree(d.hl_ptr); dfree(d.hl); dfree(d.bl); dfree(d.anchor);
+    d.halo_cap = 0;
+    invalidate_graph(c);
+    const int nbk = c->ni / PDX_BLOCK, D = c->dim;
+    if (nbk == 0 || c->slots <= 0) return PDX_OK;
+    std::vector<uint32_t> bj;
+    PDX_TRY(download(c, bj, d.bj, (size_t)c->slots));
+    std::vector<std::vector<int32_t>> other(nbk);
+    std::vector<uint16_t> bl((size_t)c->slots, (uint16_t)PDX_HNONE);
+    std::vector<pdx_d4> anc(nbk);
+    constexpr int SPB = PDX_BLOCK / PDX_SLICE;
+    auto work = [&](int b0, int b1) {
+        for (int b = b0; b < b1; ++b) {
+            const int i0 = b * PDX_BLOCK;
+            const int64_t s_lo = c->h_slptr[(size_t)b * SPB], s_hi = c->h_slptr[(size_t)(b + 1) * SPB];
+            std::vector<int32_t> &o = other[b];
+            for (int64_t s = s_lo; s < s_hi; ++s) {
+                const uint32_t e = bj[s];
+                if (e == 0xffffffffu) continue;
+                const int32_t j = (int32_t)(e & ~PDX_BROKEN);
+                if (j - i0 < 0 || j - i0 >= PDX_BLOCK) o.push_back(j);
+            }
+            std::sort(o.begin(), o.end());
+            o.erase(std::unique(o.begin(), o.end()), o.end());
+            if (PDX_BLOCK + (int64_t)o.size() > PDX_HALO_LIMIT) continue;
+            for (int64_t s = s_lo; s < s_hi; ++s) {
+                const uint32_t e = bj[s];
+                if (e == 0xffffffffu) continue;
+                const int32_t j = (int32_t)(e & ~PDX_BROKEN);
+                int32_t l = j - i0;
+                if (l < 0 || l >= PDX_BLOCK) l = PDX_BLOCK + (int32_t)(std::lower_bound(o.begin(), o.end(), j) - o.begin());
+                bl[s] = (uint16_t)(l | ((e & PDX_BROKEN) ? PDX_HBROKEN : 0u));
+            }
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            for (int t = 0; t < PDX_BLOCK; ++t) {
+                const int32_t r = c->perm[i0 + t];
+                if (r < 0) continue;
+                for (int q = 0; q < D; ++q) {
+                    lo[q] = std::min(lo[q], c->hX[(size_t)r * D + q]);
+                    hi[q] = std::max(hi[q], c->hX[(size_t)r * D + q]);
+                }
+            }
+            double m[3] = {0.0, 0.0, 0.0};
+            for (int q = 0; q < D; ++q) m[q] = lo[q] <= hi[q] ? 0.5 * (lo[q] + hi[q]) : 0.0;
+            anc[b] = d4(m[0], m[1], m[2]);
+        }
+    };
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int k = 0; k < nth; ++k) {
+        const int b0 = (int)((int64_t)nbk * k / nth), b1 = (int)((int64_t)nbk * (k + 1) / nth);
+        th.emplace_back(work, b0, b1);
+    }
+    for (auto &x : th) x.join();
+    std::vector<int32_t> ptr(nbk + 1, 0);
+    int64_t hmax = 0;
+    for (int b = 0; b < nbk; ++b) {
+        const int64_t nh = PDX_BLOCK + (int64_t)other[b].size();
+        if (nh > PDX_HALO_LIMIT) return PDX_OK;
+        hmax = std::max(hmax, nh);
+        ptr[b + 1] = ptr[b] + (int32_t)nh;
+    }
+    std::vector<int32_t> hl((size_t)ptr[nbk]);
+    for (int b = 0; b < nbk; ++b) {
+        int32_t *o = hl.data() + ptr[b];
+        for (int t = 0; t < PDX_BLOCK; ++t) o[t] = b * PDX_BLOCK + t;
+        std::copy(other[b].begin(), other[b].end(), o + PDX_BLOCK);
+    }
+    PDX_TRY(dalloc(c->stream, &d.hl_ptr, (size_t)nbk + 1));
+    PDX_TRY(dalloc(c->stream, &d.hl, hl.size()));
+    PDX_TRY(dalloc(c->stream, &d.bl, bl.size()));
+    PDX_TRY(dalloc(c->stream, &d.anchor, anc.size()));
+    PDX_TRY(upload(c, d.hl_ptr, ptr));
+    PDX_TRY(upload(c, d.hl, hl));
+    PDX_TRY(upload(c, d.bl, bl));
+    PDX_TRY(upload(c, d.anchor, anc));
+    d.halo_cap = (int32_t)((hmax + 3) & ~(int64_t)3);
+    return PDX_OK;
+}
+
 extern "C" int pdx_build_bonds(pdx_ctx *c, double horizon, int64_t *n_bonds)
 {
     CHECK_CTX(c);
@@ -609,6 +707,7 @@ extern "C" int pdx_build_bonds(pdx_ctx *c, double horizon, int64_t *n_bonds)
         memcpy(&w[i], &v, 4);
     }
     PDX_TRY(kr_put(c, w, KR_OFF_NNB, 1));
+    PDX_TRY(build_halo(c));
     c->horizon = horizon;
     c->have_bonds = true;
     c->have_K = c->have_mass = c->have_A = false;
@@ -666,6 +765,8 @@ extern "C" int pdx_set_broken(pdx_ctx *c, const uint8_t *broken)
         bj[rm.slot[t]] = broken[t] ? (e | PDX_BROKEN) : e;
     }
     PDX_TRY(upload(c, c->d.bj, bj));
+    PDX_TRY(pdx_launch_halo_bits(c));
+    PDX_HIP(hipStreamSynchronize(c->stream));
     c->have_mass = false;
     return PDX_OK;
 }
@@ -1141,13 +1242,42 @@ extern "C" int pdx_time_kernels(pdx_ctx *c, int32_t iters, double *ms)
     return PDX_OK;
 }
 
+static bool valid_variant(int32_t g)
+{
+    bool ok = g == 1 || g == 2 || g == 4 || g == (PDX_VARIANT_COOP | 1) || g == (PDX_VARIANT_COOP | 2);
+#define PDX_HALO_OK(v) ok = ok || g == (v);
+    PDX_HALO_VARIANTS(PDX_HALO_OK)
+#undef PDX_HALO_OK
+    return ok;
+}
+
+static const char *k_variant_msg = "variant must be 1, 2, 4, PDX_VARIANT_COOP | 1 or 2, or PDX_VARIANT_HALO | 1 or 2 "
+                                   "with optional PDX_VARIANT_SPLIT2 or PDX_VARIANT_SPLIT4";
+
 extern "C" int pdx_set_group(pdx_ctx *c, int32_t g)
 {
     CHECK_CTX(c);
-    if (g != 1 && g != 2 && g != 4 && g != (PDX_VARIANT_COOP | 1) && g != (PDX_VARIANT_COOP | 2))
-        return fail(PDX_EINVAL, "variant must be 1, 2, 4, PDX_VARIANT_COOP|1 or PDX_VARIANT_COOP|2");
-    c->tune_g = g;
+    if (!valid_variant(g)) return fail(PDX_EINVAL, k_variant_msg);
+    c->tune_f = c->tune_g = g;
     invalidate_graph(c);
+    return PDX_OK;
+}
+
+extern "C" int pdx_set_variants(pdx_ctx *c, int32_t fpass, int32_t force)
+{
+    CHECK_CTX(c);
+    if (!valid_variant(fpass) || !valid_variant(force)) return fail(PDX_EINVAL, k_variant_msg);
+    c->tune_f = fpass;
+    c->tune_g = force;
+    invalidate_graph(c);
+    return PDX_OK;
+}
+
+extern "C" int pdx_get_variants(pdx_ctx *c, int32_t *fpass, int32_t *force)
+{
+    CHECK_CTX(c);
+    if (fpass) *fpass = c->tune_f;
+    if (force) *force = c->tune_g;
     return PDX_OK;
 }
 
@@ -1168,21 +1298,31 @@ extern "C" int pdx_autotune(pdx_ctx *c, int32_t iters, int32_t *group_out)
     PDX_TRY(read_sc(c));
     std::vector<DScal> sc0(c->h_sc, c->h_sc + c->ncase);
     const int u0 = c->ucur;
-    double best = 1e300;
-    int best_g = c->tune_g;
-    for (int g : {1, 2, 4, PDX_VARIANT_COOP | 1, PDX_VARIANT_COOP | 2}) {
-        c->tune_g = g;
+    double best_fp = 1e300, best_fo = 1e300;
+    int best_f = c->tune_f, best_g = c->tune_g;
+    std::vector<int> cand = {1, 2, 4, PDX_VARIANT_COOP | 1, PDX_VARIANT_COOP | 2};
+#define PDX_HALO_CAND(v) cand.push_back(v);
+    if (c->d.hl_ptr) { PDX_HALO_VARIANTS(PDX_HALO_CAND) }
+#undef PDX_HALO_CAND
+    for (int g : cand) {
+        c->tune_f = c->tune_g = g;
         double ms[3];
         int r = pdx_time_kernels(c, iters, ms);
         if (r != PDX_OK) {
             hipFree(save);
             return r;
         }
-        if (ms[0] + ms[1] < best) {
-            best = ms[0] + ms[1];
+        // the passes are separate launches: keep the fastest of each
+        if (ms[0] < best_fp) {
+            best_fp = ms[0];
+            best_f = g;
+        }
+        if (ms[1] < best_fo) {
+            best_fo = ms[1];
             best_g = g;
         }
     }
+    c->tune_f = best_f;
     c->tune_g = best_g;
     c->ucur = u0;
     PDX_HIP(hipMemcpyAsync(c->d.xu[c->ucur], save_x, nx, hipMemcpyDeviceToDevice, c->stream));

@@ -27,9 +27,15 @@
 #define PDX_HD inline
 #endif
 
+#ifndef PDX_BLOCK
 #define PDX_BLOCK 256
+#endif
 #define PDX_SLICE 64
 #define PDX_BROKEN 0x80000000u
+// halo-local bond entries (16 bits): position in the block's halo, broken flag
+#define PDX_HBROKEN 0x8000u
+#define PDX_HNONE 0xFFFFu
+#define PDX_HALO_LIMIT 0x7FFF
 #define PDX_NXCD 8
 
 // flags word of KRec
@@ -129,6 +135,17 @@ struct Dev {
     int32_t *init_nb;  // bonds at build time
     int64_t *sl_ptr;
     uint32_t *bj;
+    // halo tables of the LDS-staged hot passes (null when not built): per
+    // block the particles its rows reach (hl_ptr / hl: own particles first,
+    // then the others in index order), the bond's position in that list in
+    // the SELL layout of bj (bl: PDX_HBROKEN set for a broken bond, 0xFFFF
+    // for no bond) and the FP64 anchor the staged positions are relative to
+    int32_t *hl_ptr;
+    int32_t *hl;
+    uint16_t *bl;
+    pdx_d4 *anchor;
+    int32_t halo_cap;  // LDS entries per block: max halo size rounded up to 4
+    int32_t halo_pad;
     int32_t *case_of_blk;
     int32_t *case_blk0;  // first block of each case (ncase+1)
     DScal *sc;
@@ -292,6 +309,19 @@ PDX_HD float bond_weight(float L, float invh, float Vj)
 // F-pass term of one bond: Hd += w V_j du (x) xi (FP32 product, FP64 sum).
 // H is accumulated in FP64: a 100+-term FP32 sum carries ~1e-6 relative error
 // into every A_j and from there into the residual.
+// (bond vector xi, displacement difference du, neighbour volume Vj)
+template <int D>
+PDX_HD void fpass_bond_v(const float *xi, const float *du, float Vj, float invh, bool live, double *Hd)
+{
+    float L2 = 0.f;
+    for (int r = 0; r < D; ++r) L2 = fmaf(xi[r], xi[r], L2);
+    const float w = live ? bond_weight(sqrtf(L2), invh, Vj) : 0.f;
+    for (int r = 0; r < D; ++r) {
+        const double wd = (double)(w * du[r]);
+        for (int c = 0; c < D; ++c) Hd[r * D + c] = fma(wd, (double)xi[c], Hd[r * D + c]);
+    }
+}
+
 template <int D>
 PDX_HD void fpass_bond(const XRec &pi, const XRec &pj, float invh, bool live, double *Hd)
 {
@@ -300,13 +330,7 @@ PDX_HD void fpass_bond(const XRec &pi, const XRec &pj, float invh, bool live, do
         xi[r] = (float)(pj.X[r] - pi.X[r]);
         du[r] = (float)(pj.u[r] - pi.u[r]);
     }
-    float L2 = 0.f;
-    for (int r = 0; r < D; ++r) L2 = fmaf(xi[r], xi[r], L2);
-    const float w = live ? bond_weight(sqrtf(L2), invh, pj.V) : 0.f;
-    for (int r = 0; r < D; ++r) {
-        const double wd = (double)(w * du[r]);
-        for (int c = 0; c < D; ++c) Hd[r * D + c] = fma(wd, (double)xi[c], Hd[r * D + c]);
-    }
+    fpass_bond_v<D>(xi, du, pj.V, invh, live, Hd);
 }
 
 // entry of bond slot k of a row, or "no bond" past the row's end
@@ -418,14 +442,9 @@ PDX_HD void fpass_finish(const Dev &d, int i, const KRec &kr, const double *Hd)
 // sum_j w (A_i + A_j) xi = sum_j w (A_j - A_i) xi + 2 A_i m_i, so each FP32
 // bond product is rounded relative to |A_j - A_i| rather than |A|.
 template <int D, int FORM>
-PDX_HD void force_bond(const XRec &pi, const XRec &pj, const float *Ai, const float *Aj, float invh, float Gs,
-                       bool live, double *fa)
+PDX_HD void force_bond_v(const float *xi, const float *du, float Vj, float ci, float cj, const float *Ai,
+                         const float *Aj, float invh, float Gs, bool live, double *fa)
 {
-    float xi[3], du[3];
-    for (int r = 0; r < 3; ++r) {
-        xi[r] = (float)(pj.X[r] - pi.X[r]);
-        du[r] = (float)(pj.u[r] - pi.u[r]);
-    }
     float L2 = 0.f, xd = 0.f, dd = 0.f;
     for (int r = 0; r < D; ++r) {
         L2 = fmaf(xi[r], xi[r], L2);
@@ -433,7 +452,7 @@ PDX_HD void force_bond(const XRec &pi, const XRec &pj, const float *Ai, const fl
         dd = fmaf(du[r], du[r], dd);
     }
     const float L = sqrtf(L2);
-    const float w = bond_weight(L, invh, pj.V);
+    const float w = bond_weight(L, invh, Vj);
     float eta[3];
     for (int r = 0; r < D; ++r) eta[r] = xi[r] + du[r];
     // |eta|^2 = |xi|^2 + (2 xi.du + du.du); the stretch uses the
@@ -448,14 +467,26 @@ PDX_HD void force_bond(const XRec &pi, const XRec &pj, const float *Ai, const fl
             for (int c = 0; c < D; ++c) t = fmaf(Aj[r * D + c] - Ai[r * D + c], xi[c], t);
             tb[r] = t;
         }
-        if (el > 1e-10f && L > 1e-10f) g = Gs * 0.5f * (pi.c + pj.c) * (dn / ((el + L) * L)) / el;
+        if (el > 1e-10f && L > 1e-10f) g = Gs * 0.5f * (ci + cj) * (dn / ((el + L) * L)) / el;
     } else if (el > 1e-10f) {
-        g = 0.5f * (pi.c + pj.c) * (dn / ((el + L) * L)) / el;
+        g = 0.5f * (ci + cj) * (dn / ((el + L) * L)) / el;
     }
     for (int r = 0; r < D; ++r) {
         const float contrib = w * fmaf(g, eta[r], tb[r]);
         fa[r] += live ? (double)contrib : 0.0;
     }
+}
+
+template <int D, int FORM>
+PDX_HD void force_bond(const XRec &pi, const XRec &pj, const float *Ai, const float *Aj, float invh, float Gs,
+                       bool live, double *fa)
+{
+    float xi[3], du[3];
+    for (int r = 0; r < 3; ++r) {
+        xi[r] = (float)(pj.X[r] - pi.X[r]);
+        du[r] = (float)(pj.u[r] - pi.u[r]);
+    }
+    force_bond_v<D, FORM>(xi, du, pj.V, pi.c, pj.c, Ai, Aj, invh, Gs, live, fa);
 }
 
 template <int D, int FORM, int UPD>
@@ -969,6 +1000,7 @@ PDX_HD unsigned damage_one(const Dev &d, int i, int ubuf, double s_c)
         if (L > 1e-10) st = (el - L) / L;
         if (st > s_c) {
             d.bj[a] = e | PDX_BROKEN;
+            if (d.bl) d.bl[a] |= PDX_HBROKEN;
             nb++;
         }
     }

@@ -27,7 +27,12 @@ struct pdx_ctx {
     double horizon = 0.0, G_s = 0.0;
     int64_t nb = 0, slots = 0;
     int ucur = 0;
-    int tune_g = 2;  // hot-pass variant: bond group 1, 2, 4 or PDX_VARIANT_COOP | 1, 2
+    // hot-pass variants of the force pass (tune_g) and the F-pass (tune_f):
+    // bond group 1, 2, 4, PDX_VARIANT_COOP | 1, 2 or a PDX_HALO_VARIANTS code
+    // (default: the LDS-staged kernels, which fall back to PDX_VARIANT_COOP | 2
+    // where a halo does not fit)
+    int tune_g = PDX_VARIANT_HALO | PDX_VARIANT_SPLIT4 | 2, tune_f = PDX_VARIANT_HALO | PDX_VARIANT_SPLIT2 | 2;
+    int64_t lds_dyn_max = 0;  // dynamic LDS bytes a block may use on this device
     Dev d{};
     // host mirror of slice pointers
     std::vector<int64_t> h_slptr;
@@ -57,6 +62,14 @@ int pdx_launch_shape(pdx_ctx *c, int mode, double tol);
 int pdx_launch_fpass(pdx_ctx *c, int ubuf, bool diag, bool commit, bool solve_mode);
 int pdx_launch_force(pdx_ctx *c, int ubuf, int upd, bool solve_mode, double dt, double visc);
 int pdx_launch_kd_move(pdx_ctx *c, int ubuf_in, bool solve_mode, double dt, bool apply_reset);
+int pdx_launch_halo_bits(pdx_ctx *c);
+int pdx_halo_lds(const pdx_ctx *c, int entry_bytes, int split, int nv);
+
+// the halo-staged variants: bond group 1 or 2, rows split over 1, 2 or 4 waves
+#define PDX_HALO_VARIANTS(X)                                                                   \
+    X(PDX_VARIANT_HALO | 1) X(PDX_VARIANT_HALO | 2) X(PDX_VARIANT_HALO | PDX_VARIANT_SPLIT2 | 1) \
+    X(PDX_VARIANT_HALO | PDX_VARIANT_SPLIT2 | 2) X(PDX_VARIANT_HALO | PDX_VARIANT_SPLIT4 | 1)     \
+    X(PDX_VARIANT_HALO | PDX_VARIANT_SPLIT4 | 2)
 int pdx_launch_resid(pdx_ctx *c);
 int pdx_launch_zero_free(pdx_ctx *c, int ubuf);
 int pdx_launch_kr_put(pdx_ctx *c, const float *d_src, int off, int cnt);

@@ -38,10 +38,10 @@ __device__ __forceinline__ double ld_coherent(const double *p)
 // Block sum of NV doubles into partials[lb]; returns true in the block that
 // arrives last for its case, with tot[] = case totals summed in block order
 // (a fixed order: the result does not depend on block scheduling).
-template <int NV>
+template <int NV, int NT = PDX_BLOCK>
 __device__ bool reduce_case(const Dev &d, int lb, int cs, double (&val)[NV], double (&tot)[NV])
 {
-    __shared__ double sred[PDX_BLOCK / 64][NV];
+    __shared__ double sred[NT / 64][NV];
     __shared__ int s_last;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -54,7 +54,7 @@ __device__ bool reduce_case(const Dev &d, int lb, int cs, double (&val)[NV], dou
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
             double t = 0.0;
-            for (int w = 0; w < PDX_BLOCK / 64; ++w) t += sred[w][q];
+            for (int w = 0; w < NT / 64; ++w) t += sred[w][q];
             d.part[(int64_t)lb * 4 + q] = t;
         }
         __threadfence();
@@ -69,13 +69,13 @@ __device__ bool reduce_case(const Dev &d, int lb, int cs, double (&val)[NV], dou
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         double t = 0.0;
-        for (int b = b0 + threadIdx.x; b < b1; b += PDX_BLOCK) t += ld_coherent(&d.part[(int64_t)b * 4 + q]);
+        for (int b = b0 + threadIdx.x; b < b1; b += NT) t += ld_coherent(&d.part[(int64_t)b * 4 + q]);
         t = wave_sum(t);
         __syncthreads();
         if (lane == 0) sred[wid][q] = t;
         __syncthreads();
         double s = 0.0;
-        for (int w = 0; w < PDX_BLOCK / 64; ++w) s += sred[w][q];
+        for (int w = 0; w < NT / 64; ++w) s += sred[w][q];
         tot[q] = s;
     }
     if (threadIdx.x == 0) d.sc[cs].counter = 0u;
@@ -282,6 +282,17 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_force_w(Dev d, int ubuf, int solv
         // A records have 3 quarters; lanes of quarter 3 fetch quarter 2 again
         // (same cache line, keeps the loads unconditional)
         const int qa3 = qa < 3 ? qa : 2;
+#ifdef PDX_A_FLAT
+        // 64 bonds x 3 quarters in 3 full-wave loads: load h of lane L
+        // fetches quarter t % 3 of bond t / 3, t = 64 h + L
+        int ab[3], aq[3];
+#pragma unroll
+        for (int h = 0; h < 3; ++h) {
+            const int t = 64 * h + lane;
+            ab[h] = t / 3;
+            aq[h] = t - 3 * ab[h];
+        }
+#endif
         uint32_t ec[G], en[G];
 #pragma unroll
         for (int q = 0; q < G; ++q) ec[q] = q < width ? bp[(int64_t)q * PDX_SLICE] : PDX_BROKEN;
@@ -291,20 +302,38 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_force_w(Dev d, int ubuf, int solv
                 en[q] = (k + G + q) < width ? bp[(int64_t)(k + G + q) * PDX_SLICE] : PDX_BROKEN;
             WSlot sl[G];
             int jb[G][4];
-            pdx_q4 g[G][4], ga[G][4];
+            pdx_q4 g[G][4];
+#ifdef PDX_A_FLAT
+            int ja[G][3];
+            pdx_q4 ga[G][3];
+#else
+            pdx_q4 ga[G][4];
+#endif
 #pragma unroll
             for (int q = 0; q < G; ++q) {
                 sl[q] = wslot(d, ec[q], k + q, nnb, act, i);
 #pragma unroll
                 for (int h = 0; h < 4; ++h) jb[q][h] = __shfl(sl[q].j, 16 * h + (lane >> 2), 64);
+#ifdef PDX_A_FLAT
+                if (FORM == 0)
+#pragma unroll
+                    for (int h = 0; h < 3; ++h) ja[q][h] = __shfl(sl[q].j, ab[h], 64);
+#endif
             }
 #pragma unroll
-            for (int q = 0; q < G; ++q)
+            for (int q = 0; q < G; ++q) {
 #pragma unroll
-                for (int h = 0; h < 4; ++h) {
-                    g[q][h] = Xq[(int64_t)jb[q][h] * 4 + qa];
-                    if (FORM == 0) ga[q][h] = Aq[(int64_t)jb[q][h] * 3 + qa3];
-                }
+                for (int h = 0; h < 4; ++h) g[q][h] = Xq[(int64_t)jb[q][h] * 4 + qa];
+#ifdef PDX_A_FLAT
+                if (FORM == 0)
+#pragma unroll
+                    for (int h = 0; h < 3; ++h) ga[q][h] = Aq[(int64_t)ja[q][h] * 3 + aq[h]];
+#else
+                if (FORM == 0)
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) ga[q][h] = Aq[(int64_t)jb[q][h] * 3 + qa3];
+#endif
+            }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int q = 0; q < G; ++q) {
@@ -312,8 +341,15 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_force_w(Dev d, int ubuf, int solv
                 for (int h = 0; h < 4; ++h) {
                     const int b = 16 * h + (lane >> 2);
                     tw[xq_slot(b, qa)] = g[q][h];
+#ifndef PDX_A_FLAT
                     if (FORM == 0) ta[xq_slot(b, qa)] = ga[q][h];
+#endif
                 }
+#ifdef PDX_A_FLAT
+                if (FORM == 0)
+#pragma unroll
+                    for (int h = 0; h < 3; ++h) ta[xq_slot(ab[h], aq[h])] = ga[q][h];
+#endif
                 wave_lds_sync();
                 pdx_q4 rq[4], ra[3];
                 float aj[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -334,6 +370,288 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_force_w(Dev d, int ubuf, int solv
     double tot[3];
     if (!reduce_case<3>(d, lb, cs, red, tot)) return;
     if (threadIdx.x == 0) case_update<UPD>(d.sc[cs], tot, ubuf);
+}
+
+// ---------------------------------------------------------------------------
+// halo-staged hot kernels
+// ---------------------------------------------------------------------------
+// The 256 rows of a block reach about six times as many distinct particles
+// (the block's halo), and the gathering kernels fetch each of those records
+// once per bond, ~116 times per pass over all blocks, through an L1 that
+// holds a few hundred records.  Here a block first stages its halo into LDS
+// (one 64-byte record per entry, entries in index order, so the loads are
+// mostly contiguous), then every lane walks its row reading LDS at the bond's
+// halo position (Dev::bl).  Staged per entry, in separate LDS arrays:
+//   xv  : X - anchor in FP32 (3), V                     16 B
+//   u   : u in FP64                                     24 B (16 + 8)
+//   force pass, NOSB: A (9) and c                       40 B (16 + 16 + 8)
+//   force pass, bond-based: c                            8 B
+// The anchor is an FP64 point at the block's centre, so |X - anchor| is
+// bounded by the block's extent plus the horizon and the staged bond vector
+// xv_j - xv_i differs from the FP64 difference by that extent's FP32
+// rounding (about 1e-7 of the horizon); displacement differences are FP64
+// as in the gathering kernels, and the bond arithmetic is the same
+// fpass_bond_v / force_bond_v.
+typedef float pdx_q2 __attribute__((ext_vector_type(2)));
+
+// byte offsets of the LDS arrays, cap entries each
+template <int FORCE, int FORM>
+struct HaloOff {
+    static constexpr int xv = 0, u01 = 16, u2 = 32, a0 = 40, a1 = 56, ac = FORM == 0 ? 72 : 40;
+    static constexpr int bytes = !FORCE ? 40 : (FORM == 0 ? 80 : 48);
+};
+
+template <typename T>
+__device__ __forceinline__ T *lds_at(pdx_q4 *base, int off, int cap)
+{
+    return reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (size_t)off * cap);
+}
+
+// stage the halo of block lb with NT threads (ends with a block barrier)
+template <int FORCE, int FORM, int NT>
+__device__ __forceinline__ void halo_stage(const Dev &d, int ubuf, int lb, pdx_q4 *lds)
+{
+    using O = HaloOff<FORCE, FORM>;
+    const int cap = d.halo_cap;
+    const int h0 = d.hl_ptr[lb], nh = d.hl_ptr[lb + 1] - h0;
+    const pdx_d4 an = d.anchor[lb];
+    const pdx_q4 *__restrict__ Xq = reinterpret_cast<const pdx_q4 *>(d.xu[ubuf]);
+    const pdx_q4 *__restrict__ Aq = reinterpret_cast<const pdx_q4 *>(d.AR);
+    pdx_q4 *xv = lds_at<pdx_q4>(lds, O::xv, cap);
+    pdx_q4 *u01 = lds_at<pdx_q4>(lds, O::u01, cap);
+    pdx_q2 *u2 = lds_at<pdx_q2>(lds, O::u2, cap);
+    pdx_q4 *a0 = lds_at<pdx_q4>(lds, O::a0, cap);
+    pdx_q4 *a1 = lds_at<pdx_q4>(lds, O::a1, cap);
+    pdx_q2 *ac = lds_at<pdx_q2>(lds, O::ac, cap);
+    constexpr int U = NT > PDX_BLOCK ? 2 : 4;  // entries per thread in flight
+    for (int e0 = threadIdx.x; e0 < nh; e0 += U * NT) {
+        int j[U];
+        pdx_q4 r[U][4], a[U][3];
+#pragma unroll
+        for (int v = 0; v < U; ++v) {
+            const int e = e0 + v * NT;
+            j[v] = d.hl[h0 + (e < nh ? e : 0)];
+        }
+#pragma unroll
+        for (int v = 0; v < U; ++v) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) r[v][q] = Xq[(int64_t)j[v] * 4 + q];
+            if (FORCE && FORM == 0)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) a[v][q] = Aq[(int64_t)j[v] * 3 + q];
+        }
+#pragma unroll
+        for (int v = 0; v < U; ++v) {
+            const int e = e0 + v * NT;
+            if (e < nh) {
+                const PdxD2 x01 = __builtin_bit_cast(PdxD2, r[v][0]);
+                const double x2 = __builtin_bit_cast(double, r[v][1].xy);
+                xv[e] = pdx_q4{(float)(x01.a - an.x), (float)(x01.b - an.y), (float)(x2 - an.z), r[v][1].z};
+                u01[e] = r[v][2];
+                u2[e] = r[v][3].xy;
+                if (FORCE) {
+                    if (FORM == 0) {
+                        a0[e] = a[v][0];
+                        a1[e] = a[v][1];
+                        ac[e] = pdx_q2{a[v][2].x, r[v][1].w};
+                    } else {
+                        ac[e] = pdx_q2{0.f, r[v][1].w};
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// bond vector and displacement difference of halo entry l against the own
+// entry (xo, uo)
+__device__ __forceinline__ void halo_bond(const pdx_q4 &xj, const pdx_q4 &uj01, const pdx_q2 &uj2, const pdx_q4 &xo,
+                                          const PdxD2 &uo01, double uo2, float (&xi)[3], float (&du)[3])
+{
+    xi[0] = xj.x - xo.x;
+    xi[1] = xj.y - xo.y;
+    xi[2] = xj.z - xo.z;
+    const PdxD2 a = __builtin_bit_cast(PdxD2, uj01);
+    const double b = __builtin_bit_cast(double, uj2);
+    du[0] = (float)(a.a - uo01.a);
+    du[1] = (float)(a.b - uo01.b);
+    du[2] = (float)(b - uo2);
+}
+
+// Split rows: S waves share a block's halo and rows; part p (threadIdx.x /
+// PDX_BLOCK) takes the bond slots whose group index is p mod S, and the
+// partial sums meet in LDS (reusing the halo area) in part order before the
+// per-particle finish.  More waves per halo hide the LDS and FP64 latencies
+// when the halo leaves room for one block per CU.
+template <int NV, int S>
+__device__ __forceinline__ void split_sum(pdx_q4 *lds, double (&acc)[NV])
+{
+    if constexpr (S > 1) {
+        double *sp = reinterpret_cast<double *>(lds);  // [S - 1][NV][PDX_BLOCK]
+        const int t = threadIdx.x % PDX_BLOCK, part = threadIdx.x / PDX_BLOCK;
+        __syncthreads();  // every part is done reading the halo
+        if (part > 0)
+#pragma unroll
+            for (int q = 0; q < NV; ++q) sp[((part - 1) * NV + q) * PDX_BLOCK + t] = acc[q];
+        __syncthreads();
+        if (part == 0)
+            for (int p = 1; p < S; ++p)
+#pragma unroll
+                for (int q = 0; q < NV; ++q) acc[q] += sp[((p - 1) * NV + q) * PDX_BLOCK + t];
+    }
+}
+
+template <int D, int MODEL, int G, int S>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_h(Dev d, int ubuf, int solve_mode)
+{
+    extern __shared__ pdx_q4 hlds[];
+    using O = HaloOff<0, 0>;
+    const int lb = logical_block();
+    if (solve_mode && d.sc[d.case_of_blk[lb]].converged) return;
+    if (lb * PDX_BLOCK >= d.ni) return;  // grid padding blocks (whole block)
+    halo_stage<0, 0, PDX_BLOCK * S>(d, ubuf, lb, hlds);
+    const int cap = d.halo_cap;
+    const pdx_q4 *xv = lds_at<pdx_q4>(hlds, O::xv, cap);
+    const pdx_q4 *u01 = lds_at<pdx_q4>(hlds, O::u01, cap);
+    const pdx_q2 *u2 = lds_at<pdx_q2>(hlds, O::u2, cap);
+    const int t = threadIdx.x % PDX_BLOCK, lane = t & 63, part = threadIdx.x / PDX_BLOCK;
+    const int i = lb * PDX_BLOCK + t;
+    const KRec kr = d.KR[i];
+    const bool act = (kr.flags & PDX_FL_ACTIVE) != 0;
+    const pdx_q4 xo = xv[t];
+    const PdxD2 uo01 = __builtin_bit_cast(PdxD2, u01[t]);
+    const double uo2 = __builtin_bit_cast(double, u2[t]);
+    const float invh = (float)d.inv_h;
+    double Hd[D * D];
+#pragma unroll
+    for (int q = 0; q < D * D; ++q) Hd[q] = 0.0;
+    const int64_t s0 = d.sl_ptr[i >> 6];
+    const int width = (int)((d.sl_ptr[(i >> 6) + 1] - s0) / PDX_SLICE);
+    const uint16_t *__restrict__ bp = d.bl + s0 + lane;
+    constexpr int STEP = G * S;
+    const int k0 = part * G;
+    uint32_t ec[G], en[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) ec[q] = (k0 + q) < width ? bp[(int64_t)(k0 + q) * PDX_SLICE] : PDX_HNONE;
+    for (int k = k0; k < width; k += STEP) {
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+            en[q] = (k + STEP + q) < width ? bp[(int64_t)(k + STEP + q) * PDX_SLICE] : PDX_HNONE;
+        bool live[G];
+        pdx_q4 xj[G], uj01[G];
+        pdx_q2 uj2[G];
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            live[q] = act && !(ec[q] & PDX_HBROKEN);
+            const int l = live[q] ? (int)ec[q] : t;
+            xj[q] = xv[l];
+            uj01[q] = u01[l];
+            uj2[q] = u2[l];
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            float xi[3], du[3];
+            halo_bond(xj[q], uj01[q], uj2[q], xo, uo01, uo2, xi, du);
+            fpass_bond_v<D>(xi, du, xj[q].w, invh, live[q], Hd);
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q) ec[q] = en[q];
+    }
+    split_sum<D * D, S>(hlds, Hd);
+    if (part == 0 && act) fpass_finish<D, MODEL, false, false>(d, i, kr, Hd);
+}
+
+template <int D, int FORM, int UPD, int G, int S>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int solve_mode, double dt, double visc)
+{
+    extern __shared__ pdx_q4 hlds[];
+    using O = HaloOff<1, FORM>;
+    const int lb = logical_block();
+    const int cs = d.case_of_blk[lb];
+    if (solve_mode && d.sc[cs].converged) return;
+    double red[3] = {0.0, 0.0, 0.0};
+    if (lb * PDX_BLOCK < d.ni) {  // grid padding blocks only join the reduction
+        halo_stage<1, FORM, PDX_BLOCK * S>(d, ubuf, lb, hlds);
+        const int cap = d.halo_cap;
+        const pdx_q4 *xv = lds_at<pdx_q4>(hlds, O::xv, cap);
+        const pdx_q4 *u01 = lds_at<pdx_q4>(hlds, O::u01, cap);
+        const pdx_q2 *u2 = lds_at<pdx_q2>(hlds, O::u2, cap);
+        const pdx_q4 *a0 = lds_at<pdx_q4>(hlds, O::a0, cap);
+        const pdx_q4 *a1 = lds_at<pdx_q4>(hlds, O::a1, cap);
+        const pdx_q2 *ac = lds_at<pdx_q2>(hlds, O::ac, cap);
+        const int t = threadIdx.x % PDX_BLOCK, lane = t & 63, part = threadIdx.x / PDX_BLOCK;
+        const int i = lb * PDX_BLOCK + t;
+        const KRec kr = d.KR[i];
+        const bool act = (kr.flags & PDX_FL_ACTIVE) != 0;
+        const XRec pi = d.xu[ubuf][i];
+        ARec ai{};
+        if (FORM == 0) ai = d.AR[i];
+        const pdx_q4 xo = xv[t];
+        const PdxD2 uo01 = __builtin_bit_cast(PdxD2, u01[t]);
+        const double uo2 = __builtin_bit_cast(double, u2[t]);
+        const float invh = (float)d.inv_h;
+        const float Gs = (float)d.G_s;
+        double fa[3] = {0.0, 0.0, 0.0};
+        const int64_t s0 = d.sl_ptr[i >> 6];
+        const int width = (int)((d.sl_ptr[(i >> 6) + 1] - s0) / PDX_SLICE);
+        const uint16_t *__restrict__ bp = d.bl + s0 + lane;
+        constexpr int STEP = G * S;
+        const int k0 = part * G;
+        uint32_t ec[G], en[G];
+#pragma unroll
+        for (int q = 0; q < G; ++q) ec[q] = (k0 + q) < width ? bp[(int64_t)(k0 + q) * PDX_SLICE] : PDX_HNONE;
+        for (int k = k0; k < width; k += STEP) {
+#pragma unroll
+            for (int q = 0; q < G; ++q)
+                en[q] = (k + STEP + q) < width ? bp[(int64_t)(k + STEP + q) * PDX_SLICE] : PDX_HNONE;
+            bool live[G];
+            pdx_q4 xj[G], uj01[G], aj0[G], aj1[G];
+            pdx_q2 uj2[G], ajc[G];
+#pragma unroll
+            for (int q = 0; q < G; ++q) {
+                live[q] = act && !(ec[q] & PDX_HBROKEN);
+                const int l = live[q] ? (int)ec[q] : t;
+                xj[q] = xv[l];
+                uj01[q] = u01[l];
+                uj2[q] = u2[l];
+                ajc[q] = ac[l];
+                if (FORM == 0) {
+                    aj0[q] = a0[l];
+                    aj1[q] = a1[l];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < G; ++q) {
+                float xi[3], du[3];
+                halo_bond(xj[q], uj01[q], uj2[q], xo, uo01, uo2, xi, du);
+                float aj[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                if (FORM == 0) {
+                    aj[0] = aj0[q].x; aj[1] = aj0[q].y; aj[2] = aj0[q].z; aj[3] = aj0[q].w;
+                    aj[4] = aj1[q].x; aj[5] = aj1[q].y; aj[6] = aj1[q].z; aj[7] = aj1[q].w;
+                    aj[8] = ajc[q].x;
+                }
+                force_bond_v<D, FORM>(xi, du, xj[q].w, pi.c, ajc[q].y, ai.A, aj, invh, Gs, live[q], fa);
+            }
+#pragma unroll
+            for (int q = 0; q < G; ++q) ec[q] = en[q];
+        }
+        split_sum<3, S>(hlds, fa);
+        if (part == 0 && act) force_finish<D, FORM, UPD>(d, i, kr, pi, ai, ubuf, cs, dt, visc, fa, red);
+    }
+    double tot[3];
+    if (!reduce_case<3, PDX_BLOCK * S>(d, lb, cs, red, tot)) return;
+    if (threadIdx.x == 0) case_update<UPD>(d.sc[cs], tot, ubuf);
+}
+
+// broken flags of the halo-local entries from bj (after bj was rewritten)
+__global__ __launch_bounds__(PDX_BLOCK) void k_halo_bits(Dev d, int64_t slots)
+{
+    const int64_t s = (int64_t)blockIdx.x * PDX_BLOCK + threadIdx.x;
+    if (s >= slots) return;
+    const uint16_t e = d.bl[s];
+    if (e == PDX_HNONE) return;
+    d.bl[s] = (uint16_t)((e & ~PDX_HBROKEN) | ((d.bj[s] & PDX_BROKEN) ? PDX_HBROKEN : 0u));
 }
 
 template <int D>
@@ -465,6 +783,22 @@ int pdx_launch_shape(pdx_ctx *c, int mode, double tol)
 // per-lane gathers with bond groups 1, 2, 4 and wave-cooperative gathers with
 // groups 1, 2 (PDX_VARIANT_COOP | group); diagnostic and commit passes use
 // the per-lane kernel with group 1
+// LDS bytes of a halo-staged pass (the halo, or the split partial sums
+// where larger), or 0 when it does not fit (then the pass runs the
+// cooperative group-2 kernel)
+int pdx_halo_lds(const pdx_ctx *c, int entry_bytes, int split, int nv)
+{
+    if (!c->d.hl_ptr || c->d.halo_cap <= 0) return 0;
+    const int64_t halo = (int64_t)c->d.halo_cap * entry_bytes;
+    const int64_t part = (int64_t)(split - 1) * nv * PDX_BLOCK * (int64_t)sizeof(double);
+    const int64_t bytes = std::max(halo, part);
+    return bytes <= c->lds_dyn_max ? (int)bytes : 0;
+}
+
+// variant code -> halo kernel bond group and split
+constexpr int halo_group(int v) { return v & 15; }
+constexpr int halo_split(int v) { return (v & PDX_VARIANT_SPLIT4) ? 4 : ((v & PDX_VARIANT_SPLIT2) ? 2 : 1); }
+
 template <int D, int MODEL, int G>
 static void launch_fpass_g(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm)
 {
@@ -474,17 +808,29 @@ static void launch_fpass_g(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm)
     } else if (diag) {
         hipLaunchKernelGGL((k_fpass<D, MODEL, true, false, 1>), g, b, 0, c->stream, c->d, ubuf, sm);
     } else {
-        if constexpr (G > PDX_VARIANT_COOP)
+        if constexpr ((G & PDX_VARIANT_HALO) != 0) {
+            constexpr int GH = halo_group(G), S = halo_split(G);
+            const int lds = pdx_halo_lds(c, HaloOff<0, 0>::bytes, S, D * D);
+            if (lds)
+                hipLaunchKernelGGL((k_fpass_h<D, MODEL, GH, S>), g, dim3(PDX_BLOCK * S), lds, c->stream, c->d, ubuf, sm);
+            else
+                hipLaunchKernelGGL((k_fpass_w<D, MODEL, 2>), g, b, 0, c->stream, c->d, ubuf, sm);
+        } else if constexpr (G > PDX_VARIANT_COOP) {
             hipLaunchKernelGGL((k_fpass_w<D, MODEL, G - PDX_VARIANT_COOP>), g, b, 0, c->stream, c->d, ubuf, sm);
-        else
+        } else {
             hipLaunchKernelGGL((k_fpass<D, MODEL, false, false, G>), g, b, 0, c->stream, c->d, ubuf, sm);
+        }
     }
 }
 
 template <int D, int MODEL>
 static void launch_fpass_t(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm)
 {
-    switch (c->tune_g) {
+    switch (c->tune_f) {
+#define PDX_HALO_CASE(v) \
+    case v: launch_fpass_g<D, MODEL, v>(c, ubuf, diag, commit, sm); break;
+    PDX_HALO_VARIANTS(PDX_HALO_CASE)
+#undef PDX_HALO_CASE
     case PDX_VARIANT_COOP | 2: launch_fpass_g<D, MODEL, PDX_VARIANT_COOP | 2>(c, ubuf, diag, commit, sm); break;
     case PDX_VARIANT_COOP | 1: launch_fpass_g<D, MODEL, PDX_VARIANT_COOP | 1>(c, ubuf, diag, commit, sm); break;
     case 4: launch_fpass_g<D, MODEL, 4>(c, ubuf, diag, commit, sm); break;
@@ -516,7 +862,19 @@ template <int D, int FORM, int G>
 static void launch_force_g(pdx_ctx *c, int ubuf, int upd, int sm, double dt, double visc)
 {
     dim3 g(c->nblk), b(PDX_BLOCK);
-    if constexpr (G > PDX_VARIANT_COOP) {
+    if constexpr ((G & PDX_VARIANT_HALO) != 0) {
+        constexpr int GH = halo_group(G), S = halo_split(G);
+        const int lds = pdx_halo_lds(c, HaloOff<1, FORM>::bytes, S, 3);
+        const dim3 bh(PDX_BLOCK * S);
+        if (!lds)
+            launch_force_g<D, FORM, PDX_VARIANT_COOP | 2>(c, ubuf, upd, sm, dt, visc);
+        else if (upd == 1)
+            hipLaunchKernelGGL((k_force_h<D, FORM, 1, GH, S>), g, bh, lds, c->stream, c->d, ubuf, sm, dt, visc);
+        else if (upd == 2)
+            hipLaunchKernelGGL((k_force_h<D, FORM, 2, GH, S>), g, bh, lds, c->stream, c->d, ubuf, sm, dt, visc);
+        else
+            hipLaunchKernelGGL((k_force_h<D, FORM, 0, GH, S>), g, bh, lds, c->stream, c->d, ubuf, sm, dt, visc);
+    } else if constexpr (G > PDX_VARIANT_COOP) {
         constexpr int GW = G - PDX_VARIANT_COOP;
         if (upd == 1)
             hipLaunchKernelGGL((k_force_w<D, FORM, 1, GW>), g, b, 0, c->stream, c->d, ubuf, sm, dt, visc);
@@ -538,6 +896,10 @@ template <int D, int FORM>
 static void launch_force_t(pdx_ctx *c, int ubuf, int upd, int sm, double dt, double visc)
 {
     switch (c->tune_g) {
+#define PDX_HALO_CASE(v) \
+    case v: launch_force_g<D, FORM, v>(c, ubuf, upd, sm, dt, visc); break;
+    PDX_HALO_VARIANTS(PDX_HALO_CASE)
+#undef PDX_HALO_CASE
     case PDX_VARIANT_COOP | 2: launch_force_g<D, FORM, PDX_VARIANT_COOP | 2>(c, ubuf, upd, sm, dt, visc); break;
     case PDX_VARIANT_COOP | 1: launch_force_g<D, FORM, PDX_VARIANT_COOP | 1>(c, ubuf, upd, sm, dt, visc); break;
     case 4: launch_force_g<D, FORM, 4>(c, ubuf, upd, sm, dt, visc); break;
@@ -556,6 +918,15 @@ int pdx_launch_force(pdx_ctx *c, int ubuf, int upd, bool solve_mode, double dt, 
         if (c->form == PDX_FORM_BOND) launch_force_t<2, PDX_FORM_BOND>(c, ubuf, upd, sm, dt, visc);
         else launch_force_t<2, PDX_FORM_NOSB>(c, ubuf, upd, sm, dt, visc);
     }
+    PDX_LAUNCH_CHECK();
+    return PDX_OK;
+}
+
+int pdx_launch_halo_bits(pdx_ctx *c)
+{
+    if (!c->d.bl || c->slots <= 0) return PDX_OK;
+    hipLaunchKernelGGL(k_halo_bits, dim3((unsigned)((c->slots + PDX_BLOCK - 1) / PDX_BLOCK)), dim3(PDX_BLOCK), 0,
+                       c->stream, c->d, c->slots);
     PDX_LAUNCH_CHECK();
     return PDX_OK;
 }

@@ -30,7 +30,6 @@ def _group_body(specs):
     from ..workloads import pack
     packed, off = pack(specs)
     body = from_specimen(packed, case_offsets=off)
-    body.set_group(16 | 2)
     return packed, off, body
 
 

@@ -241,9 +241,9 @@ def test_large_bond_build_is_intact(gpu):
 VARIANTS = (1, 2, 4, 16 | 1, 16 | 2)  # per-lane groups 1/2/4, cooperative groups 1/2
 
 
-def _variant_runs(body, u0, method, iters, **kw):
+def _variant_runs(body, u0, method, iters, variants=VARIANTS, **kw):
     out = {}
-    for g in VARIANTS:
+    for g in variants:
         body.set_group(g)
         body.u = u0
         # a solve keeps the accelerations of the previous one (the reference's
@@ -283,6 +283,81 @@ def test_hot_pass_variants_bitwise_identical(gpu, formulation):
             assert it == base[2] == [25, 25]
             assert np.array_equal(f, base[0]), (method, g)
             assert np.array_equal(uu, base[1]), (method, g)
+
+
+HALO = tuple(32 | s | g for s in (0, 64, 128) for g in (1, 2))  # PDX_VARIANT_HALO [| SPLIT2/4] | group
+
+
+def _halo_body(formulation):
+    A = lattice(9, 8, 7, jitter=0.2, seed=11)
+    B = lattice(6, 7, 5, jitter=0.2, seed=12) + 50.0
+    X = np.concatenate([A, B])
+    off = np.array([0, len(A), len(X)])
+    n = len(X)
+    bulk, shear = W.bulk_shear(15000.0 * np.exp(0.3 * np.random.default_rng(1).standard_normal(n)), 0.3)
+    fixed = np.zeros(n, np.uint8)
+    fixed[X[:, 2] < 0.5] = 7
+    fixed[(X[:, 2] > 49.5) & (X[:, 2] < 50.5)] = 7
+    fext = np.zeros((n, 3))
+    fext[(X[:, 2] > 5.5) & (X[:, 2] < 10.0), 2] = -20.0
+    fext[X[:, 2] > 53.5, 0] = 15.0
+    body = PDBody(X, 1.0, 3.015, bulk, shear, cstab=18 * bulk / (np.pi * 3.015 ** 4), fixed=fixed, fext=fext,
+                  formulation=formulation, model="neo_hookean", case_offsets=off)
+    return X, body
+
+
+@pytest.mark.parametrize("formulation", ["nosb", "bond"])
+def test_halo_variants_match_gathering(gpu, formulation):
+    """The LDS-staged variants stage positions relative to a per-block anchor
+    in FP32, so their bond vectors differ from the gathering kernels' FP64
+    differences by ~1e-7 relative.  Forces agree to 2e-6 with broken bonds
+    inside rows (set by the damage kernel, then rewritten by pdx_set_broken)
+    and a batch whose cases end in padding particles; groups 1 and 2 give the
+    same bits, split rows (other FP64 summation order) agree to 1e-12.
+    Adaptive DR to 1e-6 converges in the same number of iterations (+-2%) to
+    the same displacements (1e-5), and 40 kinetic-DR steps agree to 1e-5."""
+    X, body = _halo_body(formulation)
+    body.u = smooth_u(X, amp=0.02, seed=7)
+    assert body.break_bonds(0.02) > 0
+    u1 = smooth_u(X, amp=2e-3, seed=8)
+    for step in ("damage", "set_broken"):
+        if step == "set_broken":
+            _, _, br = body.bonds()
+            br = br.copy()
+            br[::7] = 1
+            br[3::11] = 0
+            body.set_broken(br)
+        body.u = u1
+        body.set_group(16 | 2)
+        f0 = body.compute_forces().copy()
+        fs = {}
+        for g in HALO:
+            body.set_group(g)
+            fs[g] = body.compute_forces().copy()
+            assert rel_l2(fs[g], f0) < 2e-6, (step, g, rel_l2(fs[g], f0))
+        for sp in (0, 64, 128):
+            assert np.array_equal(fs[32 | sp | 1], fs[32 | sp | 2]), (step, sp)
+            assert rel_l2(fs[32 | sp | 1], fs[32 | 1]) < 1e-12, (step, sp)
+    X, body = _halo_body(formulation)
+    ref = {}
+    for g in (16 | 2,) + HALO:
+        body.set_group(g)
+        body.u = np.zeros_like(X)
+        cold = body.cold_residual()
+        r = body.solve("adaptive", tol=1e-6, max_iterations=40000, refs=cold)
+        assert all(x.converged for x in r), g
+        uu = body.u.copy()
+        body.u = u1
+        body.set("v", np.zeros_like(X))
+        body.set("a", np.zeros_like(X))
+        body.solve("kinetic", tol=0.0, max_iterations=40, dt=1e-3, use_graph=False)
+        if not ref:
+            ref = {"it": [x.iterations for x in r], "u": uu, "uk": body.u.copy()}
+            continue
+        for a_, b_ in zip([x.iterations for x in r], ref["it"]):
+            assert abs(a_ - b_) <= max(3, 0.02 * b_), (g, a_, b_)
+        assert rel_l2(uu, ref["u"]) < 1e-5, (g, rel_l2(uu, ref["u"]))
+        assert rel_l2(body.u - u1, ref["uk"] - u1) < 1e-5, (g, rel_l2(body.u - u1, ref["uk"] - u1))
 
 
 def test_sliced_solve_matches_single(gpu):
