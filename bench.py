@@ -47,6 +47,17 @@ def variant_names(v):
     return f"per-lane gathers, bond group {v}", "k_fpass", "k_force"
 
 
+def apply_variant(body, spec, iters):
+    """--variant: one code for both passes, 'F,G' per pass, or 0 = autotune."""
+    v = [int(x) for x in str(spec).split(",")]
+    if v == [0]:
+        body.autotune(iters)
+    elif len(v) == 1:
+        body.set_group(v[0])
+    else:
+        body.set_variants(v[0], v[1])
+
+
 def pass_variants(body):
     """(label, F-pass kernel, force kernel) of the variants a body runs."""
     fv, gv = body.variants()
@@ -73,8 +84,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--kernel-iters", type=int, default=50)
-    ap.add_argument("--variant", type=int, default=0,
-                    help="hot-pass variant (1, 2, 4, 17, 18, 33, 34, 97, 98, 161, 162); 0 = autotune")
+    ap.add_argument("--variant", type=str, default="0",
+                    help="hot-pass variant (1, 2, 4, 17, 18, 33, 34, 97, 98, 161, 162) or 'F,G' for the F-pass "
+                         "and the force pass; 0 = autotune")
     ap.add_argument("--workload", default="vertebra", choices=["vertebra", "vertebra_j2", "batch"],
                     help="vertebra: the headline (dx 0.4, Neo-Hookean, pressure); vertebra_j2: J2, 20 "
                          "prescribed increments with breaking (dx 0.25); batch: 64 random cases in one context")
@@ -118,10 +130,7 @@ def main():
 
     cold = body.cold_residual()
     body.u = spec.u0
-    if args.variant:
-        body.set_group(args.variant)
-    else:
-        body.autotune(10)
+    apply_variant(body, args.variant, 10)
     vlabel, kf_name, kg_name = pass_variants(body)
     log(f"hot-pass variant: {vlabel}")
     # warmup + timed steps: fixed-count iterations that never stop early (tol = 0)
@@ -233,10 +242,7 @@ def run_loading(args, torch):
     setup = time.perf_counter() - t0
     nb0 = body.stats()["n_intact"]
     log(f"{spec.name}: {spec.n} particles, {nb0} bonds, setup {setup:.2f}s")
-    if args.variant:
-        body.set_group(args.variant)
-    else:
-        body.autotune(5)
+    apply_variant(body, args.variant, 5)
     drv = IncrementalLoading(body, spec.extra["top_idx"], spec.extra["u_final"], spec.extra["n_increments"],
                              critical_stretch=spec.extra["critical_stretch"], tol=args.tol,
                              max_iterations=args.max_solve_iters,
@@ -283,10 +289,7 @@ def run_batch(args, torch):
     log(f"{packed.name}: {len(specs)} cases, {packed.n} particles, {nb} bonds, setup {setup:.2f}s")
     # ref = 0: each case is normalised by its own residual at the start state,
     # which for u0 is the cold-start residual
-    if args.variant:
-        body.set_group(args.variant)
-    else:
-        body.autotune(5)
+    apply_variant(body, args.variant, 5)
     body.u = packed.u0
     body.solve("adaptive", tol=0.0, max_iterations=args.warmup, ref=0.0, chunk=64)
     torch.cuda.synchronize()

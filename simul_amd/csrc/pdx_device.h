@@ -297,6 +297,28 @@ PDX_HD void stress_j2(const double *H, double Kb_, double mu_, const J2Par &pp, 
     P[2] = P[6] = sfin[5];
 }
 
+// square root and reciprocal of the FP32 bond arithmetic: the hardware
+// V_SQRT_F32 / V_RCP_F32 (1 ulp) on the GPU, where the IEEE-rounded forms cost
+// about a dozen instructions each; the host build of this header (the CPU
+// checks) uses the library functions
+PDX_HD float bond_sqrt(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return sqrtf(x);
+#endif
+}
+
+PDX_HD float bond_rcp(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
 // influence weight w = max(0, 1 - |xi|/delta) (bonds.py:131-137) times V_j
 PDX_HD float bond_weight(float L, float invh, float Vj)
 {
@@ -315,7 +337,7 @@ PDX_HD void fpass_bond_v(const float *xi, const float *du, float Vj, float invh,
 {
     float L2 = 0.f;
     for (int r = 0; r < D; ++r) L2 = fmaf(xi[r], xi[r], L2);
-    const float w = live ? bond_weight(sqrtf(L2), invh, Vj) : 0.f;
+    const float w = live ? bond_weight(bond_sqrt(L2), invh, Vj) : 0.f;
     for (int r = 0; r < D; ++r) {
         const double wd = (double)(w * du[r]);
         for (int c = 0; c < D; ++c) Hd[r * D + c] = fma(wd, (double)xi[c], Hd[r * D + c]);
@@ -451,14 +473,14 @@ PDX_HD void force_bond_v(const float *xi, const float *du, float Vj, float ci, f
         xd = fmaf(xi[r], du[r], xd);
         dd = fmaf(du[r], du[r], dd);
     }
-    const float L = sqrtf(L2);
+    const float L = bond_sqrt(L2);
     const float w = bond_weight(L, invh, Vj);
     float eta[3];
     for (int r = 0; r < D; ++r) eta[r] = xi[r] + du[r];
     // |eta|^2 = |xi|^2 + (2 xi.du + du.du); the stretch uses the
     // cancellation-free |eta| - |xi| = (2 xi.du + du.du) / (|eta| + |xi|)
     const float dn = fmaf(2.f, xd, dd);
-    const float el = sqrtf(fmaxf(L2 + dn, 0.f));
+    const float el = bond_sqrt(fmaxf(L2 + dn, 0.f));
     float tb[3] = {0.f, 0.f, 0.f};
     float g = 0.f;
     if (FORM == 0) {
@@ -467,9 +489,9 @@ PDX_HD void force_bond_v(const float *xi, const float *du, float Vj, float ci, f
             for (int c = 0; c < D; ++c) t = fmaf(Aj[r * D + c] - Ai[r * D + c], xi[c], t);
             tb[r] = t;
         }
-        if (el > 1e-10f && L > 1e-10f) g = Gs * 0.5f * (ci + cj) * (dn / ((el + L) * L)) / el;
+        if (el > 1e-10f && L > 1e-10f) g = Gs * 0.5f * (ci + cj) * dn * bond_rcp((el + L) * L * el);
     } else if (el > 1e-10f) {
-        g = 0.5f * (ci + cj) * (dn / ((el + L) * L)) / el;
+        g = 0.5f * (ci + cj) * dn * bond_rcp((el + L) * L * el);
     }
     for (int r = 0; r < D; ++r) {
         const float contrib = w * fmaf(g, eta[r], tb[r]);
