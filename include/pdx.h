@@ -197,7 +197,8 @@ int pdx_set(pdx_ctx *ctx, int32_t field, const void *in);
 
 /* ---- measurement ------------------------------------------------------------ */
 /* times `iters` adaptive-DR iterations with HIP events around every launch on the
- * context stream; ms[0] = mean F-pass, ms[1] = mean force pass, ms[2] = total */
+ * context stream; ms[0] = mean F-pass, ms[1] = mean force pass (the force kernel; the
+ * per-case reduction that follows it is a separate small launch), ms[2] = total */
 int pdx_time_kernels(pdx_ctx *ctx, int32_t iters, double *ms);
 /* hot-pass variants: per-lane neighbour gathers with bond groups 1, 2, 4, or
  * wave-cooperative gathers (four lanes fetch one 64-byte neighbour record, staged

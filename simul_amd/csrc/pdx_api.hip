@@ -69,7 +69,7 @@ static void free_all(pdx_ctx *c)
     dfree(d.mass); dfree(d.lam); dfree(d.V64); dfree(d.K64); dfree(d.Kinv64);
     dfree(d.Fd); dfree(d.Pd); dfree(d.damage); dfree(d.init_nb);
     dfree(d.sl_ptr); dfree(d.bj); dfree(d.case_of_blk); dfree(d.case_blk0);
-    dfree(d.hl_ptr); dfree(d.hl); dfree(d.bl); dfree(d.anchor);
+    dfree(d.hl_ptr); dfree(d.hl); dfree(d.hs); dfree(d.bl); dfree(d.anchor);
     dfree(d.sc); dfree(d.part); dfree(d.grid); dfree(d.case_of_p);
     dfree(d.cell_start); dfree(d.cell_idx); dfree(d.counts);
     if (c->gexec) hipGraphExecDestroy(c->gexec);
@@ -435,7 +435,7 @@ extern "C" int pdx_set_particles(pdx_ctx *c, const double *X, const double *vol,
             if (!isfinite(X[(size_t)r * D + q])) return fail(PDX_EINVAL, "positions must be finite");
     }
     c->hX.assign(X, X + (size_t)c->n * D);
-    pdx_layout::morton_order(D, X, c->case_off, c->case_p0, c->perm, c->inv);
+    pdx_layout::morton_order(D, X, vol, c->case_off, c->case_p0, c->perm, c->inv, c->colour);
     HostMirror &m = mirror(c);
     for (int r = 0; r < c->n; ++r) {
         m.vol[r] = vol[r];
@@ -585,31 +585,40 @@ extern "C" int pdx_count_neighbors(pdx_ctx *c, double horizon, int32_t *counts)
     return PDX_OK;
 }
 
-// halo tables of the LDS-staged hot passes (Dev::hl_ptr, hl, bl, anchor):
-// per block, its own PDX_BLOCK particles, then every other particle its rows
-// reach in index order; each bond slot's position in that list; and the
-// centre of the block's own particles as the FP64 anchor.  Blocks are built
-// in parallel host threads.  A halo above PDX_HALO_LIMIT entries leaves the
-// tables unbuilt (the halo variants then run the cooperative kernels).
+// halo tables of the LDS-staged hot passes (Dev::hl_ptr, hl, hs, bl,
+// anchor).  Per block: the halo is its own PDX_BLOCK particles, then every
+// other particle its rows reach in index order (hl); each halo entry gets an
+// LDS slot (hs) whose low 4 bits are the particle's bank colour where the
+// colour classes allow it (slot 16 r + colour for the r-th entry of that
+// colour), the entries of an over-full colour take the remaining free slots,
+// and the block's slot count is the halo size rounded up to 16.  bl holds
+// each bond slot's LDS slot; the anchor is the centre of the block's own
+// particles.  Blocks are built in parallel host threads.  A halo above
+// PDX_HALO_LIMIT slots leaves the tables unbuilt (the halo variants then run
+// the cooperative kernels).
 static int build_halo(pdx_ctx *c)
 {
     Dev &d = c->d;
-    dfree(d.hl_ptr); dfree(d.hl); dfree(d.bl); dfree(d.anchor);
+    dfree(d.hl_ptr); dfree(d.hl); dfree(d.hs); dfree(d.bl); dfree(d.anchor);
     d.halo_cap = 0;
     invalidate_graph(c);
     const int nbk = c->ni / PDX_BLOCK, D = c->dim;
     if (nbk == 0 || c->slots <= 0) return PDX_OK;
     std::vector<uint32_t> bj;
     PDX_TRY(download(c, bj, d.bj, (size_t)c->slots));
-    std::vector<std::vector<int32_t>> other(nbk);
+    std::vector<std::vector<int32_t>> ent_list(nbk);
+    std::vector<std::vector<uint16_t>> slot_list(nbk);
+    std::vector<int32_t> cap_b(nbk, 0);
     std::vector<uint16_t> bl((size_t)c->slots, (uint16_t)PDX_HNONE);
     std::vector<pdx_d4> anc(nbk);
+    std::vector<uint8_t> over(nbk, 0);
     constexpr int SPB = PDX_BLOCK / PDX_SLICE;
     auto work = [&](int b0, int b1) {
+        std::vector<int32_t> o, sl, slot_of;
         for (int b = b0; b < b1; ++b) {
             const int i0 = b * PDX_BLOCK;
             const int64_t s_lo = c->h_slptr[(size_t)b * SPB], s_hi = c->h_slptr[(size_t)(b + 1) * SPB];
-            std::vector<int32_t> &o = other[b];
+            o.clear();
             for (int64_t s = s_lo; s < s_hi; ++s) {
                 const uint32_t e = bj[s];
                 if (e == 0xffffffffu) continue;
@@ -618,14 +627,51 @@ static int build_halo(pdx_ctx *c)
             }
             std::sort(o.begin(), o.end());
             o.erase(std::unique(o.begin(), o.end()), o.end());
-            if (PDX_BLOCK + (int64_t)o.size() > PDX_HALO_LIMIT) continue;
+            const int nh = PDX_BLOCK + (int)o.size();
+            const int nslot = (nh + 15) & ~15, rows = nslot / 16;
+            if (nslot > PDX_HALO_LIMIT) {
+                over[b] = 1;
+                continue;
+            }
+            std::vector<int32_t> &ent = ent_list[b];
+            ent.resize(nh);
+            for (int t = 0; t < PDX_BLOCK; ++t) ent[t] = i0 + t;
+            std::copy(o.begin(), o.end(), ent.begin() + PDX_BLOCK);
+            // slots by colour, then the overflow into the free slots
+            sl.assign(nslot, -1);
+            slot_of.assign(nh, -1);
+            int cnt[16] = {0};
+            std::vector<int> spill;
+            for (int e = 0; e < nh; ++e) {
+                const int cl = c->colour[ent[e]] & 15;
+                if (cnt[cl] < rows) {
+                    const int sidx = 16 * cnt[cl]++ + cl;
+                    sl[sidx] = ent[e];
+                    slot_of[e] = sidx;
+                } else {
+                    spill.push_back(e);
+                }
+            }
+            size_t k = 0;
+            for (int sidx = 0; sidx < nslot && k < spill.size(); ++sidx)
+                if (sl[sidx] < 0) {
+                    sl[sidx] = ent[spill[k]];
+                    slot_of[spill[k]] = sidx;
+                    ++k;
+                }
+            if (k < spill.size()) {  // cannot happen: nslot >= nh
+                over[b] = 1;
+                continue;
+            }
+            slot_list[b].assign(slot_of.begin(), slot_of.end());
+            cap_b[b] = nslot;
             for (int64_t s = s_lo; s < s_hi; ++s) {
                 const uint32_t e = bj[s];
                 if (e == 0xffffffffu) continue;
                 const int32_t j = (int32_t)(e & ~PDX_BROKEN);
                 int32_t l = j - i0;
                 if (l < 0 || l >= PDX_BLOCK) l = PDX_BLOCK + (int32_t)(std::lower_bound(o.begin(), o.end(), j) - o.begin());
-                bl[s] = (uint16_t)(l | ((e & PDX_BROKEN) ? PDX_HBROKEN : 0u));
+                bl[s] = (uint16_t)(slot_of[l] | ((e & PDX_BROKEN) ? PDX_HBROKEN : 0u));
             }
             double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
             for (int t = 0; t < PDX_BLOCK; ++t) {
@@ -651,26 +697,27 @@ static int build_halo(pdx_ctx *c)
     std::vector<int32_t> ptr(nbk + 1, 0);
     int64_t hmax = 0;
     for (int b = 0; b < nbk; ++b) {
-        const int64_t nh = PDX_BLOCK + (int64_t)other[b].size();
-        if (nh > PDX_HALO_LIMIT) return PDX_OK;
-        hmax = std::max(hmax, nh);
-        ptr[b + 1] = ptr[b] + (int32_t)nh;
+        if (over[b]) return PDX_OK;
+        hmax = std::max(hmax, (int64_t)cap_b[b]);
+        ptr[b + 1] = ptr[b] + (int32_t)ent_list[b].size();
     }
     std::vector<int32_t> hl((size_t)ptr[nbk]);
+    std::vector<uint16_t> hs((size_t)ptr[nbk]);
     for (int b = 0; b < nbk; ++b) {
-        int32_t *o = hl.data() + ptr[b];
-        for (int t = 0; t < PDX_BLOCK; ++t) o[t] = b * PDX_BLOCK + t;
-        std::copy(other[b].begin(), other[b].end(), o + PDX_BLOCK);
+        std::copy(ent_list[b].begin(), ent_list[b].end(), hl.begin() + ptr[b]);
+        std::copy(slot_list[b].begin(), slot_list[b].end(), hs.begin() + ptr[b]);
     }
     PDX_TRY(dalloc(c->stream, &d.hl_ptr, (size_t)nbk + 1));
     PDX_TRY(dalloc(c->stream, &d.hl, hl.size()));
+    PDX_TRY(dalloc(c->stream, &d.hs, hs.size()));
     PDX_TRY(dalloc(c->stream, &d.bl, bl.size()));
     PDX_TRY(dalloc(c->stream, &d.anchor, anc.size()));
     PDX_TRY(upload(c, d.hl_ptr, ptr));
     PDX_TRY(upload(c, d.hl, hl));
+    PDX_TRY(upload(c, d.hs, hs));
     PDX_TRY(upload(c, d.bl, bl));
     PDX_TRY(upload(c, d.anchor, anc));
-    d.halo_cap = (int32_t)((hmax + 3) & ~(int64_t)3);
+    d.halo_cap = (int32_t)hmax;
     return PDX_OK;
 }
 
@@ -1219,8 +1266,9 @@ extern "C" int pdx_time_kernels(pdx_ctx *c, int32_t iters, double *ms)
         PDX_HIP(hipEventRecord(ev[3 * t], c->stream));
         if (c->form == PDX_FORM_NOSB) PDX_TRY(pdx_launch_fpass(c, c->ucur, false, false, false));
         PDX_HIP(hipEventRecord(ev[3 * t + 1], c->stream));
-        PDX_TRY(pdx_launch_force(c, c->ucur, 1, false, 0.0, 0.0));
+        PDX_TRY(pdx_launch_force_pass(c, c->ucur, 1, false, 0.0, 0.0));
         PDX_HIP(hipEventRecord(ev[3 * t + 2], c->stream));
+        PDX_TRY(pdx_launch_case_reduce(c, c->ucur, 1, false));
         c->ucur ^= 1;
     }
     PDX_HIP(hipStreamSynchronize(c->stream));

@@ -137,14 +137,16 @@ struct Dev {
     uint32_t *bj;
     // halo tables of the LDS-staged hot passes (null when not built): per
     // block the particles its rows reach (hl_ptr / hl: own particles first,
-    // then the others in index order), the bond's position in that list in
-    // the SELL layout of bj (bl: PDX_HBROKEN set for a broken bond, 0xFFFF
-    // for no bond) and the FP64 anchor the staged positions are relative to
+    // then the others in index order) and each one's LDS slot (hs), the LDS
+    // slot of each bond in the SELL layout of bj (bl: PDX_HBROKEN set for a
+    // broken bond, 0xFFFF for no bond) and the FP64 anchor the staged
+    // positions are relative to
     int32_t *hl_ptr;
     int32_t *hl;
+    uint16_t *hs;
     uint16_t *bl;
     pdx_d4 *anchor;
-    int32_t halo_cap;  // LDS entries per block: max halo size rounded up to 4
+    int32_t halo_cap;  // LDS slots per block: the largest block's slot count (a multiple of 16)
     int32_t halo_pad;
     int32_t *case_of_blk;
     int32_t *case_blk0;  // first block of each case (ncase+1)
@@ -331,13 +333,15 @@ PDX_HD float bond_weight(float L, float invh, float Vj)
 // F-pass term of one bond: Hd += w V_j du (x) xi (FP32 product, FP64 sum).
 // H is accumulated in FP64: a 100+-term FP32 sum carries ~1e-6 relative error
 // into every A_j and from there into the residual.
-// (bond vector xi, displacement difference du, neighbour volume Vj)
+// (bond vector xi, displacement difference du, neighbour volume Vj).  A slot
+// without a live bond passes the particle's own record: xi = du = 0, so its
+// term is exactly zero and no select is needed.
 template <int D>
-PDX_HD void fpass_bond_v(const float *xi, const float *du, float Vj, float invh, bool live, double *Hd)
+PDX_HD void fpass_bond_v(const float *xi, const float *du, float Vj, float invh, double *Hd)
 {
     float L2 = 0.f;
     for (int r = 0; r < D; ++r) L2 = fmaf(xi[r], xi[r], L2);
-    const float w = live ? bond_weight(bond_sqrt(L2), invh, Vj) : 0.f;
+    const float w = bond_weight(bond_sqrt(L2), invh, Vj);
     for (int r = 0; r < D; ++r) {
         const double wd = (double)(w * du[r]);
         for (int c = 0; c < D; ++c) Hd[r * D + c] = fma(wd, (double)xi[c], Hd[r * D + c]);
@@ -345,14 +349,14 @@ PDX_HD void fpass_bond_v(const float *xi, const float *du, float Vj, float invh,
 }
 
 template <int D>
-PDX_HD void fpass_bond(const XRec &pi, const XRec &pj, float invh, bool live, double *Hd)
+PDX_HD void fpass_bond(const XRec &pi, const XRec &pj, float invh, double *Hd)
 {
     float xi[3], du[3];
     for (int r = 0; r < 3; ++r) {
         xi[r] = (float)(pj.X[r] - pi.X[r]);
         du[r] = (float)(pj.u[r] - pi.u[r]);
     }
-    fpass_bond_v<D>(xi, du, pj.V, invh, live, Hd);
+    fpass_bond_v<D>(xi, du, pj.V, invh, Hd);
 }
 
 // entry of bond slot k of a row, or "no bond" past the row's end
@@ -390,7 +394,7 @@ PDX_HD void fpass_one(const Dev &d, int i, int ubuf)
             live[q] = ec[q] < (uint32_t)d.ni;
             pj[q] = X[live[q] ? (int)ec[q] : i];
         }
-        for (int q = 0; q < G; ++q) fpass_bond<D>(pi, pj[q], invh, live[q], Hd);
+        for (int q = 0; q < G; ++q) fpass_bond<D>(pi, pj[q], invh, Hd);
         for (int q = 0; q < G; ++q) ec[q] = en[q];
     }
     fpass_finish<D, MODEL, DIAG, COMMIT>(d, i, kr, Hd);
@@ -465,7 +469,7 @@ PDX_HD void fpass_finish(const Dev &d, int i, const KRec &kr, const double *Hd)
 // bond product is rounded relative to |A_j - A_i| rather than |A|.
 template <int D, int FORM>
 PDX_HD void force_bond_v(const float *xi, const float *du, float Vj, float ci, float cj, const float *Ai,
-                         const float *Aj, float invh, float Gs, bool live, double *fa)
+                         const float *Aj, float invh, float Gs, double *fa)
 {
     float L2 = 0.f, xd = 0.f, dd = 0.f;
     for (int r = 0; r < D; ++r) {
@@ -495,20 +499,20 @@ PDX_HD void force_bond_v(const float *xi, const float *du, float Vj, float ci, f
     }
     for (int r = 0; r < D; ++r) {
         const float contrib = w * fmaf(g, eta[r], tb[r]);
-        fa[r] += live ? (double)contrib : 0.0;
+        fa[r] += (double)contrib;
     }
 }
 
 template <int D, int FORM>
 PDX_HD void force_bond(const XRec &pi, const XRec &pj, const float *Ai, const float *Aj, float invh, float Gs,
-                       bool live, double *fa)
+                       double *fa)
 {
     float xi[3], du[3];
     for (int r = 0; r < 3; ++r) {
         xi[r] = (float)(pj.X[r] - pi.X[r]);
         du[r] = (float)(pj.u[r] - pi.u[r]);
     }
-    force_bond_v<D, FORM>(xi, du, pj.V, pi.c, pj.c, Ai, Aj, invh, Gs, live, fa);
+    force_bond_v<D, FORM>(xi, du, pj.V, pi.c, pj.c, Ai, Aj, invh, Gs, fa);
 }
 
 template <int D, int FORM, int UPD>
@@ -542,7 +546,7 @@ PDX_HD void force_one(const Dev &d, int i, int ubuf, int cs, double dt, double v
             pj[q] = X[j];
             if (FORM == 0) aj[q] = d.AR[j];
         }
-        for (int q = 0; q < G; ++q) force_bond<D, FORM>(pi, pj[q], ai.A, aj[q].A, invh, Gs, live[q], fa);
+        for (int q = 0; q < G; ++q) force_bond<D, FORM>(pi, pj[q], ai.A, aj[q].A, invh, Gs, fa);
         for (int q = 0; q < G; ++q) ec[q] = en[q];
     }
     force_finish<D, FORM, UPD>(d, i, kr, pi, ai, ubuf, cs, dt, visc, fa, red);

@@ -20,6 +20,7 @@ struct pdx_ctx {
     std::vector<int32_t> case_p0;      // internal start per case (ncase+1)
     std::vector<int32_t> perm;         // internal -> caller row (-1 padding)
     std::vector<int32_t> inv;          // caller row -> internal
+    std::vector<uint8_t> colour;       // LDS bank colour per internal particle
     std::vector<double> hX;            // caller-order reference positions
     bool have_particles = false, have_bonds = false, have_K = false, have_mass = false;
     bool have_A = false;
@@ -60,7 +61,11 @@ int pdx_launch_count(pdx_ctx *c, int32_t *d_counts);
 int pdx_launch_fill(pdx_ctx *c);
 int pdx_launch_shape(pdx_ctx *c, int mode, double tol);
 int pdx_launch_fpass(pdx_ctx *c, int ubuf, bool diag, bool commit, bool solve_mode);
+// force pass followed by the per-case reduction and update (k_case_reduce)
 int pdx_launch_force(pdx_ctx *c, int ubuf, int upd, bool solve_mode, double dt, double visc);
+// the two launches separately (pdx_time_kernels times the force pass alone)
+int pdx_launch_force_pass(pdx_ctx *c, int ubuf, int upd, bool solve_mode, double dt, double visc);
+int pdx_launch_case_reduce(pdx_ctx *c, int ubuf, int upd, bool solve_mode);
 int pdx_launch_kd_move(pdx_ctx *c, int ubuf_in, bool solve_mode, double dt, bool apply_reset);
 int pdx_launch_halo_bits(pdx_ctx *c);
 int pdx_halo_lds(const pdx_ctx *c, int entry_bytes, int split, int nv);

@@ -35,14 +35,14 @@ __device__ __forceinline__ double ld_coherent(const double *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Block sum of NV doubles into partials[lb]; returns true in the block that
-// arrives last for its case, with tot[] = case totals summed in block order
-// (a fixed order: the result does not depend on block scheduling).
+// Block sum of NV doubles into d.part[lb * 4 + q] (waves in order).  The
+// case totals are summed by k_case_reduce, a separate launch: a last-block
+// reduction inside the pass would need an agent-scope release per block,
+// which writes back the XCD's L2 (every block's stores) on gfx950.
 template <int NV, int NT = PDX_BLOCK>
-__device__ bool reduce_case(const Dev &d, int lb, int cs, double (&val)[NV], double (&tot)[NV])
+__device__ void block_partials(const Dev &d, int lb, double (&val)[NV])
 {
     __shared__ double sred[NT / 64][NV];
-    __shared__ int s_last;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
@@ -57,29 +57,40 @@ __device__ bool reduce_case(const Dev &d, int lb, int cs, double (&val)[NV], dou
             for (int w = 0; w < NT / 64; ++w) t += sred[w][q];
             d.part[(int64_t)lb * 4 + q] = t;
         }
-        __threadfence();
-        const int nbc = d.case_blk0[cs + 1] - d.case_blk0[cs];
-        unsigned prev = atomicAdd(&d.sc[cs].counter, 1u);
-        s_last = (prev == (unsigned)(nbc - 1));
     }
-    __syncthreads();
-    if (!s_last) return false;
-    __threadfence();
+}
+
+// case totals of the block partials (strided per thread, then waves in
+// order: a fixed order, independent of block scheduling) and the per-case
+// update; MODE 0..2 = the force pass's UPD, MODE 3 = residual only.  One
+// block per case.
+template <int MODE>
+__global__ __launch_bounds__(PDX_BLOCK) void k_case_reduce(Dev d, int ubuf, int solve_mode)
+{
+    constexpr int NV = MODE == 3 ? 1 : 3;
+    __shared__ double sred[PDX_BLOCK / 64][NV];
+    const int cs = blockIdx.x;
+    if (solve_mode && d.sc[cs].converged) return;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int b0 = d.case_blk0[cs], b1 = d.case_blk0[cs + 1];
+    double tot[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         double t = 0.0;
-        for (int b = b0 + threadIdx.x; b < b1; b += NT) t += ld_coherent(&d.part[(int64_t)b * 4 + q]);
+        for (int b = b0 + threadIdx.x; b < b1; b += PDX_BLOCK) t += d.part[(int64_t)b * 4 + q];
         t = wave_sum(t);
-        __syncthreads();
         if (lane == 0) sred[wid][q] = t;
         __syncthreads();
         double s = 0.0;
-        for (int w = 0; w < NT / 64; ++w) s += sred[w][q];
+        for (int w = 0; w < PDX_BLOCK / 64; ++w) s += sred[w][q];
         tot[q] = s;
+        __syncthreads();
     }
-    if (threadIdx.x == 0) d.sc[cs].counter = 0u;
-    return true;
+    if (threadIdx.x != 0) return;
+    if (MODE == 3)
+        d.sc[cs].res = sqrt(tot[0]);
+    else
+        case_update<MODE>(d.sc[cs], tot, ubuf);
 }
 
 // ---------------------------------------------------------------------------
@@ -104,9 +115,7 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_force(Dev d, int ubuf, int solve_
     const int i = lb * PDX_BLOCK + threadIdx.x;
     double red[3] = {0.0, 0.0, 0.0};
     if (i < d.ni) force_one<D, FORM, UPD, G>(d, i, ubuf, cs, dt, visc, red);
-    double tot[3];
-    if (!reduce_case<3>(d, lb, cs, red, tot)) return;
-    if (threadIdx.x == 0) case_update<UPD>(d.sc[cs], tot, ubuf);
+    block_partials<3>(d, lb, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -241,7 +250,7 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_fpass_w(Dev d, int ubuf, int solv
 #pragma unroll
             for (int p = 0; p < 4; ++p) rq[p] = tw[xq_slot(lane, p)];
             wave_lds_sync();
-            fpass_bond<D>(pi, xrec_of(rq), invh, sl[q].live, Hd);
+            fpass_bond<D>(pi, xrec_of(rq), invh, Hd);
         }
 #pragma unroll
         for (int q = 0; q < G; ++q) ec[q] = en[q];
@@ -360,16 +369,14 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_force_w(Dev d, int ubuf, int solv
                     for (int p = 0; p < 3; ++p) ra[p] = ta[xq_slot(lane, p)];
                 wave_lds_sync();
                 if (FORM == 0) arec_of(ra, aj);
-                force_bond<D, FORM>(pi, xrec_of(rq), ai.A, aj, invh, Gs, sl[q].live, fa);
+                force_bond<D, FORM>(pi, xrec_of(rq), ai.A, aj, invh, Gs, fa);
             }
 #pragma unroll
             for (int q = 0; q < G; ++q) ec[q] = en[q];
         }
         if (act) force_finish<D, FORM, UPD>(d, i, kr, pi, ai, ubuf, cs, dt, visc, fa, red);
     }
-    double tot[3];
-    if (!reduce_case<3>(d, lb, cs, red, tot)) return;
-    if (threadIdx.x == 0) case_update<UPD>(d.sc[cs], tot, ubuf);
+    block_partials<3>(d, lb, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -379,13 +386,15 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_force_w(Dev d, int ubuf, int solv
 // (the block's halo), and the gathering kernels fetch each of those records
 // once per bond, ~116 times per pass over all blocks, through an L1 that
 // holds a few hundred records.  Here a block first stages its halo into LDS
-// (one 64-byte record per entry, entries in index order, so the loads are
-// mostly contiguous), then every lane walks its row reading LDS at the bond's
-// halo position (Dev::bl).  Staged per entry, in separate LDS arrays:
-//   xv  : X - anchor in FP32 (3), V                     16 B
-//   u   : u in FP64                                     24 B (16 + 8)
-//   force pass, NOSB: A (9) and c                       40 B (16 + 16 + 8)
-//   force pass, bond-based: c                            8 B
+// (one 64-byte record per entry), then every lane walks its row reading LDS
+// at the bond's slot (Dev::bl).  Slots follow the particles' bank colours, so
+// the 16 lanes that read together mostly hit 16 different bank groups.
+// Staged per slot, in separate LDS arrays of 16-byte entries (8 for u_z in
+// the F-pass):
+//   xv  : X - anchor in FP32 (3), V
+//   u01 : u_x, u_y in FP64
+//   F-pass: u2 = u_z in FP64;  force pass: uac = u_z in FP64, A_zz, c
+//   force pass, NOSB: a0 = A (0..3), a1 = A (4..7)
 // The anchor is an FP64 point at the block's centre, so |X - anchor| is
 // bounded by the block's extent plus the horizon and the staged bond vector
 // xv_j - xv_i differs from the FP64 difference by that extent's FP32
@@ -394,10 +403,10 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_force_w(Dev d, int ubuf, int solv
 // fpass_bond_v / force_bond_v.
 typedef float pdx_q2 __attribute__((ext_vector_type(2)));
 
-// byte offsets of the LDS arrays, cap entries each
+// byte offsets of the LDS arrays, cap slots each
 template <int FORCE, int FORM>
 struct HaloOff {
-    static constexpr int xv = 0, u01 = 16, u2 = 32, a0 = 40, a1 = 56, ac = FORM == 0 ? 72 : 40;
+    static constexpr int xv = 0, u01 = 16, u2 = 32, a0 = 48, a1 = 64;
     static constexpr int bytes = !FORCE ? 40 : (FORM == 0 ? 80 : 48);
 };
 
@@ -407,57 +416,65 @@ __device__ __forceinline__ T *lds_at(pdx_q4 *base, int off, int cap)
     return reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (size_t)off * cap);
 }
 
-// stage the halo of block lb with NT threads (ends with a block barrier)
+// stage the halo of block lb with NT threads (ends with a block barrier).
+// Four lanes per entry: lane L of a wave loads the 16-byte quarter L & 3 of
+// entry 16 k + (L >> 2), so one load instruction covers 16 consecutive halo
+// entries (mostly consecutive records, as the halo is in index order); each
+// lane converts its own quarter and writes it to the entry's LDS slot.
 template <int FORCE, int FORM, int NT>
 __device__ __forceinline__ void halo_stage(const Dev &d, int ubuf, int lb, pdx_q4 *lds)
 {
     using O = HaloOff<FORCE, FORM>;
+    constexpr int NW = NT / 64;
+    constexpr int U = 4;  // 16-entry groups per wave in flight
     const int cap = d.halo_cap;
     const int h0 = d.hl_ptr[lb], nh = d.hl_ptr[lb + 1] - h0;
     const pdx_d4 an = d.anchor[lb];
     const pdx_q4 *__restrict__ Xq = reinterpret_cast<const pdx_q4 *>(d.xu[ubuf]);
     const pdx_q4 *__restrict__ Aq = reinterpret_cast<const pdx_q4 *>(d.AR);
-    pdx_q4 *xv = lds_at<pdx_q4>(lds, O::xv, cap);
-    pdx_q4 *u01 = lds_at<pdx_q4>(lds, O::u01, cap);
-    pdx_q2 *u2 = lds_at<pdx_q2>(lds, O::u2, cap);
-    pdx_q4 *a0 = lds_at<pdx_q4>(lds, O::a0, cap);
-    pdx_q4 *a1 = lds_at<pdx_q4>(lds, O::a1, cap);
-    pdx_q2 *ac = lds_at<pdx_q2>(lds, O::ac, cap);
-    constexpr int U = NT > PDX_BLOCK ? 2 : 4;  // entries per thread in flight
-    for (int e0 = threadIdx.x; e0 < nh; e0 += U * NT) {
-        int j[U];
-        pdx_q4 r[U][4], a[U][3];
+    char *xv = reinterpret_cast<char *>(lds) + (size_t)O::xv * cap;
+    char *u01 = reinterpret_cast<char *>(lds) + (size_t)O::u01 * cap;
+    char *u2 = reinterpret_cast<char *>(lds) + (size_t)O::u2 * cap;
+    char *a0 = reinterpret_cast<char *>(lds) + (size_t)O::a0 * cap;
+    char *a1 = reinterpret_cast<char *>(lds) + (size_t)O::a1 * cap;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int q = lane & 3, sub = lane >> 2;
+    const int qa = q < 3 ? q : 2;  // A records have 3 quarters
+    for (int g0 = wv * 16; g0 < nh; g0 += U * NW * 16) {
+        int j[U], sl[U];
+        bool ok[U];
+        pdx_q4 r[U], a[U];
 #pragma unroll
         for (int v = 0; v < U; ++v) {
-            const int e = e0 + v * NT;
-            j[v] = d.hl[h0 + (e < nh ? e : 0)];
+            const int e = g0 + v * NW * 16 + sub;
+            ok[v] = e < nh;
+            j[v] = d.hl[h0 + (ok[v] ? e : 0)];
+            sl[v] = ok[v] ? (int)d.hs[h0 + e] : 0;
         }
 #pragma unroll
         for (int v = 0; v < U; ++v) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) r[v][q] = Xq[(int64_t)j[v] * 4 + q];
-            if (FORCE && FORM == 0)
-#pragma unroll
-                for (int q = 0; q < 3; ++q) a[v][q] = Aq[(int64_t)j[v] * 3 + q];
+            r[v] = Xq[(int64_t)j[v] * 4 + q];
+            if (FORCE && FORM == 0) a[v] = Aq[(int64_t)j[v] * 3 + qa];
         }
 #pragma unroll
         for (int v = 0; v < U; ++v) {
-            const int e = e0 + v * NT;
-            if (e < nh) {
-                const PdxD2 x01 = __builtin_bit_cast(PdxD2, r[v][0]);
-                const double x2 = __builtin_bit_cast(double, r[v][1].xy);
-                xv[e] = pdx_q4{(float)(x01.a - an.x), (float)(x01.b - an.y), (float)(x2 - an.z), r[v][1].z};
-                u01[e] = r[v][2];
-                u2[e] = r[v][3].xy;
-                if (FORCE) {
-                    if (FORM == 0) {
-                        a0[e] = a[v][0];
-                        a1[e] = a[v][1];
-                        ac[e] = pdx_q2{a[v][2].x, r[v][1].w};
-                    } else {
-                        ac[e] = pdx_q2{0.f, r[v][1].w};
-                    }
-                }
+            if (!ok[v]) continue;
+            const int s16 = sl[v] * 16;
+            if (q == 0) {
+                const PdxD2 x = __builtin_bit_cast(PdxD2, r[v]);
+                *reinterpret_cast<pdx_q2 *>(xv + s16) = pdx_q2{(float)(x.a - an.x), (float)(x.b - an.y)};
+                if (FORCE && FORM == 0) *reinterpret_cast<pdx_q4 *>(a0 + s16) = a[v];
+            } else if (q == 1) {
+                const double x2 = __builtin_bit_cast(double, r[v].xy);
+                *reinterpret_cast<pdx_q2 *>(xv + s16 + 8) = pdx_q2{(float)(x2 - an.z), r[v].z};
+                if (FORCE) *reinterpret_cast<float *>(u2 + s16 + 12) = r[v].w;  // c
+                if (FORCE && FORM == 0) *reinterpret_cast<pdx_q4 *>(a1 + s16) = a[v];
+            } else if (q == 2) {
+                *reinterpret_cast<pdx_q4 *>(u01 + s16) = r[v];
+                if (FORCE) *reinterpret_cast<float *>(u2 + s16 + 8) = FORM == 0 ? a[v].x : 0.f;  // A_zz
+            } else {
+                if (FORCE) *reinterpret_cast<pdx_q2 *>(u2 + s16) = r[v].xy;
+                else *reinterpret_cast<pdx_q2 *>(u2 + sl[v] * 8) = r[v].xy;
             }
         }
     }
@@ -519,9 +536,10 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_h(Dev d, int ubuf, int 
     const int i = lb * PDX_BLOCK + t;
     const KRec kr = d.KR[i];
     const bool act = (kr.flags & PDX_FL_ACTIVE) != 0;
-    const pdx_q4 xo = xv[t];
-    const PdxD2 uo01 = __builtin_bit_cast(PdxD2, u01[t]);
-    const double uo2 = __builtin_bit_cast(double, u2[t]);
+    const int so = d.hs[d.hl_ptr[lb] + t];
+    const pdx_q4 xo = xv[so];
+    const PdxD2 uo01 = __builtin_bit_cast(PdxD2, u01[so]);
+    const double uo2 = __builtin_bit_cast(double, u2[so]);
     const float invh = (float)d.inv_h;
     double Hd[D * D];
 #pragma unroll
@@ -544,7 +562,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_h(Dev d, int ubuf, int 
 #pragma unroll
         for (int q = 0; q < G; ++q) {
             live[q] = act && !(ec[q] & PDX_HBROKEN);
-            const int l = live[q] ? (int)ec[q] : t;
+            const int l = live[q] ? (int)ec[q] : so;
             xj[q] = xv[l];
             uj01[q] = u01[l];
             uj2[q] = u2[l];
@@ -553,7 +571,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_h(Dev d, int ubuf, int 
         for (int q = 0; q < G; ++q) {
             float xi[3], du[3];
             halo_bond(xj[q], uj01[q], uj2[q], xo, uo01, uo2, xi, du);
-            fpass_bond_v<D>(xi, du, xj[q].w, invh, live[q], Hd);
+            fpass_bond_v<D>(xi, du, xj[q].w, invh, Hd);
         }
 #pragma unroll
         for (int q = 0; q < G; ++q) ec[q] = en[q];
@@ -576,10 +594,9 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
         const int cap = d.halo_cap;
         const pdx_q4 *xv = lds_at<pdx_q4>(hlds, O::xv, cap);
         const pdx_q4 *u01 = lds_at<pdx_q4>(hlds, O::u01, cap);
-        const pdx_q2 *u2 = lds_at<pdx_q2>(hlds, O::u2, cap);
+        const pdx_q4 *uac = lds_at<pdx_q4>(hlds, O::u2, cap);
         const pdx_q4 *a0 = lds_at<pdx_q4>(hlds, O::a0, cap);
         const pdx_q4 *a1 = lds_at<pdx_q4>(hlds, O::a1, cap);
-        const pdx_q2 *ac = lds_at<pdx_q2>(hlds, O::ac, cap);
         const int t = threadIdx.x % PDX_BLOCK, lane = t & 63, part = threadIdx.x / PDX_BLOCK;
         const int i = lb * PDX_BLOCK + t;
         const KRec kr = d.KR[i];
@@ -587,9 +604,10 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
         const XRec pi = d.xu[ubuf][i];
         ARec ai{};
         if (FORM == 0) ai = d.AR[i];
-        const pdx_q4 xo = xv[t];
-        const PdxD2 uo01 = __builtin_bit_cast(PdxD2, u01[t]);
-        const double uo2 = __builtin_bit_cast(double, u2[t]);
+        const int so = d.hs[d.hl_ptr[lb] + t];
+        const pdx_q4 xo = xv[so];
+        const PdxD2 uo01 = __builtin_bit_cast(PdxD2, u01[so]);
+        const double uo2 = __builtin_bit_cast(double, uac[so].xy);
         const float invh = (float)d.inv_h;
         const float Gs = (float)d.G_s;
         double fa[3] = {0.0, 0.0, 0.0};
@@ -606,16 +624,14 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
             for (int q = 0; q < G; ++q)
                 en[q] = (k + STEP + q) < width ? bp[(int64_t)(k + STEP + q) * PDX_SLICE] : PDX_HNONE;
             bool live[G];
-            pdx_q4 xj[G], uj01[G], aj0[G], aj1[G];
-            pdx_q2 uj2[G], ajc[G];
+            pdx_q4 xj[G], uj01[G], ujc[G], aj0[G], aj1[G];
 #pragma unroll
             for (int q = 0; q < G; ++q) {
                 live[q] = act && !(ec[q] & PDX_HBROKEN);
-                const int l = live[q] ? (int)ec[q] : t;
+                const int l = live[q] ? (int)ec[q] : so;
                 xj[q] = xv[l];
                 uj01[q] = u01[l];
-                uj2[q] = u2[l];
-                ajc[q] = ac[l];
+                ujc[q] = uac[l];
                 if (FORM == 0) {
                     aj0[q] = a0[l];
                     aj1[q] = a1[l];
@@ -624,14 +640,14 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
 #pragma unroll
             for (int q = 0; q < G; ++q) {
                 float xi[3], du[3];
-                halo_bond(xj[q], uj01[q], uj2[q], xo, uo01, uo2, xi, du);
+                halo_bond(xj[q], uj01[q], ujc[q].xy, xo, uo01, uo2, xi, du);
                 float aj[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 if (FORM == 0) {
                     aj[0] = aj0[q].x; aj[1] = aj0[q].y; aj[2] = aj0[q].z; aj[3] = aj0[q].w;
                     aj[4] = aj1[q].x; aj[5] = aj1[q].y; aj[6] = aj1[q].z; aj[7] = aj1[q].w;
-                    aj[8] = ajc[q].x;
+                    aj[8] = ujc[q].z;
                 }
-                force_bond_v<D, FORM>(xi, du, xj[q].w, pi.c, ajc[q].y, ai.A, aj, invh, Gs, live[q], fa);
+                force_bond_v<D, FORM>(xi, du, xj[q].w, pi.c, ujc[q].w, ai.A, aj, invh, Gs, fa);
             }
 #pragma unroll
             for (int q = 0; q < G; ++q) ec[q] = en[q];
@@ -639,9 +655,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
         split_sum<3, S>(hlds, fa);
         if (part == 0 && act) force_finish<D, FORM, UPD>(d, i, kr, pi, ai, ubuf, cs, dt, visc, fa, red);
     }
-    double tot[3];
-    if (!reduce_case<3, PDX_BLOCK * S>(d, lb, cs, red, tot)) return;
-    if (threadIdx.x == 0) case_update<UPD>(d.sc[cs], tot, ubuf);
+    block_partials<3, PDX_BLOCK * S>(d, lb, red);
 }
 
 // broken flags of the halo-local entries from bj (after bj was rewritten)
@@ -669,12 +683,9 @@ template <int D>
 __global__ __launch_bounds__(PDX_BLOCK) void k_resid(Dev d)
 {
     const int lb = logical_block();
-    const int cs = d.case_of_blk[lb];
     const int i = lb * PDX_BLOCK + threadIdx.x;
     double red[1] = {i < d.ni ? resid_one<D>(d, i) : 0.0};
-    double tot[1];
-    if (!reduce_case<1>(d, lb, cs, red, tot)) return;
-    if (threadIdx.x == 0) d.sc[cs].res = sqrt(tot[0]);
+    block_partials<1>(d, lb, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -910,6 +921,13 @@ static void launch_force_t(pdx_ctx *c, int ubuf, int upd, int sm, double dt, dou
 
 int pdx_launch_force(pdx_ctx *c, int ubuf, int upd, bool solve_mode, double dt, double visc)
 {
+    const int r = pdx_launch_force_pass(c, ubuf, upd, solve_mode, dt, visc);
+    if (r != PDX_OK) return r;
+    return pdx_launch_case_reduce(c, ubuf, upd, solve_mode);
+}
+
+int pdx_launch_force_pass(pdx_ctx *c, int ubuf, int upd, bool solve_mode, double dt, double visc)
+{
     const int sm = solve_mode ? 1 : 0;
     if (c->dim == 3) {
         if (c->form == PDX_FORM_BOND) launch_force_t<3, PDX_FORM_BOND>(c, ubuf, upd, sm, dt, visc);
@@ -1006,6 +1024,21 @@ int pdx_launch_resid(pdx_ctx *c)
         hipLaunchKernelGGL((k_resid<3>), dim3(c->nblk), dim3(PDX_BLOCK), 0, c->stream, c->d);
     else
         hipLaunchKernelGGL((k_resid<2>), dim3(c->nblk), dim3(PDX_BLOCK), 0, c->stream, c->d);
+    PDX_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_case_reduce<3>), dim3(c->ncase), dim3(PDX_BLOCK), 0, c->stream, c->d, 0, 0);
+    PDX_LAUNCH_CHECK();
+    return PDX_OK;
+}
+
+int pdx_launch_case_reduce(pdx_ctx *c, int ubuf, int upd, bool solve_mode)
+{
+    const int sm = solve_mode ? 1 : 0;
+    if (upd == 1)
+        hipLaunchKernelGGL((k_case_reduce<1>), dim3(c->ncase), dim3(PDX_BLOCK), 0, c->stream, c->d, ubuf, sm);
+    else if (upd == 2)
+        hipLaunchKernelGGL((k_case_reduce<2>), dim3(c->ncase), dim3(PDX_BLOCK), 0, c->stream, c->d, ubuf, sm);
+    else
+        hipLaunchKernelGGL((k_case_reduce<0>), dim3(c->ncase), dim3(PDX_BLOCK), 0, c->stream, c->d, ubuf, sm);
     PDX_LAUNCH_CHECK();
     return PDX_OK;
 }

@@ -53,14 +53,34 @@ inline std::vector<int32_t> case_starts(const std::vector<int64_t> &off)
     return p0;
 }
 
+// Bank colour (0..15) of the integer cell q: (x mod 2, y mod 2, z mod 4) in
+// 3-D, (x mod 4, y mod 4) in 2-D.  Any box of 2x2x4 (2-D: 4x4) cells holds
+// 16 different colours, and so does every shift of it; an aligned run of 16
+// Morton keys is such a box, and a run of 16 lanes that straddles two of
+// them still covers each colour once, since the colour is periodic.  So the
+// 16 lanes that read an LDS array together, whose neighbours at one stencil
+// offset form the shifted run, mostly hit 16 different bank groups of a
+// 16-byte-per-entry array.
+inline int bank_colour(int D, const uint64_t *q)
+{
+    if (D == 3) return (int)((q[0] & 1) | ((q[1] & 1) << 1) | ((q[2] & 3) << 2));
+    return (int)((q[0] & 3) | ((q[1] & 3) << 2));
+}
+
 // Morton order of the caller rows of each case: perm[internal] = caller row
-// (-1 for padding), inv[caller row] = internal
-inline void morton_order(int D, const double *X, const std::vector<int64_t> &off, const std::vector<int32_t> &p0,
-                         std::vector<int32_t> &perm, std::vector<int32_t> &inv)
+// (-1 for padding), inv[caller row] = internal; colour[internal] = bank
+// colour of the particle's cell (0 for padding).  Positions are quantised to
+// cells of the case's median particle spacing vol^(1/D), rounded to the
+// nearest cell, so a lattice specimen maps one particle to one cell and an
+// aligned run of 16 keys is one 2x2x4 (2-D: 4x4) box of lattice sites.
+inline void morton_order(int D, const double *X, const double *vol, const std::vector<int64_t> &off,
+                         const std::vector<int32_t> &p0, std::vector<int32_t> &perm, std::vector<int32_t> &inv,
+                         std::vector<uint8_t> &colour)
 {
     const int nc = (int)off.size() - 1;
     perm.assign(p0[nc], -1);
     inv.assign(off[nc], -1);
+    colour.assign(p0[nc], 0);
     for (int k = 0; k < nc; ++k) {
         const int64_t r0 = off[k], r1 = off[k + 1], nk = r1 - r0;
         if (nk == 0) continue;
@@ -70,23 +90,27 @@ inline void morton_order(int D, const double *X, const std::vector<int64_t> &off
                 lo[q] = std::min(lo[q], X[r * D + q]);
                 hi[q] = std::max(hi[q], X[r * D + q]);
             }
-        double vol_box = 1.0;
-        for (int q = 0; q < D; ++q) vol_box *= std::max(hi[q] - lo[q], 1e-300);
-        double hq = pow(vol_box / (double)nk, 1.0 / D);
+        std::vector<double> sp(nk);
+        for (int64_t r = r0; r < r1; ++r) sp[r - r0] = pow(vol[r], 1.0 / D);
+        std::nth_element(sp.begin(), sp.begin() + nk / 2, sp.end());
+        double hq = sp[nk / 2];
         for (int q = 0; q < D; ++q) hq = std::max(hq, (hi[q] - lo[q]) / (D == 3 ? 2097151.0 : 2147483647.0));
-        if (!(hq > 0.0)) hq = 1.0;
+        if (!(hq > 0.0) || !isfinite(hq)) hq = 1.0;
         std::vector<std::pair<uint64_t, int32_t>> key(nk);
+        std::vector<uint8_t> col(nk);
         for (int64_t r = r0; r < r1; ++r) {
             uint64_t qv[3] = {0, 0, 0};
-            for (int q = 0; q < D; ++q) qv[q] = (uint64_t)floor((X[r * D + q] - lo[q]) / hq);
+            for (int q = 0; q < D; ++q) qv[q] = (uint64_t)floor((X[r * D + q] - lo[q]) / hq + 0.5);
             const uint64_t kk = D == 3 ? (spread3(qv[0]) << 2 | spread3(qv[1]) << 1 | spread3(qv[2]))
                                        : (spread2(qv[0]) << 1 | spread2(qv[1]));
             key[r - r0] = {kk, (int32_t)r};
+            col[r - r0] = (uint8_t)bank_colour(D, qv);
         }
         std::sort(key.begin(), key.end());
         for (int64_t t = 0; t < nk; ++t) {
             perm[p0[k] + t] = key[t].second;
             inv[key[t].second] = (int32_t)(p0[k] + t);
+            colour[p0[k] + t] = col[key[t].second - r0];
         }
     }
 }

@@ -155,7 +155,8 @@ int main(int argc, char **argv)
     R.p0 = pdx_layout::case_starts(off);
     const int ni = R.p0[R.ncase];
     const int nblk = ((ni / PDX_BLOCK + PDX_NXCD - 1) / PDX_NXCD) * PDX_NXCD;
-    pdx_layout::morton_order(D, X.data(), off, R.p0, R.perm, R.inv);
+    std::vector<uint8_t> colour;
+    pdx_layout::morton_order(D, X.data(), vol.data(), off, R.p0, R.perm, R.inv, colour);
     Dev &d = R.d;
     d.ni = ni;
     d.nblk = nblk;
