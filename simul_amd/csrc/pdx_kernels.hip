@@ -601,13 +601,25 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
         const int i = lb * PDX_BLOCK + t;
         const KRec kr = d.KR[i];
         const bool act = (kr.flags & PDX_FL_ACTIVE) != 0;
-        const XRec pi = d.xu[ubuf][i];
-        ARec ai{};
-        if (FORM == 0) ai = d.AR[i];
+        // the own record and A_i come from the staged halo: force_finish
+        // reads u (and force_bond_v c) of pi, not X
         const int so = d.hs[d.hl_ptr[lb] + t];
         const pdx_q4 xo = xv[so];
         const PdxD2 uo01 = __builtin_bit_cast(PdxD2, u01[so]);
-        const double uo2 = __builtin_bit_cast(double, uac[so].xy);
+        const pdx_q4 uco = uac[so];
+        const double uo2 = __builtin_bit_cast(double, uco.xy);
+        XRec pi{};
+        pi.u[0] = uo01.a;
+        pi.u[1] = uo01.b;
+        pi.u[2] = uo2;
+        pi.c = uco.w;
+        ARec ai{};
+        if (FORM == 0) {
+            const pdx_q4 p0 = a0[so], p1 = a1[so];
+            ai.A[0] = p0.x; ai.A[1] = p0.y; ai.A[2] = p0.z; ai.A[3] = p0.w;
+            ai.A[4] = p1.x; ai.A[5] = p1.y; ai.A[6] = p1.z; ai.A[7] = p1.w;
+            ai.A[8] = uco.z;
+        }
         const float invh = (float)d.inv_h;
         const float Gs = (float)d.G_s;
         double fa[3] = {0.0, 0.0, 0.0};
