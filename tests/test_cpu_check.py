@@ -83,3 +83,14 @@ def test_kernels_in_bounds_vertebra_coarse(exe, tmp_path):
     inp = _inputs(spec.X, horizon=spec.horizon, dx=spec.dx, fixed=spec.fixed, fext=spec.fext)
     inp["bulk"], inp["shear"] = spec.bulk, spec.shear
     _run(exe, tmp_path, "vertebra", model="neo_hookean", **inp)
+
+
+def test_layout_bank_colours_and_lattice_quantisation(exe):
+    """The LDS slot colours of the halo kernels: any shifted 2x2x4 (2-D: 4x4)
+    box of cells holds 16 colours, and the Morton quantisation at the median
+    spacing puts each aligned run of 16 keys of a jittered lattice on one box
+    (tests/cpu_check/layout_check.cpp)."""
+    r = subprocess.run([os.path.join(HERE, "layout_check")], capture_output=True, text=True, timeout=120,
+                       env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"))
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "layout ok" in r.stdout
