@@ -416,6 +416,21 @@ __device__ __forceinline__ T *lds_at(pdx_q4 *base, int off, int cap)
     return reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (size_t)off * cap);
 }
 
+// byte address of field f (its offset above) of LDS slot s.  The force pass
+// keeps one record of HaloOff::bytes per slot, so a bond needs one address and
+// the fields are immediate offsets; 80 and 48 bytes are 20 and 12 dwords, odd
+// multiples of 4, so 16 consecutive slots start in 16 different 4-bank groups
+// and the colour-slot layout stays conflict-free.  The F-pass keeps one array
+// per field (40 bytes per slot in all), which leaves room for two blocks per
+// CU.
+template <int FORCE, int FORM>
+__device__ __forceinline__ char *hslot(pdx_q4 *base, int s, int f, int cap)
+{
+    char *b = reinterpret_cast<char *>(base);
+    if (FORCE) return b + s * HaloOff<1, FORM>::bytes + f;
+    return b + f * cap + s * (f == HaloOff<0, 0>::u2 ? 8 : 16);
+}
+
 // stage the halo of block lb with NT threads (ends with a block barrier).
 // Four lanes per entry: lane L of a wave loads the 16-byte quarter L & 3 of
 // entry 16 k + (L >> 2), so one load instruction covers 16 consecutive halo
@@ -432,11 +447,6 @@ __device__ __forceinline__ void halo_stage(const Dev &d, int ubuf, int lb, pdx_q
     const pdx_d4 an = d.anchor[lb];
     const pdx_q4 *__restrict__ Xq = reinterpret_cast<const pdx_q4 *>(d.xu[ubuf]);
     const pdx_q4 *__restrict__ Aq = reinterpret_cast<const pdx_q4 *>(d.AR);
-    char *xv = reinterpret_cast<char *>(lds) + (size_t)O::xv * cap;
-    char *u01 = reinterpret_cast<char *>(lds) + (size_t)O::u01 * cap;
-    char *u2 = reinterpret_cast<char *>(lds) + (size_t)O::u2 * cap;
-    char *a0 = reinterpret_cast<char *>(lds) + (size_t)O::a0 * cap;
-    char *a1 = reinterpret_cast<char *>(lds) + (size_t)O::a1 * cap;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int q = lane & 3, sub = lane >> 2;
     const int qa = q < 3 ? q : 2;  // A records have 3 quarters
@@ -459,22 +469,24 @@ __device__ __forceinline__ void halo_stage(const Dev &d, int ubuf, int lb, pdx_q
 #pragma unroll
         for (int v = 0; v < U; ++v) {
             if (!ok[v]) continue;
-            const int s16 = sl[v] * 16;
+            const int sv = sl[v];
             if (q == 0) {
                 const PdxD2 x = __builtin_bit_cast(PdxD2, r[v]);
-                *reinterpret_cast<pdx_q2 *>(xv + s16) = pdx_q2{(float)(x.a - an.x), (float)(x.b - an.y)};
-                if (FORCE && FORM == 0) *reinterpret_cast<pdx_q4 *>(a0 + s16) = a[v];
+                *reinterpret_cast<pdx_q2 *>(hslot<FORCE, FORM>(lds, sv, O::xv, cap)) =
+                    pdx_q2{(float)(x.a - an.x), (float)(x.b - an.y)};
+                if (FORCE && FORM == 0) *reinterpret_cast<pdx_q4 *>(hslot<FORCE, FORM>(lds, sv, O::a0, cap)) = a[v];
             } else if (q == 1) {
                 const double x2 = __builtin_bit_cast(double, r[v].xy);
-                *reinterpret_cast<pdx_q2 *>(xv + s16 + 8) = pdx_q2{(float)(x2 - an.z), r[v].z};
-                if (FORCE) *reinterpret_cast<float *>(u2 + s16 + 12) = r[v].w;  // c
-                if (FORCE && FORM == 0) *reinterpret_cast<pdx_q4 *>(a1 + s16) = a[v];
+                *reinterpret_cast<pdx_q2 *>(hslot<FORCE, FORM>(lds, sv, O::xv, cap) + 8) =
+                    pdx_q2{(float)(x2 - an.z), r[v].z};
+                if (FORCE) *reinterpret_cast<float *>(hslot<FORCE, FORM>(lds, sv, O::u2, cap) + 12) = r[v].w;  // c
+                if (FORCE && FORM == 0) *reinterpret_cast<pdx_q4 *>(hslot<FORCE, FORM>(lds, sv, O::a1, cap)) = a[v];
             } else if (q == 2) {
-                *reinterpret_cast<pdx_q4 *>(u01 + s16) = r[v];
-                if (FORCE) *reinterpret_cast<float *>(u2 + s16 + 8) = FORM == 0 ? a[v].x : 0.f;  // A_zz
+                *reinterpret_cast<pdx_q4 *>(hslot<FORCE, FORM>(lds, sv, O::u01, cap)) = r[v];
+                if (FORCE)  // A_zz
+                    *reinterpret_cast<float *>(hslot<FORCE, FORM>(lds, sv, O::u2, cap) + 8) = FORM == 0 ? a[v].x : 0.f;
             } else {
-                if (FORCE) *reinterpret_cast<pdx_q2 *>(u2 + s16) = r[v].xy;
-                else *reinterpret_cast<pdx_q2 *>(u2 + sl[v] * 8) = r[v].xy;
+                *reinterpret_cast<pdx_q2 *>(hslot<FORCE, FORM>(lds, sv, O::u2, cap)) = r[v].xy;
             }
         }
     }
@@ -591,12 +603,9 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
     double red[3] = {0.0, 0.0, 0.0};
     if (lb * PDX_BLOCK < d.ni) {  // grid padding blocks only join the reduction
         halo_stage<1, FORM, PDX_BLOCK * S>(d, ubuf, lb, hlds);
-        const int cap = d.halo_cap;
-        const pdx_q4 *xv = lds_at<pdx_q4>(hlds, O::xv, cap);
-        const pdx_q4 *u01 = lds_at<pdx_q4>(hlds, O::u01, cap);
-        const pdx_q4 *uac = lds_at<pdx_q4>(hlds, O::u2, cap);
-        const pdx_q4 *a0 = lds_at<pdx_q4>(hlds, O::a0, cap);
-        const pdx_q4 *a1 = lds_at<pdx_q4>(hlds, O::a1, cap);
+        // slot l's record: xv at 0, u01 at 16, uac at 32, a0 at 48, a1 at 64
+        const char *hb = reinterpret_cast<const char *>(hlds);
+        auto rec = [&](int l, int f) { return *reinterpret_cast<const pdx_q4 *>(hb + l * O::bytes + f); };
         const int t = threadIdx.x % PDX_BLOCK, lane = t & 63, part = threadIdx.x / PDX_BLOCK;
         const int i = lb * PDX_BLOCK + t;
         const KRec kr = d.KR[i];
@@ -604,9 +613,9 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
         // the own record and A_i come from the staged halo: force_finish
         // reads u (and force_bond_v c) of pi, not X
         const int so = d.hs[d.hl_ptr[lb] + t];
-        const pdx_q4 xo = xv[so];
-        const PdxD2 uo01 = __builtin_bit_cast(PdxD2, u01[so]);
-        const pdx_q4 uco = uac[so];
+        const pdx_q4 xo = rec(so, O::xv);
+        const PdxD2 uo01 = __builtin_bit_cast(PdxD2, rec(so, O::u01));
+        const pdx_q4 uco = rec(so, O::u2);
         const double uo2 = __builtin_bit_cast(double, uco.xy);
         XRec pi{};
         pi.u[0] = uo01.a;
@@ -615,7 +624,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
         pi.c = uco.w;
         ARec ai{};
         if (FORM == 0) {
-            const pdx_q4 p0 = a0[so], p1 = a1[so];
+            const pdx_q4 p0 = rec(so, O::a0), p1 = rec(so, O::a1);
             ai.A[0] = p0.x; ai.A[1] = p0.y; ai.A[2] = p0.z; ai.A[3] = p0.w;
             ai.A[4] = p1.x; ai.A[5] = p1.y; ai.A[6] = p1.z; ai.A[7] = p1.w;
             ai.A[8] = uco.z;
@@ -641,12 +650,12 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
             for (int q = 0; q < G; ++q) {
                 live[q] = act && !(ec[q] & PDX_HBROKEN);
                 const int l = live[q] ? (int)ec[q] : so;
-                xj[q] = xv[l];
-                uj01[q] = u01[l];
-                ujc[q] = uac[l];
+                xj[q] = rec(l, O::xv);
+                uj01[q] = rec(l, O::u01);
+                ujc[q] = rec(l, O::u2);
                 if (FORM == 0) {
-                    aj0[q] = a0[l];
-                    aj1[q] = a1[l];
+                    aj0[q] = rec(l, O::a0);
+                    aj1[q] = rec(l, O::a1);
                 }
             }
 #pragma unroll
