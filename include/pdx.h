@@ -175,6 +175,12 @@ int pdx_commit_state(pdx_ctx *ctx);
  * (in/out n*width doubles, width 1..64): out_i = sum_j w_ij V_j in_j / sum_j w_ij V_j.
  * No reference counterpart: the message-passing step of the learned warm start. */
 int pdx_neighbor_mean(pdx_ctx *ctx, int32_t width, const double *in, double *out);
+/* feature input of the learned warm start (no reference counterpart): `hops` successive
+ * neighbour means of `in` (n x width, caller order, host) computed on the device, and
+ * [in | hop 1 | ... | hop `hops`] written as FP32 into the DEVICE array d_out (n rows of
+ * ldo floats, caller order; columns from width * (hops + 1) on are not written).  width
+ * 1..64, hops 0..4, ldo >= width * (hops + 1).  One host-to-device copy in all. */
+int pdx_neighbor_features(pdx_ctx *ctx, int32_t width, int32_t hops, const double *in, float *d_out, int32_t ldo);
 
 /* ---- dynamic relaxation ---------------------------------------------------- */
 /* NOSBSolver._estimate_stable_dt, nosb_solver.py:94-154 */

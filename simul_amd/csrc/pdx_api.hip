@@ -986,6 +986,38 @@ extern "C" int pdx_neighbor_mean(pdx_ctx *c, int32_t width, const double *in, do
     return PDX_OK;
 }
 
+extern "C" int pdx_neighbor_features(pdx_ctx *c, int32_t width, int32_t hops, const double *in, float *d_out,
+                                     int32_t ldo)
+{
+    CHECK_CTX(c);
+    if (!in || !d_out) return fail(PDX_EINVAL, "in/d_out NULL");
+    if (width < 1 || width > 64) return fail(PDX_EINVAL, "width must be 1..64");
+    if (hops < 0 || hops > 4) return fail(PDX_EINVAL, "hops must be 0..4");
+    if (ldo < width * (hops + 1)) return fail(PDX_EINVAL, "ldo must be >= width * (hops + 1)");
+    if (hops > 0 && !c->have_bonds) return fail(PDX_ESTATE, "bonds not built");
+    const size_t blk = (size_t)c->ni * width;
+    std::vector<double> h(blk, 0.0);
+    for (int i = 0; i < c->ni; ++i) {
+        const int r = c->perm[i];
+        if (r >= 0) memcpy(&h[(size_t)i * width], in + (size_t)r * width, sizeof(double) * width);
+    }
+    double *dv = nullptr;
+    int32_t *dp = nullptr;
+    PDX_HIP(hipMallocAsync((void **)&dv, sizeof(double) * blk * (hops + 1), c->stream));
+    hipError_t e = hipMallocAsync((void **)&dp, sizeof(int32_t) * c->ni, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dv, h.data(), sizeof(double) * blk, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dp, c->perm.data(), sizeof(int32_t) * c->ni, hipMemcpyHostToDevice, c->stream);
+    int r = e == hipSuccess ? PDX_OK : PDX_EDEVICE;
+    for (int k = 1; k <= hops && r == PDX_OK; ++k) r = pdx_launch_nbr_mean(c, dv + (k - 1) * blk, dv + k * blk, width);
+    if (r == PDX_OK) r = pdx_launch_feat_pack(c, dv, dp, d_out, width, hops, ldo);
+    hipFreeAsync(dv, c->stream);
+    if (dp) hipFreeAsync(dp, c->stream);
+    PDX_HIP(e);
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    return r;
+}
+
 extern "C" int pdx_local_damage(pdx_ctx *c)
 {
     CHECK_CTX(c);

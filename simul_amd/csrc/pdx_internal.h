@@ -80,6 +80,8 @@ int pdx_launch_zero_free(pdx_ctx *c, int ubuf);
 int pdx_launch_kr_put(pdx_ctx *c, const float *d_src, int off, int cnt);
 int pdx_launch_xu_geom(pdx_ctx *c, const XRec *d_src);
 int pdx_launch_nbr_mean(pdx_ctx *c, const double *d_in, double *d_out, int width);
+int pdx_launch_feat_pack(pdx_ctx *c, const double *d_hv, const int32_t *d_perm, float *d_out, int width, int hops,
+                         int ldo);
 int pdx_launch_adr_mass(pdx_ctx *c, double beta);
 int pdx_launch_dt_est(pdx_ctx *c, double modulus, double *d_out);
 int pdx_launch_damage(pdx_ctx *c, double s_c, unsigned long long *d_count);

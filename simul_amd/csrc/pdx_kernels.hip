@@ -1078,6 +1078,29 @@ int pdx_launch_nbr_mean(pdx_ctx *c, const double *d_in, double *d_out, int width
     return PDX_OK;
 }
 
+// [hop 0 | hop 1 | ...] of internal-order FP64 blocks (ni x width each) into
+// the caller-order FP32 rows of out (stride ldo)
+__global__ __launch_bounds__(PDX_BLOCK) void k_feat_pack(Dev d, const double *hv, const int32_t *perm, float *out,
+                                                        int width, int hops, int ldo)
+{
+    const int i = blockIdx.x * PDX_BLOCK + threadIdx.x;
+    if (i >= d.ni) return;
+    const int r = perm[i];
+    if (r < 0) return;
+    for (int k = 0; k <= hops; ++k)
+        for (int q = 0; q < width; ++q)
+            out[(int64_t)r * ldo + k * width + q] = (float)hv[((int64_t)k * d.ni + i) * width + q];
+}
+
+int pdx_launch_feat_pack(pdx_ctx *c, const double *d_hv, const int32_t *d_perm, float *d_out, int width, int hops,
+                         int ldo)
+{
+    hipLaunchKernelGGL(k_feat_pack, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_hv, d_perm, d_out,
+                       width, hops, ldo);
+    PDX_LAUNCH_CHECK();
+    return PDX_OK;
+}
+
 int pdx_launch_xu_geom(pdx_ctx *c, const XRec *d_src)
 {
     hipLaunchKernelGGL(k_xu_geom, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_src);

@@ -11,10 +11,15 @@ Raw features (20 per particle), all dimensionless:
   13-15 H / 25 mm, a / 20 mm, b / 20 mm
   16    specimen mean of log10(E / 1000 MPa)
   17    specimen cortical fraction
-  18    1,   19  0
+  18-19 the specimen's stiffness eccentricity: the E-weighted centroid minus
+        the geometric centre, over the lateral half-widths (x, y); an axially
+        loaded column bends in proportion to it
 Two message-passing hops over the bonds (weighted neighbour means, libpdx
 pdx_neighbor_mean) of the raw vector give 20 + 20 more; 4 zeros pad the
-input to 64 columns (the GEMM K granularity).
+input to 64 columns (the GEMM K granularity).  Column 3 enters the network
+only through the two hops: the particle's own modulus is i.i.d. noise in the
+cancellous core, and a prediction that follows it is rough at the particle
+scale.
 """
 from __future__ import annotations
 
@@ -32,9 +37,12 @@ def _per_case(spec, off):
 
 def raw_features(spec, case_offsets=None) -> np.ndarray:
     X, n = spec.X, spec.n
-    F = np.zeros((n, RAW))
+    # column-major while filling (each column is one contiguous write), one
+    # transposing copy at the end
+    F = np.zeros((RAW, n))
     logE = np.log10(spec.E / 1000.0)
     cort = (spec.material_id == 0).astype(np.float64) if spec.material_id is not None else np.zeros(n)
+    loaded_all = np.abs(spec.fext).sum(1) > 0
     for a0, b0 in _per_case(spec, case_offsets):
         Xc = X[a0:b0]
         lo, hi = Xc.min(0), Xc.max(0)
@@ -42,43 +50,53 @@ def raw_features(spec, case_offsets=None) -> np.ndarray:
         ax = max(0.5 * (hi[0] - lo[0]), 1e-9)
         by = max(0.5 * (hi[1] - lo[1]), 1e-9)
         H = max(hi[2] - lo[2], 1e-9)
-        F[a0:b0, 0] = (Xc[:, 0] - c[0]) / ax
-        F[a0:b0, 1] = (Xc[:, 1] - c[1]) / by
-        F[a0:b0, 2] = (Xc[:, 2] - lo[2]) / H
-        fe = spec.fext[a0:b0]
-        tot = fe.sum(0)
-        loaded = np.abs(fe).sum(1) > 0
+        F[0, a0:b0] = (Xc[:, 0] - c[0]) / ax
+        F[1, a0:b0] = (Xc[:, 1] - c[1]) / by
+        F[2, a0:b0] = (Xc[:, 2] - lo[2]) / H
+        tot = spec.fext[a0:b0].sum(0)
+        loaded = loaded_all[a0:b0]
         nrm = np.linalg.norm(tot)
-        F[a0:b0, 8] = loaded
+        F[8, a0:b0] = loaded
         if nrm > 0:
-            F[a0:b0, 9:12] = tot / nrm
+            F[9:12, a0:b0] = (tot / nrm)[:, None]
             # resultant pressure: |sum f| V / loaded area (area = n_loaded dx^2)
             p = nrm * spec.dx ** 3 / max(loaded.sum() * spec.dx ** 2, 1e-30)
-            F[a0:b0, 12] = np.log10(max(p, 1e-12))
-        F[a0:b0, 13] = H / 25.0
-        F[a0:b0, 14] = ax / 20.0
-        F[a0:b0, 15] = by / 20.0
-        F[a0:b0, 16] = logE[a0:b0].mean()
-        F[a0:b0, 17] = cort[a0:b0].mean()
-    F[:, 3] = logE
-    F[:, 4] = cort
+            F[12, a0:b0] = np.log10(max(p, 1e-12))
+        F[13, a0:b0] = H / 25.0
+        F[14, a0:b0] = ax / 20.0
+        F[15, a0:b0] = by / 20.0
+        F[16, a0:b0] = logE[a0:b0].mean()
+        F[17, a0:b0] = cort[a0:b0].mean()
+        Ec = spec.E[a0:b0] if np.ndim(spec.E) else np.full(b0 - a0, float(spec.E))
+        ce = (Ec @ Xc[:, :2]) / max(Ec.sum(), 1e-300)
+        F[18, a0:b0] = (ce[0] - c[0]) / ax
+        F[19, a0:b0] = (ce[1] - c[1]) / by
+    F[3] = logE
+    F[4] = cort
     for q in range(3):
-        F[:, 5 + q] = (spec.fixed >> q) & 1
-    F[:, 18] = 1.0
-    return F
+        F[5 + q] = (spec.fixed >> q) & 1
+    return np.ascontiguousarray(F.T)
+
+
+def feature_tensor(body, spec, case_offsets=None):
+    """(n, 64) float32 device tensor: raw features and two neighbour-mean hops
+    through the bonds of ``body`` (a libpdx context of the same particles),
+    built on the device by pdx_neighbor_features from one host-to-device copy
+    of the raw features."""
+    import ctypes as C
+
+    import torch
+    from .. import _native as N
+    raw = raw_features(spec, case_offsets)
+    out = torch.zeros((spec.n, FEAT), dtype=torch.float32, device="cuda")
+    body.ctx.call("pdx_neighbor_features", RAW, 2, N.ptr(raw, N._dp), C.c_void_p(out.data_ptr()), FEAT)
+    out[:, 3] = 0.0  # the local modulus only through its neighbour means
+    return out
 
 
 def feature_matrix(body, spec, case_offsets=None) -> np.ndarray:
-    """(n, 64) float32: raw features and two neighbour-mean hops through the
-    bonds of ``body`` (a libpdx context of the same particles)."""
-    raw = raw_features(spec, case_offsets)
-    m1 = body.neighbor_mean(raw)
-    m2 = body.neighbor_mean(m1)
-    out = np.zeros((spec.n, FEAT), np.float32)
-    out[:, :RAW] = raw
-    out[:, RAW:2 * RAW] = m1
-    out[:, 2 * RAW:3 * RAW] = m2
-    return out
+    """feature_tensor on the host (the training data)."""
+    return feature_tensor(body, spec, case_offsets).cpu().numpy()
 
 
 def target_scale(spec, case_offsets=None) -> np.ndarray:

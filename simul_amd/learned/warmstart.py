@@ -8,13 +8,11 @@ import time
 
 import numpy as np
 
-from .features import feature_matrix, target_scale
+from .features import feature_tensor, target_scale
 
 
 def predict_u(net, body, spec, case_offsets=None) -> np.ndarray:
-    import torch
-    F = feature_matrix(body, spec, case_offsets)
-    y = net.predict(torch.from_numpy(F).cuda())
+    y = net.predict(feature_tensor(body, spec, case_offsets))
     u = y.astype(np.float64) * target_scale(spec, case_offsets)[:, None]
     fixed = spec.fixed
     for q in range(3):

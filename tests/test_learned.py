@@ -172,3 +172,25 @@ def test_warm_start_reaches_the_cold_solution(gpu):
     rw, t = warm_solve(body, spec, net, tol=1e-6, ref=cold)
     assert rc.converged and rw.converged and t["inference_s"] > 0
     assert rel_l2(body.u, uc) < 1e-5
+
+
+def test_device_features_match_host_hops(gpu):
+    """pdx_neighbor_features (device hops, FP32 pack into the network input)
+    against the same two hops through pdx_neighbor_mean on host arrays, for a
+    two-case batch with padding between the cases: bit for bit."""
+    from simul_amd import workloads as W
+    from simul_amd.body import from_specimen
+    from simul_amd.learned.features import FEAT, RAW, feature_tensor, raw_features
+    specs = [W.random_vertebra(77, dx=1.6), W.random_vertebra(78, dx=1.6)]
+    packed, off = W.pack(specs)
+    body = from_specimen(packed, case_offsets=off)
+    F = feature_tensor(body, packed, off).cpu().numpy()
+    raw = raw_features(packed, off)
+    m1 = body.neighbor_mean(raw)
+    m2 = body.neighbor_mean(m1)
+    ref = np.zeros((packed.n, FEAT), np.float32)
+    ref[:, :RAW] = raw
+    ref[:, 3] = 0.0
+    ref[:, RAW:2 * RAW] = m1
+    ref[:, 2 * RAW:3 * RAW] = m2
+    assert np.array_equal(F, ref)
