@@ -382,3 +382,29 @@ def test_sliced_solve_matches_single(gpu):
     rb = b2.solve("adaptive", tol=1e-6, slice_iterations=128, progress=lambda d, rs: None)
     assert [r.iterations for r in ra] == [r.iterations for r in rb]
     assert np.array_equal(b1.u, b2.u)
+
+
+def test_halo_variants_2d_and_lds_fallback(gpu):
+    """2-D bodies through the halo kernels, and a horizon of 4 spacings whose
+    force-pass halo (~3 000 slots x 80 B) exceeds the 160 KB of LDS while the
+    F-pass halo (40 B per slot) fits: the force pass then runs the
+    cooperative kernel.  Forces agree with the gathering kernels to 2e-6."""
+    g = np.arange(40.0)
+    X2 = np.stack(np.meshgrid(g, g[:30], indexing="ij"), -1).reshape(-1, 2)
+    X2 = X2 + np.random.default_rng(4).uniform(-0.2, 0.2, X2.shape)
+    X3 = lattice(14, 13, 12, jitter=0.2, seed=9)
+    for X, h in ((X2, 3.015), (X3, 4.0)):
+        n, dim = X.shape
+        bulk, shear = W.bulk_shear(15000.0 * np.exp(0.3 * np.random.default_rng(2).standard_normal(n)), 0.3)
+        fixed = np.zeros(n, np.uint8)
+        fixed[X[:, -1] < 0.5] = (1 << dim) - 1
+        for form in ("nosb", "bond"):
+            body = PDBody(X, 1.0, h, bulk, shear, cstab=18 * bulk / (np.pi * h ** 4), fixed=fixed,
+                          formulation=form, model="neo_hookean")
+            body.u = 2e-3 * np.sin(X / 5.0) + 1e-4 * np.cos(X.sum(1))[:, None]
+            body.set_group(16 | 2)
+            f0 = body.compute_forces().copy()
+            for g_ in HALO:
+                body.set_group(g_)
+                f = body.compute_forces()
+                assert rel_l2(f, f0) < 2e-6, (dim, h, form, g_, rel_l2(f, f0))
