@@ -200,6 +200,13 @@ class PDBody:
     def adr_mass(self, beta=1.0):
         self.ctx.call("pdx_adr_mass", float(beta))
 
+    def tune_mass(self, iters=20, safety=2.0):
+        """Adaptive-DR mass scaled to the spectrum of each case
+        (pdx_adr_mass_auto); returns beta per case (a float for one case)."""
+        b = np.zeros(self.ncase)
+        self.ctx.call("pdx_adr_mass_auto", int(iters), float(safety), N.ptr(b, N._dp))
+        return b if self.ncase > 1 else float(b[0])
+
     def kd_step(self, dt, visc, prev_ke, ke_increasing):
         pk = np.array(np.broadcast_to(prev_ke, (self.ncase,)), np.float64)
         inc = np.array(np.broadcast_to(ke_increasing, (self.ncase,)), np.int32)
@@ -213,7 +220,7 @@ class PDBody:
 
     def solve(self, method="adaptive", tol=1e-6, max_iterations=100000, ref=0.0, dt=0.0,
               viscous_damping=0.0, cmax=1.9, use_graph=True, chunk=64, resume=False, refs=None,
-              progress=None, slice_iterations=None):
+              progress=None, slice_iterations=None, mass=None):
         """Dynamic relaxation to |f_free| / ref < tol per case.
 
         ref > 0 fixes the reference residual of every case, refs (ncase) of
@@ -222,7 +229,31 @@ class PDBody:
         (resumed on the device, so the trajectory is the same) and
         ``progress(done, result)`` is called after each slice with the slice's
         result (a list for a batch); the returned iteration counts are totals
-        over the slices."""
+        over the slices.
+
+        mass (adaptive DR): None keeps the context's mass (the Gershgorin mass
+        unless set otherwise); "gershgorin" recomputes it; "spectral" scales it
+        to each case's spectrum first (tune_mass) and, should a case diverge
+        (relative residual above 1 or not finite at the end), solves again
+        from the same start with the Gershgorin mass."""
+        if mass is not None and method == "adaptive" and not resume:
+            if mass == "gershgorin":
+                self.adr_mass(1.0)
+            elif mass == "spectral":
+                u_start = self.u
+                self.tune_mass()
+                out = self.solve(method, tol, max_iterations, ref, dt, viscous_damping, cmax, use_graph, chunk,
+                                 False, refs, progress, slice_iterations)
+                lst = out if isinstance(out, list) else [out]
+                if all(x.converged or (np.isfinite(x.relative_residual) and x.relative_residual <= 1.0)
+                       for x in lst):
+                    return out
+                self.u = u_start
+                self.adr_mass(1.0)
+                return self.solve(method, tol, max_iterations, ref, dt, viscous_damping, cmax, use_graph, chunk,
+                                  False, refs, progress, slice_iterations)
+            else:
+                raise ValueError(f"mass must be None, 'gershgorin' or 'spectral', not {mass!r}")
         if slice_iterations is None:
             return self._solve_once(method, tol, max_iterations, ref, dt, viscous_damping, cmax, use_graph,
                                     chunk, resume, refs)

@@ -1080,6 +1080,83 @@ extern "C" int pdx_adr_mass(pdx_ctx *c, double beta)
     return PDX_OK;
 }
 
+// Adaptive-DR mass from the spectrum: the Gershgorin mass (beta = 1) bounds
+// the largest eigenvalue rho of M^-1 K by 4 with a wide margin.  `iters`
+// power iterations on M^-1 K estimate rho per case (Rayleigh quotients
+// v.Kv / v.Mv; K v from the force difference at a perturbation of 1e-4
+// horizons RMS around the current u), and each case's mass is scaled by
+// beta = safety * rho / 4 (at most 1), the explicit update's stability limit
+// times the safety factor.  The iteration count of DR falls like sqrt(beta).
+extern "C" int pdx_adr_mass_auto(pdx_ctx *c, int32_t iters, double safety, double *beta_out)
+{
+    CHECK_CTX(c);
+    PDX_TRY(need_K(c));
+    if (iters < 1 || iters > 1000) return fail(PDX_EINVAL, "iters must be 1..1000");
+    if (!(safety > 0.0) || !isfinite(safety)) return fail(PDX_EINVAL, "safety must be positive");
+    PDX_TRY(pdx_adr_mass(c, 1.0));
+    const int u0 = c->ucur, nc = c->ncase;
+    pdx_d4 *f0 = nullptr, *v = nullptr;
+    double *dsc = nullptr, *dsum = nullptr;
+    PDX_HIP(hipMalloc((void **)&f0, sizeof(pdx_d4) * 2 * (size_t)c->ni));
+    v = f0 + c->ni;
+    hipError_t e = hipMalloc((void **)&dsc, sizeof(double) * 5 * (size_t)nc);
+    if (e != hipSuccess) {
+        hipFree(f0);
+        return fail(PDX_ENOMEM, "hipMalloc failed");
+    }
+    dsum = dsc + nc;
+    auto done = [&](int r) {
+        hipStreamSynchronize(c->stream);
+        hipFree(f0);
+        hipFree(dsc);
+        c->have_A = false;
+        return r;
+    };
+    auto force_at = [&](int ub) -> int {
+        if (c->form == PDX_FORM_NOSB) {
+            const int r = pdx_launch_fpass(c, ub, false, false, false);
+            if (r != PDX_OK) return r;
+        }
+        return pdx_launch_force(c, ub, 0, false, 0.0, 0.0);
+    };
+    int r = force_at(u0);
+    if (r == PDX_OK && hipMemcpyAsync(f0, c->d.f, sizeof(pdx_d4) * c->ni, hipMemcpyDeviceToDevice, c->stream) !=
+                           hipSuccess)
+        r = PDX_EDEVICE;
+    if (r == PDX_OK) r = pdx_launch_pow(c, 0, u0, v, nullptr, nullptr, nullptr);
+    std::vector<double> scale(nc, 1e-4 * c->horizon), rho(nc, 0.0), sums(4 * (size_t)nc);
+    for (int k = 0; k < iters && r == PDX_OK; ++k) {
+        if (hipMemcpyAsync(dsc, scale.data(), sizeof(double) * nc, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+            r = PDX_EDEVICE;
+            break;
+        }
+        if ((r = pdx_launch_pow(c, 1, u0, v, nullptr, dsc, nullptr)) != PDX_OK) break;
+        if ((r = force_at(u0 ^ 1)) != PDX_OK) break;
+        if ((r = pdx_launch_pow(c, 2, u0, v, f0, dsc, nullptr)) != PDX_OK) break;
+        if ((r = pdx_launch_pow(c, 3, u0, v, nullptr, nullptr, dsum)) != PDX_OK) break;
+        if (hipMemcpyAsync(sums.data(), dsum, sizeof(double) * 4 * nc, hipMemcpyDeviceToHost, c->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
+            r = PDX_EDEVICE;
+            break;
+        }
+        for (int q = 0; q < nc; ++q) {
+            const double num = sums[4 * q], den = sums[4 * q + 1], vv = sums[4 * q + 2], cnt = sums[4 * q + 3];
+            if (den > 0.0 && isfinite(num)) rho[q] = std::max(rho[q], num / den);
+            const double rms = cnt > 0.0 ? sqrt(vv / cnt) : 0.0;
+            scale[q] = rms > 0.0 && isfinite(rms) ? 1e-4 * c->horizon / rms : 1e-4 * c->horizon;
+        }
+    }
+    if (r != PDX_OK) return done(r);
+    for (int q = 0; q < nc; ++q) scale[q] = std::min(1.0, std::max(1e-3, safety * rho[q] / 4.0));
+    if (hipMemcpyAsync(dsc, scale.data(), sizeof(double) * nc, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return done(fail(PDX_EDEVICE, "hipMemcpyAsync failed"));
+    r = pdx_launch_pow(c, 4, u0, v, nullptr, dsc, nullptr);
+    if (beta_out)
+        for (int q = 0; q < nc; ++q) beta_out[q] = scale[q];
+    return done(r);
+}
+
 static int zero_v_kernel(pdx_ctx *c)
 {
     PDX_HIP(hipMemsetAsync(c->d.v, 0, sizeof(pdx_d4) * c->ni, c->stream));

@@ -1101,6 +1101,119 @@ int pdx_launch_feat_pack(pdx_ctx *c, const double *d_hv, const int32_t *d_perm, 
     return PDX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// power iteration for the largest eigenvalue of M^-1 K (adaptive-DR mass)
+// ---------------------------------------------------------------------------
+// start vector: +-1 on every free DOF of an active particle (a hash of the
+// DOF index), 0 elsewhere
+__global__ __launch_bounds__(PDX_BLOCK) void k_pow_init(Dev d, pdx_d4 *v)
+{
+    const int i = blockIdx.x * PDX_BLOCK + threadIdx.x;
+    if (i >= d.ni) return;
+    const KRec kr = d.KR[i];
+    const uint32_t fm = kr.flags & PDX_FL_FIXMASK;
+    const bool act = (kr.flags & PDX_FL_ACTIVE) != 0;
+    double e[3] = {0.0, 0.0, 0.0};
+    for (int r = 0; r < d.dim; ++r) {
+        uint32_t h = (uint32_t)(3 * i + r) * 2654435761u;
+        h ^= h >> 15;
+        h *= 2246822519u;
+        h ^= h >> 13;
+        e[r] = (act && !((fm >> r) & 1u)) ? ((h & 1u) ? 1.0 : -1.0) : 0.0;
+    }
+    v[i] = d4(e[0], e[1], e[2]);
+}
+
+// u of buffer ubuf ^ 1 = u of ubuf + s_case v
+__global__ __launch_bounds__(PDX_BLOCK) void k_pow_perturb(Dev d, int ubuf, const pdx_d4 *v, const double *scale)
+{
+    const int i = blockIdx.x * PDX_BLOCK + threadIdx.x;
+    if (i >= d.ni) return;
+    const double s = scale[d.case_of_p[i] < 0 ? 0 : d.case_of_p[i]];
+    const pdx_d4 vi = v[i];
+    const double *ui = d.xu[ubuf][i].u;
+    double *uo = d.xu[ubuf ^ 1][i].u;
+    uo[0] = ui[0] + s * vi.x;
+    uo[1] = ui[1] + s * vi.y;
+    uo[2] = ui[2] + s * vi.z;
+}
+
+// K v = -(f - f0) / s_case on the free DOFs; block partials of v.Kv, lam v.v,
+// |v_next|^2 and the free-DOF count; v_next = K v / lam
+__global__ __launch_bounds__(PDX_BLOCK) void k_pow_apply(Dev d, const pdx_d4 *f0, pdx_d4 *v, const double *scale)
+{
+    const int lb = blockIdx.x;
+    const int i = lb * PDX_BLOCK + threadIdx.x;
+    double red[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i < d.ni) {
+        const KRec kr = d.KR[i];
+        const uint32_t fm = kr.flags & PDX_FL_FIXMASK;
+        if (kr.flags & PDX_FL_ACTIVE) {
+            const double s = scale[d.case_of_blk[lb]];
+            const double lam = d.lam[i];
+            const pdx_d4 vi = v[i], fi = d.f[i], gi = f0[i];
+            const double vv[3] = {vi.x, vi.y, vi.z}, ff[3] = {fi.x, fi.y, fi.z}, gg[3] = {gi.x, gi.y, gi.z};
+            double vn[3] = {0.0, 0.0, 0.0};
+            for (int r = 0; r < d.dim; ++r) {
+                if ((fm >> r) & 1u) continue;
+                const double kv = -(ff[r] - gg[r]) / s;
+                red[0] += vv[r] * kv;
+                red[1] += lam * vv[r] * vv[r];
+                vn[r] = kv / lam;
+                red[2] += vn[r] * vn[r];
+                red[3] += 1.0;
+            }
+            v[i] = d4(vn[0], vn[1], vn[2]);
+        }
+    }
+    block_partials<4>(d, lb, red);
+}
+
+// per-case totals of the 4 block partials into out[4 case + q]
+__global__ __launch_bounds__(PDX_BLOCK) void k_pow_reduce(Dev d, double *out)
+{
+    __shared__ double sred[PDX_BLOCK / 64][4];
+    const int cs = blockIdx.x;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int b0 = d.case_blk0[cs], b1 = d.case_blk0[cs + 1];
+    for (int q = 0; q < 4; ++q) {
+        double t = 0.0;
+        for (int b = b0 + threadIdx.x; b < b1; b += PDX_BLOCK) t += d.part[(int64_t)b * 4 + q];
+        t = wave_sum(t);
+        if (lane == 0) sred[wid][q] = t;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double sum = 0.0;
+            for (int w = 0; w < PDX_BLOCK / 64; ++w) sum += sred[w][q];
+            out[4 * cs + q] = sum;
+        }
+        __syncthreads();
+    }
+}
+
+// lam *= beta of the particle's case
+__global__ __launch_bounds__(PDX_BLOCK) void k_scale_lam(Dev d, const double *beta)
+{
+    const int i = blockIdx.x * PDX_BLOCK + threadIdx.x;
+    if (i >= d.ni) return;
+    const int cs = d.case_of_p[i];
+    if (cs >= 0) d.lam[i] *= beta[cs];
+}
+
+int pdx_launch_pow(pdx_ctx *c, int what, int ubuf, pdx_d4 *v, const pdx_d4 *f0, const double *scale, double *out)
+{
+    const dim3 g(grid1(c->ni)), b(PDX_BLOCK);
+    switch (what) {
+    case 0: hipLaunchKernelGGL(k_pow_init, g, b, 0, c->stream, c->d, v); break;
+    case 1: hipLaunchKernelGGL(k_pow_perturb, g, b, 0, c->stream, c->d, ubuf, v, scale); break;
+    case 2: hipLaunchKernelGGL(k_pow_apply, dim3(c->nblk), b, 0, c->stream, c->d, f0, v, scale); break;
+    case 3: hipLaunchKernelGGL(k_pow_reduce, dim3(c->ncase), b, 0, c->stream, c->d, out); break;
+    default: hipLaunchKernelGGL(k_scale_lam, g, b, 0, c->stream, c->d, scale); break;
+    }
+    PDX_LAUNCH_CHECK();
+    return PDX_OK;
+}
+
 int pdx_launch_xu_geom(pdx_ctx *c, const XRec *d_src)
 {
     hipLaunchKernelGGL(k_xu_geom, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_src);

@@ -41,7 +41,7 @@ class IncrementalLoading:
     def __init__(self, body, indices, u_final, n_increments: int = 20, critical_stretch: Optional[float] = None,
                  tol: float = 1e-6, max_iterations: int = 200000, method: str = "adaptive",
                  max_break_rounds: int = 3, dt: float = 0.0, viscous_damping: float = 0.0,
-                 progress=None, slice_iterations: int = 4096):
+                 progress=None, slice_iterations: int = 4096, mass=None):
         if n_increments < 1:
             raise ValueError("n_increments must be >= 1")
         self.body = body
@@ -53,13 +53,15 @@ class IncrementalLoading:
         self.max_break_rounds = int(max_break_rounds)
         self.dt, self.viscous_damping = float(dt), float(viscous_damping)
         self.progress, self.slice_iterations = progress, slice_iterations
+        self.mass = mass  # adaptive-DR mass of every solve (PDBody.solve's mass=)
         self.results: List[IncrementResult] = []
 
     def _solve(self, ref=0.0):
         b = self.body
         return b.solve(self.method, tol=self.tol, max_iterations=self.max_iterations, ref=ref, dt=self.dt,
                        viscous_damping=self.viscous_damping, progress=self.progress,
-                       slice_iterations=self.slice_iterations if self.progress is not None else None)
+                       slice_iterations=self.slice_iterations if self.progress is not None else None,
+                       mass=self.mass if self.method == "adaptive" else None)
 
     def step(self, k: int) -> IncrementResult:
         """Increment k (1-based)."""

@@ -408,3 +408,41 @@ def test_halo_variants_2d_and_lds_fallback(gpu):
                 body.set_group(g_)
                 f = body.compute_forces()
                 assert rel_l2(f, f0) < 2e-6, (dim, h, form, g_, rel_l2(f, f0))
+
+
+def test_spectral_mass_same_solution_fewer_iterations(gpu):
+    """pdx_adr_mass_auto: beta in (0, 1] per case, and adaptive DR with the
+    spectral mass reaches the Gershgorin-mass solution (1e-5) in fewer
+    iterations, for one body and for a batch; the oracle-matched default
+    (the Gershgorin mass) is what a solve without mass= uses."""
+    spec, body, prob = _cube_pair(12, "neo_hookean")
+    cold = body.cold_residual()
+    body.adr_mass(1.0)
+    lam0 = body.get("drmass").copy()
+    r1 = body.solve("adaptive", tol=1e-6, ref=cold)
+    u1 = body.u
+    assert np.allclose(body.get("drmass"), lam0)
+    body.u = spec.u0
+    beta = body.tune_mass()
+    assert 0.0 < beta <= 1.0
+    assert np.allclose(body.get("drmass"), beta * lam0, rtol=1e-12)
+    body.u = spec.u0
+    r2 = body.solve("adaptive", tol=1e-6, ref=cold, mass="spectral")
+    assert r1.converged and r2.converged and r2.iterations < r1.iterations
+    assert rel_l2(body.u, u1) < 1e-5
+    specs = [W.cube(n_side=k) for k in (7, 10, 13)]
+    off = np.cumsum([0] + [s.n for s in specs])
+    bulk, shear = W.bulk_shear(15000.0, 0.3)
+    X = np.concatenate([s.X for s in specs])
+    bb = PDBody(X, 1.0, 3.015, bulk, shear, fixed=np.concatenate([s.fixed for s in specs]),
+                u0=np.concatenate([s.u0 for s in specs]), case_offsets=off, model="neo_hookean")
+    refs = np.atleast_1d(bb.cold_residual())
+    ra = bb.solve("adaptive", tol=1e-6, refs=refs)
+    ua = bb.u
+    bb.u = np.concatenate([s.u0 for s in specs])
+    rb = bb.solve("adaptive", tol=1e-6, refs=refs, mass="spectral")
+    assert all(x.converged for x in ra + rb)
+    assert all(b.iterations < a.iterations for a, b in zip(ra, rb))
+    assert rel_l2(bb.u, ua) < 1e-5
+    b3 = bb.tune_mass()
+    assert b3.shape == (3,) and np.all((b3 > 0) & (b3 <= 1))
