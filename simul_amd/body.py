@@ -39,6 +39,10 @@ def _f64(a, shape=None):
     return np.ascontiguousarray(a)
 
 
+
+class _Diverged(Exception):
+    """A solve with the spectral mass left the stable range."""
+
 class PDBody:
     def __init__(self, X, vol, horizon, bulk, shear, *, mass=None, model="linear_elastic",
                  formulation="nosb", G_s=0.1, cstab=None, fixed=None, fext=None, u0=None,
@@ -242,12 +246,24 @@ class PDBody:
             elif mass == "spectral":
                 u_start = self.u
                 self.tune_mass()
-                out = self.solve(method, tol, max_iterations, ref, dt, viscous_damping, cmax, use_graph, chunk,
-                                 False, refs, progress, slice_iterations)
-                lst = out if isinstance(out, list) else [out]
-                if all(x.converged or (np.isfinite(x.relative_residual) and x.relative_residual <= 1.0)
-                       for x in lst):
-                    return out
+
+                def watch(done, rs):  # a diverging case ends the attempt at the slice
+                    for x in (rs if isinstance(rs, list) else [rs]):
+                        if not x.converged and not (np.isfinite(x.relative_residual)
+                                                    and x.relative_residual <= 100.0):
+                            raise _Diverged()
+                    if progress is not None:
+                        progress(done, rs)
+
+                try:
+                    out = self.solve(method, tol, max_iterations, ref, dt, viscous_damping, cmax, use_graph,
+                                     chunk, False, refs, watch, slice_iterations or 2048)
+                    lst = out if isinstance(out, list) else [out]
+                    if all(x.converged or (np.isfinite(x.relative_residual) and x.relative_residual <= 1.0)
+                           for x in lst):
+                        return out
+                except _Diverged:
+                    pass
                 self.u = u_start
                 self.adr_mass(1.0)
                 return self.solve(method, tol, max_iterations, ref, dt, viscous_damping, cmax, use_graph, chunk,

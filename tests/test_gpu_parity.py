@@ -446,3 +446,20 @@ def test_spectral_mass_same_solution_fewer_iterations(gpu):
     assert rel_l2(bb.u, ua) < 1e-5
     b3 = bb.tune_mass()
     assert b3.shape == (3,) and np.all((b3 > 0) & (b3 <= 1))
+
+
+def test_spectral_mass_falls_back_when_a_case_diverges(gpu):
+    """solve(mass="spectral") with a mass far below the stability limit (the
+    tuning's safety factor forced to 0.1): the attempt diverges numerically,
+    is stopped at its first slice, and the solve falls back to the Gershgorin
+    mass from the same start, ending on the same solution as a plain solve."""
+    spec, body, _ = _cube_pair(10, "neo_hookean")
+    cold = body.cold_residual()
+    r1 = body.solve("adaptive", tol=1e-6, ref=cold)
+    u1 = body.u
+    body.u = spec.u0
+    tune = body.tune_mass
+    body.tune_mass = lambda iters=20, safety=2.0: tune(iters, 0.1)
+    r2 = body.solve("adaptive", tol=1e-6, ref=cold, mass="spectral", max_iterations=20000)
+    assert r2.converged and r2.iterations == r1.iterations
+    assert np.array_equal(body.u, u1)
