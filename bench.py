@@ -58,6 +58,10 @@ def apply_variant(body, spec, iters):
         body.set_variants(v[0], v[1])
 
 
+def mass_label(spectral):
+    return "spectral (pdx_adr_mass_auto, tuning included)" if spectral else "Gershgorin (beta = 1)"
+
+
 def pass_variants(body):
     """(label, F-pass kernel, force kernel) of the variants a body runs."""
     fv, gv = body.variants()
@@ -84,6 +88,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--kernel-iters", type=int, default=50)
+    ap.add_argument("--dr-mass", choices=["spectral", "gershgorin"], default="spectral",
+                    help="adaptive-DR mass of the time-to-solve runs (spectral: pdx_adr_mass_auto per solve, with "
+                         "the Gershgorin fallback should a case diverge)")
     ap.add_argument("--variant", type=str, default="0",
                     help="hot-pass variant (1, 2, 4, 17, 18, 33, 34, 97, 98, 161, 162) or 'F,G' for the F-pass "
                          "and the force pass; 0 = autotune")
@@ -186,12 +193,12 @@ def main():
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         rs = body.solve("adaptive", tol=args.tol, max_iterations=args.max_solve_iters, ref=cold, chunk=128,
-                        mass="spectral")
+                        mass=args.dr_mass)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         tts = {"cold_start_s": round(wall, 4), "iterations": rs.iterations, "converged": rs.converged,
                "relative_residual": rs.relative_residual, "residual_normalisation": "cold-start residual",
-               "dr_mass": "spectral (pdx_adr_mass_auto, tuning included)",
+               "dr_mass": mass_label(args.dr_mass == "spectral"),
                "cold_start_gershgorin_s": round(wall_g, 4), "iterations_gershgorin": rg.iterations,
                "warm_start_s": None, "warm_start": "no trained weights", "setup_s": round(setup_s, 3)}
         log(f"time-to-solve: {wall:.3f}s, {rs.iterations} iterations, converged={rs.converged}, "
@@ -203,7 +210,7 @@ def main():
             net = WarmStartNet.load(wfile)
             body.u = spec.u0
             rw, tw = warm_solve(body, spec, net, tol=args.tol, ref=cold, max_iterations=args.max_solve_iters,
-                                chunk=128, mass="spectral")
+                                chunk=128, mass=args.dr_mass)
             du = float(np.linalg.norm(body.u - u_cold) / np.linalg.norm(u_cold))
             tts.update({"warm_start_s": round(tw["total_s"], 4), "warm_iterations": rw.iterations,
                         "warm_converged": rw.converged, "warm_inference_s": round(tw["inference_s"], 4),
@@ -260,6 +267,7 @@ def run_loading(args, torch):
     drv = IncrementalLoading(body, spec.extra["top_idx"], spec.extra["u_final"], spec.extra["n_increments"],
                              critical_stretch=spec.extra["critical_stretch"], tol=args.tol,
                              max_iterations=args.max_solve_iters,
+                             mass=args.dr_mass,
                              progress=lambda done, r: log(f"    {done} iterations, rel {r.relative_residual:.2e}"))
     upd = 0.0
 
@@ -277,6 +285,8 @@ def run_loading(args, torch):
           {"workload": spec.name, "n_particles": spec.n, "n_bonds": nb0, "increments": len(res),
            "critical_stretch": spec.extra["critical_stretch"], "variant": pass_variants(body)[0]},
           time_to_solve={"loading_s": round(wall, 3), "setup_s": round(setup, 3),
+                         "dr_mass": mass_label(args.dr_mass == "spectral"),
+                         "broken_total": res[-1].broken_total if res else 0,
                          "all_converged": all(r.converged for r in res),
                          "increments": [{"k": r.increment, "iterations": r.iterations,
                                          "relative_residual": r.relative_residual, "broken_now": r.broken_now,
@@ -324,11 +334,12 @@ def run_batch(args, torch):
             log(f"  {done} iterations: {nconv}/{len(rs)} cases converged, worst rel. residual {worst:.2e}")
 
         rs = body.solve("adaptive", tol=args.tol, max_iterations=args.max_solve_iters, ref=0.0, chunk=128,
-                        slice_iterations=2048, progress=progress, mass="spectral")
+                        slice_iterations=2048, progress=progress,
+                        mass=args.dr_mass)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         its = [r.iterations for r in rs]
-        tts = {"all_cases_s": round(wall, 3), "dr_mass": "spectral (pdx_adr_mass_auto, tuning included)",
+        tts = {"all_cases_s": round(wall, 3), "dr_mass": mass_label(args.dr_mass == "spectral"),
                "converged": sum(r.converged for r in rs), "cases": len(rs),
                "iterations_min": min(its), "iterations_max": max(its), "setup_s": round(setup, 3)}
         log(f"time-to-solve (all cases): {wall:.2f}s, iterations {min(its)}..{max(its)}, "
