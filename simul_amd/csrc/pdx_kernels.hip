@@ -1,17 +1,19 @@
 // pdx_kernels.hip -- gfx950 kernels of the NOSB peridynamics solver step.
 //
-// Every kernel is a thin wrapper over a per-particle function of pdx_device.h
-// (one thread per particle, wave64).  Hot loop per dynamic-relaxation
-// iteration:
-//   k_fpass  : H = [sum_j w V_j du (x) xi] K^-1 (FP32 terms, FP64 sums), P(H)
+// The per-particle and per-bond arithmetic lives in pdx_device.h (one thread
+// per particle, wave64).  Hot loop per dynamic-relaxation iteration:
+//   F-pass (k_fpass_h; gathering variants k_fpass, k_fpass_w):
+//              H = [sum_j w V_j du (x) xi] K^-1 (FP32 terms, FP64 sums), P(H)
 //              (LE/NH in FP32, J2 radial return in FP64), A = P K^-1
-//   k_force  : f_i = A_i m_i + sum_j w V_j [A_j xi + G_s c_ij s eta/|eta|]
+//   force pass (k_force_h; k_force, k_force_w):
+//              f_i = A_i m_i + sum_j w V_j [A_j xi + G_s c_ij s eta/|eta|]
 //              (FP32 bond terms, FP64 per-particle sums), + external load,
-//              fused DR update, FP64 block partials and the case's last-block
-//              reduction of residual / damping / kinetic energy
+//              fused DR update, FP64 block partials of residual / damping /
+//              kinetic energy
+//   k_case_reduce: the case totals of those partials and the per-case update
 //   k_kd_move: kinetic-damping position update (reference-faithful mode)
 // Setup kernels (FP64): neighbour count/fill, shape tensor and inverse,
-// adaptive-DR mass, dt estimate, damage.
+// adaptive-DR mass and its spectral scaling, dt estimate, damage.
 #include "pdx_internal.h"
 
 // blocks are dispatched round-robin over the 8 XCDs; give every XCD one
@@ -389,12 +391,13 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_force_w(Dev d, int ubuf, int solv
 // (one 64-byte record per entry), then every lane walks its row reading LDS
 // at the bond's slot (Dev::bl).  Slots follow the particles' bank colours, so
 // the 16 lanes that read together mostly hit 16 different bank groups.
-// Staged per slot, in separate LDS arrays of 16-byte entries (8 for u_z in
-// the F-pass):
-//   xv  : X - anchor in FP32 (3), V
-//   u01 : u_x, u_y in FP64
-//   F-pass: u2 = u_z in FP64;  force pass: uac = u_z in FP64, A_zz, c
-//   force pass, NOSB: a0 = A (0..3), a1 = A (4..7)
+// Staged per slot (the F-pass in one LDS array per field, the force pass in
+// one 80-byte record per slot, see hslot):
+//   xv  : X - anchor in FP32 (3), V                              16 B
+//   u01 : u_x, u_y in FP64                                       16 B
+//   F-pass: u2 = u_z in FP64                                      8 B
+//   force pass: uac = u_z in FP64, A_zz, c                       16 B
+//   force pass, NOSB: a0 = A (0..3), a1 = A (4..7)               32 B
 // The anchor is an FP64 point at the block's centre, so |X - anchor| is
 // bounded by the block's extent plus the horizon and the staged bond vector
 // xv_j - xv_i differs from the FP64 difference by that extent's FP32
@@ -403,13 +406,14 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_force_w(Dev d, int ubuf, int solv
 // fpass_bond_v / force_bond_v.
 typedef float pdx_q2 __attribute__((ext_vector_type(2)));
 
-// byte offsets of the LDS arrays, cap slots each
+// field offsets (bytes) and bytes per slot of the staged halo
 template <int FORCE, int FORM>
 struct HaloOff {
     static constexpr int xv = 0, u01 = 16, u2 = 32, a0 = 48, a1 = 64;
     static constexpr int bytes = !FORCE ? 40 : (FORM == 0 ? 80 : 48);
 };
 
+// start of field `off`'s array in the F-pass layout (one array per field)
 template <typename T>
 __device__ __forceinline__ T *lds_at(pdx_q4 *base, int off, int cap)
 {
