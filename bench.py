@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--dr-mass", choices=["spectral", "gershgorin"], default="spectral",
                     help="adaptive-DR mass of the time-to-solve runs (spectral: pdx_adr_mass_auto per solve, with "
                          "the Gershgorin fallback should a case diverge)")
+    ap.add_argument("--predictor", choices=["secant", "none"], default="secant",
+                    help="start of each load increment of the J2 loading (secant: 2 u_(k-1) - u_(k-2))")
     ap.add_argument("--variant", type=str, default="0",
                     help="hot-pass variant (1, 2, 4, 17, 18, 33, 34, 97, 98, 161, 162) or 'F,G' for the F-pass "
                          "and the force pass; 0 = autotune")
@@ -267,7 +269,7 @@ def run_loading(args, torch):
     drv = IncrementalLoading(body, spec.extra["top_idx"], spec.extra["u_final"], spec.extra["n_increments"],
                              critical_stretch=spec.extra["critical_stretch"], tol=args.tol,
                              max_iterations=args.max_solve_iters,
-                             mass=args.dr_mass,
+                             mass=args.dr_mass, predictor=None if args.predictor == "none" else args.predictor,
                              progress=lambda done, r: log(f"    {done} iterations, rel {r.relative_residual:.2e}"))
     upd = 0.0
 
@@ -285,7 +287,7 @@ def run_loading(args, torch):
           {"workload": spec.name, "n_particles": spec.n, "n_bonds": nb0, "increments": len(res),
            "critical_stretch": spec.extra["critical_stretch"], "variant": pass_variants(body)[0]},
           time_to_solve={"loading_s": round(wall, 3), "setup_s": round(setup, 3),
-                         "dr_mass": mass_label(args.dr_mass == "spectral"),
+                         "dr_mass": mass_label(args.dr_mass == "spectral"), "predictor": args.predictor,
                          "broken_total": res[-1].broken_total if res else 0,
                          "all_converged": all(r.converged for r in res),
                          "increments": [{"k": r.increment, "iterations": r.iterations,

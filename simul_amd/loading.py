@@ -41,7 +41,7 @@ class IncrementalLoading:
     def __init__(self, body, indices, u_final, n_increments: int = 20, critical_stretch: Optional[float] = None,
                  tol: float = 1e-6, max_iterations: int = 200000, method: str = "adaptive",
                  max_break_rounds: int = 3, dt: float = 0.0, viscous_damping: float = 0.0,
-                 progress=None, slice_iterations: int = 4096, mass=None):
+                 progress=None, slice_iterations: int = 4096, mass=None, predictor=None):
         if n_increments < 1:
             raise ValueError("n_increments must be >= 1")
         self.body = body
@@ -54,6 +54,18 @@ class IncrementalLoading:
         self.dt, self.viscous_damping = float(dt), float(viscous_damping)
         self.progress, self.slice_iterations = progress, slice_iterations
         self.mass = mass  # adaptive-DR mass of every solve (PDBody.solve's mass=)
+        # start of an increment: None = the previous state with the new
+        # boundary values; "secant" = 2 u_(k-1) - u_(k-2) from the third
+        # increment on (the convergence reference stays the residual of the
+        # plain start, so both reach the same tolerance)
+        if predictor not in (None, "secant"):
+            raise ValueError("predictor must be None or 'secant'")
+        self.predictor = predictor
+        self._hist: List[np.ndarray] = []
+        # broken bonds so far: counted once here, then from each break_bonds
+        # call (pdx_stats reads the whole bond list back)
+        st = body.stats()
+        self._broken = int(st["n_bonds"] - st["n_intact"])
         self.results: List[IncrementResult] = []
 
     def _solve(self, ref=0.0):
@@ -70,7 +82,15 @@ class IncrementalLoading:
         u = b.u
         u[self.indices] = (k / self.n_increments) * self.u_final
         b.u = u
-        r = self._solve()
+        if self.predictor == "secant" and len(self._hist) == 2:
+            b.compute_forces()
+            ref0 = float(np.atleast_1d(b.residual_norm())[0])
+            up = 2.0 * self._hist[1] - self._hist[0]
+            up[self.indices] = u[self.indices]
+            b.u = up
+            r = self._solve(ref0 if ref0 > 0.0 else 0.0)
+        else:
+            r = self._solve()
         # re-equilibration after breaking is measured against the increment's
         # own start residual: the residual right after a few bonds break is
         # tiny, and 1e-6 of it lies below the FP32 force noise
@@ -91,9 +111,11 @@ class IncrementalLoading:
                 iters += r.iterations
         if b.model == "j2_plasticity":
             b.commit_state()
-        st = b.stats()
+        if self.predictor is not None:
+            self._hist = (self._hist + [b.u])[-2:]
+        self._broken += broken_now
         res = IncrementResult(k, iters, bool(r.converged), float(r.relative_residual), broken_now,
-                              int(st["n_bonds"] - st["n_intact"]), rounds, time.perf_counter() - t0)
+                              self._broken, rounds, time.perf_counter() - t0)
         self.results.append(res)
         return res
 

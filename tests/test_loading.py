@@ -118,3 +118,25 @@ def test_config3_full_size_flags_and_j2(gpu):
     prob.commit()
     assert prob.epbar.max() > 0
     assert rel_l2(body.get("epbar"), prob.epbar) < 1e-3
+
+
+def test_secant_predictor_and_spectral_mass_same_states(gpu):
+    """The J2 loading with the secant predictor and the spectral DR mass ends
+    every increment at the state of the plain driver (same tolerance against
+    the same plain-start reference) in fewer DR iterations; the oracle's
+    final state agrees to the tolerance of test_incremental_j2_matches_oracle."""
+    spec = W.vertebra_j2(dx=1.25, n_increments=6)
+    idx, uf = spec.extra["top_idx"], spec.extra["u_final"]
+    runs = {}
+    for key, kw in (("plain", {}), ("fast", {"predictor": "secant", "mass": "spectral"})):
+        body = from_specimen(spec)
+        drv = IncrementalLoading(body, idx, uf, n_increments=6, critical_stretch=None, tol=1e-6, **kw)
+        res = drv.run()
+        assert all(r.converged for r in res), key
+        runs[key] = (sum(r.iterations for r in res), body.u.copy(), body.get("epbar").copy())
+    assert runs["fast"][0] < runs["plain"][0], (runs["fast"][0], runs["plain"][0])
+    assert rel_l2(runs["fast"][1], runs["plain"][1]) < 1e-4
+    prob = _oracle(spec)
+    _oracle_increments(prob, idx, uf, 6, 1e-6)
+    assert rel_l2(runs["fast"][1], prob.u) < 1e-4
+    assert rel_l2(runs["fast"][2], prob.epbar) < 1e-3
