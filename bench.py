@@ -170,11 +170,17 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get(dom_name)
+    # the whole DR iteration (both passes) against the per-iteration model
+    # 16 B/bond + 245 B/particle
+    it_bytes = 16.0 * nb + 245.0 * n_p
+    it_gbs = it_bytes / (ms_per_step * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dom_name, "achieved": round(dom["achieved_GBs"], 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom["achieved_GBs"] / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "algorithmic_bytes": dom["bytes"], "mean_us": round(dom["mean_us"], 2),
                 "kernels": {kf_name: {k: round(v, 2) for k, v in k_f.items()},
-                            kg_name: {k: round(v, 2) for k, v in k_g.items()}}}
+                            kg_name: {k: round(v, 2) for k, v in k_g.items()}},
+                "iteration": {"algorithmic_bytes": it_bytes, "achieved_GBs": round(it_gbs, 1),
+                              "frac": round(it_gbs / HBM_PEAK_GBS, 4), "ms": round(ms_per_step, 4)}}
     log(f"kernels: fpass {k_f['mean_us']:.1f} us ({k_f['achieved_GBs']:.0f} GB/s), "
         f"force {k_g['mean_us']:.1f} us ({k_g['achieved_GBs']:.0f} GB/s)")
 
