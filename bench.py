@@ -94,7 +94,7 @@ def main():
     ap.add_argument("--predictor", choices=["secant", "none"], default="secant",
                     help="start of each load increment of the J2 loading (secant: 2 u_(k-1) - u_(k-2))")
     ap.add_argument("--variant", type=str, default="0",
-                    help="hot-pass variant (1, 2, 4, 17, 18, 33, 34, 97, 98, 161, 162) or 'F,G' for the F-pass "
+                    help="hot-pass variant (1, 2, 4, 17, 18, 33, 34, 36, 97, 98, 100, 161, 162, 164) or 'F,G' for the F-pass "
                          "and the force pass; 0 = autotune")
     ap.add_argument("--workload", default="vertebra", choices=["vertebra", "vertebra_j2", "batch"],
                     help="vertebra: the headline (dx 0.4, Neo-Hookean, pressure); vertebra_j2: J2, 20 "

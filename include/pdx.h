@@ -217,11 +217,12 @@ int pdx_time_kernels(pdx_ctx *ctx, int32_t iters, double *ms);
 /* hot-pass variants: per-lane neighbour gathers with bond groups 1, 2, 4, or
  * wave-cooperative gathers (four lanes fetch one 64-byte neighbour record, staged
  * through LDS) with bond groups 1, 2 as PDX_VARIANT_COOP | group.  These variants
- * give the same bits.  PDX_VARIANT_HALO | group (1, 2): each block stages the records
- * of every particle its rows reach into LDS once, positions relative to a per-block
- * anchor in FP32, and reads its bonds from LDS; results agree with the others to FP32
- * rounding of the bond vectors (about 1e-7 relative).  A pass whose halo does not fit
- * the device's LDS runs the PDX_VARIANT_COOP | 2 kernel instead. */
+ * give the same bits.  PDX_VARIANT_HALO | group (1, 2, 4): each block stages the
+ * records of every particle its rows reach into LDS once, positions relative to a
+ * per-block anchor in FP32, and reads its bonds from LDS; the NOSB force pass sums each
+ * group's terms in FP32 before adding them to its FP64 sum.  Results agree with the
+ * others to FP32 rounding (about 1e-7 relative).  A pass whose halo does not fit the
+ * device's LDS runs the PDX_VARIANT_COOP | 2 kernel instead. */
 #define PDX_VARIANT_COOP 16
 #define PDX_VARIANT_HALO 32
 /* with PDX_VARIANT_HALO: each row's bond slots split over 2 or 4 waves that share the
@@ -232,7 +233,7 @@ int pdx_time_kernels(pdx_ctx *ctx, int32_t iters, double *ms);
  * every hot-pass variant on this context's own data, keeps the fastest for each pass
  * and leaves the solver state as it was; group_out = the force pass's variant */
 int pdx_autotune(pdx_ctx *ctx, int32_t iters, int32_t *group_out);
-/* sets the hot-pass variant (1, 2, 4, PDX_VARIANT_COOP | 1 or 2, PDX_VARIANT_HALO | 1 or 2
+/* sets the hot-pass variant (1, 2, 4, PDX_VARIANT_COOP | 1 or 2, PDX_VARIANT_HALO | 1, 2 or 4
  * optionally | PDX_VARIANT_SPLIT2 or PDX_VARIANT_SPLIT4; the halo variants need the bonds
  * built) */
 int pdx_set_group(pdx_ctx *ctx, int32_t group);

@@ -69,7 +69,7 @@ static void free_all(pdx_ctx *c)
     dfree(d.mass); dfree(d.lam); dfree(d.V64); dfree(d.K64); dfree(d.Kinv64);
     dfree(d.Fd); dfree(d.Pd); dfree(d.damage); dfree(d.init_nb);
     dfree(d.sl_ptr); dfree(d.bj); dfree(d.case_of_blk); dfree(d.case_blk0);
-    dfree(d.hl_ptr); dfree(d.hl); dfree(d.hs); dfree(d.bl); dfree(d.anchor);
+    dfree(d.hl_ptr); dfree(d.hl); dfree(d.hs); dfree(d.bl); dfree(d.bk); dfree(d.anchor);
     dfree(d.sc); dfree(d.part); dfree(d.grid); dfree(d.case_of_p);
     dfree(d.cell_start); dfree(d.cell_idx); dfree(d.counts);
     if (c->gexec) hipGraphExecDestroy(c->gexec);
@@ -599,7 +599,7 @@ extern "C" int pdx_count_neighbors(pdx_ctx *c, double horizon, int32_t *counts)
 static int build_halo(pdx_ctx *c)
 {
     Dev &d = c->d;
-    dfree(d.hl_ptr); dfree(d.hl); dfree(d.hs); dfree(d.bl); dfree(d.anchor);
+    dfree(d.hl_ptr); dfree(d.hl); dfree(d.hs); dfree(d.bl); dfree(d.bk); dfree(d.anchor);
     d.halo_cap = 0;
     invalidate_graph(c);
     const int nbk = c->ni / PDX_BLOCK, D = c->dim;
@@ -609,7 +609,7 @@ static int build_halo(pdx_ctx *c)
     std::vector<std::vector<int32_t>> ent_list(nbk);
     std::vector<std::vector<uint16_t>> slot_list(nbk);
     std::vector<int32_t> cap_b(nbk, 0);
-    std::vector<uint16_t> bl((size_t)c->slots, (uint16_t)PDX_HNONE);
+    std::vector<uint16_t> bl((size_t)c->slots, (uint16_t)PDX_HNONE), bk((size_t)c->slots, 0);
     std::vector<pdx_d4> anc(nbk);
     std::vector<uint8_t> over(nbk, 0);
     constexpr int SPB = PDX_BLOCK / PDX_SLICE;
@@ -673,6 +673,17 @@ static int build_halo(pdx_ctx *c)
                 if (l < 0 || l >= PDX_BLOCK) l = PDX_BLOCK + (int32_t)(std::lower_bound(o.begin(), o.end(), j) - o.begin());
                 bl[s] = (uint16_t)(slot_of[l] | ((e & PDX_BROKEN) ? PDX_HBROKEN : 0u));
             }
+            // the slot each bond slot reads: its particle's for an intact bond
+            // of an active row, else the row's own (slot_of[t] for row i0 + t)
+            for (int q = 0; q < SPB; ++q) {
+                const int64_t a0 = c->h_slptr[(size_t)b * SPB + q], a1 = c->h_slptr[(size_t)b * SPB + q + 1];
+                for (int64_t s = a0; s < a1; ++s) {
+                    const int t = q * PDX_SLICE + (int)((s - a0) % PDX_SLICE);
+                    const uint16_t e = bl[s];
+                    const bool live = e != PDX_HNONE && !(e & PDX_HBROKEN) && c->perm[i0 + t] >= 0;
+                    bk[s] = live ? e : (uint16_t)slot_of[t];
+                }
+            }
             double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
             for (int t = 0; t < PDX_BLOCK; ++t) {
                 const int32_t r = c->perm[i0 + t];
@@ -711,11 +722,13 @@ static int build_halo(pdx_ctx *c)
     PDX_TRY(dalloc(c->stream, &d.hl, hl.size()));
     PDX_TRY(dalloc(c->stream, &d.hs, hs.size()));
     PDX_TRY(dalloc(c->stream, &d.bl, bl.size()));
+    PDX_TRY(dalloc(c->stream, &d.bk, bk.size()));
     PDX_TRY(dalloc(c->stream, &d.anchor, anc.size()));
     PDX_TRY(upload(c, d.hl_ptr, ptr));
     PDX_TRY(upload(c, d.hl, hl));
     PDX_TRY(upload(c, d.hs, hs));
     PDX_TRY(upload(c, d.bl, bl));
+    PDX_TRY(upload(c, d.bk, bk));
     PDX_TRY(upload(c, d.anchor, anc));
     d.halo_cap = (int32_t)hmax;
     return PDX_OK;
@@ -1408,7 +1421,7 @@ static bool valid_variant(int32_t g)
     return ok;
 }
 
-static const char *k_variant_msg = "variant must be 1, 2, 4, PDX_VARIANT_COOP | 1 or 2, or PDX_VARIANT_HALO | 1 or 2 "
+static const char *k_variant_msg = "variant must be 1, 2, 4, PDX_VARIANT_COOP | 1 or 2, or PDX_VARIANT_HALO | 1, 2 or 4 "
                                    "with optional PDX_VARIANT_SPLIT2 or PDX_VARIANT_SPLIT4";
 
 extern "C" int pdx_set_group(pdx_ctx *c, int32_t g)

@@ -139,12 +139,15 @@ struct Dev {
     // block the particles its rows reach (hl_ptr / hl: own particles first,
     // then the others in index order) and each one's LDS slot (hs), the LDS
     // slot of each bond in the SELL layout of bj (bl: PDX_HBROKEN set for a
-    // broken bond, 0xFFFF for no bond) and the FP64 anchor the staged
-    // positions are relative to
+    // broken bond, 0xFFFF for no bond), the slot the kernels read for each
+    // bond (bk: bl's slot for an intact bond of an active row, else the row's
+    // own slot, whose bond terms are exact zeros) and the FP64 anchor the
+    // staged positions are relative to
     int32_t *hl_ptr;
     int32_t *hl;
     uint16_t *hs;
     uint16_t *bl;
+    uint16_t *bk;
     pdx_d4 *anchor;
     int32_t halo_cap;  // LDS slots per block: the largest block's slot count (a multiple of 16)
     int32_t halo_pad;
@@ -327,6 +330,13 @@ PDX_HD float bond_weight(float L, float invh, float Vj)
     return (L > 0.f ? fmaxf(0.f, 1.f - L * invh) : 1.f) * Vj;
 }
 
+// the same weight without the |xi| = 0 case: the halo kernels' dead bond slots
+// (xi = 0, du = 0) contribute zero whatever their weight
+PDX_HD float bond_weight_live(float L, float invh, float Vj)
+{
+    return fmaxf(0.f, 1.f - L * invh) * Vj;
+}
+
 // ---------------------------------------------------------------------------
 // F-pass: H = [sum_j w V_j du (x) xi] K^-1, P(H), A = P K^-1
 // ---------------------------------------------------------------------------
@@ -336,12 +346,12 @@ PDX_HD float bond_weight(float L, float invh, float Vj)
 // (bond vector xi, displacement difference du, neighbour volume Vj).  A slot
 // without a live bond passes the particle's own record: xi = du = 0, so its
 // term is exactly zero and no select is needed.
-template <int D>
+template <int D, bool HALO = false>
 PDX_HD void fpass_bond_v(const float *xi, const float *du, float Vj, float invh, double *Hd)
 {
     float L2 = 0.f;
     for (int r = 0; r < D; ++r) L2 = fmaf(xi[r], xi[r], L2);
-    const float w = bond_weight(bond_sqrt(L2), invh, Vj);
+    const float w = HALO ? bond_weight_live(bond_sqrt(L2), invh, Vj) : bond_weight(bond_sqrt(L2), invh, Vj);
     for (int r = 0; r < D; ++r) {
         const double wd = (double)(w * du[r]);
         for (int c = 0; c < D; ++c) Hd[r * D + c] = fma(wd, (double)xi[c], Hd[r * D + c]);
@@ -464,12 +474,15 @@ PDX_HD void fpass_finish(const Dev &d, int i, const KRec &kr, const double *Hd)
 // numerator and denominator; UPD 2: red[1] = kinetic energy.
 // ---------------------------------------------------------------------------
 // force term of one bond, w V_j [(A_j - A_i) xi + g eta] (FP32), summed into
-// fa in FP64.  The correspondence term is in difference form:
+// fa (FP64, or FP32 for a partial sum the caller adds to FP64).  The
+// correspondence term is in difference form:
 // sum_j w (A_i + A_j) xi = sum_j w (A_j - A_i) xi + 2 A_i m_i, so each FP32
 // bond product is rounded relative to |A_j - A_i| rather than |A|.
-template <int D, int FORM>
+// HALO: the weight without the |xi| = 0 case, and ci, cj come premultiplied
+// by G_s / 2 (NOSB) or 1/2 (bond form).
+template <int D, int FORM, bool HALO = false, typename Acc = double>
 PDX_HD void force_bond_v(const float *xi, const float *du, float Vj, float ci, float cj, const float *Ai,
-                         const float *Aj, float invh, float Gs, double *fa)
+                         const float *Aj, float invh, float Gs, Acc *fa)
 {
     float L2 = 0.f, xd = 0.f, dd = 0.f;
     for (int r = 0; r < D; ++r) {
@@ -478,7 +491,7 @@ PDX_HD void force_bond_v(const float *xi, const float *du, float Vj, float ci, f
         dd = fmaf(du[r], du[r], dd);
     }
     const float L = bond_sqrt(L2);
-    const float w = bond_weight(L, invh, Vj);
+    const float w = HALO ? bond_weight_live(L, invh, Vj) : bond_weight(L, invh, Vj);
     float eta[3];
     for (int r = 0; r < D; ++r) eta[r] = xi[r] + du[r];
     // |eta|^2 = |xi|^2 + (2 xi.du + du.du); the stretch uses the
@@ -493,13 +506,14 @@ PDX_HD void force_bond_v(const float *xi, const float *du, float Vj, float ci, f
             for (int c = 0; c < D; ++c) t = fmaf(Aj[r * D + c] - Ai[r * D + c], xi[c], t);
             tb[r] = t;
         }
-        if (el > 1e-10f && L > 1e-10f) g = Gs * 0.5f * (ci + cj) * dn * bond_rcp((el + L) * L * el);
+        if (el > 1e-10f && L > 1e-10f)
+            g = (HALO ? ci + cj : Gs * 0.5f * (ci + cj)) * dn * bond_rcp((el + L) * L * el);
     } else if (el > 1e-10f) {
-        g = 0.5f * (ci + cj) * dn * bond_rcp((el + L) * L * el);
+        g = (HALO ? ci + cj : 0.5f * (ci + cj)) * dn * bond_rcp((el + L) * L * el);
     }
     for (int r = 0; r < D; ++r) {
         const float contrib = w * fmaf(g, eta[r], tb[r]);
-        fa[r] += (double)contrib;
+        fa[r] += (Acc)contrib;
     }
 }
 
@@ -1026,7 +1040,10 @@ PDX_HD unsigned damage_one(const Dev &d, int i, int ubuf, double s_c)
         if (L > 1e-10) st = (el - L) / L;
         if (st > s_c) {
             d.bj[a] = e | PDX_BROKEN;
-            if (d.bl) d.bl[a] |= PDX_HBROKEN;
+            if (d.bl) {
+                d.bl[a] |= PDX_HBROKEN;
+                d.bk[a] = d.hs[d.hl_ptr[i / PDX_BLOCK] + i % PDX_BLOCK];
+            }
             nb++;
         }
     }

@@ -32,7 +32,7 @@ struct pdx_ctx {
     // bond group 1, 2, 4, PDX_VARIANT_COOP | 1, 2 or a PDX_HALO_VARIANTS code
     // (default: the LDS-staged kernels, which fall back to PDX_VARIANT_COOP | 2
     // where a halo does not fit)
-    int tune_g = PDX_VARIANT_HALO | PDX_VARIANT_SPLIT4 | 2, tune_f = PDX_VARIANT_HALO | PDX_VARIANT_SPLIT2 | 2;
+    int tune_g = PDX_VARIANT_HALO | PDX_VARIANT_SPLIT4 | 4, tune_f = PDX_VARIANT_HALO | PDX_VARIANT_SPLIT4 | 1;
     int64_t lds_dyn_max = 0;  // dynamic LDS bytes a block may use on this device
     Dev d{};
     // host mirror of slice pointers
@@ -74,7 +74,8 @@ int pdx_halo_lds(const pdx_ctx *c, int entry_bytes, int split, int nv);
 #define PDX_HALO_VARIANTS(X)                                                                   \
     X(PDX_VARIANT_HALO | 1) X(PDX_VARIANT_HALO | 2) X(PDX_VARIANT_HALO | PDX_VARIANT_SPLIT2 | 1) \
     X(PDX_VARIANT_HALO | PDX_VARIANT_SPLIT2 | 2) X(PDX_VARIANT_HALO | PDX_VARIANT_SPLIT4 | 1)     \
-    X(PDX_VARIANT_HALO | PDX_VARIANT_SPLIT4 | 2)
+    X(PDX_VARIANT_HALO | PDX_VARIANT_SPLIT4 | 2) X(PDX_VARIANT_HALO | 4) X(PDX_VARIANT_HALO | PDX_VARIANT_SPLIT2 | 4) \
+    X(PDX_VARIANT_HALO | PDX_VARIANT_SPLIT4 | 4)
 int pdx_launch_resid(pdx_ctx *c);
 int pdx_launch_zero_free(pdx_ctx *c, int ubuf);
 int pdx_launch_kr_put(pdx_ctx *c, const float *d_src, int off, int cnt);

@@ -389,21 +389,24 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_force_w(Dev d, int ubuf, int solv
 // once per bond, ~116 times per pass over all blocks, through an L1 that
 // holds a few hundred records.  Here a block first stages its halo into LDS
 // (one 64-byte record per entry), then every lane walks its row reading LDS
-// at the bond's slot (Dev::bl).  Slots follow the particles' bank colours, so
-// the 16 lanes that read together mostly hit 16 different bank groups.
-// Staged per slot (the F-pass in one LDS array per field, the force pass in
-// one 80-byte record per slot, see hslot):
+// at the bond's slot (Dev::bk, which already points dead bonds at the row's
+// own slot, so the loops carry no broken-bond test).  Slots follow the
+// particles' bank colours, so the 16 lanes that read together mostly hit 16
+// different bank groups.  Staged per slot (the F-pass in one LDS array per
+// field, the force pass in one record per slot, see hslot):
 //   xv  : X - anchor in FP32 (3), V                              16 B
 //   u01 : u_x, u_y in FP64                                       16 B
 //   F-pass: u2 = u_z in FP64                                      8 B
-//   force pass: uac = u_z in FP64, A_zz, c                       16 B
+//   force pass: u2c = u_z in FP64, A_zz (NOSB), c'               16 B
 //   force pass, NOSB: a0 = A (0..3), a1 = A (4..7)               32 B
-// The anchor is an FP64 point at the block's centre, so |X - anchor| is
-// bounded by the block's extent plus the horizon and the staged bond vector
-// xv_j - xv_i differs from the FP64 difference by that extent's FP32
-// rounding (about 1e-7 of the horizon); displacement differences are FP64
-// as in the gathering kernels, and the bond arithmetic is the same
-// fpass_bond_v / force_bond_v.
+// where c' = c G_s / 2 (NOSB) or c / 2 (bond form).  The anchor is an FP64
+// point at the block's centre, so |X - anchor| is bounded by the block's
+// extent plus the horizon and the staged bond vector xv_j - xv_i differs from
+// the FP64 difference by that extent's FP32 rounding (about 1e-7 of the
+// horizon).  Displacement differences are taken in FP64 and rounded per
+// bond: an FP32 u - u_anchor would round once per particle, by the span of
+// u over the halo rather than over a bond, and that per-particle error put a
+// floor of a few 1e-6 under the relative residual of adaptive DR.
 typedef float pdx_q2 __attribute__((ext_vector_type(2)));
 
 // field offsets (bytes) and bytes per slot of the staged halo
@@ -454,6 +457,7 @@ __device__ __forceinline__ void halo_stage(const Dev &d, int ubuf, int lb, pdx_q
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int q = lane & 3, sub = lane >> 2;
     const int qa = q < 3 ? q : 2;  // A records have 3 quarters
+    const float csc = FORM == 0 ? (float)(0.5 * d.G_s) : 0.5f;
     for (int g0 = wv * 16; g0 < nh; g0 += U * NW * 16) {
         int j[U], sl[U];
         bool ok[U];
@@ -474,22 +478,22 @@ __device__ __forceinline__ void halo_stage(const Dev &d, int ubuf, int lb, pdx_q
         for (int v = 0; v < U; ++v) {
             if (!ok[v]) continue;
             const int sv = sl[v];
-            if (q == 0) {
+            if (q == 0) {  // X_x, X_y
                 const PdxD2 x = __builtin_bit_cast(PdxD2, r[v]);
                 *reinterpret_cast<pdx_q2 *>(hslot<FORCE, FORM>(lds, sv, O::xv, cap)) =
                     pdx_q2{(float)(x.a - an.x), (float)(x.b - an.y)};
                 if (FORCE && FORM == 0) *reinterpret_cast<pdx_q4 *>(hslot<FORCE, FORM>(lds, sv, O::a0, cap)) = a[v];
-            } else if (q == 1) {
+            } else if (q == 1) {  // X_z, V, c
                 const double x2 = __builtin_bit_cast(double, r[v].xy);
                 *reinterpret_cast<pdx_q2 *>(hslot<FORCE, FORM>(lds, sv, O::xv, cap) + 8) =
                     pdx_q2{(float)(x2 - an.z), r[v].z};
-                if (FORCE) *reinterpret_cast<float *>(hslot<FORCE, FORM>(lds, sv, O::u2, cap) + 12) = r[v].w;  // c
+                if (FORCE) *reinterpret_cast<float *>(hslot<FORCE, FORM>(lds, sv, O::u2, cap) + 12) = r[v].w * csc;
                 if (FORCE && FORM == 0) *reinterpret_cast<pdx_q4 *>(hslot<FORCE, FORM>(lds, sv, O::a1, cap)) = a[v];
-            } else if (q == 2) {
+            } else if (q == 2) {  // u_x, u_y
                 *reinterpret_cast<pdx_q4 *>(hslot<FORCE, FORM>(lds, sv, O::u01, cap)) = r[v];
-                if (FORCE)  // A_zz
-                    *reinterpret_cast<float *>(hslot<FORCE, FORM>(lds, sv, O::u2, cap) + 8) = FORM == 0 ? a[v].x : 0.f;
-            } else {
+                if (FORCE && FORM == 0)  // A_zz
+                    *reinterpret_cast<float *>(hslot<FORCE, FORM>(lds, sv, O::u2, cap) + 8) = a[v].x;
+            } else {  // u_z
                 *reinterpret_cast<pdx_q2 *>(hslot<FORCE, FORM>(lds, sv, O::u2, cap)) = r[v].xy;
             }
         }
@@ -497,8 +501,8 @@ __device__ __forceinline__ void halo_stage(const Dev &d, int ubuf, int lb, pdx_q
     __syncthreads();
 }
 
-// bond vector and displacement difference of halo entry l against the own
-// entry (xo, uo)
+// bond vector and displacement difference of halo entry (xj, uj01, uj2)
+// against the own entry (xo, uo01, uo2)
 __device__ __forceinline__ void halo_bond(const pdx_q4 &xj, const pdx_q4 &uj01, const pdx_q2 &uj2, const pdx_q4 &xo,
                                           const PdxD2 &uo01, double uo2, float (&xi)[3], float (&du)[3])
 {
@@ -510,6 +514,42 @@ __device__ __forceinline__ void halo_bond(const pdx_q4 &xj, const pdx_q4 &uj01, 
     du[0] = (float)(a.a - uo01.a);
     du[1] = (float)(a.b - uo01.b);
     du[2] = (float)(b - uo2);
+}
+
+// Walk the bond slots of a row: slot k's entry is bp[k * PDX_SLICE] below
+// the slice width, the own slot `so` past it.  Part p of S takes the groups
+// of G slots starting at p G, p G + G S, ...  Two groups are processed per
+// trip and the entries of the next two are loaded at its start, so an entry
+// load (from HBM; bk is read once per pass) has two groups' arithmetic to
+// arrive in.  width is the same for the 64 rows of a wave, so the test of
+// the second group is uniform.
+template <int G, int S, typename F>
+__device__ __forceinline__ void halo_row_loop(const uint16_t *__restrict__ bp, int width, int part, int so, F &&group)
+{
+    constexpr int STEP = G * S;
+    const int k0 = part * G;
+    auto entry = [&](int k) -> uint32_t { return k < width ? (uint32_t)bp[(int64_t)k * PDX_SLICE] : (uint32_t)so; };
+    uint32_t ea[G], eb[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+        ea[q] = entry(k0 + q);
+        eb[q] = entry(k0 + STEP + q);
+    }
+    for (int k = k0; k < width; k += 2 * STEP) {
+        uint32_t na[G], nb[G];
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            na[q] = entry(k + 2 * STEP + q);
+            nb[q] = entry(k + 3 * STEP + q);
+        }
+        group(ea);
+        if (k + STEP < width) group(eb);
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            ea[q] = na[q];
+            eb[q] = nb[q];
+        }
+    }
 }
 
 // Split rows: S waves share a block's halo and rows; part p (threadIdx.x /
@@ -562,36 +602,24 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_h(Dev d, int ubuf, int 
     for (int q = 0; q < D * D; ++q) Hd[q] = 0.0;
     const int64_t s0 = d.sl_ptr[i >> 6];
     const int width = (int)((d.sl_ptr[(i >> 6) + 1] - s0) / PDX_SLICE);
-    const uint16_t *__restrict__ bp = d.bl + s0 + lane;
-    constexpr int STEP = G * S;
-    const int k0 = part * G;
-    uint32_t ec[G], en[G];
-#pragma unroll
-    for (int q = 0; q < G; ++q) ec[q] = (k0 + q) < width ? bp[(int64_t)(k0 + q) * PDX_SLICE] : PDX_HNONE;
-    for (int k = k0; k < width; k += STEP) {
-#pragma unroll
-        for (int q = 0; q < G; ++q)
-            en[q] = (k + STEP + q) < width ? bp[(int64_t)(k + STEP + q) * PDX_SLICE] : PDX_HNONE;
-        bool live[G];
+    const uint16_t *__restrict__ bp = d.bk + s0 + lane;
+    auto group = [&](const uint32_t (&e)[G]) {
         pdx_q4 xj[G], uj01[G];
         pdx_q2 uj2[G];
 #pragma unroll
         for (int q = 0; q < G; ++q) {
-            live[q] = act && !(ec[q] & PDX_HBROKEN);
-            const int l = live[q] ? (int)ec[q] : so;
-            xj[q] = xv[l];
-            uj01[q] = u01[l];
-            uj2[q] = u2[l];
+            xj[q] = xv[e[q]];
+            uj01[q] = u01[e[q]];
+            uj2[q] = u2[e[q]];
         }
 #pragma unroll
         for (int q = 0; q < G; ++q) {
             float xi[3], du[3];
             halo_bond(xj[q], uj01[q], uj2[q], xo, uo01, uo2, xi, du);
-            fpass_bond_v<D>(xi, du, xj[q].w, invh, Hd);
+            fpass_bond_v<D, true>(xi, du, xj[q].w, invh, Hd);
         }
-#pragma unroll
-        for (int q = 0; q < G; ++q) ec[q] = en[q];
-    }
+    };
+    halo_row_loop<G, S>(bp, width, part, so, group);
     split_sum<D * D, S>(hlds, Hd);
     if (part == 0 && act) fpass_finish<D, MODEL, false, false>(d, i, kr, Hd);
 }
@@ -607,7 +635,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
     double red[3] = {0.0, 0.0, 0.0};
     if (lb * PDX_BLOCK < d.ni) {  // grid padding blocks only join the reduction
         halo_stage<1, FORM, PDX_BLOCK * S>(d, ubuf, lb, hlds);
-        // slot l's record: xv at 0, u01 at 16, uac at 32, a0 at 48, a1 at 64
+        // slot l's record: xv at 0, u01 at 16, u2c at 32, a0 at 48, a1 at 64
         const char *hb = reinterpret_cast<const char *>(hlds);
         auto rec = [&](int l, int f) { return *reinterpret_cast<const pdx_q4 *>(hb + l * O::bytes + f); };
         const int t = threadIdx.x % PDX_BLOCK, lane = t & 63, part = threadIdx.x / PDX_BLOCK;
@@ -615,7 +643,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
         const KRec kr = d.KR[i];
         const bool act = (kr.flags & PDX_FL_ACTIVE) != 0;
         // the own record and A_i come from the staged halo: force_finish
-        // reads u (and force_bond_v c) of pi, not X
+        // reads u of pi, not X
         const int so = d.hs[d.hl_ptr[lb] + t];
         const pdx_q4 xo = rec(so, O::xv);
         const PdxD2 uo01 = __builtin_bit_cast(PdxD2, rec(so, O::u01));
@@ -625,7 +653,6 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
         pi.u[0] = uo01.a;
         pi.u[1] = uo01.b;
         pi.u[2] = uo2;
-        pi.c = uco.w;
         ARec ai{};
         if (FORM == 0) {
             const pdx_q4 p0 = rec(so, O::a0), p1 = rec(so, O::a1);
@@ -634,26 +661,15 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
             ai.A[8] = uco.z;
         }
         const float invh = (float)d.inv_h;
-        const float Gs = (float)d.G_s;
         double fa[3] = {0.0, 0.0, 0.0};
         const int64_t s0 = d.sl_ptr[i >> 6];
         const int width = (int)((d.sl_ptr[(i >> 6) + 1] - s0) / PDX_SLICE);
-        const uint16_t *__restrict__ bp = d.bl + s0 + lane;
-        constexpr int STEP = G * S;
-        const int k0 = part * G;
-        uint32_t ec[G], en[G];
-#pragma unroll
-        for (int q = 0; q < G; ++q) ec[q] = (k0 + q) < width ? bp[(int64_t)(k0 + q) * PDX_SLICE] : PDX_HNONE;
-        for (int k = k0; k < width; k += STEP) {
-#pragma unroll
-            for (int q = 0; q < G; ++q)
-                en[q] = (k + STEP + q) < width ? bp[(int64_t)(k + STEP + q) * PDX_SLICE] : PDX_HNONE;
-            bool live[G];
+        const uint16_t *__restrict__ bp = d.bk + s0 + lane;
+        auto group = [&](const uint32_t (&e)[G]) {
             pdx_q4 xj[G], uj01[G], ujc[G], aj0[G], aj1[G];
 #pragma unroll
             for (int q = 0; q < G; ++q) {
-                live[q] = act && !(ec[q] & PDX_HBROKEN);
-                const int l = live[q] ? (int)ec[q] : so;
+                const int l = (int)e[q];
                 xj[q] = rec(l, O::xv);
                 uj01[q] = rec(l, O::u01);
                 ujc[q] = rec(l, O::u2);
@@ -662,35 +678,57 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
                     aj1[q] = rec(l, O::a1);
                 }
             }
+            // NOSB: the group's terms are summed in FP32 (each is a difference
+            // term, small against the particle's sum) and the group sums in
+            // FP64; the bond form adds every term in FP64
+            float fg[3] = {0.f, 0.f, 0.f};
 #pragma unroll
             for (int q = 0; q < G; ++q) {
                 float xi[3], du[3];
                 halo_bond(xj[q], uj01[q], ujc[q].xy, xo, uo01, uo2, xi, du);
-                float aj[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                if (FORM == 0) {
-                    aj[0] = aj0[q].x; aj[1] = aj0[q].y; aj[2] = aj0[q].z; aj[3] = aj0[q].w;
-                    aj[4] = aj1[q].x; aj[5] = aj1[q].y; aj[6] = aj1[q].z; aj[7] = aj1[q].w;
-                    aj[8] = ujc[q].z;
+                if constexpr (FORM == 0) {
+                    const float aj[9] = {aj0[q].x, aj0[q].y, aj0[q].z, aj0[q].w, aj1[q].x,
+                                         aj1[q].y, aj1[q].z, aj1[q].w, ujc[q].z};
+                    force_bond_v<D, FORM, true, float>(xi, du, xj[q].w, uco.w, ujc[q].w, ai.A, aj, invh, 1.f, fg);
+                } else {
+                    force_bond_v<D, FORM, true, double>(xi, du, xj[q].w, uco.w, ujc[q].w, ai.A, nullptr, invh, 1.f,
+                                                        fa);
                 }
-                force_bond_v<D, FORM>(xi, du, xj[q].w, pi.c, ujc[q].w, ai.A, aj, invh, Gs, fa);
             }
+            if constexpr (FORM == 0) {
 #pragma unroll
-            for (int q = 0; q < G; ++q) ec[q] = en[q];
-        }
+                for (int r = 0; r < 3; ++r) fa[r] += (double)fg[r];
+            }
+        };
+        halo_row_loop<G, S>(bp, width, part, so, group);
         split_sum<3, S>(hlds, fa);
         if (part == 0 && act) force_finish<D, FORM, UPD>(d, i, kr, pi, ai, ubuf, cs, dt, visc, fa, red);
     }
     block_partials<3, PDX_BLOCK * S>(d, lb, red);
 }
 
-// broken flags of the halo-local entries from bj (after bj was rewritten)
-__global__ __launch_bounds__(PDX_BLOCK) void k_halo_bits(Dev d, int64_t slots)
+// bl's broken flags from bj (after bj was rewritten) and the slots the halo
+// kernels read (bk), one row per thread
+__global__ __launch_bounds__(PDX_BLOCK) void k_halo_bits(Dev d)
 {
-    const int64_t s = (int64_t)blockIdx.x * PDX_BLOCK + threadIdx.x;
-    if (s >= slots) return;
-    const uint16_t e = d.bl[s];
-    if (e == PDX_HNONE) return;
-    d.bl[s] = (uint16_t)((e & ~PDX_HBROKEN) | ((d.bj[s] & PDX_BROKEN) ? PDX_HBROKEN : 0u));
+    const int i = blockIdx.x * PDX_BLOCK + threadIdx.x;
+    if (i >= d.ni) return;
+    const uint16_t own = d.hs[d.hl_ptr[i / PDX_BLOCK] + i % PDX_BLOCK];
+    const bool act = (d.KR[i].flags & PDX_FL_ACTIVE) != 0;
+    const int64_t s0 = d.sl_ptr[i >> 6];
+    const int width = (int)((d.sl_ptr[(i >> 6) + 1] - s0) / PDX_SLICE);
+    for (int k = 0; k < width; ++k) {
+        const int64_t s = s0 + (int64_t)k * PDX_SLICE + (i & (PDX_SLICE - 1));
+        const uint16_t e = d.bl[s];
+        if (e == PDX_HNONE) {
+            d.bk[s] = own;
+            continue;
+        }
+        const bool br = (d.bj[s] & PDX_BROKEN) != 0;
+        const uint16_t l = (uint16_t)(e & ~PDX_HBROKEN);
+        d.bl[s] = (uint16_t)(l | (br ? PDX_HBROKEN : 0u));
+        d.bk[s] = (act && !br) ? l : own;
+    }
 }
 
 template <int D>
@@ -968,8 +1006,8 @@ int pdx_launch_force_pass(pdx_ctx *c, int ubuf, int upd, bool solve_mode, double
 int pdx_launch_halo_bits(pdx_ctx *c)
 {
     if (!c->d.bl || c->slots <= 0) return PDX_OK;
-    hipLaunchKernelGGL(k_halo_bits, dim3((unsigned)((c->slots + PDX_BLOCK - 1) / PDX_BLOCK)), dim3(PDX_BLOCK), 0,
-                       c->stream, c->d, c->slots);
+    hipLaunchKernelGGL(k_halo_bits, dim3((unsigned)((c->ni + PDX_BLOCK - 1) / PDX_BLOCK)), dim3(PDX_BLOCK), 0,
+                       c->stream, c->d);
     PDX_LAUNCH_CHECK();
     return PDX_OK;
 }

@@ -285,7 +285,7 @@ def test_hot_pass_variants_bitwise_identical(gpu, formulation):
             assert np.array_equal(uu, base[1]), (method, g)
 
 
-HALO = tuple(32 | s | g for s in (0, 64, 128) for g in (1, 2))  # PDX_VARIANT_HALO [| SPLIT2/4] | group
+HALO = tuple(32 | s | g for s in (0, 64, 128) for g in (1, 2, 4))  # PDX_VARIANT_HALO [| SPLIT2/4] | group
 
 
 def _halo_body(formulation):
@@ -312,8 +312,9 @@ def test_halo_variants_match_gathering(gpu, formulation):
     in FP32, so their bond vectors differ from the gathering kernels' FP64
     differences by ~1e-7 relative.  Forces agree to 2e-6 with broken bonds
     inside rows (set by the damage kernel, then rewritten by pdx_set_broken)
-    and a batch whose cases end in padding particles; groups 1 and 2 give the
-    same bits, split rows (other FP64 summation order) agree to 1e-12.
+    and a batch whose cases end in padding particles; groups 2 and 4 (NOSB
+    force terms summed per group in FP32) agree with group 1 to 1e-7, split
+    rows (other FP64 summation order) agree to 1e-12.
     Adaptive DR to 1e-6 converges in the same number of iterations (+-2%) to
     the same displacements (1e-5), and 40 kinetic-DR steps agree to 1e-5."""
     X, body = _halo_body(formulation)
@@ -336,7 +337,8 @@ def test_halo_variants_match_gathering(gpu, formulation):
             fs[g] = body.compute_forces().copy()
             assert rel_l2(fs[g], f0) < 2e-6, (step, g, rel_l2(fs[g], f0))
         for sp in (0, 64, 128):
-            assert np.array_equal(fs[32 | sp | 1], fs[32 | sp | 2]), (step, sp)
+            for g in (2, 4):
+                assert rel_l2(fs[32 | sp | g], fs[32 | sp | 1]) < 1e-7, (step, sp, g)
             assert rel_l2(fs[32 | sp | 1], fs[32 | 1]) < 1e-12, (step, sp)
     X, body = _halo_body(formulation)
     ref = {}
@@ -386,9 +388,10 @@ def test_sliced_solve_matches_single(gpu):
 
 def test_halo_variants_2d_and_lds_fallback(gpu):
     """2-D bodies through the halo kernels, and a horizon of 4 spacings whose
-    force-pass halo (~3 000 slots x 80 B) exceeds the 160 KB of LDS while the
-    F-pass halo (40 B per slot) fits: the force pass then runs the
-    cooperative kernel.  Forces agree with the gathering kernels to 2e-6."""
+    NOSB force-pass halo (~3 000 slots x 80 B) exceeds the 160 KB of LDS
+    while the F-pass halo (40 B per slot) fits: the NOSB force pass then runs
+    the cooperative kernel.  Forces agree with the gathering kernels to
+    2e-6."""
     g = np.arange(40.0)
     X2 = np.stack(np.meshgrid(g, g[:30], indexing="ij"), -1).reshape(-1, 2)
     X2 = X2 + np.random.default_rng(4).uniform(-0.2, 0.2, X2.shape)
