@@ -609,7 +609,7 @@ static int build_halo(pdx_ctx *c)
     std::vector<std::vector<int32_t>> ent_list(nbk);
     std::vector<std::vector<uint16_t>> slot_list(nbk);
     std::vector<int32_t> cap_b(nbk, 0);
-    std::vector<uint16_t> bl((size_t)c->slots, (uint16_t)PDX_HNONE), bk((size_t)c->slots, 0);
+    std::vector<uint16_t> bl((size_t)c->slots, (uint16_t)PDX_HNONE), bk((size_t)c->slots + PDX_BK_TAIL, 0);
     std::vector<pdx_d4> anc(nbk);
     std::vector<uint8_t> over(nbk, 0);
     constexpr int SPB = PDX_BLOCK / PDX_SLICE;

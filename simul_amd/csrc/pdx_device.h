@@ -31,6 +31,11 @@
 #define PDX_BLOCK 256
 #endif
 #define PDX_SLICE 64
+// slice widths (bond slots per row of a slice) are multiples of this
+#define PDX_SLOT_ALIGN 4
+// bond slots past the end of bk that the halo kernels' look-ahead loads may
+// touch (never used): three groups of the largest row split (4 parts x 4)
+#define PDX_BK_TAIL (3 * 16 * PDX_SLICE + 4 * PDX_SLICE)
 #define PDX_BROKEN 0x80000000u
 // halo-local bond entries (16 bits): position in the block's halo, broken flag
 #define PDX_HBROKEN 0x8000u
@@ -512,8 +517,12 @@ PDX_HD void force_bond_v(const float *xi, const float *du, float Vj, float ci, f
         g = (HALO ? ci + cj : 0.5f * (ci + cj)) * dn * bond_rcp((el + L) * L * el);
     }
     for (int r = 0; r < D; ++r) {
-        const float contrib = w * fmaf(g, eta[r], tb[r]);
-        fa[r] += (Acc)contrib;
+        if constexpr (sizeof(Acc) == sizeof(float)) {
+            fa[r] = fmaf(w, fmaf(g, eta[r], tb[r]), fa[r]);
+        } else {
+            const float contrib = w * fmaf(g, eta[r], tb[r]);
+            fa[r] += (Acc)contrib;
+        }
     }
 }
 

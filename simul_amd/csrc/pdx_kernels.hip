@@ -516,19 +516,22 @@ __device__ __forceinline__ void halo_bond(const pdx_q4 &xj, const pdx_q4 &uj01, 
     du[2] = (float)(b - uo2);
 }
 
-// Walk the bond slots of a row: slot k's entry is bp[k * PDX_SLICE] below
-// the slice width, the own slot `so` past it.  Part p of S takes the groups
-// of G slots starting at p G, p G + G S, ...  Two groups are processed per
-// trip and the entries of the next two are loaded at its start, so an entry
-// load (from HBM; bk is read once per pass) has two groups' arithmetic to
-// arrive in.  width is the same for the 64 rows of a wave, so the test of
-// the second group is uniform.
+// Walk the bond slots of a row: slot k's entry is bp[k * PDX_SLICE].  Part p
+// of S takes the groups of G slots starting at p G, p G + G S, ...  Two
+// groups are processed per trip and the entries of the next two are loaded
+// at its start, so an entry load (from HBM; bk is read once per pass) has two
+// groups' arithmetic to arrive in.  The slice width is a multiple of
+// PDX_SLOT_ALIGN, so a group never straddles it, and the look-ahead loads
+// past it land in the next slice or in bk's tail and are never used.  width
+// is the same for the 64 rows of a wave, so the test of the second group is
+// uniform.
 template <int G, int S, typename F>
-__device__ __forceinline__ void halo_row_loop(const uint16_t *__restrict__ bp, int width, int part, int so, F &&group)
+__device__ __forceinline__ void halo_row_loop(const uint16_t *__restrict__ bp, int width, int part, F &&group)
 {
+    static_assert(PDX_SLOT_ALIGN % G == 0 && G * S * 3 + PDX_SLOT_ALIGN <= PDX_BK_TAIL / PDX_SLICE, "group layout");
     constexpr int STEP = G * S;
     const int k0 = part * G;
-    auto entry = [&](int k) -> uint32_t { return k < width ? (uint32_t)bp[(int64_t)k * PDX_SLICE] : (uint32_t)so; };
+    auto entry = [&](int k) -> uint32_t { return (uint32_t)bp[(int64_t)k * PDX_SLICE]; };
     uint32_t ea[G], eb[G];
 #pragma unroll
     for (int q = 0; q < G; ++q) {
@@ -619,7 +622,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_h(Dev d, int ubuf, int 
             fpass_bond_v<D, true>(xi, du, xj[q].w, invh, Hd);
         }
     };
-    halo_row_loop<G, S>(bp, width, part, so, group);
+    halo_row_loop<G, S>(bp, width, part, group);
     split_sum<D * D, S>(hlds, Hd);
     if (part == 0 && act) fpass_finish<D, MODEL, false, false>(d, i, kr, Hd);
 }
@@ -637,7 +640,10 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
         halo_stage<1, FORM, PDX_BLOCK * S>(d, ubuf, lb, hlds);
         // slot l's record: xv at 0, u01 at 16, u2c at 32, a0 at 48, a1 at 64
         const char *hb = reinterpret_cast<const char *>(hlds);
-        auto rec = [&](int l, int f) { return *reinterpret_cast<const pdx_q4 *>(hb + l * O::bytes + f); };
+        // (slots are below 2^15, so the 24-bit multiply gives the offset)
+        auto rec = [&](int l, int f) {
+            return *reinterpret_cast<const pdx_q4 *>(hb + __umul24((unsigned)l, (unsigned)O::bytes) + f);
+        };
         const int t = threadIdx.x % PDX_BLOCK, lane = t & 63, part = threadIdx.x / PDX_BLOCK;
         const int i = lb * PDX_BLOCK + t;
         const KRec kr = d.KR[i];
@@ -700,7 +706,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
                 for (int r = 0; r < 3; ++r) fa[r] += (double)fg[r];
             }
         };
-        halo_row_loop<G, S>(bp, width, part, so, group);
+        halo_row_loop<G, S>(bp, width, part, group);
         split_sum<3, S>(hlds, fa);
         if (part == 0 && act) force_finish<D, FORM, UPD>(d, i, kr, pi, ai, ubuf, cs, dt, visc, fa, red);
     }

@@ -200,6 +200,9 @@ inline int64_t sell_pointers(const std::vector<int32_t> &cnt, std::vector<int64_
             w = std::max(w, cnt[s * PDX_SLICE + t]);
             total += cnt[s * PDX_SLICE + t];
         }
+        // widths are multiples of PDX_SLOT_ALIGN, so a group of up to that
+        // many slots never straddles a slice's end
+        w = (w + PDX_SLOT_ALIGN - 1) / PDX_SLOT_ALIGN * PDX_SLOT_ALIGN;
         slptr[s + 1] = slptr[s] + (int64_t)w * PDX_SLICE;
     }
     return slptr[nsl];
