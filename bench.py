@@ -170,6 +170,23 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get(dom_name)
+    # VALU issue: wave64 VALU instructions per launch of the running variants
+    # (SQ_INSTS_VALU of the same workload, profiles/valu_*.json) over the
+    # measured launch time, against 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles
+    valu = None
+    vfile = os.path.join(ROOT, "profiles", f"valu_{spec.name}.json")
+    fv, gv = body.variants()
+    if os.path.exists(vfile):
+        with open(vfile) as fh:
+            vj = json.load(fh)
+        peak = vj["peak_wave_instructions_per_s"]
+        valu = {"unit": "G wave64 VALU instructions/s", "peak": round(peak / 1e9, 1)}
+        for key, code, kt_ms, name in (("fpass", fv, kt["fpass_ms"], kf_name), ("force", gv, kt["force_ms"], kg_name)):
+            n_ins = vj[key].get(str(code))
+            if n_ins:
+                rate = n_ins / (kt_ms * 1e-3)
+                valu[name] = {"variant": code, "instructions": n_ins, "achieved": round(rate / 1e9, 1),
+                              "frac": round(rate / peak, 4)}
     # the whole DR iteration (both passes) against the per-iteration model
     # 16 B/bond + 245 B/particle
     it_bytes = 16.0 * nb + 245.0 * n_p
@@ -180,7 +197,8 @@ def main():
                 "kernels": {kf_name: {k: round(v, 2) for k, v in k_f.items()},
                             kg_name: {k: round(v, 2) for k, v in k_g.items()}},
                 "iteration": {"algorithmic_bytes": it_bytes, "achieved_GBs": round(it_gbs, 1),
-                              "frac": round(it_gbs / HBM_PEAK_GBS, 4), "ms": round(ms_per_step, 4)}}
+                              "frac": round(it_gbs / HBM_PEAK_GBS, 4), "ms": round(ms_per_step, 4)},
+                "valu": valu}
     log(f"kernels: fpass {k_f['mean_us']:.1f} us ({k_f['achieved_GBs']:.0f} GB/s), "
         f"force {k_g['mean_us']:.1f} us ({k_g['achieved_GBs']:.0f} GB/s)")
 
