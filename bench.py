@@ -241,6 +241,10 @@ def main():
             tts.update({"warm_start_s": round(tw["total_s"], 4), "warm_iterations": rw.iterations,
                         "warm_converged": rw.converged, "warm_inference_s": round(tw["inference_s"], 4),
                         "warm_speedup": round(wall / tw["total_s"], 3),
+                        # the warm-started solver (learned start, spectral
+                        # mass) against the plain cold start (u = 0, the
+                        # Gershgorin mass of the reference-faithful DR)
+                        "warm_speedup_vs_plain_cold": round(wall_g / tw["total_s"], 3),
                         "warm_vs_cold_u_rel_l2": du,
                         "warm_start": "learned per-particle network (BF16 MFMA kernels), features, inference and "
                                       "DR included in warm_start_s"})

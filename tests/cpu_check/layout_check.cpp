@@ -4,10 +4,14 @@
 //      offset, holds 16 different colours;
 //   2. morton_order on a (jittered) lattice quantises to the lattice sites: in
 //      a lattice whose extents are multiples of the box, every aligned run of
-//      16 internal indices is one box, so its 16 colours differ.
+//      16 internal indices is one box, so its 16 colours differ;
+//   3. sell_pointers: every slice's width is its longest row rounded up to a
+//      multiple of PDX_SLOT_ALIGN (the halo kernels load groups of up to that
+//      many slots without a bounds test), and the bond total is exact.
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <algorithm>
 #include <random>
 #include <vector>
 
@@ -74,6 +78,35 @@ int main()
         }
         for (size_t r = 0; r < inv.size(); ++r)
             if (perm[inv[r]] != (int32_t)r) return fail("perm / inv not inverse");
+    }
+    // 3. slice widths
+    {
+        std::uniform_int_distribution<int> nn(0, 140);
+        std::vector<int32_t> cnt(64 * 37);
+        int64_t want = 0;
+        for (auto &c : cnt) {
+            c = nn(rng);
+            want += c;
+        }
+        for (int t = 0; t < 64; ++t) cnt[5 * 64 + t] = 0;     // an empty slice
+        for (int t = 0; t < 64; ++t) cnt[6 * 64 + t] = t % 9;  // widest row 8
+        want = 0;
+        for (int c : cnt) want += c;
+        std::vector<int64_t> slptr;
+        int64_t total = -1;
+        const int64_t slots = pdx_layout::sell_pointers(cnt, slptr, total);
+        if (total != want) return fail("sell_pointers bond total");
+        if (slots != slptr.back() || slptr[0] != 0) return fail("sell_pointers ends");
+        for (size_t s = 0; s + 1 < slptr.size(); ++s) {
+            int w = 0;
+            for (int t = 0; t < 64; ++t) w = std::max(w, cnt[s * 64 + t]);
+            const int64_t width = (slptr[s + 1] - slptr[s]) / 64;
+            if ((slptr[s + 1] - slptr[s]) % 64 || width % PDX_SLOT_ALIGN || width < w ||
+                width >= w + PDX_SLOT_ALIGN)
+                return fail("slice width not the longest row rounded up to PDX_SLOT_ALIGN");
+        }
+        if (slptr[6] != slptr[5]) return fail("empty slice has slots");
+        if (slptr[7] - slptr[6] != 8 * 64) return fail("slice of widest row 8 is not 8 wide");
     }
     printf("layout ok\n");
     return 0;

@@ -87,9 +87,10 @@ def test_kernels_in_bounds_vertebra_coarse(exe, tmp_path):
 
 def test_layout_bank_colours_and_lattice_quantisation(exe):
     """The LDS slot colours of the halo kernels: any shifted 2x2x4 (2-D: 4x4)
-    box of cells holds 16 colours, and the Morton quantisation at the median
-    spacing puts each aligned run of 16 keys of a jittered lattice on one box
-    (tests/cpu_check/layout_check.cpp)."""
+    box of cells holds 16 colours, the Morton quantisation at the median
+    spacing puts each aligned run of 16 keys of a jittered lattice on one box,
+    and every SELL slice width is its longest row rounded up to a multiple of
+    4 slots (tests/cpu_check/layout_check.cpp)."""
     r = subprocess.run([os.path.join(HERE, "layout_check")], capture_output=True, text=True, timeout=120,
                        env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"))
     assert r.returncode == 0, r.stderr[-2000:]
