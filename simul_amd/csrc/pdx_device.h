@@ -538,9 +538,30 @@ PDX_HD void force_bond(const XRec &pi, const XRec &pj, const float *Ai, const fl
     force_bond_v<D, FORM>(xi, du, pj.V, pi.c, pj.c, Ai, Aj, invh, Gs, fa);
 }
 
+// per-particle inputs of force_finish (DR mass, velocity, previous force or
+// acceleration, external load), which a kernel may load ahead of its use
+struct FinishIn {
+    double m;
+    pdx_d4 v, fp, fe;
+};
+
+template <int UPD>
+PDX_HD FinishIn finish_in(const Dev &d, int i, const KRec &kr)
+{
+    FinishIn fi{};
+    if (kr.flags & PDX_FL_LOADED) fi.fe = d.fext[i];
+    if (UPD == 1) fi.m = d.lam[i];
+    if (UPD == 2) fi.m = d.mass[i];
+    if (UPD != 0) {
+        fi.v = d.v[i];
+        fi.fp = d.fp[i];
+    }
+    return fi;
+}
+
 template <int D, int FORM, int UPD>
 PDX_HD void force_finish(const Dev &d, int i, const KRec &kr, const XRec &pi, const ARec &ai, int ubuf, int cs,
-                         double dt, double visc, double *fa, double *red);
+                         double dt, double visc, double *fa, double *red, const FinishIn &fi);
 
 template <int D, int FORM, int UPD, int G>
 PDX_HD void force_one(const Dev &d, int i, int ubuf, int cs, double dt, double visc, double *red)
@@ -572,14 +593,14 @@ PDX_HD void force_one(const Dev &d, int i, int ubuf, int cs, double dt, double v
         for (int q = 0; q < G; ++q) force_bond<D, FORM>(pi, pj[q], ai.A, aj[q].A, invh, Gs, fa);
         for (int q = 0; q < G; ++q) ec[q] = en[q];
     }
-    force_finish<D, FORM, UPD>(d, i, kr, pi, ai, ubuf, cs, dt, visc, fa, red);
+    force_finish<D, FORM, UPD>(d, i, kr, pi, ai, ubuf, cs, dt, visc, fa, red, finish_in<UPD>(d, i, kr));
 }
 
 // per-particle end of the force pass: + 2 A_i m_i, + external load, |f|^2
 // over free DOFs into red[0], then the update of UPD
 template <int D, int FORM, int UPD>
 PDX_HD void force_finish(const Dev &d, int i, const KRec &kr, const XRec &pi, const ARec &ai, int ubuf, int cs,
-                         double dt, double visc, double *fa, double *red)
+                         double dt, double visc, double *fa, double *red, const FinishIn &fi)
 {
     if (FORM == 0) {
         for (int r = 0; r < D; ++r) {
@@ -589,7 +610,7 @@ PDX_HD void force_finish(const Dev &d, int i, const KRec &kr, const XRec &pi, co
         }
     }
     if (kr.flags & PDX_FL_LOADED) {
-        const pdx_d4 fe = d.fext[i];
+        const pdx_d4 fe = fi.fe;
         fa[0] += fe.x;
         fa[1] += fe.y;
         if (D == 3) fa[2] += fe.z;
@@ -603,8 +624,8 @@ PDX_HD void force_finish(const Dev &d, int i, const KRec &kr, const XRec &pi, co
         const DScal &S = d.sc[cs];
         const double c = S.c;
         const int first = S.first;
-        const double lam = d.lam[i];
-        const pdx_d4 vo4 = d.v[i], fo4 = d.fp[i];
+        const double lam = fi.m;
+        const pdx_d4 vo4 = fi.v, fo4 = fi.fp;
         const double vo[3] = {vo4.x, vo4.y, vo4.z}, fo[3] = {fo4.x, fo4.y, fo4.z};
         const double uo[3] = {pi.u[0], pi.u[1], pi.u[2]};
         double vn[3] = {0.0, 0.0, 0.0}, un[3] = {uo[0], uo[1], uo[2]}, fn[3] = {0.0, 0.0, 0.0};
@@ -628,8 +649,8 @@ PDX_HD void force_finish(const Dev &d, int i, const KRec &kr, const XRec &pi, co
         uw[1] = un[1];
         uw[2] = un[2];
     } else {
-        const double m = d.mass[i];
-        const pdx_d4 vo4 = d.v[i], ao4 = d.fp[i];
+        const double m = fi.m;
+        const pdx_d4 vo4 = fi.v, ao4 = fi.fp;
         double vv[3] = {vo4.x, vo4.y, vo4.z}, aa[3] = {ao4.x, ao4.y, ao4.z};
         const double damp = 1.0 - visc;
         for (int r = 0; r < D; ++r) {

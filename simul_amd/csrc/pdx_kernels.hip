@@ -376,7 +376,7 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_force_w(Dev d, int ubuf, int solv
 #pragma unroll
             for (int q = 0; q < G; ++q) ec[q] = en[q];
         }
-        if (act) force_finish<D, FORM, UPD>(d, i, kr, pi, ai, ubuf, cs, dt, visc, fa, red);
+        if (act) force_finish<D, FORM, UPD>(d, i, kr, pi, ai, ubuf, cs, dt, visc, fa, red, finish_in<UPD>(d, i, kr));
     }
     block_partials<3>(d, lb, red);
 }
@@ -707,8 +707,12 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
             }
         };
         halo_row_loop<G, S>(bp, width, part, group);
+        // part 0 finishes its rows first (a SIMD issues to its oldest wave
+        // first) and loads the update's inputs while the other parts end
+        FinishIn fin{};
+        if (part == 0 && act) fin = finish_in<UPD>(d, i, kr);
         split_sum<3, S>(hlds, fa);
-        if (part == 0 && act) force_finish<D, FORM, UPD>(d, i, kr, pi, ai, ubuf, cs, dt, visc, fa, red);
+        if (part == 0 && act) force_finish<D, FORM, UPD>(d, i, kr, pi, ai, ubuf, cs, dt, visc, fa, red, fin);
     }
     block_partials<3, PDX_BLOCK * S>(d, lb, red);
 }
