@@ -604,6 +604,9 @@ static int build_halo(pdx_ctx *c)
     invalidate_graph(c);
     const int nbk = c->ni / PDX_BLOCK, D = c->dim;
     if (nbk == 0 || c->slots <= 0) return PDX_OK;
+    // the halo staging addresses particle records and halo lists with 32-bit
+    // byte offsets (at32); larger problems run the gathering kernels
+    if ((uint64_t)c->ni * sizeof(XRec) > UINT32_MAX) return PDX_OK;
     std::vector<uint32_t> bj;
     PDX_TRY(download(c, bj, d.bj, (size_t)c->slots));
     std::vector<std::vector<int32_t>> ent_list(nbk);
@@ -707,10 +710,13 @@ static int build_halo(pdx_ctx *c)
     for (auto &x : th) x.join();
     std::vector<int32_t> ptr(nbk + 1, 0);
     int64_t hmax = 0;
+    int64_t nent = 0;
     for (int b = 0; b < nbk; ++b) {
         if (over[b]) return PDX_OK;
         hmax = std::max(hmax, (int64_t)cap_b[b]);
-        ptr[b + 1] = ptr[b] + (int32_t)ent_list[b].size();
+        nent += (int64_t)ent_list[b].size();
+        if (nent * (int64_t)sizeof(int32_t) > (int64_t)UINT32_MAX) return PDX_OK;
+        ptr[b + 1] = (int32_t)nent;
     }
     std::vector<int32_t> hl((size_t)ptr[nbk]);
     std::vector<uint16_t> hs((size_t)ptr[nbk]);

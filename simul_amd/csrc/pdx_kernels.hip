@@ -438,6 +438,18 @@ __device__ __forceinline__ char *hslot(pdx_q4 *base, int s, int f, int cap)
     return b + f * cap + s * (f == HaloOff<0, 0>::u2 ? 8 : 16);
 }
 
+// element at a 32-bit byte offset from a base that is the same for the whole
+// wave: the load takes the base from SGPRs and the offset from one VGPR, with
+// no 64-bit address arithmetic per lane.  The halo tables are built only when
+// every offset used this way fits in 32 bits (build_halo: particle records
+// and halo lists).  The row loops keep a per-lane pointer instead, whose
+// slot loads differ by immediate offsets.
+template <typename T, typename B>
+__device__ __forceinline__ T at32(const B *base, uint32_t byte_off)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
 // stage the halo of block lb with NT threads (ends with a block barrier).
 // Four lanes per entry: lane L of a wave loads the 16-byte quarter L & 3 of
 // entry 16 k + (L >> 2), so one load instruction covers 16 consecutive halo
@@ -466,13 +478,14 @@ __device__ __forceinline__ void halo_stage(const Dev &d, int ubuf, int lb, pdx_q
         for (int v = 0; v < U; ++v) {
             const int e = g0 + v * NW * 16 + sub;
             ok[v] = e < nh;
-            j[v] = d.hl[h0 + (ok[v] ? e : 0)];
-            sl[v] = ok[v] ? (int)d.hs[h0 + e] : 0;
+            const uint32_t he = (uint32_t)(h0 + (ok[v] ? e : 0));
+            j[v] = at32<int32_t>(d.hl, he * 4u);
+            sl[v] = ok[v] ? (int)at32<uint16_t>(d.hs, he * 2u) : 0;
         }
 #pragma unroll
         for (int v = 0; v < U; ++v) {
-            r[v] = Xq[(int64_t)j[v] * 4 + q];
-            if (FORCE && FORM == 0) a[v] = Aq[(int64_t)j[v] * 3 + qa];
+            r[v] = at32<pdx_q4>(Xq, (uint32_t)j[v] * 64u + (uint32_t)q * 16u);
+            if (FORCE && FORM == 0) a[v] = at32<pdx_q4>(Aq, (uint32_t)j[v] * 48u + (uint32_t)qa * 16u);
         }
 #pragma unroll
         for (int v = 0; v < U; ++v) {
