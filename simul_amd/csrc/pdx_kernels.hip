@@ -450,6 +450,15 @@ __device__ __forceinline__ T at32(const B *base, uint32_t byte_off)
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
 }
 
+// x of lane P_k of this lane's quad (k = lane & 3), by DPP quad permutation;
+// every lane of the quad must be active
+template <int P0, int P1, int P2, int P3>
+__device__ __forceinline__ float dpp_quad(float x)
+{
+    constexpr int ctrl = P0 | (P1 << 2) | (P2 << 4) | (P3 << 6);
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), ctrl, 0xF, 0xF, false));
+}
+
 // stage the halo of block lb with NT threads (ends with a block barrier).
 // Four lanes per entry: lane L of a wave loads the 16-byte quarter L & 3 of
 // entry 16 k + (L >> 2), so one load instruction covers 16 consecutive halo
@@ -486,6 +495,33 @@ __device__ __forceinline__ void halo_stage(const Dev &d, int ubuf, int lb, pdx_q
         for (int v = 0; v < U; ++v) {
             r[v] = at32<pdx_q4>(Xq, (uint32_t)j[v] * 64u + (uint32_t)q * 16u);
             if (FORCE && FORM == 0) a[v] = at32<pdx_q4>(Aq, (uint32_t)j[v] * 48u + (uint32_t)qa * 16u);
+        }
+        if constexpr (FORCE && FORM == 0) {
+            // NOSB record: the quad's lanes trade three values with DPP and
+            // each lane writes one 16-byte field (lane 0 also a0): xv from
+            // lane 0 (its x, y and lane 1's z, V), a1 from lane 1, u01 from
+            // lane 2, u2 from lane 3 (its u_z and lane 2's A_zz, lane 1's c')
+#pragma unroll
+            for (int v = 0; v < U; ++v) {
+                const PdxD2 x = __builtin_bit_cast(PdxD2, r[v]);
+                const float t0 = (float)(x.a - (q == 0 ? an.x : an.z));  // lane 0: x - a_x; lane 1: z - a_z
+                const float t1 = (float)(x.b - an.y);                     // lane 0: y - a_y
+                const float m1 = dpp_quad<1, 1, 1, 2>(q == 1 ? t0 : a[v].x);  // lane 0: z; lane 3: A_zz
+                const float mv = dpp_quad<1, 1, 1, 1>(r[v].z);                 // lane 0: V
+                const float mc = dpp_quad<1, 1, 1, 1>(r[v].w * csc);           // lane 3: c'
+                pdx_q4 out;
+                if (q == 0) out = pdx_q4{t0, t1, m1, mv};
+                else if (q == 1) out = a[v];
+                else if (q == 2) out = r[v];
+                else out = pdx_q4{r[v].x, r[v].y, m1, mc};
+                const int off = q == 0 ? O::xv : (q == 1 ? O::a1 : (q == 2 ? O::u01 : O::u2));
+                if (ok[v]) {
+                    char *b = hslot<FORCE, FORM>(lds, sl[v], 0, cap);
+                    *reinterpret_cast<pdx_q4 *>(b + off) = out;
+                    if (q == 0) *reinterpret_cast<pdx_q4 *>(b + O::a0) = a[v];
+                }
+            }
+            continue;
         }
 #pragma unroll
         for (int v = 0; v < U; ++v) {
