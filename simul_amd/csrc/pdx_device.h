@@ -363,6 +363,22 @@ PDX_HD void fpass_bond_v(const float *xi, const float *du, float Vj, float invh,
     }
 }
 
+// the halo F-pass's form of the same term: S[r D + c] += u_j[r] wx_c and
+// S[D D + c] += wx_c with wx = w V_j xi rounded to FP32 once (the caller
+// subtracts u_i (x) sum wx); dead slots (xi = 0) add zeros
+template <int D>
+PDX_HD void fpass_bond_u(const float *xi, const double *uj, float Vj, float invh, double *S)
+{
+    float L2 = 0.f;
+    for (int r = 0; r < D; ++r) L2 = fmaf(xi[r], xi[r], L2);
+    const float w = bond_weight_live(bond_sqrt(L2), invh, Vj);
+    for (int c = 0; c < D; ++c) {
+        const double wx = (double)(w * xi[c]);
+        for (int r = 0; r < D; ++r) S[r * D + c] = fma(uj[r], wx, S[r * D + c]);
+        S[D * D + c] += wx;
+    }
+}
+
 template <int D>
 PDX_HD void fpass_bond(const XRec &pi, const XRec &pj, float invh, double *Hd)
 {

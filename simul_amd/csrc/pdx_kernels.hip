@@ -649,9 +649,14 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_h(Dev d, int ubuf, int 
     const PdxD2 uo01 = __builtin_bit_cast(PdxD2, u01[so]);
     const double uo2 = __builtin_bit_cast(double, u2[so]);
     const float invh = (float)d.inv_h;
-    double Hd[D * D];
+    // sum_j w V_j (u_j - u_i) (x) xi as sum_j u_j (x) wx_j - u_i (x) sum_j wx_j
+    // with wx = w V_j xi rounded to FP32 once and both sums in FP64: the
+    // rounding of wx is shared by the two sums, so their difference is the
+    // FP64 sum of (u_j - u_i) (x) wx without rounding u_j - u_i (Sd[D * D + c]
+    // holds sum_j wx_c)
+    double Sd[D * D + D];
 #pragma unroll
-    for (int q = 0; q < D * D; ++q) Hd[q] = 0.0;
+    for (int q = 0; q < D * D + D; ++q) Sd[q] = 0.0;
     const int64_t s0 = d.sl_ptr[i >> 6];
     const int width = (int)((d.sl_ptr[(i >> 6) + 1] - s0) / PDX_SLICE);
     const uint16_t *__restrict__ bp = d.bk + s0 + lane;
@@ -666,14 +671,23 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_h(Dev d, int ubuf, int 
         }
 #pragma unroll
         for (int q = 0; q < G; ++q) {
-            float xi[3], du[3];
-            halo_bond(xj[q], uj01[q], uj2[q], xo, uo01, uo2, xi, du);
-            fpass_bond_v<D, true>(xi, du, xj[q].w, invh, Hd);
+            const float xi[3] = {xj[q].x - xo.x, xj[q].y - xo.y, xj[q].z - xo.z};
+            const PdxD2 a = __builtin_bit_cast(PdxD2, uj01[q]);
+            const double uj[3] = {a.a, a.b, __builtin_bit_cast(double, uj2[q])};
+            fpass_bond_u<D>(xi, uj, xj[q].w, invh, Sd);
         }
     };
     halo_row_loop<G, S>(bp, width, part, group);
-    split_sum<D * D, S>(hlds, Hd);
-    if (part == 0 && act) fpass_finish<D, MODEL, false, false>(d, i, kr, Hd);
+    split_sum<D * D + D, S>(hlds, Sd);
+    if (part == 0 && act) {
+        const double ui[3] = {uo01.a, uo01.b, uo2};
+        double Hd[D * D];
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int c = 0; c < D; ++c) Hd[r * D + c] = Sd[r * D + c] - ui[r] * Sd[D * D + c];
+        fpass_finish<D, MODEL, false, false>(d, i, kr, Hd);
+    }
 }
 
 template <int D, int FORM, int UPD, int G, int S>
@@ -943,7 +957,7 @@ static void launch_fpass_g(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm)
     } else {
         if constexpr ((G & PDX_VARIANT_HALO) != 0) {
             constexpr int GH = halo_group(G), S = halo_split(G);
-            const int lds = pdx_halo_lds(c, HaloOff<0, 0>::bytes, S, D * D);
+            const int lds = pdx_halo_lds(c, HaloOff<0, 0>::bytes, S, D * D + D);
             if (lds)
                 hipLaunchKernelGGL((k_fpass_h<D, MODEL, GH, S>), g, dim3(PDX_BLOCK * S), lds, c->stream, c->d, ubuf, sm);
             else
