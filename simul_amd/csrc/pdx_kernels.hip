@@ -469,7 +469,9 @@ __device__ __forceinline__ void halo_stage(const Dev &d, int ubuf, int lb, pdx_q
 {
     using O = HaloOff<FORCE, FORM>;
     constexpr int NW = NT / 64;
-    constexpr int U = 4;  // 16-entry groups per wave in flight
+    // 16-entry groups per wave in flight: the force pass's 112 bytes per entry
+    // are fetched best two groups at a time, the F-pass's 64 four at a time
+    constexpr int U = FORCE ? 2 : 4;
     const int cap = d.halo_cap;
     const int h0 = d.hl_ptr[lb], nh = d.hl_ptr[lb + 1] - h0;
     const pdx_d4 an = d.anchor[lb];
