@@ -40,23 +40,28 @@ __device__ __forceinline__ double ld_coherent(const double *p)
 // Block sum of NV doubles into d.part[lb * 4 + q] (waves in order).  The
 // case totals are summed by k_case_reduce, a separate launch: a last-block
 // reduction inside the pass would need an agent-scope release per block,
-// which writes back the XCD's L2 (every block's stores) on gfx950.
-template <int NV, int NT = PDX_BLOCK>
+// which writes back the XCD's L2 (every block's stores) on gfx950.  Only the
+// first NA waves hold values (the others' are zero and are left out, which
+// does not change the sum).
+template <int NV, int NT = PDX_BLOCK, int NA = NT / 64>
 __device__ void block_partials(const Dev &d, int lb, double (&val)[NV])
 {
-    __shared__ double sred[NT / 64][NV];
+    __shared__ double sred[NA][NV];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (wid < NA) {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        double t = wave_sum(val[q]);
-        if (lane == 0) sred[wid][q] = t;
+        for (int q = 0; q < NV; ++q) {
+            double t = wave_sum(val[q]);
+            if (lane == 0) sred[wid][q] = t;
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
             double t = 0.0;
-            for (int w = 0; w < NT / 64; ++w) t += sred[w][q];
+#pragma unroll
+            for (int w = 0; w < NA; ++w) t += sred[w][q];
             d.part[(int64_t)lb * 4 + q] = t;
         }
     }
@@ -606,6 +611,7 @@ __device__ __forceinline__ void halo_row_loop(const uint16_t *__restrict__ bp, i
     }
 }
 
+
 // Split rows: S waves share a block's halo and rows; part p (threadIdx.x /
 // PDX_BLOCK) takes the bond slots whose group index is p mod S, and the
 // partial sums meet in LDS (reusing the halo area) in part order before the
@@ -779,7 +785,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
         split_sum<3, S>(hlds, fa);
         if (part == 0 && act) force_finish<D, FORM, UPD>(d, i, kr, pi, ai, ubuf, cs, dt, visc, fa, red, fin);
     }
-    block_partials<3, PDX_BLOCK * S>(d, lb, red);
+    block_partials<3, PDX_BLOCK * S, PDX_BLOCK / 64>(d, lb, red);  // part 0's waves
 }
 
 // bl's broken flags from bj (after bj was rewritten) and the slots the halo
