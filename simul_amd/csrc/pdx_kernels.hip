@@ -611,7 +611,6 @@ __device__ __forceinline__ void halo_row_loop(const uint16_t *__restrict__ bp, i
     }
 }
 
-
 // Split rows: S waves share a block's halo and rows; part p (threadIdx.x /
 // PDX_BLOCK) takes the bond slots whose group index is p mod S, and the
 // partial sums meet in LDS (reusing the halo area) in part order before the
