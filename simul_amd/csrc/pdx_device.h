@@ -345,27 +345,16 @@ PDX_HD float bond_weight_live(float L, float invh, float Vj)
 // ---------------------------------------------------------------------------
 // F-pass: H = [sum_j w V_j du (x) xi] K^-1, P(H), A = P K^-1
 // ---------------------------------------------------------------------------
-// F-pass term of one bond: Hd += w V_j du (x) xi (FP32 product, FP64 sum).
-// H is accumulated in FP64: a 100+-term FP32 sum carries ~1e-6 relative error
-// into every A_j and from there into the residual.
-// (bond vector xi, displacement difference du, neighbour volume Vj).  A slot
-// without a live bond passes the particle's own record: xi = du = 0, so its
-// term is exactly zero and no select is needed.
-template <int D, bool HALO = false>
-PDX_HD void fpass_bond_v(const float *xi, const float *du, float Vj, float invh, double *Hd)
-{
-    float L2 = 0.f;
-    for (int r = 0; r < D; ++r) L2 = fmaf(xi[r], xi[r], L2);
-    const float w = HALO ? bond_weight_live(bond_sqrt(L2), invh, Vj) : bond_weight(bond_sqrt(L2), invh, Vj);
-    for (int r = 0; r < D; ++r) {
-        const double wd = (double)(w * du[r]);
-        for (int c = 0; c < D; ++c) Hd[r * D + c] = fma(wd, (double)xi[c], Hd[r * D + c]);
-    }
-}
-
-// the halo F-pass's form of the same term: S[r D + c] += u_j[r] wx_c and
-// S[D D + c] += wx_c with wx = w V_j xi rounded to FP32 once (the caller
-// subtracts u_i (x) sum wx); dead slots (xi = 0) add zeros
+// F-pass term of one bond.  The sum N = sum_j w V_j (u_j - u_i) (x) xi is
+// formed as S - u_i (x) m with S = sum_j u_j (x) wx and m = sum_j wx in FP64,
+// wx = w V_j xi rounded to FP32 once: both sums see the same rounded wx, so
+// N is the FP64 sum without rounding u_j - u_i, and it is linear in u to FP64
+// rounding (rounding u_j - u_i per bond made adaptive DR take ~10% more
+// iterations to 1e-6).  FP64 sums: a 100+-term FP32 sum carries ~1e-6
+// relative error into every A_j and from there into the residual.  A slot
+// without a live bond passes the particle's own record (xi = 0), so its terms
+// are exactly zero and no select is needed.
+// S[r D + c] += u_j[r] wx_c and S[D D + c] += wx_c (fpass_sum_to_h forms N)
 template <int D>
 PDX_HD void fpass_bond_u(const float *xi, const double *uj, float Vj, float invh, double *S)
 {
@@ -379,15 +368,23 @@ PDX_HD void fpass_bond_u(const float *xi, const double *uj, float Vj, float invh
     }
 }
 
+// gathering kernels' F-pass term: the same sums as the halo F-pass (S of
+// fpass_bond_u; fpass_sum_to_h forms the displacement-gradient sum)
 template <int D>
-PDX_HD void fpass_bond(const XRec &pi, const XRec &pj, float invh, double *Hd)
+PDX_HD void fpass_bond(const XRec &pi, const XRec &pj, float invh, double *S)
 {
-    float xi[3], du[3];
-    for (int r = 0; r < 3; ++r) {
-        xi[r] = (float)(pj.X[r] - pi.X[r]);
-        du[r] = (float)(pj.u[r] - pi.u[r]);
-    }
-    fpass_bond_v<D>(xi, du, pj.V, invh, Hd);
+    float xi[3];
+    for (int r = 0; r < 3; ++r) xi[r] = (float)(pj.X[r] - pi.X[r]);
+    fpass_bond_u<D>(xi, pj.u, pj.V, invh, S);
+}
+
+// sum_j w V_j (u_j - u_i) (x) xi = S - u_i (x) sum_j wx from the sums of
+// fpass_bond_u (FP64)
+template <int D>
+PDX_HD void fpass_sum_to_h(const double *S, const double *ui, double *Hd)
+{
+    for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) Hd[r * D + c] = S[r * D + c] - ui[r] * S[D * D + c];
 }
 
 // entry of bond slot k of a row, or "no bond" past the row's end
@@ -407,8 +404,8 @@ PDX_HD void fpass_one(const Dev &d, int i, int ubuf)
     const XRec *__restrict__ X = d.xu[ubuf];
     const XRec pi = X[i];
     const float invh = (float)d.inv_h;
-    double Hd[D * D];
-    for (int q = 0; q < D * D; ++q) Hd[q] = 0.0;
+    double Sd[D * D + D];
+    for (int q = 0; q < D * D + D; ++q) Sd[q] = 0.0;
     // bond loop in groups of G: the entries of the next group are loaded while
     // the current group's neighbour records are gathered
     const uint32_t *__restrict__ bp = d.bj + slot0(d, i);
@@ -425,9 +422,11 @@ PDX_HD void fpass_one(const Dev &d, int i, int ubuf)
             live[q] = ec[q] < (uint32_t)d.ni;
             pj[q] = X[live[q] ? (int)ec[q] : i];
         }
-        for (int q = 0; q < G; ++q) fpass_bond<D>(pi, pj[q], invh, Hd);
+        for (int q = 0; q < G; ++q) fpass_bond<D>(pi, pj[q], invh, Sd);
         for (int q = 0; q < G; ++q) ec[q] = en[q];
     }
+    double Hd[D * D];
+    fpass_sum_to_h<D>(Sd, pi.u, Hd);
     fpass_finish<D, MODEL, DIAG, COMMIT>(d, i, kr, Hd);
 }
 

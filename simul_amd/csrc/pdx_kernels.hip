@@ -217,9 +217,9 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_fpass_w(Dev d, int ubuf, int solv
     const pdx_q4 *__restrict__ Xq = reinterpret_cast<const pdx_q4 *>(X);
     const XRec pi = X[i];
     const float invh = (float)d.inv_h;
-    double Hd[D * D];
+    double Sd[D * D + D];
 #pragma unroll
-    for (int q = 0; q < D * D; ++q) Hd[q] = 0.0;
+    for (int q = 0; q < D * D + D; ++q) Sd[q] = 0.0;
     const int64_t s0 = d.sl_ptr[i >> 6];
     const int width = (int)((d.sl_ptr[(i >> 6) + 1] - s0) / PDX_SLICE);  // same in all lanes
     const uint32_t *__restrict__ bp = d.bj + s0 + lane;
@@ -257,12 +257,16 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_fpass_w(Dev d, int ubuf, int solv
 #pragma unroll
             for (int p = 0; p < 4; ++p) rq[p] = tw[xq_slot(lane, p)];
             wave_lds_sync();
-            fpass_bond<D>(pi, xrec_of(rq), invh, Hd);
+            fpass_bond<D>(pi, xrec_of(rq), invh, Sd);
         }
 #pragma unroll
         for (int q = 0; q < G; ++q) ec[q] = en[q];
     }
-    if (act) fpass_finish<D, MODEL, false, false>(d, i, kr, Hd);
+    if (act) {
+        double Hd[D * D];
+        fpass_sum_to_h<D>(Sd, pi.u, Hd);
+        fpass_finish<D, MODEL, false, false>(d, i, kr, Hd);
+    }
 }
 
 template <int D, int FORM, int UPD, int G>
@@ -689,10 +693,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_h(Dev d, int ubuf, int 
     if (part == 0 && act) {
         const double ui[3] = {uo01.a, uo01.b, uo2};
         double Hd[D * D];
-#pragma unroll
-        for (int r = 0; r < D; ++r)
-#pragma unroll
-            for (int c = 0; c < D; ++c) Hd[r * D + c] = Sd[r * D + c] - ui[r] * Sd[D * D + c];
+        fpass_sum_to_h<D>(Sd, ui, Hd);
         fpass_finish<D, MODEL, false, false>(d, i, kr, Hd);
     }
 }
