@@ -208,6 +208,13 @@ int pdx_dr_solve(pdx_ctx *ctx, const pdx_dr_params *params, pdx_dr_result *res);
  * (particles.py:37-62, 221-231); layout per field id above, caller order */
 int pdx_get(pdx_ctx *ctx, int32_t field, void *out);
 int pdx_set(pdx_ctx *ctx, int32_t field, const void *in);
+/* the displacement (PDX_FIELD_u) between the context and caller-order device memory
+ * (n*dim doubles on the context's device); no reference counterpart: for callers that
+ * keep displacement fields on the GPU (load-stepping drivers, warm starts), where the
+ * host round trip of pdx_get/pdx_set costs more than a DR increment.  Both return after
+ * the copy has completed; d_in must be complete when pdx_set_u_device is called. */
+int pdx_get_u_device(pdx_ctx *ctx, double *d_out);
+int pdx_set_u_device(pdx_ctx *ctx, const double *d_in);
 
 /* ---- measurement ------------------------------------------------------------ */
 /* times `iters` adaptive-DR iterations with HIP events around every launch on the

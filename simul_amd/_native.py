@@ -109,6 +109,8 @@ SIGNATURES = {
                     C.c_float, _vp],
     "pdx_nn_to_bf16": [_vp, C.c_int64, _vp, _vp],
     "pdx_get": [_vp, C.c_int32, _vp],
+    "pdx_get_u_device": [_vp, _vp],
+    "pdx_set_u_device": [_vp, _vp],
     "pdx_set": [_vp, C.c_int32, _vp],
 }
 

@@ -128,6 +128,25 @@ class PDBody:
     def u(self, value):
         self.ctx.set("u", value)
 
+    def get_u_device(self):
+        """The displacement as a (n, dim) float64 torch tensor on the GPU
+        (pdx_get_u_device: no host round trip)."""
+        import torch
+        t = torch.empty((self.n, self.dim), dtype=torch.float64, device="cuda")
+        torch.cuda.current_stream().synchronize()
+        self.ctx.call("pdx_get_u_device", C.c_void_p(t.data_ptr()))
+        return t
+
+    def set_u_device(self, t):
+        """Sets the displacement from a (n, dim) float64 GPU tensor
+        (pdx_set_u_device)."""
+        import torch
+        if not (t.is_cuda and t.dtype == torch.float64 and tuple(t.shape) == (self.n, self.dim)):
+            raise ValueError(f"expected a ({self.n}, {self.dim}) float64 CUDA tensor")
+        t = t.contiguous()
+        torch.cuda.current_stream().synchronize()
+        self.ctx.call("pdx_set_u_device", C.c_void_p(t.data_ptr()))
+
     @property
     def x(self):
         return self.ctx.get("x")
@@ -161,9 +180,11 @@ class PDBody:
     def compute_F_stress(self):
         self.ctx.call("pdx_compute_F_stress")
 
-    def compute_forces(self):
+    def compute_forces(self, download=True):
+        """Force density of the current state (host copy unless download is
+        False; residual_norm() reads the per-case norms either way)."""
         self.ctx.call("pdx_compute_forces")
-        return self.ctx.get("f")
+        return self.ctx.get("f") if download else None
 
     def residual_norm(self):
         out = np.zeros(self.ncase)
@@ -244,7 +265,7 @@ class PDBody:
             if mass == "gershgorin":
                 self.adr_mass(1.0)
             elif mass == "spectral":
-                u_start = self.u
+                u_start = self.get_u_device()
                 self.tune_mass()
 
                 def watch(done, rs):  # a diverging case ends the attempt at the slice
@@ -264,7 +285,7 @@ class PDBody:
                         return out
                 except _Diverged:
                     pass
-                self.u = u_start
+                self.set_u_device(u_start)
                 self.adr_mass(1.0)
                 return self.solve(method, tol, max_iterations, ref, dt, viscous_damping, cmax, use_graph, chunk,
                                   False, refs, progress, slice_iterations)
@@ -277,7 +298,7 @@ class PDBody:
             if ref > 0.0:
                 refs = np.full(self.ncase, float(ref))
             else:
-                self.compute_forces()
+                self.compute_forces(download=False)
                 refs = np.atleast_1d(self.residual_norm()).astype(np.float64)
                 refs = np.where(refs < 1e-20, 1.0, refs)
         refs = np.ascontiguousarray(np.broadcast_to(np.asarray(refs, np.float64), (self.ncase,)))

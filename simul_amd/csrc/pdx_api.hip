@@ -70,7 +70,7 @@ static void free_all(pdx_ctx *c)
     dfree(d.Fd); dfree(d.Pd); dfree(d.damage); dfree(d.init_nb);
     dfree(d.sl_ptr); dfree(d.bj); dfree(d.case_of_blk); dfree(d.case_blk0);
     dfree(d.hl_ptr); dfree(d.hl); dfree(d.hs); dfree(d.bl); dfree(d.bk); dfree(d.anchor);
-    dfree(d.sc); dfree(d.part); dfree(d.grid); dfree(d.case_of_p);
+    dfree(d.sc); dfree(d.part); dfree(d.grid); dfree(d.case_of_p); dfree(d.perm);
     dfree(d.cell_start); dfree(d.cell_idx); dfree(d.counts);
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     c->gexec = nullptr;
@@ -175,6 +175,7 @@ static int create_common(int32_t ncase, const int64_t *off, int32_t dim, pdx_ctx
         (r = dalloc(c->stream, &d.lam, ni)) || (r = dalloc(c->stream, &d.V64, ni)) || (r = dalloc(c->stream, &d.K64, ni * 9)) ||
         (r = dalloc(c->stream, &d.Kinv64, ni * 9)) || (r = dalloc(c->stream, &d.Fd, ni * 9)) || (r = dalloc(c->stream, &d.Pd, ni * 9)) ||
         (r = dalloc(c->stream, &d.damage, ni)) || (r = dalloc(c->stream, &d.init_nb, ni)) || (r = dalloc(c->stream, &d.case_of_p, ni)) ||
+        (r = dalloc(c->stream, &d.perm, ni)) ||
         (r = dalloc(c->stream, &d.counts, ni)) || (r = dalloc(c->stream, &d.case_of_blk, (size_t)c->nblk)) ||
         (r = dalloc(c->stream, &d.case_blk0, (size_t)ncase + 1)) || (r = dalloc(c->stream, &d.sc, (size_t)ncase)) ||
         (r = dalloc(c->stream, &d.part, (size_t)c->nblk * 4)) || (r = dalloc(c->stream, &d.grid, (size_t)ncase))) {
@@ -451,6 +452,7 @@ extern "C" int pdx_set_particles(pdx_ctx *c, const double *X, const double *vol,
     PDX_HIP(hipMemsetAsync(c->d.xu[1], 0, sizeof(XRec) * c->ni, c->stream));
     PDX_TRY(push_prec(c));
     PDX_TRY(upload(c, c->d.case_of_p, cop));
+    PDX_TRY(upload(c, c->d.perm, c->perm));
     PDX_TRY(put_scalar(c, c->d.mass, m.mass.data()));
     PDX_HIP(hipMemsetAsync(c->d.v, 0, sizeof(pdx_d4) * c->ni, c->stream));
     PDX_HIP(hipMemsetAsync(c->d.fp, 0, sizeof(pdx_d4) * c->ni, c->stream));
@@ -1002,6 +1004,27 @@ extern "C" int pdx_neighbor_mean(pdx_ctx *c, int32_t width, const double *in, do
         const int rr = c->perm[i];
         if (rr >= 0) memcpy(out + (size_t)rr * width, &h[(size_t)i * width], sizeof(double) * width);
     }
+    return PDX_OK;
+}
+
+extern "C" int pdx_get_u_device(pdx_ctx *c, double *d_out)
+{
+    CHECK_CTX(c);
+    if (!d_out) return fail(PDX_EINVAL, "d_out is NULL");
+    if (!c->have_particles) return fail(PDX_ESTATE, "set particles first");
+    PDX_TRY(pdx_launch_u_device(c, c->ucur, d_out, nullptr));
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    return PDX_OK;
+}
+
+extern "C" int pdx_set_u_device(pdx_ctx *c, const double *d_in)
+{
+    CHECK_CTX(c);
+    if (!d_in) return fail(PDX_EINVAL, "d_in is NULL");
+    if (!c->have_particles) return fail(PDX_ESTATE, "set particles first");
+    c->have_A = false;
+    PDX_TRY(pdx_launch_u_device(c, c->ucur, nullptr, d_in));
+    PDX_HIP(hipStreamSynchronize(c->stream));
     return PDX_OK;
 }
 

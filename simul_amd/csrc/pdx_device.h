@@ -163,6 +163,7 @@ struct Dev {
     // neighbour search
     GridP *grid;
     int32_t *case_of_p;  // case of each internal particle
+    int32_t *perm;       // caller index of each internal particle (-1: padding)
     int32_t *cell_start; // per cell start into cell_idx (+1 sentinel per case grid)
     int32_t *cell_idx;
     int32_t *counts;

@@ -1201,6 +1201,29 @@ int pdx_launch_nbr_mean(pdx_ctx *c, const double *d_in, double *d_out, int width
     return PDX_OK;
 }
 
+// u of buffer ubuf between the internal records and a caller-order device
+// array (n x dim FP64): out != null reads into it, else in is written
+__global__ __launch_bounds__(PDX_BLOCK) void k_u_device(Dev d, int ubuf, double *out, const double *in)
+{
+    const int i = blockIdx.x * PDX_BLOCK + threadIdx.x;
+    if (i >= d.ni) return;
+    const int r = d.perm[i];
+    if (r < 0) return;
+    double *u = d.xu[ubuf][i].u;
+    if (out) {
+        for (int q = 0; q < d.dim; ++q) out[(int64_t)r * d.dim + q] = u[q];
+    } else {
+        for (int q = 0; q < d.dim; ++q) u[q] = in[(int64_t)r * d.dim + q];
+    }
+}
+
+int pdx_launch_u_device(pdx_ctx *c, int ubuf, double *out, const double *in)
+{
+    hipLaunchKernelGGL(k_u_device, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, ubuf, out, in);
+    PDX_LAUNCH_CHECK();
+    return PDX_OK;
+}
+
 // [hop 0 | hop 1 | ...] of internal-order FP64 blocks (ni x width each) into
 // the caller-order FP32 rows of out (stride ldo)
 __global__ __launch_bounds__(PDX_BLOCK) void k_feat_pack(Dev d, const double *hv, const int32_t *perm, float *out,

@@ -61,7 +61,11 @@ class IncrementalLoading:
         if predictor not in (None, "secant"):
             raise ValueError("predictor must be None or 'secant'")
         self.predictor = predictor
-        self._hist: List[np.ndarray] = []
+        # displacement states stay on the GPU (pdx_get_u_device / pdx_set_u_device)
+        import torch
+        self._idx = torch.from_numpy(self.indices).cuda()
+        self._u_final = torch.from_numpy(self.u_final).cuda()
+        self._hist: list = []
         # broken bonds so far: counted once here, then from each break_bonds
         # call (pdx_stats reads the whole bond list back)
         st = body.stats()
@@ -79,15 +83,15 @@ class IncrementalLoading:
         """Increment k (1-based)."""
         b = self.body
         t0 = time.perf_counter()
-        u = b.u
-        u[self.indices] = (k / self.n_increments) * self.u_final
-        b.u = u
+        u = b.get_u_device()
+        u[self._idx] = (k / self.n_increments) * self._u_final
+        b.set_u_device(u)
         if self.predictor == "secant" and len(self._hist) == 2:
-            b.compute_forces()
+            b.compute_forces(download=False)
             ref0 = float(np.atleast_1d(b.residual_norm())[0])
             up = 2.0 * self._hist[1] - self._hist[0]
-            up[self.indices] = u[self.indices]
-            b.u = up
+            up[self._idx] = u[self._idx]
+            b.set_u_device(up)
             r = self._solve(ref0 if ref0 > 0.0 else 0.0)
         else:
             r = self._solve()
@@ -112,7 +116,7 @@ class IncrementalLoading:
         if b.model == "j2_plasticity":
             b.commit_state()
         if self.predictor is not None:
-            self._hist = (self._hist + [b.u])[-2:]
+            self._hist = (self._hist + [b.get_u_device()])[-2:]
         self._broken += broken_now
         res = IncrementResult(k, iters, bool(r.converged), float(r.relative_residual), broken_now,
                               self._broken, rounds, time.perf_counter() - t0)
