@@ -1007,10 +1007,24 @@ extern "C" int pdx_neighbor_mean(pdx_ctx *c, int32_t width, const double *in, do
     return PDX_OK;
 }
 
+// true when p is device memory (or managed memory) that the context's
+// device can address; a host pointer passed as a device array would fault
+// the kernel that reads it
+static bool is_device_ptr(const pdx_ctx *c, const void *p)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // an unregistered host pointer reports an error
+        return false;
+    }
+    return (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged) && a.device == c->device;
+}
+
 extern "C" int pdx_get_u_device(pdx_ctx *c, double *d_out)
 {
     CHECK_CTX(c);
     if (!d_out) return fail(PDX_EINVAL, "d_out is NULL");
+    if (!is_device_ptr(c, d_out)) return fail(PDX_EINVAL, "d_out is not device memory of the context's device");
     if (!c->have_particles) return fail(PDX_ESTATE, "set particles first");
     PDX_TRY(pdx_launch_u_device(c, c->ucur, d_out, nullptr));
     PDX_HIP(hipStreamSynchronize(c->stream));
@@ -1021,6 +1035,7 @@ extern "C" int pdx_set_u_device(pdx_ctx *c, const double *d_in)
 {
     CHECK_CTX(c);
     if (!d_in) return fail(PDX_EINVAL, "d_in is NULL");
+    if (!is_device_ptr(c, d_in)) return fail(PDX_EINVAL, "d_in is not device memory of the context's device");
     if (!c->have_particles) return fail(PDX_ESTATE, "set particles first");
     c->have_A = false;
     PDX_TRY(pdx_launch_u_device(c, c->ucur, nullptr, d_in));
@@ -1033,6 +1048,7 @@ extern "C" int pdx_neighbor_features(pdx_ctx *c, int32_t width, int32_t hops, co
 {
     CHECK_CTX(c);
     if (!in || !d_out) return fail(PDX_EINVAL, "in/d_out NULL");
+    if (!is_device_ptr(c, d_out)) return fail(PDX_EINVAL, "d_out is not device memory of the context's device");
     if (width < 1 || width > 64) return fail(PDX_EINVAL, "width must be 1..64");
     if (hops < 0 || hops > 4) return fail(PDX_EINVAL, "hops must be 0..4");
     if (ldo < width * (hops + 1)) return fail(PDX_EINVAL, "ldo must be >= width * (hops + 1)");

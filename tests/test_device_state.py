@@ -34,6 +34,13 @@ def test_device_displacement_matches_host_path(gpu):
         body.set_u_device(v.float())
     with pytest.raises(ValueError):
         body.set_u_device(v[:-1])
+    # host memory given to the device entry points is refused before any launch
+    import ctypes as C
+    h = np.zeros((packed.n, 3))
+    for fn in ("pdx_set_u_device", "pdx_get_u_device"):
+        with pytest.raises(Exception, match="not device memory"):
+            body.ctx.call(fn, C.c_void_p(h.ctypes.data))
+    assert np.array_equal(body.u, v.cpu().numpy())
 
 
 def test_device_state_solves_like_host_state(gpu):
