@@ -534,6 +534,28 @@ __device__ __forceinline__ void halo_stage(const Dev &d, int ubuf, int lb, pdx_q
             }
             continue;
         }
+        if constexpr (!FORCE) {
+            // F-pass arrays: lane 0 of the quad writes xv (its x, y and lane
+            // 1's z, V), lane 2 u01 (its u_x, u_y), lane 3 u2 (its u_z)
+#pragma unroll
+            for (int v = 0; v < U; ++v) {
+                const PdxD2 x = __builtin_bit_cast(PdxD2, r[v]);
+                const float t0 = (float)(x.a - (q == 0 ? an.x : an.z));  // lane 0: x - a_x; lane 1: z - a_z
+                const float t1 = (float)(x.b - an.y);                     // lane 0: y - a_y
+                const float mz = dpp_quad<1, 1, 1, 1>(t0);
+                const float mv = dpp_quad<1, 1, 1, 1>(r[v].z);
+                if (ok[v] && q != 1) {
+                    const int sv = sl[v];
+                    if (q == 3) {
+                        *reinterpret_cast<pdx_q2 *>(hslot<FORCE, FORM>(lds, sv, O::u2, cap)) = r[v].xy;
+                    } else {
+                        const pdx_q4 out = q == 0 ? pdx_q4{t0, t1, mz, mv} : r[v];
+                        *reinterpret_cast<pdx_q4 *>(hslot<FORCE, FORM>(lds, sv, q == 0 ? O::xv : O::u01, cap)) = out;
+                    }
+                }
+            }
+            continue;
+        }
 #pragma unroll
         for (int v = 0; v < U; ++v) {
             if (!ok[v]) continue;
