@@ -480,7 +480,8 @@ __device__ __forceinline__ void halo_stage(const Dev &d, int ubuf, int lb, pdx_q
     constexpr int NW = NT / 64;
     // 16-entry groups per wave in flight: the force pass's 112 bytes per entry
     // are fetched best two groups at a time, the F-pass's 64 four at a time
-    constexpr int U = FORCE ? 2 : 4;
+    // with 1024 threads per block and eight with fewer
+    constexpr int U = FORCE ? 2 : (NT >= 1024 ? 4 : 8);
     const int cap = d.halo_cap;
     const int h0 = d.hl_ptr[lb], nh = d.hl_ptr[lb + 1] - h0;
     const pdx_d4 an = d.anchor[lb];
