@@ -181,6 +181,10 @@ int pdx_neighbor_mean(pdx_ctx *ctx, int32_t width, const double *in, double *out
  * ldo floats, caller order; columns from width * (hops + 1) on are not written).  width
  * 1..64, hops 0..4, ldo >= width * (hops + 1).  One host-to-device copy in all. */
 int pdx_neighbor_features(pdx_ctx *ctx, int32_t width, int32_t hops, const double *in, float *d_out, int32_t ldo);
+/* the same with the input rows already in device memory (d_in, n x width, caller order;
+ * no reference counterpart): no host copy at all */
+int pdx_neighbor_features_device(pdx_ctx *ctx, int32_t width, int32_t hops, const double *d_in, float *d_out,
+                                 int32_t ldo);
 
 /* ---- dynamic relaxation ---------------------------------------------------- */
 /* NOSBSolver._estimate_stable_dt, nosb_solver.py:94-154 */

@@ -1043,37 +1043,55 @@ extern "C" int pdx_set_u_device(pdx_ctx *c, const double *d_in)
     return PDX_OK;
 }
 
-extern "C" int pdx_neighbor_features(pdx_ctx *c, int32_t width, int32_t hops, const double *in, float *d_out,
-                                     int32_t ldo)
+// features of the learned warm start: caller-order rows of width values on
+// the device (d_in), then `hops` weighted neighbour means over the intact
+// bonds, packed into the caller-order FP32 rows of d_out
+static int neighbor_features(pdx_ctx *c, int32_t width, int32_t hops, const double *d_in, float *d_out, int32_t ldo)
 {
-    CHECK_CTX(c);
+    const size_t blk = (size_t)c->ni * width;
+    double *dv = nullptr;
+    PDX_HIP(hipMallocAsync((void **)&dv, sizeof(double) * blk * (hops + 1), c->stream));
+    int r = pdx_launch_rows_internal(c, d_in, dv, width);
+    for (int k = 1; k <= hops && r == PDX_OK; ++k) r = pdx_launch_nbr_mean(c, dv + (k - 1) * blk, dv + k * blk, width);
+    if (r == PDX_OK) r = pdx_launch_feat_pack(c, dv, c->d.perm, d_out, width, hops, ldo);
+    hipFreeAsync(dv, c->stream);
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    return r;
+}
+
+static int check_features(pdx_ctx *c, int32_t width, int32_t hops, const void *in, float *d_out, int32_t ldo)
+{
     if (!in || !d_out) return fail(PDX_EINVAL, "in/d_out NULL");
     if (!is_device_ptr(c, d_out)) return fail(PDX_EINVAL, "d_out is not device memory of the context's device");
     if (width < 1 || width > 64) return fail(PDX_EINVAL, "width must be 1..64");
     if (hops < 0 || hops > 4) return fail(PDX_EINVAL, "hops must be 0..4");
     if (ldo < width * (hops + 1)) return fail(PDX_EINVAL, "ldo must be >= width * (hops + 1)");
     if (hops > 0 && !c->have_bonds) return fail(PDX_ESTATE, "bonds not built");
-    const size_t blk = (size_t)c->ni * width;
-    std::vector<double> h(blk, 0.0);
-    for (int i = 0; i < c->ni; ++i) {
-        const int r = c->perm[i];
-        if (r >= 0) memcpy(&h[(size_t)i * width], in + (size_t)r * width, sizeof(double) * width);
-    }
-    double *dv = nullptr;
-    int32_t *dp = nullptr;
-    PDX_HIP(hipMallocAsync((void **)&dv, sizeof(double) * blk * (hops + 1), c->stream));
-    hipError_t e = hipMallocAsync((void **)&dp, sizeof(int32_t) * c->ni, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(dv, h.data(), sizeof(double) * blk, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(dp, c->perm.data(), sizeof(int32_t) * c->ni, hipMemcpyHostToDevice, c->stream);
-    int r = e == hipSuccess ? PDX_OK : PDX_EDEVICE;
-    for (int k = 1; k <= hops && r == PDX_OK; ++k) r = pdx_launch_nbr_mean(c, dv + (k - 1) * blk, dv + k * blk, width);
-    if (r == PDX_OK) r = pdx_launch_feat_pack(c, dv, dp, d_out, width, hops, ldo);
-    hipFreeAsync(dv, c->stream);
-    if (dp) hipFreeAsync(dp, c->stream);
+    return PDX_OK;
+}
+
+extern "C" int pdx_neighbor_features(pdx_ctx *c, int32_t width, int32_t hops, const double *in, float *d_out,
+                                     int32_t ldo)
+{
+    CHECK_CTX(c);
+    PDX_TRY(check_features(c, width, hops, in, d_out, ldo));
+    double *din = nullptr;
+    PDX_HIP(hipMallocAsync((void **)&din, sizeof(double) * (size_t)c->n * width, c->stream));
+    hipError_t e = hipMemcpyAsync(din, in, sizeof(double) * (size_t)c->n * width, hipMemcpyHostToDevice, c->stream);
+    int r = e == hipSuccess ? neighbor_features(c, width, hops, din, d_out, ldo) : PDX_EDEVICE;
+    hipFreeAsync(din, c->stream);
     PDX_HIP(e);
     PDX_HIP(hipStreamSynchronize(c->stream));
     return r;
+}
+
+extern "C" int pdx_neighbor_features_device(pdx_ctx *c, int32_t width, int32_t hops, const double *d_in, float *d_out,
+                                            int32_t ldo)
+{
+    CHECK_CTX(c);
+    PDX_TRY(check_features(c, width, hops, d_in, d_out, ldo));
+    if (!is_device_ptr(c, d_in)) return fail(PDX_EINVAL, "d_in is not device memory of the context's device");
+    return neighbor_features(c, width, hops, d_in, d_out, ldo);
 }
 
 extern "C" int pdx_local_damage(pdx_ctx *c)

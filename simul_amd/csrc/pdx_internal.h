@@ -85,6 +85,7 @@ int pdx_launch_nbr_mean(pdx_ctx *c, const double *d_in, double *d_out, int width
 // K and the block partials, 3 per-case totals into out, 4 scale lam by scale[case]
 int pdx_launch_pow(pdx_ctx *c, int what, int ubuf, pdx_d4 *v, const pdx_d4 *f0, const double *scale, double *out);
 int pdx_launch_u_device(pdx_ctx *c, int ubuf, double *out, const double *in);
+int pdx_launch_rows_internal(pdx_ctx *c, const double *d_in, double *d_out, int width);
 int pdx_launch_feat_pack(pdx_ctx *c, const double *d_hv, const int32_t *d_perm, float *d_out, int width, int hops,
                          int ldo);
 int pdx_launch_adr_mass(pdx_ctx *c, double beta);

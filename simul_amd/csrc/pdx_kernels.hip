@@ -1224,6 +1224,23 @@ int pdx_launch_nbr_mean(pdx_ctx *c, const double *d_in, double *d_out, int width
     return PDX_OK;
 }
 
+// caller-order rows (n x width FP64) into internal order (padding rows 0)
+__global__ __launch_bounds__(PDX_BLOCK) void k_rows_internal(Dev d, const double *in, double *out, int width)
+{
+    const int i = blockIdx.x * PDX_BLOCK + threadIdx.x;
+    if (i >= d.ni) return;
+    const int r = d.perm[i];
+    for (int q = 0; q < width; ++q)
+        out[(int64_t)i * width + q] = r >= 0 ? in[(int64_t)r * width + q] : 0.0;
+}
+
+int pdx_launch_rows_internal(pdx_ctx *c, const double *d_in, double *d_out, int width)
+{
+    hipLaunchKernelGGL(k_rows_internal, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_in, d_out, width);
+    PDX_LAUNCH_CHECK();
+    return PDX_OK;
+}
+
 // u of buffer ubuf between the internal records and a caller-order device
 // array (n x dim FP64): out != null reads into it, else in is written
 __global__ __launch_bounds__(PDX_BLOCK) void k_u_device(Dev d, int ubuf, double *out, const double *in)

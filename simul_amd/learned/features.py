@@ -78,18 +78,68 @@ def raw_features(spec, case_offsets=None) -> np.ndarray:
     return np.ascontiguousarray(F.T)
 
 
+def raw_features_tensor(spec, case_offsets=None):
+    """raw_features computed on the GPU: (n, 20) float64 CUDA tensor (the same
+    values to FP64 rounding)."""
+    import torch
+    dev = "cuda"
+    n = spec.n
+    X = torch.from_numpy(np.ascontiguousarray(spec.X, np.float64)).to(dev)
+    E = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(np.asarray(spec.E, np.float64), (n,)))).to(dev)
+    fext = torch.from_numpy(np.ascontiguousarray(spec.fext, np.float64)).to(dev)
+    fixed = torch.from_numpy(np.ascontiguousarray(spec.fixed, np.uint8)).to(dev).to(torch.int32)
+    cort = (torch.from_numpy(np.ascontiguousarray(spec.material_id)).to(dev) == 0).to(torch.float64) \
+        if spec.material_id is not None else torch.zeros(n, dtype=torch.float64, device=dev)
+    F = torch.zeros((n, RAW), dtype=torch.float64, device=dev)
+    logE = torch.log10(E / 1000.0)
+    loaded_all = (fext.abs().sum(1) > 0).to(torch.float64)
+    for a0, b0 in _per_case(spec, case_offsets):
+        Xc = X[a0:b0]
+        lo, hi = Xc.min(0).values, Xc.max(0).values
+        c = 0.5 * (lo + hi)
+        ax = torch.clamp(0.5 * (hi[0] - lo[0]), min=1e-9)
+        by = torch.clamp(0.5 * (hi[1] - lo[1]), min=1e-9)
+        H = torch.clamp(hi[2] - lo[2], min=1e-9)
+        F[a0:b0, 0] = (Xc[:, 0] - c[0]) / ax
+        F[a0:b0, 1] = (Xc[:, 1] - c[1]) / by
+        F[a0:b0, 2] = (Xc[:, 2] - lo[2]) / H
+        tot = fext[a0:b0].sum(0)
+        loaded = loaded_all[a0:b0]
+        nrm = torch.linalg.norm(tot)
+        F[a0:b0, 8] = loaded
+        if float(nrm) > 0:
+            F[a0:b0, 9:12] = (tot / nrm)[None, :]
+            p = nrm * spec.dx ** 3 / torch.clamp(loaded.sum() * spec.dx ** 2, min=1e-30)
+            F[a0:b0, 12] = torch.log10(torch.clamp(p, min=1e-12))
+        F[a0:b0, 13] = H / 25.0
+        F[a0:b0, 14] = ax / 20.0
+        F[a0:b0, 15] = by / 20.0
+        F[a0:b0, 16] = logE[a0:b0].mean()
+        F[a0:b0, 17] = cort[a0:b0].mean()
+        Ec = E[a0:b0]
+        ce = (Ec @ Xc[:, :2]) / torch.clamp(Ec.sum(), min=1e-300)
+        F[a0:b0, 18] = (ce[0] - c[0]) / ax
+        F[a0:b0, 19] = (ce[1] - c[1]) / by
+    F[:, 3] = logE
+    F[:, 4] = cort
+    for q in range(3):
+        F[:, 5 + q] = ((fixed >> q) & 1).to(torch.float64)
+    return F
+
+
 def feature_tensor(body, spec, case_offsets=None):
     """(n, 64) float32 device tensor: raw features and two neighbour-mean hops
     through the bonds of ``body`` (a libpdx context of the same particles),
-    built on the device by pdx_neighbor_features from one host-to-device copy
-    of the raw features."""
+    built on the device (raw_features_tensor, pdx_neighbor_features_device)
+    with no host copy."""
     import ctypes as C
 
     import torch
-    from .. import _native as N
-    raw = raw_features(spec, case_offsets)
+    raw = raw_features_tensor(spec, case_offsets)
     out = torch.zeros((spec.n, FEAT), dtype=torch.float32, device="cuda")
-    body.ctx.call("pdx_neighbor_features", RAW, 2, N.ptr(raw, N._dp), C.c_void_p(out.data_ptr()), FEAT)
+    torch.cuda.current_stream().synchronize()
+    body.ctx.call("pdx_neighbor_features_device", RAW, 2, C.c_void_p(raw.data_ptr()), C.c_void_p(out.data_ptr()),
+                  FEAT)
     out[:, 3] = 0.0  # the local modulus only through its neighbour means
     return out
 
@@ -97,6 +147,23 @@ def feature_tensor(body, spec, case_offsets=None):
 def feature_matrix(body, spec, case_offsets=None) -> np.ndarray:
     """feature_tensor on the host (the training data)."""
     return feature_tensor(body, spec, case_offsets).cpu().numpy()
+
+
+def target_scale_tensor(spec, case_offsets=None):
+    """target_scale computed on the GPU: (n,) float64 CUDA tensor."""
+    import torch
+    X = torch.from_numpy(np.ascontiguousarray(spec.X, np.float64)).cuda()
+    fext = torch.from_numpy(np.ascontiguousarray(spec.fext, np.float64)).cuda()
+    s = torch.zeros(spec.n, dtype=torch.float64, device="cuda")
+    for a, b in _per_case(spec, case_offsets):
+        z = X[a:b, 2]
+        H = z.max() - z.min()
+        fe = fext[a:b]
+        loaded = (fe.abs().sum(1) > 0).to(torch.float64).sum()
+        nrm = torch.linalg.norm(fe.sum(0))
+        p = nrm * spec.dx ** 3 / torch.clamp(loaded * spec.dx ** 2, min=1e-30)
+        s[a:b] = torch.clamp(p, min=1e-12) * H / 1000.0
+    return s
 
 
 def target_scale(spec, case_offsets=None) -> np.ndarray:

@@ -158,8 +158,13 @@ class WarmStartNet:
 
     def predict(self, X_f32, chunk=1 << 20):
         """(M, 64) float32 device features -> (M, 3) float32 host array of u / s."""
-        out = []
-        for a in range(0, X_f32.shape[0], chunk):
-            y = self.forward(self.to_bf16(X_f32[a:a + chunk].contiguous()))
-            out.append(y[:, :NOUT].cpu().numpy())
-        return np.concatenate(out) if out else np.zeros((0, NOUT), np.float32)
+        return self.predict_device(X_f32, chunk).cpu().numpy()
+
+    def predict_device(self, X_f32, chunk=1 << 20):
+        """(M, 64) float32 device features -> (M, 3) float32 device tensor of u / s."""
+        import torch
+        out = [self.forward(self.to_bf16(X_f32[a:a + chunk].contiguous()))[:, :NOUT]
+               for a in range(0, X_f32.shape[0], chunk)]
+        if not out:
+            return torch.zeros((0, NOUT), dtype=torch.float32, device="cuda")
+        return out[0].contiguous() if len(out) == 1 else torch.cat(out)

@@ -8,17 +8,25 @@ import time
 
 import numpy as np
 
-from .features import feature_tensor, target_scale
+from .features import feature_tensor, target_scale_tensor
+
+
+def predict_u_device(net, body, spec, case_offsets=None):
+    """The predicted displacement as a (n, 3) float64 CUDA tensor (features,
+    inference and scaling on the GPU); prescribed DOFs keep spec.u0."""
+    import torch
+    y = net.predict_device(feature_tensor(body, spec, case_offsets))
+    u = y.to(torch.float64) * target_scale_tensor(spec, case_offsets)[:, None]
+    fixed = torch.from_numpy(np.ascontiguousarray(spec.fixed, np.uint8)).cuda().to(torch.int32)
+    u0 = torch.from_numpy(np.ascontiguousarray(spec.u0, np.float64)).cuda()
+    for q in range(3):
+        m = ((fixed >> q) & 1).bool()
+        u[:, q] = torch.where(m, u0[:, q], u[:, q])
+    return u
 
 
 def predict_u(net, body, spec, case_offsets=None) -> np.ndarray:
-    y = net.predict(feature_tensor(body, spec, case_offsets))
-    u = y.astype(np.float64) * target_scale(spec, case_offsets)[:, None]
-    fixed = spec.fixed
-    for q in range(3):
-        m = ((fixed >> q) & 1).astype(bool)
-        u[m, q] = spec.u0[m, q]
-    return u
+    return predict_u_device(net, body, spec, case_offsets).cpu().numpy()
 
 
 def warm_solve(body, spec, net, tol=1e-6, ref=None, case_offsets=None, max_iterations=200000, **kw):
@@ -29,8 +37,7 @@ def warm_solve(body, spec, net, tol=1e-6, ref=None, case_offsets=None, max_itera
         ref = body.cold_residual()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    u = predict_u(net, body, spec, case_offsets)
-    body.u = u
+    body.set_u_device(predict_u_device(net, body, spec, case_offsets))
     torch.cuda.synchronize()
     t_inf = time.perf_counter() - t0
     refs = np.atleast_1d(np.asarray(ref, np.float64))

@@ -175,17 +175,30 @@ def test_warm_start_reaches_the_cold_solution(gpu):
 
 
 def test_device_features_match_host_hops(gpu):
-    """pdx_neighbor_features (device hops, FP32 pack into the network input)
-    against the same two hops through pdx_neighbor_mean on host arrays, for a
-    two-case batch with padding between the cases: bit for bit."""
+    """The raw features computed on the GPU (raw_features_tensor) against the
+    host ones to FP64 rounding, and pdx_neighbor_features_device (device hops,
+    FP32 pack into the network input) against the same two hops through
+    pdx_neighbor_mean on host arrays, for a two-case batch with padding
+    between the cases: bit for bit; the host-input pdx_neighbor_features gives
+    the same bits."""
+    import ctypes as C
+
+    import torch
+    from simul_amd import _native as N
     from simul_amd import workloads as W
     from simul_amd.body import from_specimen
-    from simul_amd.learned.features import FEAT, RAW, feature_tensor, raw_features
+    from simul_amd.learned.features import FEAT, RAW, feature_tensor, raw_features, raw_features_tensor
     specs = [W.random_vertebra(77, dx=1.6), W.random_vertebra(78, dx=1.6)]
     packed, off = W.pack(specs)
     body = from_specimen(packed, case_offsets=off)
+    raw = raw_features_tensor(packed, off).cpu().numpy()
+    assert np.allclose(raw, raw_features(packed, off), rtol=1e-12, atol=1e-12)
     F = feature_tensor(body, packed, off).cpu().numpy()
-    raw = raw_features(packed, off)
+    Fh = torch.zeros((packed.n, FEAT), dtype=torch.float32, device="cuda")
+    body.ctx.call("pdx_neighbor_features", RAW, 2, N.ptr(np.ascontiguousarray(raw), N._dp),
+                  C.c_void_p(Fh.data_ptr()), FEAT)
+    Fh[:, 3] = 0.0
+    assert np.array_equal(F, Fh.cpu().numpy())
     m1 = body.neighbor_mean(raw)
     m2 = body.neighbor_mean(m1)
     ref = np.zeros((packed.n, FEAT), np.float32)
