@@ -185,6 +185,13 @@ int pdx_neighbor_features(pdx_ctx *ctx, int32_t width, int32_t hops, const doubl
  * no reference counterpart): no host copy at all */
 int pdx_neighbor_features_device(pdx_ctx *ctx, int32_t width, int32_t hops, const double *d_in, float *d_out,
                                  int32_t ldo);
+/* raw inputs of the learned warm start (no reference counterpart): 20 FP64 columns per
+ * particle (n x 20, caller order) into the DEVICE array d_out, from the host arrays of the
+ * specimen: X (n x 3), E (n), fext (n x 3 force density), fixed (n DOF masks), material_id
+ * (n, 0 = cortical; NULL: none cortical) and the particle spacing dx.  Per-case box, load
+ * resultant and modulus statistics over the context's cases.  3-D contexts only. */
+int pdx_warm_raw_features(pdx_ctx *ctx, const double *X, const double *E, const double *fext, const uint8_t *fixed,
+                          const int32_t *material_id, double dx, double *d_out);
 
 /* ---- dynamic relaxation ---------------------------------------------------- */
 /* NOSBSolver._estimate_stable_dt, nosb_solver.py:94-154 */

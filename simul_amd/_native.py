@@ -102,6 +102,7 @@ SIGNATURES = {
     "pdx_adr_mass_auto": [_vp, C.c_int32, C.c_double, _dp],
     "pdx_neighbor_features": [_vp, C.c_int32, C.c_int32, _dp, _vp, C.c_int32],
     "pdx_neighbor_features_device": [_vp, C.c_int32, C.c_int32, _vp, _vp, C.c_int32],
+    "pdx_warm_raw_features": [_vp, _dp, _dp, _dp, _u8p, _i32p, C.c_double, _vp],
     "pdx_nn_linear_fwd": [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, C.c_int, _vp],
     "pdx_nn_linear_dgrad": [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp],
     "pdx_nn_linear_wgrad": [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp],

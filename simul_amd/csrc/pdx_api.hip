@@ -1094,6 +1094,68 @@ extern "C" int pdx_neighbor_features_device(pdx_ctx *c, int32_t width, int32_t h
     return neighbor_features(c, width, hops, d_in, d_out, ldo);
 }
 
+extern "C" int pdx_warm_raw_features(pdx_ctx *c, const double *X, const double *E, const double *fext,
+                                     const uint8_t *fixed, const int32_t *material_id, double dx, double *d_out)
+{
+    CHECK_CTX(c);
+    if (!c->have_particles) return fail(PDX_ESTATE, "particles not set");
+    if (c->dim != 3) return fail(PDX_EINVAL, "raw features are defined for 3-D specimens");
+    if (!X || !E || !fext || !fixed || !d_out) return fail(PDX_EINVAL, "X/E/fext/fixed/d_out NULL");
+    if (!(dx > 0.0)) return fail(PDX_EINVAL, "dx must be > 0");
+    if (!is_device_ptr(c, d_out)) return fail(PDX_EINVAL, "d_out is not device memory of the context's device");
+    // row chunks that never span two cases
+    const int64_t CH = 4096;
+    std::vector<int64_t> chunk;
+    std::vector<int32_t> first(c->ncase + 1, 0);
+    for (int k = 0; k < c->ncase; ++k) {
+        first[k] = (int32_t)(chunk.size() / 2);
+        for (int64_t r = c->case_off[k]; r < c->case_off[k + 1]; r += CH) {
+            chunk.push_back(r);
+            chunk.push_back(std::min(r + CH, c->case_off[k + 1]));
+        }
+    }
+    first[c->ncase] = (int32_t)(chunk.size() / 2);
+    const int nch = (int)(chunk.size() / 2);
+    const size_t n = (size_t)c->n;
+    // one device block: X, E, fext (FP64), case offsets, chunks, partials,
+    // case constants, then material ids, chunk starts and DOF masks
+    const size_t off_X = 0, off_E = off_X + 24 * n, off_f = off_E + 8 * n, off_co = off_f + 24 * n;
+    const size_t off_ch = off_co + 8 * (size_t)(c->ncase + 1), off_p = off_ch + 16 * (size_t)nch;
+    const size_t off_cc = off_p + 8 * 15 * (size_t)nch, off_m = off_cc + 8 * 16 * (size_t)c->ncase;
+    const size_t off_fi = off_m + (material_id ? 4 * n : 0), off_fx = off_fi + 4 * (size_t)(c->ncase + 1);
+    const size_t total = off_fx + n;
+    char *buf = nullptr;
+    PDX_HIP(hipMallocAsync((void **)&buf, total, c->stream));
+    hipError_t e = hipMemcpyAsync(buf + off_X, X, 24 * n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(buf + off_E, E, 8 * n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(buf + off_f, fext, 24 * n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(buf + off_co, c->case_off.data(), 8 * (size_t)(c->ncase + 1), hipMemcpyHostToDevice,
+                           c->stream);
+    if (e == hipSuccess && nch > 0)
+        e = hipMemcpyAsync(buf + off_ch, chunk.data(), 16 * (size_t)nch, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && material_id)
+        e = hipMemcpyAsync(buf + off_m, material_id, 4 * n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(buf + off_fi, first.data(), 4 * (size_t)(c->ncase + 1), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(buf + off_fx, fixed, n, hipMemcpyHostToDevice, c->stream);
+    int r = PDX_EDEVICE;
+    if (e == hipSuccess) {
+        PdxRawIn in{(const double *)(buf + off_X), (const double *)(buf + off_E), (const double *)(buf + off_f),
+                    (const uint8_t *)(buf + off_fx), material_id ? (const int32_t *)(buf + off_m) : nullptr};
+        r = pdx_launch_raw_features(c, in, (const int64_t *)(buf + off_ch), nch, (const int32_t *)(buf + off_fi),
+                                    (const int64_t *)(buf + off_co), dx, (double *)(buf + off_p),
+                                    (double *)(buf + off_cc), d_out);
+    }
+    // the host vectors stay alive until the copies have run
+    hipError_t es = hipStreamSynchronize(c->stream);
+    hipFreeAsync(buf, c->stream);
+    PDX_HIP(e);
+    PDX_HIP(es);
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    return r;
+}
+
 extern "C" int pdx_local_damage(pdx_ctx *c)
 {
     CHECK_CTX(c);

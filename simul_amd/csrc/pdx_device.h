@@ -1141,13 +1141,15 @@ PDX_HD void bond_data_one(const Dev &d, int i, double *xi_o, double *len_o, doub
 // weighted neighbour mean of a per-particle feature vector (width <= 64) over
 // the intact bonds: out_i = sum_j w_ij V_j in_j / sum_j w_ij V_j (FP64), the
 // message-passing step of the learned warm start; a particle without intact
-// bonds keeps its own value
-PDX_HD void nbr_mean_one(const Dev &d, int i, const double *in, double *out, int width)
+// bonds keeps its own value.  Columns [q0, q0 + CH) per call: CH is a
+// compile-time count so that the accumulators stay in registers.
+template <int CH>
+PDX_HD void nbr_mean_one(const Dev &d, int i, const double *in, double *out, int width, int q0)
 {
     const KRec kr = d.KR[i];
     const XRec pi = d.xu[0][i];
-    double acc[64];
-    for (int q = 0; q < width; ++q) acc[q] = 0.0;
+    double acc[CH];
+    for (int q = 0; q < CH; ++q) acc[q] = 0.0;
     double ws = 0.0;
     const int64_t base = slot0(d, i);
     if (kr.flags & PDX_FL_ACTIVE) {
@@ -1159,12 +1161,27 @@ PDX_HD void nbr_mean_one(const Dev &d, int i, const double *in, double *out, int
             for (int q = 0; q < 3; ++q) s += (pj.X[q] - pi.X[q]) * (pj.X[q] - pi.X[q]);
             const double L = sqrt(s);
             const double w = (L > 0 ? fmax(0.0, 1.0 - L / d.h) : 1.0) * d.V64[(int)e];
-            for (int q = 0; q < width; ++q) acc[q] += w * in[(int64_t)e * width + q];
+            const double *row = in + (int64_t)e * width + q0;
+            if constexpr (CH == 20) {
+                // launched for width 20 only: the 160-byte row in 16-byte loads
+                struct alignas(16) Pair {
+                    double a, b;
+                };
+                for (int q = 0; q < CH; q += 2) {
+                    const Pair t = *(const Pair *)(row + q);
+                    acc[q] += w * t.a;
+                    acc[q + 1] += w * t.b;
+                }
+            } else {
+                for (int q = 0; q < CH; ++q)
+                    if (q0 + q < width) acc[q] += w * row[q];
+            }
             ws += w;
         }
     }
-    for (int q = 0; q < width; ++q)
-        out[(int64_t)i * width + q] = ws > 0.0 ? acc[q] / ws : in[(int64_t)i * width + q];
+    for (int q = 0; q < CH; ++q)
+        if (q0 + q < width)
+            out[(int64_t)i * width + q0 + q] = ws > 0.0 ? acc[q] / ws : in[(int64_t)i * width + q0 + q];
 }
 
 // |f|^2 over free DOFs from the stored force

@@ -86,6 +86,15 @@ int pdx_launch_nbr_mean(pdx_ctx *c, const double *d_in, double *d_out, int width
 int pdx_launch_pow(pdx_ctx *c, int what, int ubuf, pdx_d4 *v, const pdx_d4 *f0, const double *scale, double *out);
 int pdx_launch_u_device(pdx_ctx *c, int ubuf, double *out, const double *in);
 int pdx_launch_rows_internal(pdx_ctx *c, const double *d_in, double *d_out, int width);
+// specimen arrays of the warm-start raw features (device, caller order; mat may be null)
+struct PdxRawIn {
+    const double *X, *E, *fext;
+    const uint8_t *fixed;
+    const int32_t *mat;
+};
+int pdx_launch_raw_features(pdx_ctx *c, const PdxRawIn &in, const int64_t *d_chunk, int nchunk,
+                            const int32_t *d_chunk_first, const int64_t *d_case_off, double dx, double *d_part,
+                            double *d_cc, double *d_out);
 int pdx_launch_feat_pack(pdx_ctx *c, const double *d_hv, const int32_t *d_perm, float *d_out, int width, int hops,
                          int ldo);
 int pdx_launch_adr_mass(pdx_ctx *c, double beta);

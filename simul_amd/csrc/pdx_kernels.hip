@@ -874,9 +874,17 @@ template <int D> PDX_PER_PARTICLE(k_adr_mass_b(Dev d, const double *cs, const do
 template <int D> PDX_PER_PARTICLE(k_dt_est(Dev d, double modulus, double *out), (dt_est_one<D>(d, i, modulus, out)))
 template <int D> PDX_PER_PARTICLE(k_bond_data(Dev d, double *xi, double *len, double *w, const int64_t *rowi),
                                   (bond_data_one<D>(d, i, xi, len, w, rowi)))
+template <int CH>
+__device__ __forceinline__ void nbr_mean_cols(const Dev &d, int i, const double *in, double *out, int width)
+{
+    for (int q0 = 0; q0 < width; q0 += CH) nbr_mean_one<CH>(d, i, in, out, width, q0);
+}
+
 PDX_PER_PARTICLE(k_local_damage(Dev d), (local_damage_one(d, i)))
 PDX_PER_PARTICLE(k_zero_free(Dev d, int ubuf), (zero_free_one(d, i, ubuf)))
-PDX_PER_PARTICLE(k_nbr_mean(Dev d, const double *in, double *out, int width), (nbr_mean_one(d, i, in, out, width)))
+template <int CH>
+PDX_PER_PARTICLE(k_nbr_mean(Dev d, const double *in, double *out, int width),
+                 (nbr_mean_cols<CH>(d, i, in, out, width)))
 
 template <int D>
 __global__ __launch_bounds__(PDX_BLOCK) void k_damage(Dev d, int ubuf, double s_c, unsigned long long *count)
@@ -1219,7 +1227,14 @@ int pdx_launch_zero_free(pdx_ctx *c, int ubuf)
 
 int pdx_launch_nbr_mean(pdx_ctx *c, const double *d_in, double *d_out, int width)
 {
-    hipLaunchKernelGGL(k_nbr_mean, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_in, d_out, width);
+    // the warm-start features (20 columns) in one pass over the bonds; other
+    // widths four columns per pass
+    if (width == 20)
+        hipLaunchKernelGGL(k_nbr_mean<20>, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_in, d_out,
+                           width);
+    else
+        hipLaunchKernelGGL(k_nbr_mean<4>, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_in, d_out,
+                           width);
     PDX_LAUNCH_CHECK();
     return PDX_OK;
 }
@@ -1283,6 +1298,151 @@ int pdx_launch_feat_pack(pdx_ctx *c, const double *d_hv, const int32_t *d_perm, 
 {
     hipLaunchKernelGGL(k_feat_pack, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_hv, d_perm, d_out,
                        width, hops, ldo);
+    PDX_LAUNCH_CHECK();
+    return PDX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// raw inputs of the learned warm start: 20 FP64 columns per particle in caller
+// order, from the specimen's positions, moduli, loads, DOF masks and phases.
+// Per-case sums over row chunks (k_raw_partials), per-case constants summed
+// in chunk order (k_raw_cases, deterministic), then one thread per row
+// (k_raw_fill).  Columns:
+//   0-2   (x - cx) / a, (y - cy) / b, (z - z_min) / H   (bounding box of the case)
+//   3     log10(E / 1000)          4  1 for phase 0 (cortical)
+//   5-7   fixed-DOF bits           8  1 on loaded rows (any load component != 0)
+//   9-11  unit resultant load      12 log10 of the resultant pressure |sum f| dx / n_loaded
+//   13-15 H / 25, a / 20, b / 20   16 case mean of column 3, 17 case mean of column 4
+//   18-19 E-weighted centroid minus the box centre, over a and b
+// ---------------------------------------------------------------------------
+enum { RS_MIN = 0, RS_MAX = 3, RS_F = 6, RS_NLOAD = 9, RS_LOGE = 10, RS_CORT = 11, RS_E = 12, RS_EX = 13,
+       RS_N = 15 };
+enum { RC_C = 0, RC_AX = 3, RC_BY = 4, RC_H = 5, RC_Z0 = 6, RC_DIR = 7, RC_LOGP = 10, RC_LOGE = 11, RC_CORT = 12,
+       RC_ECC = 13, RC_N = 16 };
+
+__global__ __launch_bounds__(PDX_BLOCK) void k_raw_partials(PdxRawIn in, const int64_t *chunk, double *part)
+{
+    __shared__ double sh[RS_N][PDX_BLOCK / 64];
+    const int64_t r0 = chunk[2 * blockIdx.x], r1 = chunk[2 * blockIdx.x + 1];
+    double v[RS_N];
+    for (int q = 0; q < 3; ++q) {
+        v[RS_MIN + q] = INFINITY;
+        v[RS_MAX + q] = -INFINITY;
+    }
+    for (int q = RS_F; q < RS_N; ++q) v[q] = 0.0;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += PDX_BLOCK) {
+        const double *x = in.X + 3 * r, *f = in.fext + 3 * r;
+        const double E = in.E[r];
+        for (int q = 0; q < 3; ++q) {
+            v[RS_MIN + q] = fmin(v[RS_MIN + q], x[q]);
+            v[RS_MAX + q] = fmax(v[RS_MAX + q], x[q]);
+            v[RS_F + q] += f[q];
+        }
+        v[RS_NLOAD] += (fabs(f[0]) + fabs(f[1]) + fabs(f[2]) > 0.0) ? 1.0 : 0.0;
+        v[RS_LOGE] += log10(E / 1000.0);
+        v[RS_CORT] += (in.mat && in.mat[r] == 0) ? 1.0 : 0.0;
+        v[RS_E] += E;
+        v[RS_EX] += E * x[0];
+        v[RS_EX + 1] += E * x[1];
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int q = 0; q < RS_N; ++q) {
+        double t = v[q];
+        for (int o = 32; o > 0; o >>= 1) {
+            const double u = __shfl_xor(t, o, 64);
+            t = q < RS_MAX ? fmin(t, u) : q < RS_F ? fmax(t, u) : t + u;
+        }
+        if (lane == 0) sh[q][wv] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < RS_N) {
+        const int q = threadIdx.x;
+        double t = sh[q][0];
+        for (int w = 1; w < PDX_BLOCK / 64; ++w)
+            t = q < RS_MAX ? fmin(t, sh[q][w]) : q < RS_F ? fmax(t, sh[q][w]) : t + sh[q][w];
+        part[(int64_t)blockIdx.x * RS_N + q] = t;
+    }
+}
+
+__global__ void k_raw_cases(const double *part, const int32_t *chunk_first, const int64_t *case_off, int ncase,
+                            double dx, double *cc)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncase) return;
+    double v[RS_N];
+    for (int q = 0; q < 3; ++q) {
+        v[RS_MIN + q] = INFINITY;
+        v[RS_MAX + q] = -INFINITY;
+    }
+    for (int q = RS_F; q < RS_N; ++q) v[q] = 0.0;
+    for (int b = chunk_first[k]; b < chunk_first[k + 1]; ++b)
+        for (int q = 0; q < RS_N; ++q) {
+            const double t = part[(int64_t)b * RS_N + q];
+            v[q] = q < RS_MAX ? fmin(v[q], t) : q < RS_F ? fmax(v[q], t) : v[q] + t;
+        }
+    const double cnt = (double)(case_off[k + 1] - case_off[k]);
+    double *o = cc + (int64_t)k * RC_N;
+    for (int q = 0; q < 3; ++q) o[RC_C + q] = 0.5 * (v[RS_MIN + q] + v[RS_MAX + q]);
+    o[RC_AX] = fmax(0.5 * (v[RS_MAX] - v[RS_MIN]), 1e-9);
+    o[RC_BY] = fmax(0.5 * (v[RS_MAX + 1] - v[RS_MIN + 1]), 1e-9);
+    o[RC_H] = fmax(v[RS_MAX + 2] - v[RS_MIN + 2], 1e-9);
+    o[RC_Z0] = v[RS_MIN + 2];
+    const double nrm = sqrt(v[RS_F] * v[RS_F] + v[RS_F + 1] * v[RS_F + 1] + v[RS_F + 2] * v[RS_F + 2]);
+    for (int q = 0; q < 3; ++q) o[RC_DIR + q] = nrm > 0.0 ? v[RS_F + q] / nrm : 0.0;
+    // resultant pressure: |sum f| V / loaded area, area = n_loaded dx^2
+    o[RC_LOGP] = nrm > 0.0 ? log10(fmax(nrm * dx * dx * dx / fmax(v[RS_NLOAD] * dx * dx, 1e-30), 1e-12)) : 0.0;
+    o[RC_LOGE] = v[RS_LOGE] / cnt;
+    o[RC_CORT] = v[RS_CORT] / cnt;
+    const double es = fmax(v[RS_E], 1e-300);
+    o[RC_ECC] = (v[RS_EX] / es - o[RC_C]) / o[RC_AX];
+    o[RC_ECC + 1] = (v[RS_EX + 1] / es - o[RC_C + 1]) / o[RC_BY];
+}
+
+__global__ __launch_bounds__(PDX_BLOCK) void k_raw_fill(PdxRawIn in, const int64_t *case_off, int ncase,
+                                                       const double *cc, double *out)
+{
+    const int64_t r = (int64_t)blockIdx.x * PDX_BLOCK + threadIdx.x;
+    if (r >= case_off[ncase]) return;
+    int lo = 0, hi = ncase;  // case k with case_off[k] <= r < case_off[k + 1]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (case_off[mid] <= r) lo = mid;
+        else hi = mid;
+    }
+    const double *o = cc + (int64_t)lo * RC_N;
+    const double *x = in.X + 3 * r, *f = in.fext + 3 * r;
+    double *y = out + r * 20;
+    y[0] = (x[0] - o[RC_C]) / o[RC_AX];
+    y[1] = (x[1] - o[RC_C + 1]) / o[RC_BY];
+    y[2] = (x[2] - o[RC_Z0]) / o[RC_H];
+    y[3] = log10(in.E[r] / 1000.0);
+    y[4] = (in.mat && in.mat[r] == 0) ? 1.0 : 0.0;
+    for (int q = 0; q < 3; ++q) y[5 + q] = (double)((in.fixed[r] >> q) & 1);
+    y[8] = (fabs(f[0]) + fabs(f[1]) + fabs(f[2]) > 0.0) ? 1.0 : 0.0;
+    for (int q = 0; q < 3; ++q) y[9 + q] = o[RC_DIR + q];
+    y[12] = o[RC_LOGP];
+    y[13] = o[RC_H] / 25.0;
+    y[14] = o[RC_AX] / 20.0;
+    y[15] = o[RC_BY] / 20.0;
+    y[16] = o[RC_LOGE];
+    y[17] = o[RC_CORT];
+    y[18] = o[RC_ECC];
+    y[19] = o[RC_ECC + 1];
+}
+
+int pdx_launch_raw_features(pdx_ctx *c, const PdxRawIn &in, const int64_t *d_chunk, int nchunk,
+                            const int32_t *d_chunk_first, const int64_t *d_case_off, double dx, double *d_part,
+                            double *d_cc, double *d_out)
+{
+    if (nchunk > 0) {
+        hipLaunchKernelGGL(k_raw_partials, dim3(nchunk), dim3(PDX_BLOCK), 0, c->stream, in, d_chunk, d_part);
+        PDX_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_raw_cases, dim3((c->ncase + 63) / 64), dim3(64), 0, c->stream, d_part, d_chunk_first,
+                       d_case_off, c->ncase, dx, d_cc);
+    PDX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_raw_fill, dim3((unsigned)((c->n + PDX_BLOCK - 1) / PDX_BLOCK)), dim3(PDX_BLOCK), 0,
+                       c->stream, in, d_case_off, c->ncase, d_cc, d_out);
     PDX_LAUNCH_CHECK();
     return PDX_OK;
 }

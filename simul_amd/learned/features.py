@@ -78,64 +78,36 @@ def raw_features(spec, case_offsets=None) -> np.ndarray:
     return np.ascontiguousarray(F.T)
 
 
-def raw_features_tensor(spec, case_offsets=None):
-    """raw_features computed on the GPU: (n, 20) float64 CUDA tensor (the same
-    values to FP64 rounding)."""
+def raw_features_tensor(body, spec):
+    """raw_features computed on the GPU (libpdx pdx_warm_raw_features over the
+    cases of ``body``, a libpdx context of the same particles): (n, 20)
+    float64 CUDA tensor, the same values to FP64 rounding."""
+    import ctypes as C
+
     import torch
-    dev = "cuda"
+    from .. import _native as N
     n = spec.n
-    X = torch.from_numpy(np.ascontiguousarray(spec.X, np.float64)).to(dev)
-    E = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(np.asarray(spec.E, np.float64), (n,)))).to(dev)
-    fext = torch.from_numpy(np.ascontiguousarray(spec.fext, np.float64)).to(dev)
-    fixed = torch.from_numpy(np.ascontiguousarray(spec.fixed, np.uint8)).to(dev).to(torch.int32)
-    cort = (torch.from_numpy(np.ascontiguousarray(spec.material_id)).to(dev) == 0).to(torch.float64) \
-        if spec.material_id is not None else torch.zeros(n, dtype=torch.float64, device=dev)
-    F = torch.zeros((n, RAW), dtype=torch.float64, device=dev)
-    logE = torch.log10(E / 1000.0)
-    loaded_all = (fext.abs().sum(1) > 0).to(torch.float64)
-    for a0, b0 in _per_case(spec, case_offsets):
-        Xc = X[a0:b0]
-        lo, hi = Xc.min(0).values, Xc.max(0).values
-        c = 0.5 * (lo + hi)
-        ax = torch.clamp(0.5 * (hi[0] - lo[0]), min=1e-9)
-        by = torch.clamp(0.5 * (hi[1] - lo[1]), min=1e-9)
-        H = torch.clamp(hi[2] - lo[2], min=1e-9)
-        F[a0:b0, 0] = (Xc[:, 0] - c[0]) / ax
-        F[a0:b0, 1] = (Xc[:, 1] - c[1]) / by
-        F[a0:b0, 2] = (Xc[:, 2] - lo[2]) / H
-        tot = fext[a0:b0].sum(0)
-        loaded = loaded_all[a0:b0]
-        nrm = torch.linalg.norm(tot)
-        F[a0:b0, 8] = loaded
-        if float(nrm) > 0:
-            F[a0:b0, 9:12] = (tot / nrm)[None, :]
-            p = nrm * spec.dx ** 3 / torch.clamp(loaded.sum() * spec.dx ** 2, min=1e-30)
-            F[a0:b0, 12] = torch.log10(torch.clamp(p, min=1e-12))
-        F[a0:b0, 13] = H / 25.0
-        F[a0:b0, 14] = ax / 20.0
-        F[a0:b0, 15] = by / 20.0
-        F[a0:b0, 16] = logE[a0:b0].mean()
-        F[a0:b0, 17] = cort[a0:b0].mean()
-        Ec = E[a0:b0]
-        ce = (Ec @ Xc[:, :2]) / torch.clamp(Ec.sum(), min=1e-300)
-        F[a0:b0, 18] = (ce[0] - c[0]) / ax
-        F[a0:b0, 19] = (ce[1] - c[1]) / by
-    F[:, 3] = logE
-    F[:, 4] = cort
-    for q in range(3):
-        F[:, 5 + q] = ((fixed >> q) & 1).to(torch.float64)
-    return F
+    X = np.ascontiguousarray(spec.X, np.float64)
+    E = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.E, np.float64), (n,)))
+    fext = np.ascontiguousarray(spec.fext, np.float64)
+    fixed = np.ascontiguousarray(spec.fixed, np.uint8)
+    mat = None if spec.material_id is None else np.ascontiguousarray(spec.material_id, np.int32)
+    out = torch.empty((n, RAW), dtype=torch.float64, device="cuda")
+    torch.cuda.current_stream().synchronize()
+    body.ctx.call("pdx_warm_raw_features", N.ptr(X, N._dp), N.ptr(E, N._dp), N.ptr(fext, N._dp),
+                  N.ptr(fixed, N._u8p), N.ptr(mat, N._i32p), float(spec.dx), C.c_void_p(out.data_ptr()))
+    return out
 
 
 def feature_tensor(body, spec, case_offsets=None):
     """(n, 64) float32 device tensor: raw features and two neighbour-mean hops
     through the bonds of ``body`` (a libpdx context of the same particles),
-    built on the device (raw_features_tensor, pdx_neighbor_features_device)
-    with no host copy."""
+    built on the device (pdx_warm_raw_features, pdx_neighbor_features_device)
+    from one upload of the specimen arrays."""
     import ctypes as C
 
     import torch
-    raw = raw_features_tensor(spec, case_offsets)
+    raw = raw_features_tensor(body, spec)
     out = torch.zeros((spec.n, FEAT), dtype=torch.float32, device="cuda")
     torch.cuda.current_stream().synchronize()
     body.ctx.call("pdx_neighbor_features_device", RAW, 2, C.c_void_p(raw.data_ptr()), C.c_void_p(out.data_ptr()),

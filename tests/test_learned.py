@@ -175,8 +175,8 @@ def test_warm_start_reaches_the_cold_solution(gpu):
 
 
 def test_device_features_match_host_hops(gpu):
-    """The raw features computed on the GPU (raw_features_tensor) against the
-    host ones to FP64 rounding, and pdx_neighbor_features_device (device hops,
+    """The raw features computed on the GPU (pdx_warm_raw_features) against
+    the host ones to FP64 rounding, and pdx_neighbor_features_device (device hops,
     FP32 pack into the network input) against the same two hops through
     pdx_neighbor_mean on host arrays, for a two-case batch with padding
     between the cases: bit for bit; the host-input pdx_neighbor_features gives
@@ -191,7 +191,7 @@ def test_device_features_match_host_hops(gpu):
     specs = [W.random_vertebra(77, dx=1.6), W.random_vertebra(78, dx=1.6)]
     packed, off = W.pack(specs)
     body = from_specimen(packed, case_offsets=off)
-    raw = raw_features_tensor(packed, off).cpu().numpy()
+    raw = raw_features_tensor(body, packed).cpu().numpy()
     assert np.allclose(raw, raw_features(packed, off), rtol=1e-12, atol=1e-12)
     F = feature_tensor(body, packed, off).cpu().numpy()
     Fh = torch.zeros((packed.n, FEAT), dtype=torch.float32, device="cuda")
@@ -207,3 +207,18 @@ def test_device_features_match_host_hops(gpu):
     ref[:, RAW:2 * RAW] = m1
     ref[:, 2 * RAW:3 * RAW] = m2
     assert np.array_equal(F, ref)
+
+
+@pytest.mark.parametrize("which", ["cube", "vertebra_single"])
+def test_device_raw_features_single_case(gpu, which):
+    """pdx_warm_raw_features on one-case contexts: the unloaded cube without
+    phases (zero load resultant, scalar modulus) and one vertebral body,
+    against raw_features to FP64 rounding."""
+    from simul_amd import workloads as W
+    from simul_amd.body import from_specimen
+    from simul_amd.learned.features import raw_features, raw_features_tensor
+    spec = W.cube(n_side=12) if which == "cube" else W.random_vertebra(79, dx=1.6)
+    body = from_specimen(spec)
+    raw = raw_features_tensor(body, spec).cpu().numpy()
+    ref = raw_features(spec)
+    assert np.allclose(raw, ref, rtol=1e-12, atol=1e-12), np.abs(raw - ref).max(0)
