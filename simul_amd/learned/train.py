@@ -88,34 +88,40 @@ def train(net, F, T, Wt, epochs, batch_rows, lr0, lr_min, seed=0):
     return hist
 
 
-def evaluate(net, seeds, dx, group, tol, max_iterations):
-    """Cold vs warm-started batched solves of the held-out cases."""
+def evaluate(net, seeds, dx, group, tol, max_iterations, mass=None):
+    """Cold vs warm-started batched solves of the held-out cases (the same DR
+    mass for both: PDBody.solve's ``mass``).  Features and inference run on
+    the device (predict_u_device, pdx_set_u_device); one untimed inference
+    on the first group loads their GPU code first."""
     import torch
     from ..workloads import random_vertebra
-    from .warmstart import predict_u
+    from .warmstart import predict_u_device
     out = []
     for g0 in range(0, len(seeds), group):
         specs = [random_vertebra(int(s), dx=dx) for s in seeds[g0:g0 + group]]
         packed, off, body = _group_body(specs)
+        if g0 == 0:
+            predict_u_device(net, body, packed, off)
         cold = np.atleast_1d(body.cold_residual())
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         rc = body.solve("adaptive", tol=tol, max_iterations=max_iterations, refs=cold, slice_iterations=2048,
-                        progress=lambda d, r: None)
+                        progress=lambda d, r: None, mass=mass)
         torch.cuda.synchronize()
         t_cold = time.perf_counter() - t0
         u_cold = body.u
         body.u = packed.u0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        body.u = predict_u(net, body, packed, off)
-        u_pred = body.u
+        body.set_u_device(predict_u_device(net, body, packed, off))
         torch.cuda.synchronize()
         t_inf = time.perf_counter() - t0
+        u_pred = body.u
+        t_get = time.perf_counter() - t0 - t_inf  # the host copy of the prediction is not part of the warm start
         rw = body.solve("adaptive", tol=tol, max_iterations=max_iterations, refs=cold, slice_iterations=2048,
-                        progress=lambda d, r: None)
+                        progress=lambda d, r: None, mass=mass)
         torch.cuda.synchronize()
-        t_warm = time.perf_counter() - t0
+        t_warm = time.perf_counter() - t0 - t_get
         u_warm = body.u
         err_pred = [float(np.linalg.norm(u_pred[a:b] - u_cold[a:b]) / np.linalg.norm(u_cold[a:b]))
                     for a, b in zip(off[:-1], off[1:])]
@@ -147,6 +153,8 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--lr-min", type=float, default=2e-5)
     ap.add_argument("--seed", type=int, default=5000)
+    ap.add_argument("--mass", default=None, choices=["gershgorin", "spectral"],
+                    help="adaptive-DR mass of the held-out cold and warm solves")
     ap.add_argument("--out", default=None)
     ap.add_argument("--report", default=None)
     args = ap.parse_args(argv)
@@ -173,7 +181,7 @@ def main(argv=None):
         net.save(args.out, dx=np.array(args.dx), train_cases=args.train, epochs=args.epochs, final_loss=hist[-1])
         log(f"weights -> {args.out}")
     log(f"evaluating on {args.heldout} held-out cases")
-    ev = evaluate(net, held_seeds, args.dx[0], args.group, args.tol, args.max_iterations)
+    ev = evaluate(net, held_seeds, args.dx[0], args.group, args.tol, args.max_iterations, mass=args.mass)
     cold = sum(e["cold_s"] for e in ev)
     warm = sum(e["warm_s"] for e in ev)
     ic = [i for e in ev for i in e["cold_iterations"]]
