@@ -178,7 +178,7 @@ static int create_common(int32_t ncase, const int64_t *off, int32_t dim, pdx_ctx
         (r = dalloc(c->stream, &d.perm, ni)) ||
         (r = dalloc(c->stream, &d.counts, ni)) || (r = dalloc(c->stream, &d.case_of_blk, (size_t)c->nblk)) ||
         (r = dalloc(c->stream, &d.case_blk0, (size_t)ncase + 1)) || (r = dalloc(c->stream, &d.sc, (size_t)ncase)) ||
-        (r = dalloc(c->stream, &d.part, (size_t)c->nblk * 4)) || (r = dalloc(c->stream, &d.grid, (size_t)ncase))) {
+        (r = dalloc(c->stream, &d.part, (size_t)c->nblk * PDX_PW * 4)) || (r = dalloc(c->stream, &d.grid, (size_t)ncase))) {
         free_all(c);
         hipStreamDestroy(c->stream);
         delete c;

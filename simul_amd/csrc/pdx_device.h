@@ -30,6 +30,8 @@
 #ifndef PDX_BLOCK
 #define PDX_BLOCK 256
 #endif
+// waves per block that write reduction partials (PDX_BLOCK / 64)
+#define PDX_PW 4
 #define PDX_SLICE 64
 // slice widths (bond slots per row of a slice) are multiples of this
 #define PDX_SLOT_ALIGN 4
@@ -159,7 +161,7 @@ struct Dev {
     int32_t *case_of_blk;
     int32_t *case_blk0;  // first block of each case (ncase+1)
     DScal *sc;
-    double *part;      // per-block partial sums (4 per block)
+    double *part;      // per-wave partial sums (PDX_PW waves x 4 per block)
     // neighbour search
     GridP *grid;
     int32_t *case_of_p;  // case of each internal particle
@@ -191,7 +193,7 @@ PDX_HD void matmul_f(const float *A, const float *B, float *C)
 template <int D, typename T>
 PDX_HD T lame(T Kb, T mu)
 {
-    return D == 3 ? Kb - T(2) * mu / T(3) : Kb - mu;
+    return D == 3 ? Kb - mu * (T(2) / T(3)) : Kb - mu;
 }
 
 // linear elastic: eps = sym(H), P = lam tr(eps) I + 2 mu eps   (nosb.py:218-227)
@@ -645,15 +647,17 @@ PDX_HD void force_finish(const Dev &d, int i, const KRec &kr, const XRec &pi, co
         const double vo[3] = {vo4.x, vo4.y, vo4.z}, fo[3] = {fo4.x, fo4.y, fo4.z};
         const double uo[3] = {pi.u[0], pi.u[1], pi.u[2]};
         double vn[3] = {0.0, 0.0, 0.0}, un[3] = {uo[0], uo[1], uo[2]}, fn[3] = {0.0, 0.0, 0.0};
+        // two divisions per particle instead of two per DOF
+        const double rl = 1.0 / lam, rc = 1.0 / (2.0 + c);
         for (int r = 0; r < D; ++r) {
             if ((fm >> r) & 1u) continue;
             const double F = fa[r];
             if (first) {
-                vn[r] = F / (2.0 * lam);
+                vn[r] = 0.5 * F * rl;
             } else {
                 red[1] -= vo[r] * (F - fo[r]);
                 red[2] += lam * vo[r] * vo[r];
-                vn[r] = ((2.0 - c) * vo[r] + 2.0 * F / lam) / (2.0 + c);
+                vn[r] = ((2.0 - c) * vo[r] + 2.0 * F * rl) * rc;
             }
             fn[r] = F;
             un[r] = uo[r] + vn[r];

@@ -37,7 +37,8 @@ __device__ __forceinline__ double ld_coherent(const double *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Block sum of NV doubles into d.part[lb * 4 + q] (waves in order).  The
+// Wave sums of NV doubles into d.part[(lb * PDX_PW + w) * 4 + q], one entry
+// per wave w < NA, written by each wave as it ends (no block barrier).  The
 // case totals are summed by k_case_reduce, a separate launch: a last-block
 // reduction inside the pass would need an agent-scope release per block,
 // which writes back the XCD's L2 (every block's stores) on gfx950.  Only the
@@ -46,23 +47,13 @@ __device__ __forceinline__ double ld_coherent(const double *p)
 template <int NV, int NT = PDX_BLOCK, int NA = NT / 64>
 __device__ void block_partials(const Dev &d, int lb, double (&val)[NV])
 {
-    __shared__ double sred[NA][NV];
+    static_assert(NA == PDX_PW, "one partial per wave slot");
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (wid < NA) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
-            double t = wave_sum(val[q]);
-            if (lane == 0) sred[wid][q] = t;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) {
-            double t = 0.0;
-#pragma unroll
-            for (int w = 0; w < NA; ++w) t += sred[w][q];
-            d.part[(int64_t)lb * 4 + q] = t;
+            const double t = wave_sum(val[q]);
+            if (lane == 0) d.part[((int64_t)lb * PDX_PW + wid) * 4 + q] = t;
         }
     }
 }
@@ -84,7 +75,8 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_case_reduce(Dev d, int ubuf, int 
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         double t = 0.0;
-        for (int b = b0 + threadIdx.x; b < b1; b += PDX_BLOCK) t += d.part[(int64_t)b * 4 + q];
+        for (int64_t e = (int64_t)b0 * PDX_PW + threadIdx.x; e < (int64_t)b1 * PDX_PW; e += PDX_BLOCK)
+            t += d.part[e * 4 + q];
         t = wave_sum(t);
         if (lane == 0) sred[wid][q] = t;
         __syncthreads();
@@ -1524,7 +1516,8 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_pow_reduce(Dev d, double *out)
     const int b0 = d.case_blk0[cs], b1 = d.case_blk0[cs + 1];
     for (int q = 0; q < 4; ++q) {
         double t = 0.0;
-        for (int b = b0 + threadIdx.x; b < b1; b += PDX_BLOCK) t += d.part[(int64_t)b * 4 + q];
+        for (int64_t e = (int64_t)b0 * PDX_PW + threadIdx.x; e < (int64_t)b1 * PDX_PW; e += PDX_BLOCK)
+            t += d.part[e * 4 + q];
         t = wave_sum(t);
         if (lane == 0) sred[wid][q] = t;
         __syncthreads();
