@@ -222,3 +222,47 @@ def test_device_raw_features_single_case(gpu, which):
     raw = raw_features_tensor(body, spec).cpu().numpy()
     ref = raw_features(spec)
     assert np.allclose(raw, ref, rtol=1e-12, atol=1e-12), np.abs(raw - ref).max(0)
+
+
+def test_device_features_refuse_bad_arguments(gpu):
+    """pdx_warm_raw_features and pdx_neighbor_features_device check their
+    arguments before any launch: NULL arrays, host memory where device memory
+    is required, a non-positive spacing, widths and hops out of range."""
+    import ctypes as C
+
+    import torch
+    from simul_amd import _native as N
+    from simul_amd import workloads as W
+    from simul_amd.body import from_specimen
+    spec = W.random_vertebra(80, dx=1.6)
+    body = from_specimen(spec)
+    n = spec.n
+    X = np.ascontiguousarray(spec.X, np.float64)
+    E = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.E, np.float64), (n,)))
+    fext = np.ascontiguousarray(spec.fext, np.float64)
+    fixed = np.ascontiguousarray(spec.fixed, np.uint8)
+    out = torch.zeros((n, 20), dtype=torch.float64, device="cuda")
+    host = np.zeros((n, 64))
+
+    def raw(x, dx, dst):
+        body.ctx.call("pdx_warm_raw_features", x, N.ptr(E, N._dp), N.ptr(fext, N._dp), N.ptr(fixed, N._u8p),
+                      None, dx, dst)
+
+    with pytest.raises(N.FEAValidationError, match="NULL"):
+        raw(None, spec.dx, C.c_void_p(out.data_ptr()))
+    with pytest.raises(N.FEAValidationError, match="dx"):
+        raw(N.ptr(X, N._dp), 0.0, C.c_void_p(out.data_ptr()))
+    with pytest.raises(N.FEAValidationError, match="not device memory"):
+        raw(N.ptr(X, N._dp), spec.dx, C.c_void_p(host.ctypes.data))
+    assert float(out.abs().sum()) == 0.0  # nothing was written
+    raw(N.ptr(X, N._dp), spec.dx, C.c_void_p(out.data_ptr()))
+    F = torch.zeros((n, 64), dtype=torch.float32, device="cuda")
+    for width, hops, ldo, d_in, msg in ((0, 2, 64, out, "width"), (20, 5, 64, out, "hops"),
+                                        (20, 2, 40, out, "ldo"), (20, 2, 64, None, "NULL")):
+        with pytest.raises(N.FEAValidationError, match=msg):
+            body.ctx.call("pdx_neighbor_features_device", width, hops,
+                          None if d_in is None else C.c_void_p(d_in.data_ptr()), C.c_void_p(F.data_ptr()), ldo)
+    with pytest.raises(N.FEAValidationError, match="not device memory"):
+        body.ctx.call("pdx_neighbor_features_device", 20, 2, C.c_void_p(host.ctypes.data), C.c_void_p(F.data_ptr()),
+                      64)
+    assert float(F.abs().sum()) == 0.0
