@@ -58,34 +58,46 @@ __device__ void block_partials(const Dev &d, int lb, double (&val)[NV])
     }
 }
 
-// case totals of the block partials (strided per thread, then waves in
-// order: a fixed order, independent of block scheduling) and the per-case
-// update; MODE 0..2 = the force pass's UPD, MODE 3 = residual only.  One
-// block per case.
+// case totals of the wave partials (strided per thread, one 32-byte entry per
+// load, then waves in order: a fixed order, independent of block scheduling)
+// and the per-case update; MODE 0..2 = the force pass's UPD, MODE 3 =
+// residual only.  One block of PDX_RED_THREADS per case.
+#define PDX_RED_THREADS 1024
 template <int MODE>
-__global__ __launch_bounds__(PDX_BLOCK) void k_case_reduce(Dev d, int ubuf, int solve_mode)
+__global__ __launch_bounds__(PDX_RED_THREADS) void k_case_reduce(Dev d, int ubuf, int solve_mode)
 {
     constexpr int NV = MODE == 3 ? 1 : 3;
-    __shared__ double sred[PDX_BLOCK / 64][NV];
+    __shared__ double sred[PDX_RED_THREADS / 64][NV];
     const int cs = blockIdx.x;
     if (solve_mode && d.sc[cs].converged) return;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int b0 = d.case_blk0[cs], b1 = d.case_blk0[cs + 1];
+    const pdx_d4 *pp = reinterpret_cast<const pdx_d4 *>(d.part);
+    double t[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) t[q] = 0.0;
+    for (int64_t e = (int64_t)b0 * PDX_PW + threadIdx.x; e < (int64_t)b1 * PDX_PW; e += PDX_RED_THREADS) {
+        const pdx_d4 v = pp[e];
+        t[0] += v.x;
+        if constexpr (NV == 3) {
+            t[1] += v.y;
+            t[2] += v.z;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        const double w = wave_sum(t[q]);
+        if (lane == 0) sred[wid][q] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     double tot[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
-        double t = 0.0;
-        for (int64_t e = (int64_t)b0 * PDX_PW + threadIdx.x; e < (int64_t)b1 * PDX_PW; e += PDX_BLOCK)
-            t += d.part[e * 4 + q];
-        t = wave_sum(t);
-        if (lane == 0) sred[wid][q] = t;
-        __syncthreads();
         double s = 0.0;
-        for (int w = 0; w < PDX_BLOCK / 64; ++w) s += sred[w][q];
+        for (int w = 0; w < PDX_RED_THREADS / 64; ++w) s += sred[w][q];
         tot[q] = s;
-        __syncthreads();
     }
-    if (threadIdx.x != 0) return;
     if (MODE == 3)
         d.sc[cs].res = sqrt(tot[0]);
     else
@@ -1192,7 +1204,7 @@ int pdx_launch_resid(pdx_ctx *c)
     else
         hipLaunchKernelGGL((k_resid<2>), dim3(c->nblk), dim3(PDX_BLOCK), 0, c->stream, c->d);
     PDX_LAUNCH_CHECK();
-    hipLaunchKernelGGL((k_case_reduce<3>), dim3(c->ncase), dim3(PDX_BLOCK), 0, c->stream, c->d, 0, 0);
+    hipLaunchKernelGGL((k_case_reduce<3>), dim3(c->ncase), dim3(PDX_RED_THREADS), 0, c->stream, c->d, 0, 0);
     PDX_LAUNCH_CHECK();
     return PDX_OK;
 }
@@ -1201,11 +1213,11 @@ int pdx_launch_case_reduce(pdx_ctx *c, int ubuf, int upd, bool solve_mode)
 {
     const int sm = solve_mode ? 1 : 0;
     if (upd == 1)
-        hipLaunchKernelGGL((k_case_reduce<1>), dim3(c->ncase), dim3(PDX_BLOCK), 0, c->stream, c->d, ubuf, sm);
+        hipLaunchKernelGGL((k_case_reduce<1>), dim3(c->ncase), dim3(PDX_RED_THREADS), 0, c->stream, c->d, ubuf, sm);
     else if (upd == 2)
-        hipLaunchKernelGGL((k_case_reduce<2>), dim3(c->ncase), dim3(PDX_BLOCK), 0, c->stream, c->d, ubuf, sm);
+        hipLaunchKernelGGL((k_case_reduce<2>), dim3(c->ncase), dim3(PDX_RED_THREADS), 0, c->stream, c->d, ubuf, sm);
     else
-        hipLaunchKernelGGL((k_case_reduce<0>), dim3(c->ncase), dim3(PDX_BLOCK), 0, c->stream, c->d, ubuf, sm);
+        hipLaunchKernelGGL((k_case_reduce<0>), dim3(c->ncase), dim3(PDX_RED_THREADS), 0, c->stream, c->d, ubuf, sm);
     PDX_LAUNCH_CHECK();
     return PDX_OK;
 }
