@@ -254,7 +254,8 @@ def main():
                         "warm_speedup_vs_plain_cold": round(wall_g / tw["total_s"], 3),
                         "warm_vs_cold_u_rel_l2": du,
                         "warm_start": "learned per-particle network (BF16 MFMA kernels), features, inference and "
-                                      "DR included in warm_start_s"})
+                                      "DR included in warm_start_s; one untimed inference before it loads the "
+                                      "process's GPU code for that path"})
             log(f"warm start: {tw['total_s']:.3f}s ({tw['inference_s']:.3f}s features+inference), "
                 f"{rw.iterations} iterations, converged={rw.converged}, |u_warm - u_cold|/|u_cold| = {du:.2e}")
 
