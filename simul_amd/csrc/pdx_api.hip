@@ -1347,12 +1347,19 @@ extern "C" int pdx_kd_step(pdx_ctx *c, double dt, double visc, double *prev_ke, 
     return PDX_OK;
 }
 
-// one iteration of the chosen method on the context stream
-static int launch_iter(pdx_ctx *c, int method, double dt, double visc, bool solve_mode)
+// one iteration of the chosen method on the context stream.  Adaptive DR
+// leaves the case reduction of its force pass pending (*pend = the pass's
+// input buffer): the next iteration's F-pass runs it first, fused into the
+// halo kernel, and flush_reduce ends a sequence of iterations
+static int launch_iter(pdx_ctx *c, int method, double dt, double visc, bool solve_mode, int *pend)
 {
     if (method == PDX_DR_ADAPTIVE) {
-        if (c->form == PDX_FORM_NOSB) PDX_TRY(pdx_launch_fpass(c, c->ucur, false, false, solve_mode));
-        PDX_TRY(pdx_launch_force(c, c->ucur, 1, solve_mode, 0.0, 0.0));
+        if (c->form == PDX_FORM_NOSB)
+            PDX_TRY(pdx_launch_fpass(c, c->ucur, false, false, solve_mode, *pend));
+        else if (*pend >= 0)
+            PDX_TRY(pdx_launch_case_reduce(c, *pend, 1, solve_mode));
+        PDX_TRY(pdx_launch_force_pass(c, c->ucur, 1, solve_mode, 0.0, 0.0));
+        *pend = c->ucur;
         c->ucur ^= 1;
     } else {
         PDX_TRY(pdx_launch_kd_move(c, c->ucur, solve_mode, dt, true));
@@ -1361,6 +1368,14 @@ static int launch_iter(pdx_ctx *c, int method, double dt, double visc, bool solv
         PDX_TRY(pdx_launch_force(c, c->ucur, 2, solve_mode, dt, visc));
     }
     return PDX_OK;
+}
+
+static int flush_reduce(pdx_ctx *c, bool solve_mode, int *pend)
+{
+    if (*pend < 0) return PDX_OK;
+    const int r = pdx_launch_case_reduce(c, *pend, 1, solve_mode);
+    *pend = -1;
+    return r;
 }
 
 extern "C" int pdx_dr_solve(pdx_ctx *c, const pdx_dr_params *p, pdx_dr_result *res)
@@ -1438,7 +1453,10 @@ extern "C" int pdx_dr_solve(pdx_ctx *c, const pdx_dr_params *p, pdx_dr_result *r
                 const int u0 = c->ucur;
                 PDX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
                 int r = PDX_OK;
-                for (int t = 0; t < chunk && r == PDX_OK; ++t) r = launch_iter(c, method, p->dt, p->viscous_damping, true);
+                int pend = -1;
+                for (int t = 0; t < chunk && r == PDX_OK; ++t)
+                    r = launch_iter(c, method, p->dt, p->viscous_damping, true, &pend);
+                if (r == PDX_OK) r = flush_reduce(c, true, &pend);
                 hipError_t ec = hipStreamEndCapture(c->stream, &g);
                 c->ucur = u0;
                 if (r != PDX_OK) return r;
@@ -1452,7 +1470,9 @@ extern "C" int pdx_dr_solve(pdx_ctx *c, const pdx_dr_params *p, pdx_dr_result *r
             launched += chunk;
         } else {
             const int64_t m = std::min<int64_t>(left, chunk);
-            for (int64_t t = 0; t < m; ++t) PDX_TRY(launch_iter(c, method, p->dt, p->viscous_damping, true));
+            int pend = -1;
+            for (int64_t t = 0; t < m; ++t) PDX_TRY(launch_iter(c, method, p->dt, p->viscous_damping, true, &pend));
+            PDX_TRY(flush_reduce(c, true, &pend));
             launched += m;
         }
         PDX_TRY(read_sc(c));

@@ -60,7 +60,9 @@ void pdx_set_error(const std::string &msg);
 int pdx_launch_count(pdx_ctx *c, int32_t *d_counts);
 int pdx_launch_fill(pdx_ctx *c);
 int pdx_launch_shape(pdx_ctx *c, int mode, double tol);
-int pdx_launch_fpass(pdx_ctx *c, int ubuf, bool diag, bool commit, bool solve_mode);
+// red_ubuf >= 0: the adaptive-DR case reduction of the force pass that read buffer red_ubuf
+// runs first (fused into the halo F-pass where it runs)
+int pdx_launch_fpass(pdx_ctx *c, int ubuf, bool diag, bool commit, bool solve_mode, int red_ubuf = -1);
 // force pass followed by the per-case reduction and update (k_case_reduce)
 int pdx_launch_force(pdx_ctx *c, int ubuf, int upd, bool solve_mode, double dt, double visc);
 // the two launches separately (pdx_time_kernels times the force pass alone)

@@ -19,11 +19,13 @@
 // blocks are dispatched round-robin over the 8 XCDs; give every XCD one
 // contiguous range of Morton blocks so the neighbour halo of a block mostly
 // hits the same L2
-__device__ __forceinline__ int logical_block()
+__device__ __forceinline__ int logical_block_n(int nb)
 {
-    const int b = blockIdx.x, nb = gridDim.x;
+    const int b = blockIdx.x;
     return (b % PDX_NXCD) * (nb / PDX_NXCD) + b / PDX_NXCD;
 }
+
+__device__ __forceinline__ int logical_block() { return logical_block_n(gridDim.x); }
 
 __device__ __forceinline__ double wave_sum(double x)
 {
@@ -60,15 +62,13 @@ __device__ void block_partials(const Dev &d, int lb, double (&val)[NV])
 
 // case totals of the wave partials (strided per thread, one 32-byte entry per
 // load, then waves in order: a fixed order, independent of block scheduling)
-// and the per-case update; MODE 0..2 = the force pass's UPD, MODE 3 =
-// residual only.  One block of PDX_RED_THREADS per case.
-#define PDX_RED_THREADS 1024
-template <int MODE>
-__global__ __launch_bounds__(PDX_RED_THREADS) void k_case_reduce(Dev d, int ubuf, int solve_mode)
+// and the per-case update of case cs, by one block of NT threads; MODE 0..2 =
+// the force pass's UPD, MODE 3 = residual only
+template <int MODE, int NT>
+__device__ void case_reduce_block(const Dev &d, int cs, int ubuf, int solve_mode)
 {
     constexpr int NV = MODE == 3 ? 1 : 3;
-    __shared__ double sred[PDX_RED_THREADS / 64][NV];
-    const int cs = blockIdx.x;
+    __shared__ double sred[NT / 64][NV];
     if (solve_mode && d.sc[cs].converged) return;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int b0 = d.case_blk0[cs], b1 = d.case_blk0[cs + 1];
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(PDX_RED_THREADS) void k_case_reduce(Dev d, int ubuf
     double t[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) t[q] = 0.0;
-    for (int64_t e = (int64_t)b0 * PDX_PW + threadIdx.x; e < (int64_t)b1 * PDX_PW; e += PDX_RED_THREADS) {
+    for (int64_t e = (int64_t)b0 * PDX_PW + threadIdx.x; e < (int64_t)b1 * PDX_PW; e += NT) {
         const pdx_d4 v = pp[e];
         t[0] += v.x;
         if constexpr (NV == 3) {
@@ -95,13 +95,21 @@ __global__ __launch_bounds__(PDX_RED_THREADS) void k_case_reduce(Dev d, int ubuf
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         double s = 0.0;
-        for (int w = 0; w < PDX_RED_THREADS / 64; ++w) s += sred[w][q];
+        for (int w = 0; w < NT / 64; ++w) s += sred[w][q];
         tot[q] = s;
     }
     if (MODE == 3)
         d.sc[cs].res = sqrt(tot[0]);
     else
         case_update<MODE>(d.sc[cs], tot, ubuf);
+}
+
+// one block of PDX_RED_THREADS per case
+#define PDX_RED_THREADS 1024
+template <int MODE>
+__global__ __launch_bounds__(PDX_RED_THREADS) void k_case_reduce(Dev d, int ubuf, int solve_mode)
+{
+    case_reduce_block<MODE, PDX_RED_THREADS>(d, blockIdx.x, ubuf, solve_mode);
 }
 
 // ---------------------------------------------------------------------------
@@ -665,13 +673,26 @@ __device__ __forceinline__ void split_sum(pdx_q4 *lds, double (&acc)[NV])
     }
 }
 
-template <int D, int MODEL, int G, int S>
-__global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_h(Dev d, int ubuf, int solve_mode)
+// RED: the grid has ncase more blocks, which run the adaptive-DR case
+// reduction of the force pass before this F-pass (its input buffer red_ubuf)
+// instead of a launch of their own; the F-pass blocks may then read a case's
+// converged flag before or after that update (a converged case's force pass
+// is skipped either way)
+template <int D, int MODEL, int G, int S, bool RED = false>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_h(Dev d, int ubuf, int solve_mode, int red_ubuf)
 {
     extern __shared__ pdx_q4 hlds[];
     using O = HaloOff<0, 0>;
-    const int lb = logical_block();
-    if (solve_mode && d.sc[d.case_of_blk[lb]].converged) return;
+    if constexpr (RED) {
+        if ((int)blockIdx.x >= d.nblk) {
+            case_reduce_block<1, PDX_BLOCK * S>(d, (int)blockIdx.x - d.nblk, red_ubuf, solve_mode);
+            return;
+        }
+    }
+    const int lb = logical_block_n(d.nblk);
+    if (solve_mode &&
+        __hip_atomic_load(&d.sc[d.case_of_blk[lb]].converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        return;
     if (lb * PDX_BLOCK >= d.ni) return;  // grid padding blocks (whole block)
     halo_stage<0, 0, PDX_BLOCK * S>(d, ubuf, lb, hlds);
     const int cap = d.halo_cap;
@@ -990,9 +1011,13 @@ constexpr int halo_group(int v) { return v & 15; }
 constexpr int halo_split(int v) { return (v & PDX_VARIANT_SPLIT4) ? 4 : ((v & PDX_VARIANT_SPLIT2) ? 2 : 1); }
 
 template <int D, int MODEL, int G>
-static void launch_fpass_g(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm)
+static void launch_fpass_g(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm, int red_ubuf)
 {
     dim3 g(c->nblk), b(PDX_BLOCK);
+    // red_ubuf >= 0: the adaptive-DR case reduction of the preceding force
+    // pass goes first, fused into the halo F-pass, else as its own launch
+    if (red_ubuf >= 0 && ((G & PDX_VARIANT_HALO) == 0 || diag || commit))
+        pdx_launch_case_reduce(c, red_ubuf, 1, sm != 0);
     if (commit) {
         hipLaunchKernelGGL((k_fpass<D, MODEL, false, true, 1>), g, b, 0, c->stream, c->d, ubuf, sm);
     } else if (diag) {
@@ -1001,10 +1026,16 @@ static void launch_fpass_g(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm)
         if constexpr ((G & PDX_VARIANT_HALO) != 0) {
             constexpr int GH = halo_group(G), S = halo_split(G);
             const int lds = pdx_halo_lds(c, HaloOff<0, 0>::bytes, S, D * D + D);
-            if (lds)
-                hipLaunchKernelGGL((k_fpass_h<D, MODEL, GH, S>), g, dim3(PDX_BLOCK * S), lds, c->stream, c->d, ubuf, sm);
-            else
+            if (lds && red_ubuf >= 0)
+                hipLaunchKernelGGL((k_fpass_h<D, MODEL, GH, S, true>), dim3(c->nblk + c->ncase), dim3(PDX_BLOCK * S),
+                                   lds, c->stream, c->d, ubuf, sm, red_ubuf);
+            else if (lds)
+                hipLaunchKernelGGL((k_fpass_h<D, MODEL, GH, S>), g, dim3(PDX_BLOCK * S), lds, c->stream, c->d, ubuf, sm,
+                                   -1);
+            else {
+                if (red_ubuf >= 0) pdx_launch_case_reduce(c, red_ubuf, 1, sm != 0);
                 hipLaunchKernelGGL((k_fpass_w<D, MODEL, 2>), g, b, 0, c->stream, c->d, ubuf, sm);
+            }
         } else if constexpr (G > PDX_VARIANT_COOP) {
             hipLaunchKernelGGL((k_fpass_w<D, MODEL, G - PDX_VARIANT_COOP>), g, b, 0, c->stream, c->d, ubuf, sm);
         } else {
@@ -1014,34 +1045,35 @@ static void launch_fpass_g(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm)
 }
 
 template <int D, int MODEL>
-static void launch_fpass_t(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm)
+static void launch_fpass_t(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm, int red)
 {
     switch (c->tune_f) {
 #define PDX_HALO_CASE(v) \
-    case v: launch_fpass_g<D, MODEL, v>(c, ubuf, diag, commit, sm); break;
+    case v: launch_fpass_g<D, MODEL, v>(c, ubuf, diag, commit, sm, red); break;
     PDX_HALO_VARIANTS(PDX_HALO_CASE)
 #undef PDX_HALO_CASE
-    case PDX_VARIANT_COOP | 2: launch_fpass_g<D, MODEL, PDX_VARIANT_COOP | 2>(c, ubuf, diag, commit, sm); break;
-    case PDX_VARIANT_COOP | 1: launch_fpass_g<D, MODEL, PDX_VARIANT_COOP | 1>(c, ubuf, diag, commit, sm); break;
-    case 4: launch_fpass_g<D, MODEL, 4>(c, ubuf, diag, commit, sm); break;
-    case 2: launch_fpass_g<D, MODEL, 2>(c, ubuf, diag, commit, sm); break;
-    default: launch_fpass_g<D, MODEL, 1>(c, ubuf, diag, commit, sm); break;
+    case PDX_VARIANT_COOP | 2: launch_fpass_g<D, MODEL, PDX_VARIANT_COOP | 2>(c, ubuf, diag, commit, sm, red); break;
+    case PDX_VARIANT_COOP | 1: launch_fpass_g<D, MODEL, PDX_VARIANT_COOP | 1>(c, ubuf, diag, commit, sm, red); break;
+    case 4: launch_fpass_g<D, MODEL, 4>(c, ubuf, diag, commit, sm, red); break;
+    case 2: launch_fpass_g<D, MODEL, 2>(c, ubuf, diag, commit, sm, red); break;
+    default: launch_fpass_g<D, MODEL, 1>(c, ubuf, diag, commit, sm, red); break;
     }
 }
 
-int pdx_launch_fpass(pdx_ctx *c, int ubuf, bool diag, bool commit, bool solve_mode)
+int pdx_launch_fpass(pdx_ctx *c, int ubuf, bool diag, bool commit, bool solve_mode, int red_ubuf)
 {
     const int sm = solve_mode ? 1 : 0;
+    const int red = red_ubuf;
     if (c->dim == 3) {
         switch (c->model) {
-        case PDX_MODEL_NEO_HOOKEAN: launch_fpass_t<3, PDX_MODEL_NEO_HOOKEAN>(c, ubuf, diag, commit, sm); break;
-        case PDX_MODEL_J2: launch_fpass_t<3, PDX_MODEL_J2>(c, ubuf, diag, commit, sm); break;
-        default: launch_fpass_t<3, PDX_MODEL_LINEAR_ELASTIC>(c, ubuf, diag, commit, sm); break;
+        case PDX_MODEL_NEO_HOOKEAN: launch_fpass_t<3, PDX_MODEL_NEO_HOOKEAN>(c, ubuf, diag, commit, sm, red); break;
+        case PDX_MODEL_J2: launch_fpass_t<3, PDX_MODEL_J2>(c, ubuf, diag, commit, sm, red); break;
+        default: launch_fpass_t<3, PDX_MODEL_LINEAR_ELASTIC>(c, ubuf, diag, commit, sm, red); break;
         }
     } else {
         switch (c->model) {
-        case PDX_MODEL_NEO_HOOKEAN: launch_fpass_t<2, PDX_MODEL_NEO_HOOKEAN>(c, ubuf, diag, commit, sm); break;
-        default: launch_fpass_t<2, PDX_MODEL_LINEAR_ELASTIC>(c, ubuf, diag, commit, sm); break;
+        case PDX_MODEL_NEO_HOOKEAN: launch_fpass_t<2, PDX_MODEL_NEO_HOOKEAN>(c, ubuf, diag, commit, sm, red); break;
+        default: launch_fpass_t<2, PDX_MODEL_LINEAR_ELASTIC>(c, ubuf, diag, commit, sm, red); break;
         }
     }
     PDX_LAUNCH_CHECK();
