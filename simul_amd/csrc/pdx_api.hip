@@ -69,7 +69,7 @@ static void free_all(pdx_ctx *c)
     dfree(d.mass); dfree(d.lam); dfree(d.V64); dfree(d.K64); dfree(d.Kinv64);
     dfree(d.Fd); dfree(d.Pd); dfree(d.damage); dfree(d.init_nb);
     dfree(d.sl_ptr); dfree(d.bj); dfree(d.case_of_blk); dfree(d.case_blk0);
-    dfree(d.hl_ptr); dfree(d.hl); dfree(d.hs); dfree(d.bl); dfree(d.bk); dfree(d.anchor);
+    dfree(d.hl_ptr); dfree(d.hl); dfree(d.hs); dfree(d.bl); dfree(d.bk); dfree(d.anchor); dfree(d.m64);
     dfree(d.sc); dfree(d.part); dfree(d.grid); dfree(d.case_of_p); dfree(d.perm);
     dfree(d.cell_start); dfree(d.cell_idx); dfree(d.counts);
     if (c->gexec) hipGraphExecDestroy(c->gexec);
@@ -401,6 +401,7 @@ static int get_tensor(pdx_ctx *c, const T *src, double *dst)
 // internal order; the displacement words stay as they are
 static int push_prec(pdx_ctx *c)
 {
+    c->have_m = false;  // volumes and positions enter the halo F-pass's sum_j wx
     HostMirror &m = mirror(c);
     const int D = c->dim;
     std::vector<XRec> hp(c->ni);
@@ -601,7 +602,7 @@ extern "C" int pdx_count_neighbors(pdx_ctx *c, double horizon, int32_t *counts)
 static int build_halo(pdx_ctx *c)
 {
     Dev &d = c->d;
-    dfree(d.hl_ptr); dfree(d.hl); dfree(d.hs); dfree(d.bl); dfree(d.bk); dfree(d.anchor);
+    dfree(d.hl_ptr); dfree(d.hl); dfree(d.hs); dfree(d.bl); dfree(d.bk); dfree(d.anchor); dfree(d.m64);
     d.halo_cap = 0;
     invalidate_graph(c);
     const int nbk = c->ni / PDX_BLOCK, D = c->dim;
@@ -732,6 +733,8 @@ static int build_halo(pdx_ctx *c)
     PDX_TRY(dalloc(c->stream, &d.bl, bl.size()));
     PDX_TRY(dalloc(c->stream, &d.bk, bk.size()));
     PDX_TRY(dalloc(c->stream, &d.anchor, anc.size()));
+    PDX_TRY(dalloc(c->stream, &d.m64, (size_t)c->ni));
+    c->have_m = false;
     PDX_TRY(upload(c, d.hl_ptr, ptr));
     PDX_TRY(upload(c, d.hl, hl));
     PDX_TRY(upload(c, d.hs, hs));
@@ -1388,6 +1391,9 @@ extern "C" int pdx_dr_solve(pdx_ctx *c, const pdx_dr_params *p, pdx_dr_result *r
     if (p->max_iterations < 0) return fail(PDX_EINVAL, "max_iterations < 0");
     if (method == PDX_DR_KINETIC && !(p->dt > 0.0)) return fail(PDX_EINVAL, "kinetic DR needs dt > 0");
     if (method == PDX_DR_ADAPTIVE && !c->have_mass) PDX_TRY(pdx_adr_mass(c, 1.0));
+    // the halo F-pass's per-row sum_j wx for the current bonds, outside any
+    // captured graph (the graphs read it from device memory)
+    if (!c->have_m) PDX_TRY(pdx_build_m64(c));
     int chunk = p->chunk > 0 ? p->chunk : 64;
     if (chunk & 1) chunk += 1;
     // velocities to zero (nosb_solver.py:282); adaptive DR also clears its previous force

@@ -156,6 +156,11 @@ struct Dev {
     uint16_t *bl;
     uint16_t *bk;
     pdx_d4 *anchor;
+    // per row, sum_j w V_j xi of the halo F-pass (wx as the pass rounds it,
+    // FP64): a function of the reference positions, volumes and intact bonds,
+    // kept so that the pass need not sum it (valid while the context's
+    // have_m is set)
+    pdx_d4 *m64;
     int32_t halo_cap;  // LDS slots per block: the largest block's slot count (a multiple of 16)
     int32_t halo_pad;
     int32_t *case_of_blk;
@@ -357,8 +362,9 @@ PDX_HD float bond_weight_live(float L, float invh, float Vj)
 // relative error into every A_j and from there into the residual.  A slot
 // without a live bond passes the particle's own record (xi = 0), so its terms
 // are exactly zero and no select is needed.
-// S[r D + c] += u_j[r] wx_c and S[D D + c] += wx_c (fpass_sum_to_h forms N)
-template <int D>
+// S[r D + c] += u_j[r] wx_c and, with M, S[D D + c] += wx_c (fpass_sum_to_h
+// forms N; without M the caller has sum_j wx from a pass that summed it)
+template <int D, bool M = true>
 PDX_HD void fpass_bond_u(const float *xi, const double *uj, float Vj, float invh, double *S)
 {
     float L2 = 0.f;
@@ -367,7 +373,7 @@ PDX_HD void fpass_bond_u(const float *xi, const double *uj, float Vj, float invh
     for (int c = 0; c < D; ++c) {
         const double wx = (double)(w * xi[c]);
         for (int r = 0; r < D; ++r) S[r * D + c] = fma(uj[r], wx, S[r * D + c]);
-        S[D * D + c] += wx;
+        if (M) S[D * D + c] += wx;
     }
 }
 

@@ -24,6 +24,7 @@ struct pdx_ctx {
     std::vector<double> hX;            // caller-order reference positions
     bool have_particles = false, have_bonds = false, have_K = false, have_mass = false;
     bool have_A = false;
+    bool have_m = false;  // d.m64 holds the halo F-pass's sum_j wx for the current bonds
     int32_t model = PDX_MODEL_LINEAR_ELASTIC, form = PDX_FORM_NOSB;
     double horizon = 0.0, G_s = 0.0;
     int64_t nb = 0, slots = 0;
@@ -68,6 +69,8 @@ int pdx_launch_force(pdx_ctx *c, int ubuf, int upd, bool solve_mode, double dt, 
 // the two launches separately (pdx_time_kernels times the force pass alone)
 int pdx_launch_force_pass(pdx_ctx *c, int ubuf, int upd, bool solve_mode, double dt, double visc);
 int pdx_launch_case_reduce(pdx_ctx *c, int ubuf, int upd, bool solve_mode);
+// d.m64 from the halo F-pass's own sums (sets have_m; no-op without a halo)
+int pdx_build_m64(pdx_ctx *c);
 int pdx_launch_kd_move(pdx_ctx *c, int ubuf_in, bool solve_mode, double dt, bool apply_reset);
 int pdx_launch_halo_bits(pdx_ctx *c);
 int pdx_halo_lds(const pdx_ctx *c, int entry_bytes, int split, int nv);

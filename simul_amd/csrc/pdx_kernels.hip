@@ -677,8 +677,9 @@ __device__ __forceinline__ void split_sum(pdx_q4 *lds, double (&acc)[NV])
 // reduction of the force pass before this F-pass (its input buffer red_ubuf)
 // instead of a launch of their own; the F-pass blocks may then read a case's
 // converged flag before or after that update (a converged case's force pass
-// is skipped either way)
-template <int D, int MODEL, int G, int S, bool RED = false>
+// is skipped either way).  MM 0: sum_j wx summed here; 1: read from d.m64;
+// 2: summed here and written to d.m64, nothing else (pdx_build_m64)
+template <int D, int MODEL, int G, int S, bool RED = false, int MM = 0>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_h(Dev d, int ubuf, int solve_mode, int red_ubuf)
 {
     extern __shared__ pdx_q4 hlds[];
@@ -713,9 +714,10 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_h(Dev d, int ubuf, int 
     // rounding of wx is shared by the two sums, so their difference is the
     // FP64 sum of (u_j - u_i) (x) wx without rounding u_j - u_i (Sd[D * D + c]
     // holds sum_j wx_c)
-    double Sd[D * D + D];
+    constexpr int NS = MM == 1 ? D * D : D * D + D;
+    double Sd[NS];
 #pragma unroll
-    for (int q = 0; q < D * D + D; ++q) Sd[q] = 0.0;
+    for (int q = 0; q < NS; ++q) Sd[q] = 0.0;
     const int64_t s0 = d.sl_ptr[i >> 6];
     const int width = (int)((d.sl_ptr[(i >> 6) + 1] - s0) / PDX_SLICE);
     const uint16_t *__restrict__ bp = d.bk + s0 + lane;
@@ -733,16 +735,28 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_h(Dev d, int ubuf, int 
             const float xi[3] = {xj[q].x - xo.x, xj[q].y - xo.y, xj[q].z - xo.z};
             const PdxD2 a = __builtin_bit_cast(PdxD2, uj01[q]);
             const double uj[3] = {a.a, a.b, __builtin_bit_cast(double, uj2[q])};
-            fpass_bond_u<D>(xi, uj, xj[q].w, invh, Sd);
+            fpass_bond_u<D, MM != 1>(xi, uj, xj[q].w, invh, Sd);
         }
     };
     halo_row_loop<G, S>(bp, width, part, group);
-    split_sum<D * D + D, S>(hlds, Sd);
+    pdx_d4 mi{};
+    if (MM == 1 && part == 0 && act) mi = d.m64[i];
+    split_sum<NS, S>(hlds, Sd);
     if (part == 0 && act) {
-        const double ui[3] = {uo01.a, uo01.b, uo2};
-        double Hd[D * D];
-        fpass_sum_to_h<D>(Sd, ui, Hd);
-        fpass_finish<D, MODEL, false, false>(d, i, kr, Hd);
+        if constexpr (MM == 2) {
+            d.m64[i] = d4(Sd[NS - D], Sd[NS - D + 1], D == 3 ? Sd[NS - 1] : 0.0);
+        } else {
+            double Sf[D * D + D];
+#pragma unroll
+            for (int q = 0; q < D * D; ++q) Sf[q] = Sd[q];
+            const double mv[3] = {mi.x, mi.y, mi.z};
+#pragma unroll
+            for (int c = 0; c < D; ++c) Sf[D * D + c] = MM == 1 ? mv[c] : Sd[(D * D + c) % NS];
+            const double ui[3] = {uo01.a, uo01.b, uo2};
+            double Hd[D * D];
+            fpass_sum_to_h<D>(Sf, ui, Hd);
+            fpass_finish<D, MODEL, false, false>(d, i, kr, Hd);
+        }
     }
 }
 
@@ -1026,12 +1040,17 @@ static void launch_fpass_g(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm,
         if constexpr ((G & PDX_VARIANT_HALO) != 0) {
             constexpr int GH = halo_group(G), S = halo_split(G);
             const int lds = pdx_halo_lds(c, HaloOff<0, 0>::bytes, S, D * D + D);
-            if (lds && red_ubuf >= 0)
-                hipLaunchKernelGGL((k_fpass_h<D, MODEL, GH, S, true>), dim3(c->nblk + c->ncase), dim3(PDX_BLOCK * S),
-                                   lds, c->stream, c->d, ubuf, sm, red_ubuf);
+            const dim3 bh(PDX_BLOCK * S), gr(c->nblk + c->ncase);
+            if (lds && red_ubuf >= 0 && c->have_m)
+                hipLaunchKernelGGL((k_fpass_h<D, MODEL, GH, S, true, 1>), gr, bh, lds, c->stream, c->d, ubuf, sm,
+                                   red_ubuf);
+            else if (lds && red_ubuf >= 0)
+                hipLaunchKernelGGL((k_fpass_h<D, MODEL, GH, S, true>), gr, bh, lds, c->stream, c->d, ubuf, sm,
+                                   red_ubuf);
+            else if (lds && c->have_m)
+                hipLaunchKernelGGL((k_fpass_h<D, MODEL, GH, S, false, 1>), g, bh, lds, c->stream, c->d, ubuf, sm, -1);
             else if (lds)
-                hipLaunchKernelGGL((k_fpass_h<D, MODEL, GH, S>), g, dim3(PDX_BLOCK * S), lds, c->stream, c->d, ubuf, sm,
-                                   -1);
+                hipLaunchKernelGGL((k_fpass_h<D, MODEL, GH, S>), g, bh, lds, c->stream, c->d, ubuf, sm, -1);
             else {
                 if (red_ubuf >= 0) pdx_launch_case_reduce(c, red_ubuf, 1, sm != 0);
                 hipLaunchKernelGGL((k_fpass_w<D, MODEL, 2>), g, b, 0, c->stream, c->d, ubuf, sm);
@@ -1078,6 +1097,30 @@ int pdx_launch_fpass(pdx_ctx *c, int ubuf, bool diag, bool commit, bool solve_mo
     }
     PDX_LAUNCH_CHECK();
     return PDX_OK;
+}
+
+// d.m64 by the halo F-pass of the tuned split (the bond group only orders the
+// loads), as the pass would sum it; its model does not enter the sum
+template <int D, int S>
+static int build_m64_s(pdx_ctx *c)
+{
+    const int lds = pdx_halo_lds(c, HaloOff<0, 0>::bytes, S, D * D + D);
+    if (!lds) return PDX_OK;
+    hipLaunchKernelGGL((k_fpass_h<D, PDX_MODEL_LINEAR_ELASTIC, 1, S, false, 2>), dim3(c->nblk), dim3(PDX_BLOCK * S),
+                       lds, c->stream, c->d, c->ucur, 0, -1);
+    PDX_LAUNCH_CHECK();
+    c->have_m = true;
+    return PDX_OK;
+}
+
+int pdx_build_m64(pdx_ctx *c)
+{
+    c->have_m = false;
+    if (!c->d.m64 || !c->d.hl_ptr || c->form != PDX_FORM_NOSB) return PDX_OK;
+    const int split = (c->tune_f & PDX_VARIANT_HALO) ? halo_split(c->tune_f) : 4;
+    if (c->dim == 3)
+        return split == 4 ? build_m64_s<3, 4>(c) : split == 2 ? build_m64_s<3, 2>(c) : build_m64_s<3, 1>(c);
+    return split == 4 ? build_m64_s<2, 4>(c) : split == 2 ? build_m64_s<2, 2>(c) : build_m64_s<2, 1>(c);
 }
 
 template <int D, int FORM, int G>
@@ -1153,6 +1196,7 @@ int pdx_launch_force_pass(pdx_ctx *c, int ubuf, int upd, bool solve_mode, double
 
 int pdx_launch_halo_bits(pdx_ctx *c)
 {
+    c->have_m = false;  // the intact bonds change
     if (!c->d.bl || c->slots <= 0) return PDX_OK;
     hipLaunchKernelGGL(k_halo_bits, dim3((unsigned)((c->ni + PDX_BLOCK - 1) / PDX_BLOCK)), dim3(PDX_BLOCK), 0,
                        c->stream, c->d);
@@ -1201,6 +1245,7 @@ int pdx_launch_dt_est(pdx_ctx *c, double modulus, double *d_out)
 
 int pdx_launch_damage(pdx_ctx *c, double s_c, unsigned long long *d_count)
 {
+    c->have_m = false;  // bonds may break
     dim3 g(grid1(c->ni)), b(PDX_BLOCK);
     if (c->dim == 3)
         hipLaunchKernelGGL((k_damage<3>), g, b, 0, c->stream, c->d, c->ucur, s_c, d_count);
