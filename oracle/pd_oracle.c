@@ -431,11 +431,10 @@ static void stress_j2(const double *F, double Kb, double mu, double sy0, double 
         beta_t[m] = beta[m] + sqrt(2.0 / 3.0) * Hk * dg * nflow;
     }
     *epbar_t = epbar + dg;
-    for (int m = 0; m < 9; ++m) {
-        double sc = beta[m] + fac * eta[m];
-        if (Hk != 0.0) sc = beta_t[m] + fac * eta[m];
-        P[m] = sc + ((m % 4) == 0 ? pr : 0.0);
-    }
+    /* s = s_trial - 2 mu d(eps_p) = beta_n + fac (s_trial - beta_n): the deviator
+     * stays 2 mu dev(eps - eps_p) whatever the hardening; the back stress moves
+     * the yield surface, not the elastic law */
+    for (int m = 0; m < 9; ++m) P[m] = beta[m] + fac * eta[m] + ((m % 4) == 0 ? pr : 0.0);
 }
 
 void orc_stress(orc_pd *p)

@@ -148,7 +148,7 @@ class Problem:
         self.bulk, self.shear = full(bulk), full(shear)
         if cstab is None:
             cstab = 18.0 * self.bulk / (np.pi * horizon ** 4) if dim == 3 else \
-                9.0 * (9.0 * self.bulk * self.shear / (3 * self.bulk + self.shear)) / (np.pi * horizon ** 3)
+                9.0 * (4.0 * self.bulk * self.shear / (self.bulk + self.shear)) / (np.pi * horizon ** 3)
         self.cstab = full(cstab)
         self.model = MODELS[model] if isinstance(model, str) else int(model)
         self.formulation = {"nosb": 0, "bond": 1}[formulation] if isinstance(formulation, str) else int(formulation)

@@ -59,9 +59,15 @@ class PDBody:
         self.ctx.call("pdx_set_particles", N.ptr(X, N._dp), N.ptr(vol, N._dp), N.ptr(mass, N._dp))
         self.set_materials(model, bulk, shear, yield_stress, h_iso, h_kin)
         if cstab is None:
+            # NOSBMaterial.set_stabilization_modulus (nosb.py:335-346): 18 K / (pi d^4)
+            # in 3-D, 9 E / (pi t d^3) in 2-D (t = 1), with the plane-stress
+            # E = 4 K mu / (K + mu) recovered from K = E / (2 (1 - nu)), mu = E / (2 (1 + nu))
             b = _f64(bulk, (n,))
-            cstab = 18.0 * b / (np.pi * self.horizon ** 4) if dim == 3 else \
-                9.0 * (9.0 * b * _f64(shear, (n,)) / (3 * b + _f64(shear, (n,)))) / (np.pi * self.horizon ** 3)
+            if dim == 3:
+                cstab = 18.0 * b / (np.pi * self.horizon ** 4)
+            else:
+                s = _f64(shear, (n,))
+                cstab = 9.0 * (4.0 * b * s / (b + s)) / (np.pi * self.horizon ** 3)
         self.set_stabilization(G_s, cstab)
         self.ctx.call("pdx_set_formulation", FORMS[formulation])
         self.formulation = formulation

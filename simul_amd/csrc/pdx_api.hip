@@ -1525,6 +1525,7 @@ extern "C" int pdx_time_kernels(pdx_ctx *c, int32_t iters, double *ms)
     PDX_TRY(need_K(c));
     if (iters <= 0) return fail(PDX_EINVAL, "iters must be positive");
     if (!c->have_mass) PDX_TRY(pdx_adr_mass(c, 1.0));
+    if (!c->have_m) PDX_TRY(pdx_build_m64(c));  // time the F-pass the solver runs
     PDX_TRY(read_sc(c));
     for (int k = 0; k < c->ncase; ++k) {
         c->h_sc[k].converged = 0;
@@ -1579,6 +1580,7 @@ extern "C" int pdx_set_group(pdx_ctx *c, int32_t g)
 {
     CHECK_CTX(c);
     if (!valid_variant(g)) return fail(PDX_EINVAL, k_variant_msg);
+    if (g != c->tune_f) c->have_m = false;  // m64 is summed in the F-pass variant's order
     c->tune_f = c->tune_g = g;
     invalidate_graph(c);
     return PDX_OK;
@@ -1588,6 +1590,7 @@ extern "C" int pdx_set_variants(pdx_ctx *c, int32_t fpass, int32_t force)
 {
     CHECK_CTX(c);
     if (!valid_variant(fpass) || !valid_variant(force)) return fail(PDX_EINVAL, k_variant_msg);
+    if (fpass != c->tune_f) c->have_m = false;  // m64 is summed in the F-pass variant's order
     c->tune_f = fpass;
     c->tune_g = force;
     invalidate_graph(c);
@@ -1627,6 +1630,7 @@ extern "C" int pdx_autotune(pdx_ctx *c, int32_t iters, int32_t *group_out)
 #undef PDX_HALO_CAND
     for (int g : cand) {
         c->tune_f = c->tune_g = g;
+        c->have_m = false;  // each candidate times its own m64 (pdx_time_kernels builds it)
         double ms[3];
         int r = pdx_time_kernels(c, iters, ms);
         if (r != PDX_OK) {
@@ -1645,6 +1649,7 @@ extern "C" int pdx_autotune(pdx_ctx *c, int32_t iters, int32_t *group_out)
     }
     c->tune_f = best_f;
     c->tune_g = best_g;
+    c->have_m = false;  // the last candidate's m64: rebuilt for best_f before the next solve
     c->ucur = u0;
     PDX_HIP(hipMemcpyAsync(c->d.xu[c->ucur], save_x, nx, hipMemcpyDeviceToDevice, c->stream));
     PDX_HIP(hipMemcpyAsync(c->d.v, save, nv, hipMemcpyDeviceToDevice, c->stream));

@@ -304,7 +304,8 @@ PDX_HD void stress_j2(const double *H, double Kb_, double mu_, const J2Par &pp, 
             ns.ep[q] = (float)((double)st.ep[q] + sqrt(1.5) * dg * nf);
             const double bn = (double)st.beta[q] + sqrt(2.0 / 3.0) * (double)pp.hkin * dg * nf;
             ns.beta[q] = (float)bn;
-            sfin[q] = bn + fac * eta[q] + (q < 3 ? pr : 0.0);
+            // the deviator stays 2 mu dev(eps - eps_p): beta_n + fac (s_trial - beta_n)
+            sfin[q] = (double)st.beta[q] + fac * eta[q] + (q < 3 ? pr : 0.0);
         }
         ns.epbar = (float)(epbar + dg);
         if (out) *out = ns;

@@ -4,7 +4,7 @@
 // per particle, wave64).  Hot loop per dynamic-relaxation iteration:
 //   F-pass (k_fpass_h; gathering variants k_fpass, k_fpass_w):
 //              H = [sum_j w V_j du (x) xi] K^-1 (FP32 terms, FP64 sums), P(H)
-//              (LE/NH in FP32, J2 radial return in FP64), A = P K^-1
+//              (LE, NH and the J2 radial return in FP64), A = P K^-1 stored in FP32
 //   force pass (k_force_h; k_force, k_force_w):
 //              f_i = A_i m_i + sum_j w V_j [A_j xi + G_s c_ij s eta/|eta|]
 //              (FP32 bond terms, FP64 per-particle sums), + external load,
@@ -314,17 +314,6 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_force_w(Dev d, int ubuf, int solv
         // A records have 3 quarters; lanes of quarter 3 fetch quarter 2 again
         // (same cache line, keeps the loads unconditional)
         const int qa3 = qa < 3 ? qa : 2;
-#ifdef PDX_A_FLAT
-        // 64 bonds x 3 quarters in 3 full-wave loads: load h of lane L
-        // fetches quarter t % 3 of bond t / 3, t = 64 h + L
-        int ab[3], aq[3];
-#pragma unroll
-        for (int h = 0; h < 3; ++h) {
-            const int t = 64 * h + lane;
-            ab[h] = t / 3;
-            aq[h] = t - 3 * ab[h];
-        }
-#endif
         uint32_t ec[G], en[G];
 #pragma unroll
         for (int q = 0; q < G; ++q) ec[q] = q < width ? bp[(int64_t)q * PDX_SLICE] : PDX_BROKEN;
@@ -335,36 +324,20 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_force_w(Dev d, int ubuf, int solv
             WSlot sl[G];
             int jb[G][4];
             pdx_q4 g[G][4];
-#ifdef PDX_A_FLAT
-            int ja[G][3];
-            pdx_q4 ga[G][3];
-#else
             pdx_q4 ga[G][4];
-#endif
 #pragma unroll
             for (int q = 0; q < G; ++q) {
                 sl[q] = wslot(d, ec[q], k + q, nnb, act, i);
 #pragma unroll
                 for (int h = 0; h < 4; ++h) jb[q][h] = __shfl(sl[q].j, 16 * h + (lane >> 2), 64);
-#ifdef PDX_A_FLAT
-                if (FORM == 0)
-#pragma unroll
-                    for (int h = 0; h < 3; ++h) ja[q][h] = __shfl(sl[q].j, ab[h], 64);
-#endif
             }
 #pragma unroll
             for (int q = 0; q < G; ++q) {
 #pragma unroll
                 for (int h = 0; h < 4; ++h) g[q][h] = Xq[(int64_t)jb[q][h] * 4 + qa];
-#ifdef PDX_A_FLAT
-                if (FORM == 0)
-#pragma unroll
-                    for (int h = 0; h < 3; ++h) ga[q][h] = Aq[(int64_t)ja[q][h] * 3 + aq[h]];
-#else
                 if (FORM == 0)
 #pragma unroll
                     for (int h = 0; h < 4; ++h) ga[q][h] = Aq[(int64_t)jb[q][h] * 3 + qa3];
-#endif
             }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -373,15 +346,8 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_force_w(Dev d, int ubuf, int solv
                 for (int h = 0; h < 4; ++h) {
                     const int b = 16 * h + (lane >> 2);
                     tw[xq_slot(b, qa)] = g[q][h];
-#ifndef PDX_A_FLAT
                     if (FORM == 0) ta[xq_slot(b, qa)] = ga[q][h];
-#endif
                 }
-#ifdef PDX_A_FLAT
-                if (FORM == 0)
-#pragma unroll
-                    for (int h = 0; h < 3; ++h) ta[xq_slot(ab[h], aq[h])] = ga[q][h];
-#endif
                 wave_lds_sync();
                 pdx_q4 rq[4], ra[3];
                 float aj[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -1099,15 +1065,19 @@ int pdx_launch_fpass(pdx_ctx *c, int ubuf, bool diag, bool commit, bool solve_mo
     return PDX_OK;
 }
 
-// d.m64 by the halo F-pass of the tuned split (the bond group only orders the
-// loads), as the pass would sum it; its model does not enter the sum
-template <int D, int S>
-static int build_m64_s(pdx_ctx *c)
+// d.m64 by the halo F-pass of the tuned variant, summed in the order that
+// variant sums it: with rows split over S waves, part p takes the slot groups
+// p G, p G + G S, ..., so the FP64 order depends on the bond group G as well
+// as on S.  Its model does not enter the sum.  m64 is valid only for the
+// variant it was built with: every change of the F-pass variant clears have_m.
+template <int D, int V>
+static int build_m64_v(pdx_ctx *c)
 {
+    constexpr int GH = halo_group(V), S = halo_split(V);
     const int lds = pdx_halo_lds(c, HaloOff<0, 0>::bytes, S, D * D + D);
     if (!lds) return PDX_OK;
-    hipLaunchKernelGGL((k_fpass_h<D, PDX_MODEL_LINEAR_ELASTIC, 1, S, false, 2>), dim3(c->nblk), dim3(PDX_BLOCK * S),
-                       lds, c->stream, c->d, c->ucur, 0, -1);
+    hipLaunchKernelGGL((k_fpass_h<D, PDX_MODEL_LINEAR_ELASTIC, GH, S, false, 2>), dim3(c->nblk),
+                       dim3(PDX_BLOCK * S), lds, c->stream, c->d, c->ucur, 0, -1);
     PDX_LAUNCH_CHECK();
     c->have_m = true;
     return PDX_OK;
@@ -1116,11 +1086,13 @@ static int build_m64_s(pdx_ctx *c)
 int pdx_build_m64(pdx_ctx *c)
 {
     c->have_m = false;
-    if (!c->d.m64 || !c->d.hl_ptr || c->form != PDX_FORM_NOSB) return PDX_OK;
-    const int split = (c->tune_f & PDX_VARIANT_HALO) ? halo_split(c->tune_f) : 4;
-    if (c->dim == 3)
-        return split == 4 ? build_m64_s<3, 4>(c) : split == 2 ? build_m64_s<3, 2>(c) : build_m64_s<3, 1>(c);
-    return split == 4 ? build_m64_s<2, 4>(c) : split == 2 ? build_m64_s<2, 2>(c) : build_m64_s<2, 1>(c);
+    // only the halo F-pass reads m64
+    if (!c->d.m64 || !c->d.hl_ptr || c->form != PDX_FORM_NOSB || !(c->tune_f & PDX_VARIANT_HALO)) return PDX_OK;
+#define PDX_M64_CASE(v)                                                          \
+    if (c->tune_f == (v)) return c->dim == 3 ? build_m64_v<3, (v)>(c) : build_m64_v<2, (v)>(c);
+    PDX_HALO_VARIANTS(PDX_M64_CASE)
+#undef PDX_M64_CASE
+    return PDX_OK;
 }
 
 template <int D, int FORM, int G>

@@ -1,7 +1,7 @@
 """Training data, training and evaluation of the learned warm start.
 
     python -m simul_amd.learned.train --train 1024 --heldout 64 --dx 0.8 \
-        --out simul_amd/learned/weights_vertebra.npz --report gpurun_out/warmstart_report.json
+        --out simul_amd/learned/weights_vertebra.npz --report warmstart_report.json
 
 Data: random vertebral bodies (simul_amd.workloads.random_vertebra: size,
 cancellous field, pressure magnitude and direction within 20 deg), solved by

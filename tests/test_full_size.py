@@ -1,0 +1,119 @@
+"""Solve parity at the benchmark workloads' full sizes.
+
+* The dx 0.4 mm Neo-Hookean vertebral body (365,056 particles, 42,517,012
+  bonds): the GPU solve to 1e-6 of the cold-start residual, with the library's
+  default hot-pass variants and with the variants the bench tunes to, against
+  the FP64 oracle: the oracle's own residual at the GPU's displacement, and the
+  oracle's adaptive DR continued from that displacement to 1e-7 (its own
+  solution, ten times tighter than the GPU's tolerance) within 1e-5
+  relative-L2 of the GPU's.
+* The 64-case batch (7,045,942 particles in one context): every case reaches
+  1e-6, three cases solved alone with the batch's per-case DR mass give the
+  batch's iterations and bits, and the three smallest cases solved by the
+  oracle from the cold start (same mass, same criterion) agree to 1e-5.
+
+The oracle runs on the host cores (OpenMP) as the checker.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from helpers import rel_l2
+from oracle import pdoracle as po
+from simul_amd import workloads as W
+from simul_amd.body import from_specimen
+
+pytestmark = pytest.mark.gpu
+
+# PDX_VARIANT_HALO | group, | PDX_VARIANT_SPLIT2 (64) / SPLIT4 (128)
+DEFAULT_VARIANTS = (32 | 128 | 1, 32 | 128 | 4)
+BENCH_VARIANTS = (32 | 64 | 4, 32 | 128 | 4)
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _oracle_problem(spec):
+    p = po.Problem(spec.X, spec.vol, spec.horizon, spec.bulk, spec.shear, mass=spec.vol * spec.rho,
+                   model=spec.model, fixed=spec.fixed, fext=spec.fext)
+    p.shape_tensor("relative", 1e-12)
+    return p
+
+
+def test_config2_vertebra_solve_matches_fp64_oracle(gpu):
+    t0 = time.time()
+    spec = W.vertebra(0.4)
+    body = from_specimen(spec)
+    assert body.n_bonds == 42517012
+    cold = body.cold_residual()
+    sols = {}
+    for fv, gv in (DEFAULT_VARIANTS, BENCH_VARIANTS):
+        body.set_variants(fv, gv)
+        body.u = spec.u0
+        r = body.solve("adaptive", tol=1e-6, max_iterations=20000, ref=cold, mass="spectral")
+        assert r.converged and r.relative_residual < 1e-6, (fv, gv, r)
+        sols[(fv, gv)] = (body.u.copy(), r.iterations)
+        _say(f"config2 GPU solve, variants {fv}/{gv}: {r.iterations} iterations")
+    lam = body.get("drmass")
+    prob = _oracle_problem(spec)
+    prob.set_u(spec.u0)
+    prob.compute_forces()
+    cold_o = prob.residual_norm()
+    assert abs(cold_o - cold) < 1e-5 * cold
+    for key, (u, _) in sols.items():
+        prob.set_u(u)
+        prob.compute_forces()
+        rel = prob.residual_norm() / cold_o
+        _say(f"config2 oracle FP64 residual at the GPU solution {key}: {rel:.3e}")
+        assert rel < 2e-6, (key, rel)
+    u_gpu = sols[DEFAULT_VARIANTS][0]
+    prob.set_u(u_gpu)
+    ro = prob.adr_solve(lam, max_it=3000, tol=1e-7, ref=cold_o)
+    _say(f"config2 oracle DR from the GPU solution to 1e-7: {ro['iterations']} iterations, "
+         f"{time.time() - t0:.1f}s so far")
+    assert ro["converged"], ro
+    for key, (u, _) in sols.items():
+        err = rel_l2(u, prob.u)
+        _say(f"config2 rel-L2(u_gpu {key}, u_oracle) = {err:.3e}")
+        assert err < 1e-5, (key, err)
+    a, b = sols[DEFAULT_VARIANTS][0], sols[BENCH_VARIANTS][0]
+    assert rel_l2(a, b) < 1e-5
+
+
+def test_config4_batch_converges_matches_single_cases_and_oracle(gpu):
+    t0 = time.time()
+    specs = W.batch()
+    packed, off = W.pack(specs)
+    assert off[-1] == 7045942
+    body = from_specimen(packed, case_offsets=off)
+    _say(f"config4 setup {time.time() - t0:.1f}s, {body.n_bonds} bonds")
+    cold = body.cold_residual()
+    rs = body.solve("adaptive", tol=1e-6, max_iterations=20000, refs=cold, mass="spectral")
+    its = np.array([r.iterations for r in rs])
+    _say(f"config4 batch solve: {its.min()}..{its.max()} iterations, {time.time() - t0:.1f}s so far")
+    assert all(r.converged and r.relative_residual < 1e-6 for r in rs)
+    lam = body.get("drmass")
+    ub = body.u
+    sizes = np.array([s.n for s in specs])
+    smallest = [int(k) for k in np.argsort(sizes)[:3]]
+    for k in smallest:
+        s = specs[k]
+        sl = slice(off[k], off[k + 1])
+        single = from_specimen(s)
+        single.set("drmass", lam[sl])
+        r = single.solve("adaptive", tol=1e-6, max_iterations=20000, ref=float(cold[k]))
+        assert r.iterations == rs[k].iterations, (k, r.iterations, rs[k].iterations)
+        assert np.array_equal(single.u, ub[sl]), k
+        p = _oracle_problem(s)
+        p.set_u(s.u0)
+        p.compute_forces()
+        cold_o = p.residual_norm()
+        assert abs(cold_o - cold[k]) < 1e-5 * cold[k]
+        ro = p.adr_solve(lam[sl], max_it=20000, tol=1e-6, ref=cold_o)
+        assert ro["converged"], (k, ro)
+        err = rel_l2(ub[sl], p.u)
+        _say(f"config4 case {k} ({s.n} particles): GPU {rs[k].iterations} / oracle {ro['iterations']} "
+             f"iterations, rel-L2 {err:.3e}")
+        assert err < 1e-5, (k, err)

@@ -185,13 +185,15 @@ def test_damage_flags_bitexact(gpu):
     assert np.array_equal(body.local_damage(), prob.local_damage())
 
 
-def test_j2_stress_and_commit(gpu):
+@pytest.mark.parametrize("h_kin", [None, 0.08])
+def test_j2_stress_and_commit(gpu, h_kin):
     X = lattice(7, 7, 7, jitter=0.05, seed=1)
     n = len(X)
     E = 1000.0
     sy = np.full(n, 2.0)
     hi = np.full(n, 0.05 * E)
-    body, prob = pair(X, 3.015, E=E, model="j2_plasticity", yield_stress=sy, h_iso=hi)
+    hk = None if h_kin is None else np.full(n, h_kin * E)
+    body, prob = pair(X, 3.015, E=E, model="j2_plasticity", yield_stress=sy, h_iso=hi, h_kin=hk)
     G = np.diag([-0.004, 0.001, 0.0015]) + 1e-3 * np.random.default_rng(2).uniform(-1, 1, (3, 3))
     u = X @ G.T + smooth_u(X, amp=5e-4, seed=3)
     body.u = u
@@ -205,6 +207,22 @@ def test_j2_stress_and_commit(gpu):
     prob.commit()
     assert rel_l2(body.get("epbar"), prob.epbar) < 1e-4
     assert rel_l2(body.get("ep").reshape(-1, 9), prob.ep) < 1e-4
+    if hk is None:
+        return
+    assert rel_l2(body.get("back").reshape(-1, 9), prob.beta) < 1e-4
+    # reverse loading from the committed state: the yield test and the stress see
+    # the back stress and the plastic strain of the first step
+    u2 = -0.6 * u
+    body.u = u2
+    prob.set_u(u2)
+    body.compute_F_stress()
+    prob.deformation_gradient()
+    prob.stress()
+    assert prob.epbar_t.max() > prob.epbar.max()
+    assert rel_l2(body.get("P"), prob.P) < 2e-5
+    body.commit_state()
+    prob.commit()
+    assert rel_l2(body.get("back").reshape(-1, 9), prob.beta) < 1e-4
 
 
 def test_batch_matches_single_cases(gpu):
@@ -362,6 +380,28 @@ def test_halo_variants_match_gathering(gpu, formulation):
         assert rel_l2(body.u - u1, ref["uk"] - u1) < 1e-5, (g, rel_l2(body.u - u1, ref["uk"] - u1))
 
 
+def test_m64_reading_fpass_equals_summing_fpass(gpu):
+    """The halo F-pass either sums each row's FP64 m = sum_j w V_j xi itself or
+    reads it from the per-row array a solve builds first (in the order the
+    F-pass variant sums it).  For every halo variant the two paths give the
+    same F and the same forces bit for bit, also after a change of variant
+    (which invalidates the array), with broken bonds and batch padding."""
+    X, body = _halo_body("nosb")
+    body.u = smooth_u(X, amp=0.02, seed=7)
+    assert body.break_bonds(0.02) > 0
+    u1 = smooth_u(X, amp=2e-3, seed=8)
+    for g in HALO + (32 | 1, 32 | 128 | 4):
+        body.set_group(g)
+        body.u = u1
+        f_sum = body.compute_forces().copy()       # no per-row array yet: summed in the pass
+        F_sum = body.get("F").copy()
+        body.solve("adaptive", tol=1e-6, max_iterations=0)   # builds the per-row array
+        body.u = u1
+        f_read = body.compute_forces().copy()
+        assert np.array_equal(body.get("F"), F_sum), g
+        assert np.array_equal(f_read, f_sum), g
+
+
 def test_sliced_solve_matches_single(gpu):
     """Resumed slices of the device DR loop follow the same trajectory: same
     iteration counts and the same bits as one uninterrupted solve."""
@@ -411,6 +451,55 @@ def test_halo_variants_2d_and_lds_fallback(gpu):
                 body.set_group(g_)
                 f = body.compute_forces()
                 assert rel_l2(f, f0) < 2e-6, (dim, h, form, g_, rel_l2(f, f0))
+
+
+@pytest.mark.parametrize("model", ["linear_elastic", "neo_hookean"])
+def test_2d_forces_and_dr_match_oracle(gpu, model):
+    """A 2-D plane-stress plate against the oracle: the default stabilisation
+    modulus 9 E / (pi t d^3) (nosb.py:335-346, t = 1), NOSB and bond forces at
+    the force tolerance of the 3-D tests, and the adaptive-DR solution of a
+    compressed plate to 1e-5 relative-L2."""
+    g = np.arange(26.0)
+    X = np.stack(np.meshgrid(g, g[:18], indexing="ij"), -1).reshape(-1, 2)
+    X = X + np.random.default_rng(21).uniform(-0.15, 0.15, X.shape)
+    n = len(X)
+    E, nu, h = 15000.0, 0.3, 3.015
+    K2, mu = E / (2 * (1 - nu)), E / (2 * (1 + nu))
+    mass = np.full(n, W.RHO_BONE)
+    fixed = np.zeros(n, np.uint8)
+    bot, top = X[:, 1] < 0.5, X[:, 1] > 16.5
+    fixed[bot] = 3
+    fixed[top] = 3
+    u0 = np.zeros_like(X)
+    u0[top, 1] = -0.004 * 17.0
+    body = PDBody(X, 1.0, h, K2, mu, mass=mass, model=model, fixed=fixed)
+    prob = po.Problem(X, 1.0, h, K2, mu, mass=mass, model=model, fixed=fixed)
+    prob.shape_tensor("relative", 1e-12)
+    c_ref = 9.0 * E / (np.pi * h ** 3)
+    assert np.allclose(prob.cstab, c_ref, rtol=1e-12)
+    assert np.allclose(body.get("cstab"), c_ref, rtol=1e-6)
+    u = smooth_u(X, amp=2e-3, seed=5)
+    body.u = u
+    prob.set_u(u)
+    f = body.compute_forces()
+    prob.compute_forces()
+    assert rel_l2(f, prob.f) < 5e-5, rel_l2(f, prob.f)
+    body.u = u0
+    prob.set_u(u0)
+    cold = body.cold_residual()
+    prob.compute_forces()
+    assert abs(prob.residual_norm() - cold) < 1e-5 * cold
+    r = body.solve("adaptive", tol=1e-6, max_iterations=50000, ref=cold)
+    ro = prob.adr_solve(prob.adr_mass(1.0), max_it=50000, tol=1e-6, ref=cold)
+    assert r.converged and ro["converged"]
+    assert rel_l2(body.u, prob.u) < 1e-5, rel_l2(body.u, prob.u)
+    pb = po.Problem(X, 1.0, h, K2, mu, mass=mass, formulation="bond", fixed=fixed)
+    bb = PDBody(X, 1.0, h, K2, mu, mass=mass, formulation="bond", fixed=fixed)
+    bb.u = u
+    pb.set_u(u)
+    fb = bb.compute_forces()
+    pb.compute_forces()
+    assert rel_l2(fb, pb.f) < 5e-5, rel_l2(fb, pb.f)
 
 
 def test_spectral_mass_same_solution_fewer_iterations(gpu):

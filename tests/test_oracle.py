@@ -200,6 +200,66 @@ def test_j2_uniaxial_onset_closed_form():
     assert _j2(np.diag([1 + 1.001 * e_y, 1, 1]), sy, 0.0).epbar_t[0] > 0.0
 
 
+_DEV_D = np.diag([2.0, -1.0, -1.0]) / 3.0  # deviator of diag(1, 0, 0); |dev D| = sqrt(2/3)
+
+
+@pytest.mark.parametrize("hk", [0.0, 4e3, 2.5e4])
+def test_j2_kinematic_uniaxial_closed_form(hk):
+    # uniaxial strain e past yield from the virgin state: the trial deviator is
+    # 2 mu e dev D with Mises value q = 2 mu e, and one radial-return step with
+    # linear isotropic (Hi) and kinematic (hk) hardening gives
+    #   dgamma = (2 mu e - sy) / (3 mu + Hi + hk),   n = dev D / |dev D|
+    #   eps_p = sqrt(3/2) dgamma n = 1.5 dgamma dev D
+    #   beta  = sqrt(2/3) hk dgamma n = hk dgamma dev D
+    #   s     = 2 mu dev(eps - eps_p) = (2 mu e - 3 mu dgamma) dev D   (elasticity)
+    # and the relative stress s - beta, Mises value q - (3 mu + hk) dgamma, lies on
+    # the isotropically hardened surface sy + Hi dgamma
+    E, nu, sy, Hi = 200e3, 0.3, 250.0, 1.5e3
+    Kb, mu = W.bulk_shear(E, nu)
+    e = 4.0 * sy / (2 * mu)
+    p = _j2(np.diag([1 + e, 1, 1]), sy, Hi, hk=hk)
+    dg = (2 * mu * e - sy) / (3 * mu + Hi + hk)
+    assert abs(p.epbar_t[0] - dg) < 1e-12 * dg
+    assert np.allclose(p.ep_t[0].reshape(3, 3), 1.5 * dg * _DEV_D, rtol=1e-11, atol=1e-18)
+    assert np.allclose(p.beta_t[0].reshape(3, 3), hk * dg * _DEV_D, rtol=1e-11, atol=1e-12)
+    # the pressure stays elastic (tr eps_p = 0): tr P / 3 = Kb e
+    assert abs(np.trace(p.P[0]) / 3 - Kb * e) < 1e-10 * Kb * e
+    s = p.P[0] - np.trace(p.P[0]) / 3 * np.eye(3)
+    assert np.allclose(s, 2 * mu * (e * _DEV_D - p.ep_t[0].reshape(3, 3)), rtol=1e-10, atol=1e-9)
+    assert np.allclose(s, (2 * mu * e - 3 * mu * dg) * _DEV_D, rtol=1e-10, atol=1e-9)
+    assert abs(_mises(s - p.beta_t[0].reshape(3, 3)) - (sy + Hi * dg)) < 1e-9 * sy
+
+
+def test_j2_kinematic_bauschinger_reverse_onset():
+    # forward uniaxial strain e1 (one plastic step, committed: eps_p = 1.5 g dev D,
+    # beta = hk g dev D, epbar = g), then the strain is taken back down.  The trial
+    # relative stress at strain e is [2 mu (e - 1.5 g) - hk g] dev D, Mises value
+    # |2 mu (e - 1.5 g) - hk g|, so reverse yield starts at
+    #   e_rev = 1.5 g + (hk g - sy - Hi g) / (2 mu),
+    # above the onset e_iso = 1.5 g - (sy + Hi g) / (2 mu) that the same plastic
+    # strain gives without the back stress.  The onset therefore pins beta.
+    E, nu, sy, Hi, hk = 200e3, 0.3, 250.0, 1.5e3, 2.0e4
+    Kb, mu = W.bulk_shear(E, nu)
+    e1 = 3.0 * sy / (2 * mu)
+    p = _j2(np.diag([1 + e1, 1, 1]), sy, Hi, hk=hk)
+    g = float(p.epbar_t[0])
+    assert g > 0
+    p.commit()
+    e_rev = 1.5 * g + (hk * g - sy - Hi * g) / (2 * mu)
+    e_iso = 1.5 * g - (sy + Hi * g) / (2 * mu)
+    assert e_rev - e_iso > 0.1 * sy / (2 * mu)
+    for side, plastic in ((+1.0, False), (-1.0, True)):
+        p.F[0] = np.diag([1 + e_rev + side * 5e-3 * abs(e_rev), 1, 1])
+        p.stress()
+        assert (p.epbar_t[0] > g) == plastic, (side, p.epbar_t[0], g)
+    # between the two onsets: plastic with the back stress (elastic without it)
+    p.F[0] = np.diag([1 + 0.5 * (e_rev + e_iso), 1, 1])
+    p.stress()
+    assert p.epbar_t[0] > g
+    # the reverse step moves the back stress against the first one
+    assert p.beta_t[0][0] < p.beta[0][0]
+
+
 # -- a6 forces ---------------------------------------------------------------------
 def _brute_force_f(X, u, h, Kb, mu, Gs, c):
     """Independent pure-Python restatement of nosb.py:59-275 for tiny inputs."""
@@ -273,6 +333,95 @@ def test_bond_based_force_closed_form():
 
 
 # -- a7 dynamic relaxation ------------------------------------------------------------
+def _brute_dt(X, h, V, m, modulus):
+    """Independent pure-Python restatement of nosb_solver.py:94-154 on tiny inputs:
+    bonds by brute force, K_i = sum w xi xi^T V_j (nosb.py:59-94) inverted by LAPACK,
+    dpsi_k = w K_i^-1 xi_k (no V_j, as the reference loop has it),
+    k_eff = modulus V_i (|sum dpsi|^2 + sum |dpsi|^2), dt = 0.5 * 2 / sqrt(max k_eff / m)."""
+    n = len(X)
+    lmax = 0.0
+    for i in range(n):
+        if m[i] < 1e-30:
+            continue
+        nb = [j for j in range(n) if j != i and np.sum((X[j] - X[i]) ** 2) < h * h]
+        K = sum(((1 - np.linalg.norm(X[j] - X[i]) / h) * np.outer(X[j] - X[i], X[j] - X[i]) * V[j]
+                 for j in nb), np.zeros((3, 3)))
+        Ki = np.linalg.inv(K)
+        d = [(1 - np.linalg.norm(X[j] - X[i]) / h) * Ki @ (X[j] - X[i]) for j in nb]
+        keff = modulus * V[i] * (np.sum(np.sum(d, axis=0) ** 2) + sum(float(v @ v) for v in d))
+        lmax = max(lmax, keff / m[i])
+    return 0.5 * 2.0 / math.sqrt(lmax)
+
+
+def test_estimate_dt_matches_pure_python_brute_force():
+    # jittered 4^3 lattice, non-uniform volumes and masses: a dropped V_i, a V_j
+    # in dpsi, a wrong safety factor or a missing sum of |dpsi|^2 all change dt
+    X = lattice(4, 4, 4, jitter=0.2, seed=11)
+    rng = np.random.default_rng(12)
+    V = rng.uniform(0.6, 1.4, len(X))
+    m = V * rng.uniform(0.8, 1.2, len(X)) * 1.9e-3
+    Kb, mu = 1300.0, 600.0
+    p = po.Problem(X, V, 2.2, Kb, mu, mass=m)
+    p.shape_tensor("absolute", 1e-12)
+    assert not p.sing.any()
+    modulus = (Kb - 2 * mu / 3) + 2 * mu
+    assert abs(p.estimate_dt(Kb, mu, 1.0) - _brute_dt(X, 2.2, V, m, modulus)) < 1e-12 * p.estimate_dt(Kb, mu, 1.0)
+
+
+def test_estimate_dt_interior_closed_form():
+    # regular lattice, uniform volume V != 1, only the centre particle of a 9^3
+    # block has a finite mass m0 (all others 1e30 -> negligible eigenvalue).  Its
+    # neighbourhood is the full stencil: K = k I with k = V sum w |v|^2 / 3,
+    # sum dpsi = 0 by symmetry, sum |dpsi|^2 = sum w^2 |v|^2 / k^2, so
+    #   lambda = modulus V (sum w^2 |v|^2) / k^2 / m0,  dt = 0.5 * 2 / sqrt(lambda)
+    h, V, m0 = 3.015, 0.7, 2.3e-3
+    X = lattice(9, 9, 9)
+    n = len(X)
+    c = 4 * 81 + 4 * 9 + 4
+    m = np.full(n, 1e30)
+    m[c] = m0
+    Kb, mu = 2000.0, 900.0
+    p = po.Problem(X, V, h, Kb, mu, mass=m)
+    p.shape_tensor("absolute", 1e-12)
+    r = np.arange(-3, 4)
+    S = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3).astype(float)
+    L = np.linalg.norm(S, axis=1)
+    sel = (L > 0) & (L < h)
+    w = 1 - L[sel] / h
+    k = V * np.sum(w * L[sel] ** 2) / 3
+    lam = (Kb + 4 * mu / 3) * V * np.sum(w ** 2 * L[sel] ** 2) / k ** 2 / m0
+    dt = p.estimate_dt(Kb, mu, 1.0)
+    assert abs(dt - 1.0 / math.sqrt(lam)) < 1e-12 * dt
+    # 2-D: modulus lam + 2 mu with lam = K - mu (nosb_solver.py:112-115)
+    g = np.arange(7, dtype=float)
+    X2 = np.stack(np.meshgrid(g, g, indexing="ij"), -1).reshape(-1, 2)
+    m2 = np.full(len(X2), 1e30)
+    c2 = 3 * 7 + 3
+    m2[c2] = m0
+    p2 = po.Problem(X2, V, h, Kb, mu, mass=m2)
+    p2.shape_tensor("absolute", 1e-12)
+    S2 = np.stack(np.meshgrid(r, r, indexing="ij"), -1).reshape(-1, 2).astype(float)
+    L2 = np.linalg.norm(S2, axis=1)
+    sel2 = (L2 > 0) & (L2 < h)
+    w2 = 1 - L2[sel2] / h
+    k2 = V * np.sum(w2 * L2[sel2] ** 2) / 2
+    lam2 = (Kb + mu) * V * np.sum(w2 ** 2 * L2[sel2] ** 2) / k2 ** 2 / m0
+    dt2 = p2.estimate_dt(Kb, mu, 1.0)
+    assert abs(dt2 - 1.0 / math.sqrt(lam2)) < 1e-12 * dt2
+
+
+def test_estimate_dt_fallback_cfl():
+    # no particle with mass >= 1e-30: the conservative CFL branch,
+    # 0.1 * spacing / sqrt((K + 4 mu / 3) / rho) with spacing = mean(V)^(1/3)
+    X = lattice(3, 3, 3)
+    V = np.linspace(0.5, 1.5, len(X))
+    p = po.Problem(X, V, 1.5, 1000.0, 300.0, mass=np.zeros(len(X)))
+    p.shape_tensor()
+    rho = 1.9e-9
+    expect = 0.1 * np.mean(V) ** (1 / 3) / math.sqrt((1000.0 + 400.0) / rho)
+    assert abs(p.estimate_dt(1000.0, 300.0, rho) - expect) < 1e-9 * expect
+
+
 def test_kinetic_dr_single_dof_closed_form():
     # two particles, one fixed; kinetic DR of nosb_solver.py:156-256 converges to
     # the equilibrium of the PMB spring: s = fext / (c w V) (+ geometric exactness)
