@@ -164,6 +164,15 @@ int pdx_residual_norm(pdx_ctx *ctx, double *out);
 /* residual at the cold-start state (free DOFs at u = 0, prescribed DOFs as set),
  * per case; the state is left unchanged */
 int pdx_cold_residual(pdx_ctx *ctx, double *out);
+/* No counterpart in the reference: pdx_cold_residual for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same results, bit for bit, without any allocation: the records it
+ * overwrites are kept in a library-owned device array of PDX_NOALLOC_MAX_PARTICLES
+ * records (32 MiB per device, held until the process ends).  Contexts with more
+ * internal particles (cases padded to multiples of 256) get PDX_EINVAL.  All calls on
+ * one device share that array: calls that overlap on one device race. */
+#define PDX_NOALLOC_MAX_PARTICLES 524288
+int pdx_cold_residual_noalloc(pdx_ctx *ctx, double *out);
 /* DamageModel.update_damage + compute_local_damage, damage.py:94-152 (FP64 stretch) */
 int pdx_break_bonds(pdx_ctx *ctx, double critical_stretch, int64_t *n_broken_now);
 int pdx_local_damage(pdx_ctx *ctx);

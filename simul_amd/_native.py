@@ -85,6 +85,7 @@ SIGNATURES = {
     "pdx_compute_forces": [_vp],
     "pdx_residual_norm": [_vp, _dp],
     "pdx_cold_residual": [_vp, _dp],
+    "pdx_cold_residual_noalloc": [_vp, _dp],
     "pdx_break_bonds": [_vp, C.c_double, _i64p],
     "pdx_local_damage": [_vp],
     "pdx_commit_state": [_vp],

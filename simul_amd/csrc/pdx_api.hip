@@ -6,8 +6,10 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <numeric>
 #include <thread>
+#include <unordered_map>
 
 #include "pdx_internal.h"
 #include "pdx_layout.h"
@@ -94,14 +96,43 @@ struct HostMirror {
 static HostMirror &mirror(pdx_ctx *c);
 
 // keyed by context pointer; the context struct stays ABI-stable for the kernels file
-#include <unordered_map>
-#include <mutex>
 static std::mutex g_mx;
 static std::unordered_map<pdx_ctx *, HostMirror> g_mirror;
 static HostMirror &mirror(pdx_ctx *c)
 {
     std::lock_guard<std::mutex> l(g_mx);
     return g_mirror[c];
+}
+
+// ---------------------------------------------------------------------------
+// scratch of the allocation-free entry point (pdx_cold_residual_noalloc)
+// ---------------------------------------------------------------------------
+// The displacement records that the cold-residual evaluation overwrites are
+// kept here instead of in a hipMalloc'd copy: PDX_NOALLOC_MAX_PARTICLES
+// records of 64 B (32 MiB), one copy per device, held until the process ends.
+// In host code `scratch` is only a host-side stand-in for the symbol, never a
+// device address: the address comes from hipGetSymbolAddress, once per
+// device with that device current (scratch_address, at context creation),
+// and only that address is passed to copies and kernels.  Every call on one
+// device shares the array, so calls that overlap on one device race.
+static __device__ __align__(256) unsigned char scratch[(size_t)PDX_NOALLOC_MAX_PARTICLES * sizeof(XRec)];
+static std::mutex g_scratch_mx;
+static std::unordered_map<int, void *> g_scratch_addr;
+
+// this device's address of `scratch` (the device must be current); null when
+// the runtime cannot resolve it
+static void *scratch_address(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mx);
+    auto it = g_scratch_addr.find(device);
+    if (it != g_scratch_addr.end()) return it->second;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    g_scratch_addr[device] = p;
+    return p;
 }
 
 // ---------------------------------------------------------------------------
@@ -138,6 +169,7 @@ static int create_common(int32_t ncase, const int64_t *off, int32_t dim, pdx_ctx
         delete c;
         return fail(PDX_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
+    c->scratch = scratch_address(c->device);
     e = hipHostMalloc((void **)&c->h_sc, sizeof(DScal) * ncase, hipHostMallocDefault);
     if (e != hipSuccess) {
         hipStreamDestroy(c->stream);
@@ -960,6 +992,37 @@ extern "C" int pdx_cold_residual(pdx_ctx *c, double *out)
     hipStreamSynchronize(c->stream);
     hipFree(save);
     c->have_A = false;
+    return r;
+}
+
+// pdx_cold_residual without any allocation: the saved records go to this
+// device's `scratch` (through its device address, never the host-side name)
+extern "C" int pdx_cold_residual_noalloc(pdx_ctx *c, double *out)
+{
+    CHECK_CTX(c);
+    if (!out) return fail(PDX_EINVAL, "out is NULL");
+    if (c->ni > PDX_NOALLOC_MAX_PARTICLES)
+        return fail(PDX_EINVAL, "pdx_cold_residual_noalloc: " + std::to_string(c->ni) +
+                                    " internal particles (cases padded to multiples of " + std::to_string(PDX_BLOCK) +
+                                    ") exceed the scratch limit PDX_NOALLOC_MAX_PARTICLES = " +
+                                    std::to_string(PDX_NOALLOC_MAX_PARTICLES) + "; use pdx_cold_residual");
+    PDX_TRY(need_K(c));
+    if (!c->scratch) return fail(PDX_EDEVICE, "the scratch array's device address could not be resolved");
+    XRec *save = static_cast<XRec *>(c->scratch);
+    const size_t bytes = sizeof(XRec) * (size_t)c->ni;
+    PDX_HIP(hipMemcpyAsync(save, c->d.xu[c->ucur], bytes, hipMemcpyDeviceToDevice, c->stream));
+    int r = pdx_launch_zero_free(c, c->ucur);
+    if (r == PDX_OK && c->form == PDX_FORM_NOSB) r = pdx_launch_fpass(c, c->ucur, false, false, false);
+    if (r == PDX_OK) r = pdx_launch_force(c, c->ucur, 0, false, 0.0, 0.0);
+    if (r == PDX_OK) r = read_sc(c);
+    if (r == PDX_OK)
+        for (int k = 0; k < c->ncase; ++k) out[k] = c->h_sc[k].res;
+    const hipError_t e1 = hipMemcpyAsync(c->d.xu[c->ucur], save, bytes, hipMemcpyDeviceToDevice, c->stream);
+    const hipError_t e2 = hipStreamSynchronize(c->stream);
+    c->have_A = false;
+    if (r == PDX_OK && (e1 != hipSuccess || e2 != hipSuccess))
+        return fail(PDX_EDEVICE, std::string("restoring the displacement: ") +
+                                     hipGetErrorString(e1 != hipSuccess ? e1 : e2));
     return r;
 }
 

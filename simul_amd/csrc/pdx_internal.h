@@ -43,6 +43,9 @@ struct pdx_ctx {
     int64_t gkey = -1;
     // pinned host scalars
     DScal *h_sc = nullptr;
+    // device address of this device's copy of the allocation-free entry
+    // point's scratch array (hipGetSymbolAddress at creation; null if that failed)
+    void *scratch = nullptr;
 };
 
 // error handling -------------------------------------------------------------
