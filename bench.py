@@ -19,6 +19,16 @@ own stream; algorithmic bytes per launch from the per-unit model
 cpu_baseline: the FP64 oracle in reference-faithful mode (kinetic damping,
 P_j K_j^-1 recomputed per bond) on the host cores, a bounded number of
 iterations of the same specimen.
+
+The default run also measures the two other solver workloads and attaches
+them to the same JSON line under "configs", each with its own roofline (HIP
+event means of its hot kernels against the same per-unit model; J2 adds
+52 B/particle of plastic state to the F-pass) and its own CPU baseline:
+  vertebra_j2_dx0.25  J2 plasticity, 1,507,600 particles, 2 % compression in
+                      20 increments solved to 1e-6 with bond breaking
+  batch64_dx0.5       64 random specimens (7,045,942 particles) in one context,
+                      every case solved to 1e-6 (setup included in the
+                      reported time-to-solve)
 """
 from __future__ import annotations
 
@@ -101,6 +111,8 @@ def main():
                          "prescribed increments with breaking (dx 0.25); batch: 64 random cases in one context")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device copy (scales rocprof FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="measure only the dx 0.4 vertebral body (no J2-loading and batch blocks)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -120,10 +132,10 @@ def main():
         copy_calibration(torch)
         return
     if args.workload == "vertebra_j2":
-        run_loading(args, torch)
+        _print(run_loading(args, torch))
         return
     if args.workload == "batch":
-        run_batch(args, torch)
+        _print(run_batch(args, torch))
         return
 
     t0 = time.perf_counter()
@@ -226,6 +238,8 @@ def main():
                "relative_residual": rs.relative_residual, "residual_normalisation": "cold-start residual",
                "dr_mass": mass_label(args.dr_mass == "spectral"),
                "cold_start_gershgorin_s": round(wall_g, 4), "iterations_gershgorin": rg.iterations,
+               # what the spectral mass alone gains over the Gershgorin mass, both cold
+               "spectral_mass_speedup": round(wall_g / wall, 3),
                "warm_start_s": None, "warm_start": "no trained weights", "setup_s": round(setup_s, 3)}
         log(f"time-to-solve: {wall:.3f}s, {rs.iterations} iterations, converged={rs.converged}, "
             f"rel={rs.relative_residual:.2e}")
@@ -247,11 +261,10 @@ def main():
             du = float(np.linalg.norm(body.u - u_cold) / np.linalg.norm(u_cold))
             tts.update({"warm_start_s": round(tw["total_s"], 4), "warm_iterations": rw.iterations,
                         "warm_converged": rw.converged, "warm_inference_s": round(tw["inference_s"], 4),
+                        # the learned start's own gain: warm against cold,
+                        # both with the same (spectral) DR mass
                         "warm_speedup": round(wall / tw["total_s"], 3),
-                        # the warm-started solver (learned start, spectral
-                        # mass) against the plain cold start (u = 0, the
-                        # Gershgorin mass of the reference-faithful DR)
-                        "warm_speedup_vs_plain_cold": round(wall_g / tw["total_s"], 3),
+                        "warm_iteration_ratio": round(rs.iterations / max(rw.iterations, 1), 3),
                         "warm_vs_cold_u_rel_l2": du,
                         "warm_start": "learned per-particle network (BF16 MFMA kernels), features, inference and "
                                       "DR included in warm_start_s; one untimed inference before it loads the "
@@ -263,6 +276,17 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(spec, args.cpu_seconds)
 
+    configs = None
+    if not args.headline_only:
+        import gc
+        del body
+        gc.collect()
+        configs = {}
+        for fn in (run_loading, run_batch):
+            blk = fn(args, torch)
+            configs[blk["config"]["workload"]] = blk
+            gc.collect()
+
     out = {
         "metric": METRIC, "value": value, "unit": "bond-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
@@ -273,6 +297,12 @@ def main():
                    "precision": "fp32 bonds, fp64 K^-1 and reductions"},
         "time_to_solve": tts, "roofline": roofline, "cpu_baseline": cpu,
     }
+    if configs is not None:
+        out["configs"] = configs
+    _print(out)
+
+
+def _print(out):
     print(json.dumps(out), flush=True)
 
 
@@ -281,7 +311,27 @@ def _line(metric, value, unit, steps, warmup, ms, config, **extra):
            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
            "data": "synthetic", "config": config}
     out.update(extra)
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def hot_roofline(body, n_p, nb, iters, state_bytes=0.0):
+    """Roofline of the two hot passes from HIP events on the solver stream:
+    algorithmic bytes per launch (F-pass 8 B/bond + (112 + state) B/particle,
+    force pass 8 B/bond + 133 B/particle) over each kernel's mean duration;
+    the dominant kernel heads the record."""
+    kt = body.time_kernels(iters)
+    _, kf_name, kg_name = pass_variants(body)
+    bytes_f = 8.0 * nb + (112.0 + state_bytes) * n_p
+    bytes_g = 8.0 * nb + 133.0 * n_p
+    k = {kf_name: (kt["fpass_ms"], bytes_f), kg_name: (kt["force_ms"], bytes_g)}
+    dom = max(k, key=lambda name: k[name][0])
+    ms, b = k[dom]
+    ach = b / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": b,
+            "mean_us": round(ms * 1e3, 2),
+            "kernels": {name: {"mean_us": round(v[0] * 1e3, 2), "bytes": v[1],
+                               "achieved_GBs": round(v[1] / (v[0] * 1e-3) / 1e9, 1)} for name, v in k.items()}}
 
 
 def run_loading(args, torch):
@@ -307,7 +357,6 @@ def run_loading(args, torch):
                              max_iterations=args.max_solve_iters,
                              mass=args.dr_mass, predictor=None if args.predictor == "none" else args.predictor,
                              progress=lambda done, r: log(f"    {done} iterations, rel {r.relative_residual:.2e}"))
-    upd = 0.0
 
     def progress(r):
         log(f"increment {r.increment:2d}: {r.iterations} iterations, rel {r.relative_residual:.2e}, "
@@ -319,17 +368,33 @@ def run_loading(args, torch):
     wall = time.perf_counter() - t0
     iters = sum(r.iterations for r in res)
     upd = nb0 * iters  # intact bonds at the start; broken bonds are a small fraction
-    _line(METRIC, upd / wall, "bond-updates/s", iters, 0, 1e3 * wall / max(iters, 1),
-          {"workload": spec.name, "n_particles": spec.n, "n_bonds": nb0, "increments": len(res),
-           "critical_stretch": spec.extra["critical_stretch"], "variant": pass_variants(body)[0]},
-          time_to_solve={"loading_s": round(wall, 3), "setup_s": round(setup, 3),
-                         "dr_mass": mass_label(args.dr_mass == "spectral"), "predictor": args.predictor,
-                         "broken_total": res[-1].broken_total if res else 0,
-                         "all_converged": all(r.converged for r in res),
-                         "increments": [{"k": r.increment, "iterations": r.iterations,
-                                         "relative_residual": r.relative_residual, "broken_now": r.broken_now,
-                                         "seconds": round(r.elapsed_s, 3)} for r in res]},
-          roofline=None, cpu_baseline=None)
+    nb_end = body.stats()["n_intact"]
+    # hot kernels at the final (yielded, partly broken) state; J2 adds 52 B per
+    # particle of plastic state (eps_p 6, epbar 1, beta 6 floats) to the F-pass
+    roof = hot_roofline(body, spec.n, nb_end, args.kernel_iters, state_bytes=52.0)
+    log(f"kernels: {roof['kernels']}")
+    variant = pass_variants(body)[0]
+    del body
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the first increment's boundary values, kinetic DR from there
+        u1 = spec.u0.copy()
+        u1[spec.extra["top_idx"]] = spec.extra["u_final"] / spec.extra["n_increments"]
+        cpu = cpu_baseline(spec, args.cpu_seconds, u_start=u1,
+                           what="kinetic-DR iterations of the first load increment (top endplate at 1/20 of "
+                                "the final compression) of the full J2 specimen")
+    return _line(METRIC, upd / wall, "bond-updates/s", iters, 0, 1e3 * wall / max(iters, 1),
+                 {"workload": spec.name, "n_particles": spec.n, "n_bonds": nb0, "increments": len(res),
+                  "critical_stretch": spec.extra["critical_stretch"], "variant": variant},
+                 time_to_solve={"loading_s": round(wall, 3), "setup_s": round(setup, 3),
+                                "dr_mass": mass_label(args.dr_mass == "spectral"), "predictor": args.predictor,
+                                "broken_total": res[-1].broken_total if res else 0,
+                                "all_converged": all(r.converged for r in res),
+                                "increments": [{"k": r.increment, "iterations": r.iterations,
+                                                "relative_residual": r.relative_residual,
+                                                "broken_now": r.broken_now,
+                                                "seconds": round(r.elapsed_s, 3)} for r in res]},
+                 roofline=roof, cpu_baseline=cpu)
 
 
 def run_batch(args, torch):
@@ -337,7 +402,8 @@ def run_batch(args, torch):
     dx 0.5 mm, two-phase Neo-Hookean, pressure 0.5-3 MPa within 20 deg of the
     axis) packed into one context: every launch covers all cases.  value =
     intact bonds of all cases x iterations / wall time; time-to-solve = wall
-    time until every case reaches 1e-6."""
+    time until every case reaches 1e-6, reported with and without the setup
+    (neighbour search, bond layout, shape tensors) of the 64 cases."""
     from simul_amd import workloads as W
     from simul_amd.body import from_specimen
 
@@ -366,6 +432,7 @@ def run_batch(args, torch):
         body.u = packed.u0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+
         def progress(done, rs):
             nconv = sum(r.converged for r in rs)
             worst = max((r.relative_residual for r in rs if not r.converged), default=0.0)
@@ -377,15 +444,24 @@ def run_batch(args, torch):
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         its = [r.iterations for r in rs]
-        tts = {"all_cases_s": round(wall, 3), "dr_mass": mass_label(args.dr_mass == "spectral"),
+        tts = {"all_cases_s": round(wall, 3), "all_cases_with_setup_s": round(wall + setup, 3),
+               "dr_mass": mass_label(args.dr_mass == "spectral"),
                "converged": sum(r.converged for r in rs), "cases": len(rs),
                "iterations_min": min(its), "iterations_max": max(its), "setup_s": round(setup, 3)}
-        log(f"time-to-solve (all cases): {wall:.2f}s, iterations {min(its)}..{max(its)}, "
+        log(f"time-to-solve (all cases): {wall:.2f}s (+ setup {setup:.2f}s), iterations {min(its)}..{max(its)}, "
             f"{tts['converged']}/{len(rs)} converged")
-    _line(METRIC, nb * steps / dt, "bond-updates/s", steps, args.warmup, 1e3 * dt / steps,
-          {"workload": packed.name, "cases": len(specs), "n_particles": packed.n, "n_bonds": nb,
-           "variant": pass_variants(body)[0]},
-          time_to_solve=tts, roofline=None, cpu_baseline=None)
+    body.u = packed.u0
+    roof = hot_roofline(body, packed.n, nb, args.kernel_iters)
+    log(f"kernels: {roof['kernels']}")
+    variant = pass_variants(body)[0]
+    del body
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_cases(specs, args.cpu_seconds)
+    return _line(METRIC, nb * steps / dt, "bond-updates/s", steps, args.warmup, 1e3 * dt / steps,
+                 {"workload": packed.name, "cases": len(specs), "n_particles": packed.n, "n_bonds": nb,
+                  "variant": variant},
+                 time_to_solve=tts, roofline=roof, cpu_baseline=cpu)
 
 
 def copy_calibration(torch):
@@ -401,20 +477,32 @@ def copy_calibration(torch):
     log(f"copy calibration: read {4 * n} B, wrote {4 * n} B in one elementwise kernel")
 
 
-def cpu_baseline(spec, seconds):
-    """Oracle in reference-faithful mode on the host cores, bounded sample."""
+def _oracle_kd(spec, u_start=None):
+    """The specimen in the oracle, reference-faithful mode, and its stable dt."""
     import numpy as np
 
     from oracle import pdoracle as po
 
+    j2 = {}
+    if spec.model == "j2_plasticity":
+        j2 = dict(sy0=spec.extra["yield"], h_iso=spec.extra["h_iso"], h_kin=spec.extra.get("h_kin"))
+    prob = po.Problem(spec.X, spec.vol, spec.horizon, spec.bulk, spec.shear, mass=spec.vol * spec.rho,
+                      model=spec.model, fixed=spec.fixed, fext=spec.fext, nthreads=po.default_threads(),
+                      per_bond_recompute=True, **j2)
+    prob.shape_tensor("absolute", 1e-12)
+    prob.set_u(spec.u0 if u_start is None else u_start)
+    dt = prob.estimate_dt(float(np.mean(spec.bulk)), float(np.mean(spec.shear)), spec.rho)
+    return prob, dt
+
+
+def cpu_baseline(spec, seconds, u_start=None, what="kinetic-DR iterations of the full specimen"):
+    """Oracle in reference-faithful mode on the host cores, bounded sample."""
+    from oracle import pdoracle as po
+
     cores = po.default_threads()
     t0 = time.perf_counter()
-    prob = po.Problem(spec.X, spec.vol, spec.horizon, spec.bulk, spec.shear, mass=spec.vol * spec.rho,
-                      model=spec.model, fixed=spec.fixed, fext=spec.fext, nthreads=cores,
-                      per_bond_recompute=True)
-    prob.shape_tensor("absolute", 1e-12)
+    prob, dt = _oracle_kd(spec, u_start)
     setup = time.perf_counter() - t0
-    dt = prob.estimate_dt(float(np.mean(spec.bulk)), float(np.mean(spec.shear)), spec.rho)
     t0 = time.perf_counter()
     prob.kd_solve(dt, 0.0, max_it=1, tol=0.0)
     t1 = time.perf_counter() - t0
@@ -425,15 +513,45 @@ def cpu_baseline(spec, seconds):
     nb = prob.nb
     # kd_solve evaluates the forces once before the loop and once per iteration
     v = nb * (iters + 1) / el
-    log(f"cpu baseline: {cores} threads, {iters} iterations in {el:.1f}s -> {v / 1e6:.1f} M bond-updates/s, "
-        f"relative residual reached {r['relative_residual']:.3e}")
+    log(f"cpu baseline ({spec.name}): {cores} threads, {iters} iterations in {el:.1f}s -> {v / 1e6:.1f} M "
+        f"bond-updates/s, relative residual reached {r['relative_residual']:.3e}")
     return {"value": v, "unit": "bond-updates/s", "cores": cores, "kind": "port",
             "ms_per_iteration": round(1e3 * el / iters, 2),
             "relative_residual_reached": float(r["relative_residual"]),
-            "sample": f"{iters} kinetic-DR iterations of the full specimen (plus 1 untimed) from the cold start, "
-                      f"reference-faithful FP64 C/OpenMP restatement; setup {setup:.1f}s untimed; the "
-                      f"reference's kinetic DR needs about 300 000 iterations for 1e-4, so 1e-6 is not "
-                      f"reached within the sample"}
+            "sample": f"{iters} {what} (plus 1 untimed) from the cold start, reference-faithful FP64 "
+                      f"C/OpenMP restatement; setup {setup:.1f}s untimed; the reference's kinetic DR needs "
+                      f"about 300 000 iterations for 1e-4, so 1e-6 is not reached within the sample"}
+
+
+def cpu_baseline_cases(specs, seconds):
+    """The oracle on the batch's cases one after another (a case at a time:
+    the reference solves one body per solver), about `seconds` of kinetic-DR
+    iterations spread over the first cases."""
+    from oracle import pdoracle as po
+
+    cores = po.default_threads()
+    upd, el_tot, used, its_tot = 0.0, 0.0, [], 0
+    per_case = seconds / 4
+    for spec in specs:
+        if el_tot >= seconds or len(used) >= 4:
+            break
+        prob, dt = _oracle_kd(spec)
+        t0 = time.perf_counter()
+        prob.kd_solve(dt, 0.0, max_it=1, tol=0.0)
+        t1 = time.perf_counter() - t0
+        iters = max(1, int(per_case / max(t1, 1e-3)))
+        t0 = time.perf_counter()
+        prob.kd_solve(dt, 0.0, max_it=iters, tol=0.0)
+        el = time.perf_counter() - t0
+        upd += prob.nb * (iters + 1)
+        el_tot += el
+        its_tot += iters
+        used.append(f"{spec.name} ({spec.n} particles, {iters} iterations)")
+    v = upd / el_tot
+    log(f"cpu baseline (batch): {cores} threads, {len(used)} cases in {el_tot:.1f}s -> {v / 1e6:.1f} M bond-updates/s")
+    return {"value": v, "unit": "bond-updates/s", "cores": cores, "kind": "port",
+            "sample": "kinetic-DR iterations from the cold start of the first cases of the batch, one case at a "
+                      "time, reference-faithful FP64 C/OpenMP restatement (setup untimed): " + "; ".join(used)}
 
 
 if __name__ == "__main__":

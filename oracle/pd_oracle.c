@@ -661,6 +661,58 @@ int64_t orc_kd_solve(orc_pd *p, double dt, double visc, int64_t max_it, double t
 }
 
 /* ------------------------------------------------------------------ */
+/* Explicit dynamics with a per-step damage schedule:                   */
+/* ExplicitSolver.step (peridynamics/solver/explicit.py:62-71, 120-158) */
+/* followed by DamageModel.step (core/damage.py:94-152) when s_c > 0.   */
+/* Velocity Verlet: a_old = a; x += v dt + a dt^2 / 2 on free DOFs;      */
+/* forces; a = f / m (m > 0); v = (1 - damping)(v + (a_old + a) dt / 2); */
+/* fixed DOFs v = a = 0.  Starts from the current x, v, a.  Returns the  */
+/* bonds broken over the run; out[0] = kinetic energy sum m v^2 / 2     */
+/* after the last step (explicit.py:226-233).                           */
+/* ------------------------------------------------------------------ */
+int64_t orc_update_damage(orc_pd *p, double s_c);
+
+int64_t orc_explicit_run(orc_pd *p, double dt, double damping, int64_t n_steps, double s_c, double *out)
+{
+    int dim = p->dim;
+    int64_t n = p->n, broken = 0;
+    double *a_old = (double *)malloc(sizeof(double) * (size_t)(n * dim));
+    for (int64_t step = 0; step < n_steps; ++step) {
+#pragma omp parallel for schedule(static) num_threads(nthr(p->nthreads))
+        for (int64_t i = 0; i < n; ++i)
+            for (int d = 0; d < dim; ++d) {
+                int64_t k = i * dim + d;
+                a_old[k] = p->a[k];
+                if (!((p->fixed[i] >> d) & 1)) p->x[k] += p->v[k] * dt + 0.5 * p->a[k] * dt * dt;
+            }
+        orc_compute_forces(p);
+#pragma omp parallel for schedule(static) num_threads(nthr(p->nthreads))
+        for (int64_t i = 0; i < n; ++i) {
+            double m = p->mass[i];
+            for (int d = 0; d < dim; ++d) {
+                int64_t k = i * dim + d;
+                if (!((p->fixed[i] >> d) & 1)) {
+                    if (m > 0.0) p->a[k] = p->f[k] / m;
+                    p->v[k] = (1.0 - damping) * (p->v[k] + 0.5 * (a_old[k] + p->a[k]) * dt);
+                } else {
+                    p->v[k] = 0.0;
+                    p->a[k] = 0.0;
+                }
+            }
+        }
+        if (s_c > 0.0) broken += orc_update_damage(p, s_c);
+    }
+    if (out) {
+        double ke = 0.0;
+        for (int64_t i = 0; i < n; ++i)
+            for (int d = 0; d < dim; ++d) ke += 0.5 * p->mass[i] * p->v[i * dim + d] * p->v[i * dim + d];
+        out[0] = ke;
+    }
+    free(a_old);
+    return broken;
+}
+
+/* ------------------------------------------------------------------ */
 /* Adaptive dynamic relaxation (no reference counterpart; Kilic & Madenci */
 /* 2010, Theor. Appl. Fract. Mech. 53:194-204) -- the same algorithm the */
 /* HIP solver runs, here in FP64.                                       */

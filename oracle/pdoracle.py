@@ -85,6 +85,8 @@ def lib():
         L.orc_adr_solve_mode.argtypes = [P, _dp, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int32, _dp]
         L.orc_adr_solve.restype = C.c_int64
         L.orc_adr_solve.argtypes = [P, _dp, C.c_int64, C.c_double, C.c_double, C.c_double, _dp]
+        L.orc_explicit_run.restype = C.c_int64
+        L.orc_explicit_run.argtypes = [P, C.c_double, C.c_double, C.c_int64, C.c_double, _dp]
         L.orc_update_damage.restype = C.c_int64
         L.orc_update_damage.argtypes = [P, C.c_double]
         L.orc_local_damage.restype = None
@@ -240,6 +242,15 @@ class Problem:
                                       _ptr(out, _dp))
         return {"converged": it >= 0 and out[1] < tol, "iterations": abs(int(it)), "residual": out[0],
                 "relative_residual": out[1], "reference_residual": out[2], "damping": out[3]}
+
+    def explicit_run(self, dt, damping=0.0, n_steps=1, critical_stretch=0.0):
+        """Velocity-Verlet explicit dynamics (+ per-step bond breaking when
+        critical_stretch > 0) from the current x, v, a; returns
+        (bonds broken, kinetic energy after the last step)."""
+        out = np.zeros(1)
+        nb = lib().orc_explicit_run(C.byref(self._s), dt, damping, int(n_steps), critical_stretch,
+                                    _ptr(out, _dp))
+        return int(nb), float(out[0])
 
     def update_damage(self, s_c: float) -> int:
         return int(lib().orc_update_damage(C.byref(self._s), s_c))

@@ -459,6 +459,69 @@ def test_adaptive_dr_reaches_newton_equilibrium():
     assert np.linalg.norm(u_adr.reshape(-1) - un) / np.linalg.norm(un) < 1e-8
 
 
+# -- explicit dynamics (explicit.py:62-158) + damage schedule (damage.py:144-152) ----------
+def _spring_pair(c=3.0, V=2.0, m=0.5, h=1.5, v0=1e-3):
+    # particle 0 fixed, particle 1 at +1 on x, free; PMB bond along x: for motion
+    # along the bond f_1 = -c w V u (|xi| = 1), a linear spring k = c w V / m
+    X = np.array([[0.0, 0, 0], [1.0, 0, 0]])
+    fixed = np.array([7, 6], np.uint8)
+    p = po.Problem(X, V, h, 1.0, 1.0, formulation="bond", cstab=c, fixed=fixed, mass=np.full(2, m))
+    p.v[1, 0] = v0
+    return p, c * (1 - 1 / h) * V / m
+
+
+def test_explicit_verlet_spring_closed_form():
+    # velocity Verlet from u = 0, v = v0, a = 0: u_n = v0 dt sin(n th) / sin(th),
+    # cos(th) = 1 - k dt^2 / 2 (the exact discrete solution of the recurrence
+    # u_(n+1) = 2 u_n - u_(n-1) - k dt^2 u_n)
+    p, k = _spring_pair()
+    dt, v0 = 0.05, 1e-3
+    th = math.acos(1 - k * dt * dt / 2)
+    for n_done, n in ((0, 7), (7, 40)):
+        nb, ke = p.explicit_run(dt, 0.0, n - n_done)
+        assert nb == 0
+        assert abs(p.u[1, 0] - v0 * dt * math.sin(n * th) / math.sin(th)) < 1e-12 * v0
+    assert np.all(p.u[0] == 0) and np.all(p.u[1, 1:] == 0)
+
+
+def test_explicit_damping_free_particle_closed_form():
+    # a particle without bonds: a = 0, v_n = (1 - d)^n v0, u_n = v0 dt sum_{k<n} (1-d)^k
+    X = np.array([[0.0, 0, 0], [10.0, 0, 0]])
+    p = po.Problem(X, 1.0, 1.5, 1.0, 1.0, formulation="bond", cstab=1.0, mass=np.ones(2))
+    p.v[:] = [[2e-3, -1e-3, 5e-4], [0, 0, 0]]
+    d, dt, n = 0.03, 0.1, 25
+    _, ke = p.explicit_run(dt, d, n)
+    q = (1 - d) ** np.arange(n)
+    assert np.allclose(p.u[0], np.array([2e-3, -1e-3, 5e-4]) * dt * q.sum(), rtol=1e-13)
+    v = np.array([2e-3, -1e-3, 5e-4]) * (1 - d) ** n
+    assert np.allclose(p.v[0], v, rtol=1e-13)
+    assert abs(ke - 0.5 * float(v @ v)) < 1e-13 * ke
+
+
+def test_explicit_damage_schedule_breaks_at_the_closed_form_step():
+    # the spring pair kicked hard: the bond breaks at the first step whose stretch
+    # u_n / |xi| exceeds s_c (both directed entries), after which the particle
+    # keeps the velocity of that step plus half a step of its last acceleration
+    p, k = _spring_pair(v0=0.05)
+    dt, v0, s_c = 0.05, 0.05, 0.015
+    th = math.acos(1 - k * dt * dt / 2)
+    un = [v0 * dt * math.sin(n * th) / math.sin(th) for n in range(200)]
+    n_star = next(n for n in range(200) if un[n] > s_c)
+    nb, _ = p.explicit_run(dt, 0.0, n_star - 1, critical_stretch=s_c)
+    assert nb == 0 and not p.broken[:2].any()
+    nb, _ = p.explicit_run(dt, 0.0, 1, critical_stretch=s_c)
+    assert nb == 2 and p.broken[:2].all()
+    a_last = -k * un[n_star]
+    assert abs(p.a[1, 0] - a_last) < 1e-12
+    v_last, u0 = p.v[1, 0], p.u[1, 0]
+    # next step: moves with v and a, then no force (a = 0) and v + a dt / 2 from there on
+    p.explicit_run(dt, 0.0, 5, critical_stretch=s_c)
+    v_after = v_last + 0.5 * a_last * dt
+    assert abs(p.v[1, 0] - v_after) < 1e-15
+    assert abs(p.u[1, 0] - (u0 + v_last * dt + 0.5 * a_last * dt * dt + 4 * v_after * dt)) < 1e-14
+    assert np.allclose(p.local_damage(), [1.0, 1.0])
+
+
 # -- a9 damage ------------------------------------------------------------------------
 def test_damage_closed_form_uniform_stretch():
     # u = e x1: a bond along direction n stretches sqrt((1+e)^2 n1^2 + 1 - n1^2) - 1
