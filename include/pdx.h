@@ -223,6 +223,21 @@ int pdx_kd_step(pdx_ctx *ctx, double dt, double viscous_damping, double *prev_ke
  * (res has ncase entries) */
 int pdx_dr_solve(pdx_ctx *ctx, const pdx_dr_params *params, pdx_dr_result *res);
 
+/* ---- explicit dynamics ------------------------------------------------------------ */
+/* ExplicitSolver.run, backend/fea/peridynamics/solver/explicit.py:62-158, with
+ * DamageModel.step (core/damage.py:144-152) after every step when critical_stretch > 0:
+ * n_steps of velocity-Verlet dynamics from the context's current u, v and a
+ * (PDX_FIELD_a): x += v dt + a dt^2 / 2, forces, a = f / m, v = (1 - damping)(v +
+ * (a_old + a) dt / 2) on free DOFs (fixed DOFs keep their u with v = a = 0), then every
+ * intact bond with stretch > critical_stretch breaks.  The force is the context's
+ * formulation: PDX_FORM_BOND is the reference's explicit PMB force c s w V_j eta/|eta|
+ * with c the per-bond stabilisation modulus (pdx_set_stabilization, G_s = 1), plus the
+ * external force density.  The shape tensor is not recomputed during the run.  Local
+ * damage is current at return.  n_broken: bonds broken during the run (may be NULL);
+ * ke: per case, sum m v^2 / 2 after the last step (explicit.py:226-233; may be NULL). */
+int pdx_explicit_run(pdx_ctx *ctx, double dt, double damping, int64_t n_steps, double critical_stretch,
+                     int64_t *n_broken, double *ke);
+
 /* ---- field access -------------------------------------------------------------- */
 /* field.to_numpy() / field.from_numpy() of the reference's Taichi fields
  * (particles.py:37-62, 221-231); layout per field id above, caller order */

@@ -93,6 +93,7 @@ SIGNATURES = {
     "pdx_adr_mass": [_vp, C.c_double],
     "pdx_kd_step": [_vp, C.c_double, C.c_double, _dp, _i32p, _dp, _dp, _i32p],
     "pdx_dr_solve": [_vp, C.POINTER(DRParams), C.POINTER(DRResult)],
+    "pdx_explicit_run": [_vp, C.c_double, C.c_double, C.c_int64, C.c_double, _i64p, _dp],
     "pdx_time_kernels": [_vp, C.c_int32, _dp],
     "pdx_stats": [_vp, _i64p],
     "pdx_autotune": [_vp, C.c_int32, _i32p],

@@ -249,6 +249,17 @@ class PDBody:
         return {"kinetic_energy": ke, "residual": res, "velocity_reset": rs.astype(bool),
                 "prev_ke": pk, "ke_increasing": inc.astype(bool)}
 
+    def explicit_run(self, dt, damping=0.0, n_steps=1, critical_stretch=0.0):
+        """Velocity-Verlet explicit dynamics from the current u, v and a
+        (pdx_explicit_run), with bonds breaking after every step when
+        critical_stretch > 0.  Returns (bonds broken during the run, kinetic
+        energy per case after the last step; a float for one case)."""
+        nb = C.c_int64()
+        ke = np.zeros(self.ncase)
+        self.ctx.call("pdx_explicit_run", float(dt), float(damping), int(n_steps), float(critical_stretch),
+                      C.byref(nb), N.ptr(ke, N._dp))
+        return nb.value, (ke if self.ncase > 1 else float(ke[0]))
+
     def solve(self, method="adaptive", tol=1e-6, max_iterations=100000, ref=0.0, dt=0.0,
               viscous_damping=0.0, cmax=1.9, use_graph=True, chunk=64, resume=False, refs=None,
               progress=None, slice_iterations=None, mass=None):

@@ -1580,6 +1580,56 @@ extern "C" int pdx_dr_solve(pdx_ctx *c, const pdx_dr_params *p, pdx_dr_result *r
 }
 
 // ---------------------------------------------------------------------------
+// explicit dynamics
+// ---------------------------------------------------------------------------
+// ExplicitSolver.run (explicit.py:62-158) with DamageModel.step after every
+// step (damage.py:144-152): each step is the kinetic-DR move without its
+// kinetic-energy reset, the force of the context's formulation with the
+// velocity-Verlet update a = f / m, v = (1 - damping)(v + (a_old + a) dt / 2),
+// then the critical-stretch test at the new positions.  Local damage is
+// refreshed once at the end (the reference recomputes it every step; the
+// flags, and so the final field, are the same).
+extern "C" int pdx_explicit_run(pdx_ctx *c, double dt, double damping, int64_t n_steps, double s_c,
+                                int64_t *n_broken, double *ke)
+{
+    CHECK_CTX(c);
+    PDX_TRY(need_K(c));
+    if (!(dt > 0.0) || !isfinite(dt)) return fail(PDX_EINVAL, "dt must be positive");
+    if (!(damping >= 0.0 && damping < 1.0)) return fail(PDX_EINVAL, "damping must be in [0, 1)");
+    if (n_steps < 0) return fail(PDX_EINVAL, "n_steps < 0");
+    if (!isfinite(s_c)) return fail(PDX_EINVAL, "critical_stretch must be finite");
+    unsigned long long *cnt = nullptr;
+    if (s_c > 0.0) {
+        PDX_HIP(hipMalloc((void **)&cnt, sizeof(unsigned long long)));
+        PDX_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
+    }
+    int r = PDX_OK;
+    for (int64_t t = 0; t < n_steps && r == PDX_OK; ++t) {
+        r = pdx_launch_kd_move(c, c->ucur, false, dt, false);
+        if (r != PDX_OK) break;
+        c->ucur ^= 1;
+        if (c->form == PDX_FORM_NOSB) r = pdx_launch_fpass(c, c->ucur, false, false, false);
+        if (r == PDX_OK) r = pdx_launch_force(c, c->ucur, 2, false, dt, damping);
+        if (r == PDX_OK && cnt) r = pdx_launch_damage(c, s_c, cnt);
+    }
+    unsigned long long h = 0;
+    if (r == PDX_OK && cnt) {
+        r = pdx_launch_local_damage(c);
+        if (r == PDX_OK && hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            r = fail(PDX_EDEVICE, "reading the broken-bond count");
+    }
+    if (r == PDX_OK) r = read_sc(c);  // synchronizes the stream
+    if (cnt) hipFree(cnt);
+    if (r != PDX_OK) return r;
+    if (n_broken) *n_broken = (int64_t)h;
+    if (h) c->have_mass = false;
+    if (ke)
+        for (int k = 0; k < c->ncase; ++k) ke[k] = n_steps > 0 ? c->h_sc[k].num : 0.0;
+    c->have_A = n_steps > 0;
+    return PDX_OK;
+}
+
+// ---------------------------------------------------------------------------
 // measurement
 // ---------------------------------------------------------------------------
 extern "C" int pdx_time_kernels(pdx_ctx *c, int32_t iters, double *ms)

@@ -102,7 +102,7 @@ struct DScal {
     double ref, tol, res, rel;
     double c, cmax;
     double ke, prev_ke;
-    double num, den;
+    double num, den;  // adaptive: Rayleigh sums; kinetic: num = kinetic energy of the last pass
     int64_t iter, conv_iter, resets, pad64;
     int32_t converged, sol_buf, first, inc;
     int32_t reset_pending, pad32;
@@ -721,6 +721,7 @@ PDX_HD void case_update(DScal &S, const double *tot, int ubuf)
         S.iter += 1;
     } else if (UPD == 2) {
         double ke = tot[1];
+        S.num = ke;  // the pass's kinetic energy, kept when a reset zeroes S.ke
         if (ke < S.prev_ke && S.inc) {
             S.reset_pending = 1;
             ke = 0.0;

@@ -12,8 +12,9 @@ from .core.damage import DamageModel
 from .core.neighbor import NeighborSearch
 from .core.nosb import NOSBCompute, NOSBMaterial
 from .core.particles import ParticleSystem
+from .solver.explicit import ExplicitSolver
 from .solver.nosb_solver import NOSBSolver
 from .solver.quasi_static import LoadControl, QuasiStaticSolver
 
 __all__ = ["ParticleSystem", "NeighborSearch", "BondSystem", "NOSBCompute", "NOSBMaterial", "DamageModel",
-           "NOSBSolver", "QuasiStaticSolver", "LoadControl"]
+           "NOSBSolver", "QuasiStaticSolver", "LoadControl", "ExplicitSolver"]

@@ -1,1 +1,2 @@
-"""Mirror of backend/fea/peridynamics/solver (NOSB kinetic-damping DR, bond-based quasi-static)."""
+"""Mirror of backend/fea/peridynamics/solver (NOSB kinetic-damping DR, bond-based quasi-static,
+explicit dynamics)."""
