@@ -151,6 +151,22 @@ int pdx_get_bond_data(pdx_ctx *ctx, double *xi, double *xi_len, double *omega);
 /* overwrite broken flags (nb, pdx_get_bonds order), BondSystem.broken, bonds.py:76 */
 int pdx_set_broken(pdx_ctx *ctx, const uint8_t *broken);
 
+/* bond layout builder (no reference counterpart; the reference builds its
+ * neighbour lists on the device, neighbor.py:119-319, bonds.py:101-137): the halo
+ * tables of the LDS-staged hot passes are built on the device (PDX_LAYOUT_DEVICE,
+ * the default) or by host threads (PDX_LAYOUT_HOST, the same tables; also the
+ * fallback for blocks with more than 6 144 distinct out-of-block neighbours).
+ * Takes effect at the next pdx_build_bonds. */
+#define PDX_LAYOUT_DEVICE 0
+#define PDX_LAYOUT_HOST 1
+int pdx_set_layout_builder(pdx_ctx *ctx, int32_t where);
+/* the halo tables (internal order; diagnostic, no reference counterpart).
+ * sizes[5] = blocks, halo entries, bl words, bk words (bl words + tail), LDS slot
+ * cap; all 0 when no tables are built.  hl_ptr (blocks + 1), hl and hs (entries),
+ * bl, bk, anchor (3 per block) may each be NULL. */
+int pdx_get_halo_tables(pdx_ctx *ctx, int64_t *sizes, int32_t *hl_ptr, int32_t *hl, uint16_t *hs,
+                        uint16_t *bl, uint16_t *bk, double *anchor);
+
 /* ---- kernels ------------------------------------------------------------- */
 /* NOSBCompute.compute_shape_tensor, nosb.py:59-94 (FP64 K and inverse) */
 int pdx_shape_tensor(pdx_ctx *ctx, int32_t singular_mode, double tol);

@@ -80,6 +80,8 @@ SIGNATURES = {
     "pdx_get_bonds": [_vp, _i64p, _i32p, _u8p],
     "pdx_get_bond_data": [_vp, _dp, _dp, _dp],
     "pdx_set_broken": [_vp, _u8p],
+    "pdx_set_layout_builder": [_vp, C.c_int32],
+    "pdx_get_halo_tables": [_vp, _i64p, _i32p, _i32p, _vp, _vp, _vp, _dp],
     "pdx_shape_tensor": [_vp, C.c_int32, C.c_double],
     "pdx_compute_F_stress": [_vp],
     "pdx_compute_forces": [_vp],

@@ -620,121 +620,61 @@ extern "C" int pdx_count_neighbors(pdx_ctx *c, double horizon, int32_t *counts)
     return PDX_OK;
 }
 
-// halo tables of the LDS-staged hot passes (Dev::hl_ptr, hl, hs, bl,
-// anchor).  Per block: the halo is its own PDX_BLOCK particles, then every
-// other particle its rows reach in index order (hl); each halo entry gets an
-// LDS slot (hs) whose low 4 bits are the particle's bank colour where the
-// colour classes allow it (slot 16 r + colour for the r-th entry of that
-// colour), the entries of an over-full colour take the remaining free slots,
-// and the block's slot count is the halo size rounded up to 16.  bl holds
-// each bond slot's LDS slot; the anchor is the centre of the block's own
-// particles.  Blocks are built in parallel host threads.  A halo above
-// PDX_HALO_LIMIT slots leaves the tables unbuilt (the halo variants then run
+// halo tables of the LDS-staged hot passes (Dev::hl_ptr, hl, hs, bl, bk,
+// anchor; the layout is described at pdx_layout::halo_block).  The device
+// builder (k_halo_build) makes them by default; the host builder (parallel
+// host threads over pdx_layout::halo_block, the same tables) runs for blocks
+// with more neighbours than the device builder's hash set takes, or when the
+// caller asks for it (pdx_set_layout_builder).  A block with more than
+// PDX_HALO_LIMIT slots, or a context whose records or halo lists exceed
+// 32-bit byte offsets, leaves the tables unbuilt (the halo variants then run
 // the cooperative kernels).
-static int build_halo(pdx_ctx *c)
+static int alloc_halo(pdx_ctx *c, int nbk, size_t nent)
 {
     Dev &d = c->d;
-    dfree(d.hl_ptr); dfree(d.hl); dfree(d.hs); dfree(d.bl); dfree(d.bk); dfree(d.anchor); dfree(d.m64);
-    d.halo_cap = 0;
-    invalidate_graph(c);
+    PDX_TRY(dalloc(c->stream, &d.hl_ptr, (size_t)nbk + 1));
+    PDX_TRY(dalloc(c->stream, &d.hl, std::max<size_t>(nent, 1)));
+    PDX_TRY(dalloc(c->stream, &d.hs, std::max<size_t>(nent, 1)));
+    PDX_TRY(dalloc(c->stream, &d.bl, (size_t)c->slots));
+    PDX_TRY(dalloc(c->stream, &d.bk, (size_t)c->slots + PDX_BK_TAIL));
+    PDX_TRY(dalloc(c->stream, &d.anchor, (size_t)nbk));
+    PDX_TRY(dalloc(c->stream, &d.m64, (size_t)c->ni));
+    c->have_m = false;
+    return PDX_OK;
+}
+
+// block pointers of the halo lists from the per-block halo sizes; false when
+// the tables cannot be built (a block over PDX_HALO_LIMIT slots or lists past
+// 32-bit byte offsets)
+static bool halo_ptr(const std::vector<int32_t> &nh, std::vector<int32_t> &ptr, int64_t &hmax)
+{
+    const int nbk = (int)nh.size();
+    ptr.assign(nbk + 1, 0);
+    hmax = 0;
+    int64_t nent = 0;
+    for (int b = 0; b < nbk; ++b) {
+        const int64_t nslot = (nh[b] + 15) & ~15;
+        if (nslot > PDX_HALO_LIMIT) return false;
+        hmax = std::max(hmax, nslot);
+        nent += nh[b];
+        if (nent * (int64_t)sizeof(int32_t) > (int64_t)UINT32_MAX) return false;
+        ptr[b + 1] = (int32_t)nent;
+    }
+    return true;
+}
+
+static int build_halo_host(pdx_ctx *c)
+{
+    Dev &d = c->d;
     const int nbk = c->ni / PDX_BLOCK, D = c->dim;
-    if (nbk == 0 || c->slots <= 0) return PDX_OK;
-    // the halo staging addresses particle records and halo lists with 32-bit
-    // byte offsets (at32); larger problems run the gathering kernels
-    if ((uint64_t)c->ni * sizeof(XRec) > UINT32_MAX) return PDX_OK;
     std::vector<uint32_t> bj;
     PDX_TRY(download(c, bj, d.bj, (size_t)c->slots));
-    std::vector<std::vector<int32_t>> ent_list(nbk);
-    std::vector<std::vector<uint16_t>> slot_list(nbk);
-    std::vector<int32_t> cap_b(nbk, 0);
+    std::vector<pdx_layout::HaloBlock> blk(nbk);
     std::vector<uint16_t> bl((size_t)c->slots, (uint16_t)PDX_HNONE), bk((size_t)c->slots + PDX_BK_TAIL, 0);
-    std::vector<pdx_d4> anc(nbk);
-    std::vector<uint8_t> over(nbk, 0);
-    constexpr int SPB = PDX_BLOCK / PDX_SLICE;
     auto work = [&](int b0, int b1) {
-        std::vector<int32_t> o, sl, slot_of;
-        for (int b = b0; b < b1; ++b) {
-            const int i0 = b * PDX_BLOCK;
-            const int64_t s_lo = c->h_slptr[(size_t)b * SPB], s_hi = c->h_slptr[(size_t)(b + 1) * SPB];
-            o.clear();
-            for (int64_t s = s_lo; s < s_hi; ++s) {
-                const uint32_t e = bj[s];
-                if (e == 0xffffffffu) continue;
-                const int32_t j = (int32_t)(e & ~PDX_BROKEN);
-                if (j - i0 < 0 || j - i0 >= PDX_BLOCK) o.push_back(j);
-            }
-            std::sort(o.begin(), o.end());
-            o.erase(std::unique(o.begin(), o.end()), o.end());
-            const int nh = PDX_BLOCK + (int)o.size();
-            const int nslot = (nh + 15) & ~15, rows = nslot / 16;
-            if (nslot > PDX_HALO_LIMIT) {
-                over[b] = 1;
-                continue;
-            }
-            std::vector<int32_t> &ent = ent_list[b];
-            ent.resize(nh);
-            for (int t = 0; t < PDX_BLOCK; ++t) ent[t] = i0 + t;
-            std::copy(o.begin(), o.end(), ent.begin() + PDX_BLOCK);
-            // slots by colour, then the overflow into the free slots
-            sl.assign(nslot, -1);
-            slot_of.assign(nh, -1);
-            int cnt[16] = {0};
-            std::vector<int> spill;
-            for (int e = 0; e < nh; ++e) {
-                const int cl = c->colour[ent[e]] & 15;
-                if (cnt[cl] < rows) {
-                    const int sidx = 16 * cnt[cl]++ + cl;
-                    sl[sidx] = ent[e];
-                    slot_of[e] = sidx;
-                } else {
-                    spill.push_back(e);
-                }
-            }
-            size_t k = 0;
-            for (int sidx = 0; sidx < nslot && k < spill.size(); ++sidx)
-                if (sl[sidx] < 0) {
-                    sl[sidx] = ent[spill[k]];
-                    slot_of[spill[k]] = sidx;
-                    ++k;
-                }
-            if (k < spill.size()) {  // cannot happen: nslot >= nh
-                over[b] = 1;
-                continue;
-            }
-            slot_list[b].assign(slot_of.begin(), slot_of.end());
-            cap_b[b] = nslot;
-            for (int64_t s = s_lo; s < s_hi; ++s) {
-                const uint32_t e = bj[s];
-                if (e == 0xffffffffu) continue;
-                const int32_t j = (int32_t)(e & ~PDX_BROKEN);
-                int32_t l = j - i0;
-                if (l < 0 || l >= PDX_BLOCK) l = PDX_BLOCK + (int32_t)(std::lower_bound(o.begin(), o.end(), j) - o.begin());
-                bl[s] = (uint16_t)(slot_of[l] | ((e & PDX_BROKEN) ? PDX_HBROKEN : 0u));
-            }
-            // the slot each bond slot reads: its particle's for an intact bond
-            // of an active row, else the row's own (slot_of[t] for row i0 + t)
-            for (int q = 0; q < SPB; ++q) {
-                const int64_t a0 = c->h_slptr[(size_t)b * SPB + q], a1 = c->h_slptr[(size_t)b * SPB + q + 1];
-                for (int64_t s = a0; s < a1; ++s) {
-                    const int t = q * PDX_SLICE + (int)((s - a0) % PDX_SLICE);
-                    const uint16_t e = bl[s];
-                    const bool live = e != PDX_HNONE && !(e & PDX_HBROKEN) && c->perm[i0 + t] >= 0;
-                    bk[s] = live ? e : (uint16_t)slot_of[t];
-                }
-            }
-            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-            for (int t = 0; t < PDX_BLOCK; ++t) {
-                const int32_t r = c->perm[i0 + t];
-                if (r < 0) continue;
-                for (int q = 0; q < D; ++q) {
-                    lo[q] = std::min(lo[q], c->hX[(size_t)r * D + q]);
-                    hi[q] = std::max(hi[q], c->hX[(size_t)r * D + q]);
-                }
-            }
-            double m[3] = {0.0, 0.0, 0.0};
-            for (int q = 0; q < D; ++q) m[q] = lo[q] <= hi[q] ? 0.5 * (lo[q] + hi[q]) : 0.0;
-            anc[b] = d4(m[0], m[1], m[2]);
-        }
+        for (int b = b0; b < b1; ++b)
+            pdx_layout::halo_block(b, D, bj.data(), c->h_slptr, c->colour, c->perm, c->hX.data(), bl.data(),
+                                   bk.data(), blk[b]);
     };
     const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<std::thread> th;
@@ -743,38 +683,93 @@ static int build_halo(pdx_ctx *c)
         th.emplace_back(work, b0, b1);
     }
     for (auto &x : th) x.join();
-    std::vector<int32_t> ptr(nbk + 1, 0);
-    int64_t hmax = 0;
-    int64_t nent = 0;
+    std::vector<int32_t> nh(nbk), ptr;
     for (int b = 0; b < nbk; ++b) {
-        if (over[b]) return PDX_OK;
-        hmax = std::max(hmax, (int64_t)cap_b[b]);
-        nent += (int64_t)ent_list[b].size();
-        if (nent * (int64_t)sizeof(int32_t) > (int64_t)UINT32_MAX) return PDX_OK;
-        ptr[b + 1] = (int32_t)nent;
+        if (blk[b].over) return PDX_OK;
+        nh[b] = (int32_t)blk[b].ent.size();
     }
+    int64_t hmax = 0;
+    if (!halo_ptr(nh, ptr, hmax)) return PDX_OK;
     std::vector<int32_t> hl((size_t)ptr[nbk]);
     std::vector<uint16_t> hs((size_t)ptr[nbk]);
+    std::vector<pdx_d4> anc(nbk);
     for (int b = 0; b < nbk; ++b) {
-        std::copy(ent_list[b].begin(), ent_list[b].end(), hl.begin() + ptr[b]);
-        std::copy(slot_list[b].begin(), slot_list[b].end(), hs.begin() + ptr[b]);
+        std::copy(blk[b].ent.begin(), blk[b].ent.end(), hl.begin() + ptr[b]);
+        std::copy(blk[b].slot.begin(), blk[b].slot.end(), hs.begin() + ptr[b]);
+        anc[b] = d4(blk[b].anchor[0], blk[b].anchor[1], blk[b].anchor[2]);
     }
-    PDX_TRY(dalloc(c->stream, &d.hl_ptr, (size_t)nbk + 1));
-    PDX_TRY(dalloc(c->stream, &d.hl, hl.size()));
-    PDX_TRY(dalloc(c->stream, &d.hs, hs.size()));
-    PDX_TRY(dalloc(c->stream, &d.bl, bl.size()));
-    PDX_TRY(dalloc(c->stream, &d.bk, bk.size()));
-    PDX_TRY(dalloc(c->stream, &d.anchor, anc.size()));
-    PDX_TRY(dalloc(c->stream, &d.m64, (size_t)c->ni));
-    c->have_m = false;
+    PDX_TRY(alloc_halo(c, nbk, hl.size()));
     PDX_TRY(upload(c, d.hl_ptr, ptr));
-    PDX_TRY(upload(c, d.hl, hl));
-    PDX_TRY(upload(c, d.hs, hs));
+    if (!hl.empty()) PDX_TRY(upload(c, d.hl, hl));
+    if (!hs.empty()) PDX_TRY(upload(c, d.hs, hs));
     PDX_TRY(upload(c, d.bl, bl));
     PDX_TRY(upload(c, d.bk, bk));
     PDX_TRY(upload(c, d.anchor, anc));
     d.halo_cap = (int32_t)hmax;
     return PDX_OK;
+}
+
+// the device builder; *fallback = true when a block has more neighbours than
+// its hash set takes (the host builder then makes the tables)
+static int build_halo_device(pdx_ctx *c, bool *fallback)
+{
+    Dev &d = c->d;
+    const int nbk = c->ni / PDX_BLOCK;
+    *fallback = false;
+    uint8_t *dcol = nullptr;
+    int32_t *dnh = nullptr;
+    PDX_HIP(hipMallocAsync((void **)&dcol, (size_t)c->ni, c->stream));
+    hipError_t e = hipMallocAsync((void **)&dnh, sizeof(int32_t) * (size_t)nbk, c->stream);
+    auto done = [&](int r) {
+        hipFreeAsync(dcol, c->stream);
+        if (dnh) hipFreeAsync(dnh, c->stream);
+        hipStreamSynchronize(c->stream);
+        return r;
+    };
+    if (e != hipSuccess) {
+        dnh = nullptr;
+        return done(fail(PDX_ENOMEM, "hipMallocAsync failed"));
+    }
+    if (hipMemcpyAsync(dcol, c->colour.data(), (size_t)c->ni, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return done(fail(PDX_EDEVICE, "hipMemcpyAsync failed"));
+    int r = pdx_launch_halo_build(c, 0, dcol, dnh, nullptr);
+    if (r != PDX_OK) return done(r);
+    std::vector<int32_t> nh(nbk), ptr;
+    if (hipMemcpyAsync(nh.data(), dnh, sizeof(int32_t) * nbk, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return done(fail(PDX_EDEVICE, "reading the halo sizes"));
+    for (int b = 0; b < nbk; ++b)
+        if (nh[b] < 0) {
+            *fallback = true;
+            return done(PDX_OK);
+        }
+    int64_t hmax = 0;
+    if (!halo_ptr(nh, ptr, hmax)) return done(PDX_OK);
+    if ((r = alloc_halo(c, nbk, (size_t)ptr[nbk])) != PDX_OK) return done(r);
+    if ((r = upload(c, d.hl_ptr, ptr)) != PDX_OK) return done(r);
+    if ((r = pdx_launch_halo_build(c, 1, dcol, dnh, d.hl_ptr)) != PDX_OK) return done(r);
+    d.halo_cap = (int32_t)hmax;
+    return done(PDX_OK);
+}
+
+static int build_halo(pdx_ctx *c)
+{
+    Dev &d = c->d;
+    dfree(d.hl_ptr); dfree(d.hl); dfree(d.hs); dfree(d.bl); dfree(d.bk); dfree(d.anchor); dfree(d.m64);
+    d.halo_cap = 0;
+    c->have_m = false;
+    invalidate_graph(c);
+    const int nbk = c->ni / PDX_BLOCK;
+    if (nbk == 0 || c->slots <= 0) return PDX_OK;
+    // the halo staging addresses particle records and halo lists with 32-bit
+    // byte offsets (at32); larger problems run the gathering kernels
+    if ((uint64_t)c->ni * sizeof(XRec) > UINT32_MAX) return PDX_OK;
+    if (!c->layout_host) {
+        bool fallback = false;
+        const int r = build_halo_device(c, &fallback);
+        if (r != PDX_OK || !fallback) return r;
+    }
+    return build_halo_host(c);
 }
 
 extern "C" int pdx_build_bonds(pdx_ctx *c, double horizon, int64_t *n_bonds)
@@ -816,6 +811,55 @@ extern "C" int pdx_build_bonds(pdx_ctx *c, double horizon, int64_t *n_bonds)
     c->have_K = c->have_mass = c->have_A = false;
     invalidate_graph(c);
     if (n_bonds) *n_bonds = tot;
+    return PDX_OK;
+}
+
+extern "C" int pdx_set_layout_builder(pdx_ctx *c, int32_t where)
+{
+    CHECK_CTX(c);
+    if (where != PDX_LAYOUT_DEVICE && where != PDX_LAYOUT_HOST) return fail(PDX_EINVAL, "unknown layout builder");
+    c->layout_host = where == PDX_LAYOUT_HOST;
+    return PDX_OK;
+}
+
+extern "C" int pdx_get_halo_tables(pdx_ctx *c, int64_t *sizes, int32_t *hl_ptr, int32_t *hl, uint16_t *hs,
+                                   uint16_t *bl, uint16_t *bk, double *anchor)
+{
+    CHECK_CTX(c);
+    if (!sizes) return fail(PDX_EINVAL, "sizes is NULL");
+    const Dev &d = c->d;
+    const int nbk = c->ni / PDX_BLOCK;
+    if (!c->have_bonds || !d.hl_ptr) {
+        for (int k = 0; k < 5; ++k) sizes[k] = 0;
+        return PDX_OK;
+    }
+    std::vector<int32_t> ptr;
+    PDX_TRY(download(c, ptr, d.hl_ptr, (size_t)nbk + 1));
+    sizes[0] = nbk;
+    sizes[1] = ptr[nbk];
+    sizes[2] = c->slots;
+    sizes[3] = c->slots + PDX_BK_TAIL;
+    sizes[4] = d.halo_cap;
+    auto get = [&](auto *dst, const auto *src, size_t n) -> int {
+        if (!dst || n == 0) return PDX_OK;
+        PDX_HIP(hipMemcpyAsync(dst, src, sizeof(*dst) * n, hipMemcpyDeviceToHost, c->stream));
+        return PDX_OK;
+    };
+    if (hl_ptr) std::copy(ptr.begin(), ptr.end(), hl_ptr);
+    PDX_TRY(get(hl, d.hl, (size_t)ptr[nbk]));
+    PDX_TRY(get(hs, d.hs, (size_t)ptr[nbk]));
+    PDX_TRY(get(bl, d.bl, (size_t)c->slots));
+    PDX_TRY(get(bk, d.bk, (size_t)(c->slots + PDX_BK_TAIL)));
+    if (anchor) {
+        std::vector<pdx_d4> a;
+        PDX_TRY(download(c, a, d.anchor, (size_t)nbk));
+        for (int b = 0; b < nbk; ++b) {
+            anchor[3 * b] = a[b].x;
+            anchor[3 * b + 1] = a[b].y;
+            anchor[3 * b + 2] = a[b].z;
+        }
+    }
+    PDX_HIP(hipStreamSynchronize(c->stream));
     return PDX_OK;
 }
 

@@ -316,6 +316,45 @@ PDX_HD void stress_j2(const double *H, double Kb_, double mu_, const J2Par &pp, 
     P[2] = P[6] = sfin[5];
 }
 
+// Walk the bond slots of a row: slot k's entry is bp[k * PDX_SLICE].  Part p
+// of S takes the groups of G slots starting at p G, p G + G S, ...  Two
+// groups are processed per trip and the entries of the next two are loaded
+// at its start, so an entry load (from HBM; bk is read once per pass) has two
+// groups' arithmetic to arrive in.  The slice width is a multiple of
+// PDX_SLOT_ALIGN, so a group never straddles it, and the look-ahead loads
+// past it land in the next slice or in bk's tail and are never used.  width
+// is the same for the 64 rows of a wave, so the test of the second group is
+// uniform.
+template <int G, int S, typename F>
+PDX_HD void halo_row_loop(const uint16_t *__restrict__ bp, int width, int part, F &&group)
+{
+    static_assert(PDX_SLOT_ALIGN % G == 0 && G * S * 3 + PDX_SLOT_ALIGN <= PDX_BK_TAIL / PDX_SLICE, "group layout");
+    constexpr int STEP = G * S;
+    const int k0 = part * G;
+    auto entry = [&](int k) -> uint32_t { return (uint32_t)bp[(int64_t)k * PDX_SLICE]; };
+    uint32_t ea[G], eb[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+        ea[q] = entry(k0 + q);
+        eb[q] = entry(k0 + STEP + q);
+    }
+    for (int k = k0; k < width; k += 2 * STEP) {
+        uint32_t na[G], nb[G];
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            na[q] = entry(k + 2 * STEP + q);
+            nb[q] = entry(k + 3 * STEP + q);
+        }
+        group(ea);
+        if (k + STEP < width) group(eb);
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            ea[q] = na[q];
+            eb[q] = nb[q];
+        }
+    }
+}
+
 // square root and reciprocal of the FP32 bond arithmetic: the hardware
 // V_SQRT_F32 / V_RCP_F32 (1 ulp) on the GPU, where the IEEE-rounded forms cost
 // about a dozen instructions each; the host build of this header (the CPU

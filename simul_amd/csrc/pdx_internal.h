@@ -25,6 +25,7 @@ struct pdx_ctx {
     bool have_particles = false, have_bonds = false, have_K = false, have_mass = false;
     bool have_A = false;
     bool have_m = false;  // d.m64 holds the halo F-pass's sum_j wx for the current bonds
+    bool layout_host = false;  // build the halo tables on the host (pdx_set_layout_builder)
     int32_t model = PDX_MODEL_LINEAR_ELASTIC, form = PDX_FORM_NOSB;
     double horizon = 0.0, G_s = 0.0;
     int64_t nb = 0, slots = 0;
@@ -76,6 +77,10 @@ int pdx_launch_case_reduce(pdx_ctx *c, int ubuf, int upd, bool solve_mode);
 int pdx_build_m64(pdx_ctx *c);
 int pdx_launch_kd_move(pdx_ctx *c, int ubuf_in, bool solve_mode, double dt, bool apply_reset);
 int pdx_launch_halo_bits(pdx_ctx *c);
+// device builder of the halo tables: mode 0 counts each block's halo into d_nh
+// (-1: too many neighbours for the device builder), mode 1 writes the tables
+// (hl / hs at d_ptr[b], bl, bk, anchor)
+int pdx_launch_halo_build(pdx_ctx *c, int mode, const uint8_t *d_colour, int32_t *d_nh, const int32_t *d_ptr);
 int pdx_halo_lds(const pdx_ctx *c, int entry_bytes, int split, int nv);
 
 // the halo-staged variants: bond group 1 or 2, rows split over 1, 2 or 4 waves

@@ -577,45 +577,6 @@ __device__ __forceinline__ void halo_bond(const pdx_q4 &xj, const pdx_q4 &uj01, 
     du[2] = (float)(b - uo2);
 }
 
-// Walk the bond slots of a row: slot k's entry is bp[k * PDX_SLICE].  Part p
-// of S takes the groups of G slots starting at p G, p G + G S, ...  Two
-// groups are processed per trip and the entries of the next two are loaded
-// at its start, so an entry load (from HBM; bk is read once per pass) has two
-// groups' arithmetic to arrive in.  The slice width is a multiple of
-// PDX_SLOT_ALIGN, so a group never straddles it, and the look-ahead loads
-// past it land in the next slice or in bk's tail and are never used.  width
-// is the same for the 64 rows of a wave, so the test of the second group is
-// uniform.
-template <int G, int S, typename F>
-__device__ __forceinline__ void halo_row_loop(const uint16_t *__restrict__ bp, int width, int part, F &&group)
-{
-    static_assert(PDX_SLOT_ALIGN % G == 0 && G * S * 3 + PDX_SLOT_ALIGN <= PDX_BK_TAIL / PDX_SLICE, "group layout");
-    constexpr int STEP = G * S;
-    const int k0 = part * G;
-    auto entry = [&](int k) -> uint32_t { return (uint32_t)bp[(int64_t)k * PDX_SLICE]; };
-    uint32_t ea[G], eb[G];
-#pragma unroll
-    for (int q = 0; q < G; ++q) {
-        ea[q] = entry(k0 + q);
-        eb[q] = entry(k0 + STEP + q);
-    }
-    for (int k = k0; k < width; k += 2 * STEP) {
-        uint32_t na[G], nb[G];
-#pragma unroll
-        for (int q = 0; q < G; ++q) {
-            na[q] = entry(k + 2 * STEP + q);
-            nb[q] = entry(k + 3 * STEP + q);
-        }
-        group(ea);
-        if (k + STEP < width) group(eb);
-#pragma unroll
-        for (int q = 0; q < G; ++q) {
-            ea[q] = na[q];
-            eb[q] = nb[q];
-        }
-    }
-}
-
 // Split rows: S waves share a block's halo and rows; part p (threadIdx.x /
 // PDX_BLOCK) takes the bond slots whose group index is p mod S, and the
 // partial sums meet in LDS (reusing the halo area) in part order before the
@@ -814,6 +775,212 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
         if (part == 0 && act) force_finish<D, FORM, UPD>(d, i, kr, pi, ai, ubuf, cs, dt, visc, fa, red, fin);
     }
     block_partials<3, PDX_BLOCK * S, PDX_BLOCK / 64>(d, lb, red);  // part 0's waves
+}
+
+// ---------------------------------------------------------------------------
+// device builder of the halo tables: the tables of the host builder
+// (pdx_layout::halo_block), one workgroup of PDX_BLOCK threads per block of
+// rows.  The block's distinct out-of-block neighbours go into an LDS hash set
+// (linear probing, compare-and-swap), are sorted by a bitonic sort and ranked
+// per bank colour with wave ballots in entry order; the rare entries of an
+// over-full colour then take the free slots in ascending order (one thread,
+// in entry order).  MODE 0 only counts the halo (nh[b]: own + distinct
+// neighbours, -1 when more than PDX_HB_MAXOUT neighbours: the host builder
+// takes over); MODE 1 writes hl / hs at ptr[b], bl / bk of the block's slots
+// and its anchor.
+// ---------------------------------------------------------------------------
+#define PDX_HB_HASH 8192
+#define PDX_HB_LOG 13
+#define PDX_HB_MAXOUT 6144
+
+__device__ __forceinline__ void hb_insert(int32_t *tab, int32_t j, int *count)
+{
+    uint32_t h = ((uint32_t)j * 2654435761u) >> (32 - PDX_HB_LOG);
+    for (int probe = 0; probe < PDX_HB_HASH; ++probe) {
+        const int32_t old = atomicCAS(&tab[h], -1, j);
+        if (old == -1) {
+            atomicAdd(count, 1);
+            return;
+        }
+        if (old == j) return;
+        h = (h + 1) & (PDX_HB_HASH - 1);
+    }
+}
+
+template <int D, int MODE>
+__global__ __launch_bounds__(PDX_BLOCK) void k_halo_build(Dev d, const uint8_t *__restrict__ colour,
+                                                          int32_t *__restrict__ nh_out, const int32_t *__restrict__ ptr)
+{
+    extern __shared__ int32_t hb[];
+    int32_t *tab = hb, *arr = hb + PDX_HB_HASH;
+    __shared__ int cnt_s, ext_s;
+    __shared__ int run[16], wtot[PDX_BLOCK / 64][16];
+    __shared__ uint8_t act_s[PDX_BLOCK];
+    __shared__ double rlo[PDX_BLOCK / 64][3], rhi[PDX_BLOCK / 64][3];
+    constexpr int SPB = PDX_BLOCK / PDX_SLICE, NW = PDX_BLOCK / 64;
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int i0 = b * PDX_BLOCK;
+    const int64_t s_lo = d.sl_ptr[(int64_t)b * SPB], s_hi = d.sl_ptr[(int64_t)(b + 1) * SPB];
+    for (int k = tid; k < PDX_HB_HASH; k += PDX_BLOCK) tab[k] = -1;
+    if (tid == 0) {
+        cnt_s = 0;
+        ext_s = 0;
+    }
+    __syncthreads();
+    for (int64_t s = s_lo + tid; s < s_hi; s += PDX_BLOCK) {
+        const uint32_t e = d.bj[s];
+        if (e == 0xffffffffu) continue;
+        const int32_t j = (int32_t)(e & ~PDX_BROKEN);
+        if ((unsigned)(j - i0) >= (unsigned)PDX_BLOCK && __hip_atomic_load(&cnt_s, __ATOMIC_RELAXED,
+                                                                             __HIP_MEMORY_SCOPE_WORKGROUP) <= PDX_HB_MAXOUT)
+            hb_insert(tab, j, &cnt_s);
+    }
+    __syncthreads();
+    const int nu = cnt_s;
+    if constexpr (MODE == 0) {
+        if (tid == 0) nh_out[b] = nu > PDX_HB_MAXOUT ? -1 : PDX_BLOCK + nu;
+        return;
+    } else {
+        if (nu > PDX_HB_MAXOUT) return;  // the host does not launch this mode then
+        // the distinct neighbours in ascending order
+        int P = 1;
+        while (P < nu) P <<= 1;
+        for (int k = tid; k < PDX_HB_HASH; k += PDX_BLOCK) {
+            const int32_t v = tab[k];
+            if (v >= 0) arr[atomicAdd(&ext_s, 1)] = v;
+        }
+        __syncthreads();
+        for (int k = nu + tid; k < P; k += PDX_BLOCK) arr[k] = 0x7fffffff;
+        __syncthreads();
+        for (int k = 2; k <= P; k <<= 1)
+            for (int jj = k >> 1; jj > 0; jj >>= 1) {
+                for (int i = tid; i < P; i += PDX_BLOCK) {
+                    const int ixj = i ^ jj;
+                    if (ixj > i) {
+                        const int32_t x = arr[i], y = arr[ixj];
+                        if ((x > y) == ((i & k) == 0)) {
+                            arr[i] = y;
+                            arr[ixj] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        const int nh = PDX_BLOCK + nu, nslot = (nh + 15) & ~15, rows = nslot >> 4;
+        uint16_t *slot_of = reinterpret_cast<uint16_t *>(tab);  // the hash set is no longer needed
+        auto ent = [&](int e) { return e < PDX_BLOCK ? i0 + e : arr[e - PDX_BLOCK]; };
+        if (tid < 16) run[tid] = 0;
+        act_s[tid] = (d.KR[i0 + tid].flags & PDX_FL_ACTIVE) ? 1 : 0;
+        __syncthreads();
+        // colour ranks in entry order: slot 16 r + colour for r < rows
+        for (int c0 = 0; c0 < nh; c0 += PDX_BLOCK) {
+            const int e = c0 + tid;
+            const bool ok = e < nh;
+            const int cl = ok ? (colour[ent(e)] & 15) : 16;
+            uint64_t mine = 0;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const uint64_t m = __ballot(cl == k);
+                if (cl == k) mine = m;
+                if (lane == k) wtot[wv][k] = __popcll(m);
+            }
+            __syncthreads();
+            if (ok) {
+                int r = run[cl] + __popcll(mine & ((1ull << lane) - 1ull));
+                for (int w = 0; w < wv; ++w) r += wtot[w][cl];
+                slot_of[e] = r < rows ? (uint16_t)(16 * r + cl) : (uint16_t)0xFFFFu;
+            }
+            __syncthreads();
+            if (tid < 16) {
+                int t = 0;
+                for (int w = 0; w < NW; ++w) t += wtot[w][tid];
+                run[tid] += t;
+            }
+            __syncthreads();
+        }
+        // entries of an over-full colour: the free slots, ascending, in entry order
+        if (tid == 0) {
+            bool any = false;
+            for (int c = 0; c < 16; ++c) any |= run[c] > rows;
+            if (any) {
+                int pe = 0;
+                for (int sidx = 0; sidx < nslot; ++sidx) {
+                    const int c = sidx & 15;
+                    if ((sidx >> 4) < min(run[c], rows)) continue;  // taken by its colour
+                    while (pe < nh && slot_of[pe] != 0xFFFFu) ++pe;
+                    if (pe >= nh) break;
+                    slot_of[pe++] = (uint16_t)sidx;
+                }
+            }
+        }
+        __syncthreads();
+        const int32_t p0 = ptr[b];
+        for (int e = tid; e < nh; e += PDX_BLOCK) {
+            d.hl[p0 + e] = ent(e);
+            d.hs[p0 + e] = slot_of[e];
+        }
+        // each bond slot: its particle's LDS slot (bl) and the slot the kernels read (bk)
+        const int64_t a1 = d.sl_ptr[(int64_t)b * SPB + 1], a2 = d.sl_ptr[(int64_t)b * SPB + 2],
+                      a3 = d.sl_ptr[(int64_t)b * SPB + 3];
+        for (int64_t s = s_lo + tid; s < s_hi; s += PDX_BLOCK) {
+            const int q = s < a1 ? 0 : (s < a2 ? 1 : (s < a3 ? 2 : 3));
+            const int64_t aq = q == 0 ? s_lo : (q == 1 ? a1 : (q == 2 ? a2 : a3));
+            const int t = q * PDX_SLICE + (int)((s - aq) & (PDX_SLICE - 1));
+            const uint32_t e = d.bj[s];
+            uint16_t blv = PDX_HNONE, bkv = slot_of[t];
+            if (e != 0xffffffffu) {
+                const int32_t j = (int32_t)(e & ~PDX_BROKEN);
+                int l = j - i0;
+                if ((unsigned)l >= (unsigned)PDX_BLOCK) {
+                    int lo = 0, hi = nu;  // lower bound of j in arr[0, nu)
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (arr[mid] < j) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    l = PDX_BLOCK + lo;
+                }
+                blv = (uint16_t)(slot_of[l] | ((e & PDX_BROKEN) ? PDX_HBROKEN : 0u));
+                if (!(e & PDX_BROKEN) && act_s[t]) bkv = blv;
+            }
+            d.bl[s] = blv;
+            d.bk[s] = bkv;
+        }
+        // anchor: centre of the bounding box of the block's active particles
+        double lo[3], hi[3];
+        const XRec xr = d.xu[0][i0 + tid];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            lo[q] = act_s[tid] ? xr.X[q] : 1e300;
+            hi[q] = act_s[tid] ? xr.X[q] : -1e300;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                lo[q] = fmin(lo[q], __shfl_xor(lo[q], o, 64));
+                hi[q] = fmax(hi[q], __shfl_xor(hi[q], o, 64));
+            }
+        if (lane == 0)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                rlo[wv][q] = lo[q];
+                rhi[wv][q] = hi[q];
+            }
+        __syncthreads();
+        if (tid == 0) {
+            double m[3] = {0.0, 0.0, 0.0};
+            for (int q = 0; q < D; ++q) {
+                double l = rlo[0][q], h = rhi[0][q];
+                for (int w = 1; w < NW; ++w) {
+                    l = fmin(l, rlo[w][q]);
+                    h = fmax(h, rhi[w][q]);
+                }
+                m[q] = l <= h ? 0.5 * (l + h) : 0.0;
+            }
+            d.anchor[b] = d4(m[0], m[1], m[2]);
+        }
+    }
 }
 
 // bl's broken flags from bj (after bj was rewritten) and the slots the halo
@@ -1628,3 +1795,21 @@ int pdx_launch_kr_put(pdx_ctx *c, const float *d_src, int off, int cnt)
     return PDX_OK;
 }
 
+int pdx_launch_halo_build(pdx_ctx *c, int mode, const uint8_t *d_colour, int32_t *d_nh, const int32_t *d_ptr)
+{
+    const int nbk = c->ni / PDX_BLOCK;
+    const size_t lds = sizeof(int32_t) * 2 * PDX_HB_HASH;
+    if (c->dim == 3) {
+        if (mode == 0)
+            hipLaunchKernelGGL((k_halo_build<3, 0>), dim3(nbk), dim3(PDX_BLOCK), lds, c->stream, c->d, d_colour, d_nh, d_ptr);
+        else
+            hipLaunchKernelGGL((k_halo_build<3, 1>), dim3(nbk), dim3(PDX_BLOCK), lds, c->stream, c->d, d_colour, d_nh, d_ptr);
+    } else {
+        if (mode == 0)
+            hipLaunchKernelGGL((k_halo_build<2, 0>), dim3(nbk), dim3(PDX_BLOCK), lds, c->stream, c->d, d_colour, d_nh, d_ptr);
+        else
+            hipLaunchKernelGGL((k_halo_build<2, 1>), dim3(nbk), dim3(PDX_BLOCK), lds, c->stream, c->d, d_colour, d_nh, d_ptr);
+    }
+    PDX_LAUNCH_CHECK();
+    return PDX_OK;
+}

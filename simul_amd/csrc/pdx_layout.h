@@ -220,6 +220,172 @@ inline void block_cases(const std::vector<int32_t> &p0, int nblk, std::vector<in
     }
 }
 
+// ---------------------------------------------------------------------------
+// halo tables of the LDS-staged hot passes (Dev::hl_ptr, hl, hs, bl, bk,
+// anchor), the host builder.  The device builder (k_halo_build in
+// pdx_kernels.hip) produces the same tables; this one is kept for contexts it
+// cannot take and as the reference its output is compared with.
+//
+// Per block of PDX_BLOCK rows: the halo is the block's own particles, then
+// every other particle its rows reach, in index order (hl).  Each halo entry
+// gets an LDS slot (hs) whose low 4 bits are the particle's bank colour where
+// the colour classes allow it (slot 16 r + colour for the r-th entry of that
+// colour, r < rows = slots / 16); the entries of an over-full colour take the
+// remaining free slots in ascending order.  The block's slot count is the
+// halo size rounded up to 16.  bl holds each bond slot's LDS slot (PDX_HBROKEN
+// for a broken bond, PDX_HNONE for no bond); bk the slot the kernels read:
+// bl's slot for an intact bond of an active row, else the row's own slot.
+// The anchor is the centre of the bounding box of the block's own active
+// particles (0 on an axis without any).
+// ---------------------------------------------------------------------------
+struct HaloBlock {
+    std::vector<int32_t> ent;      // halo entries: own particles, then the others ascending
+    std::vector<uint16_t> slot;    // LDS slot of each entry
+    int nslot = 0;                 // slots of the block (halo rounded up to 16)
+    bool over = false;             // more than PDX_HALO_LIMIT slots: no tables
+    double anchor[3] = {0.0, 0.0, 0.0};
+};
+
+// one block's halo; writes the block's bl and bk words (bl must hold
+// PDX_HNONE, bk 0 in the block's slot range beforehand).  perm maps internal
+// to caller rows (-1: padding, an inactive row); X holds the caller-order
+// reference positions (D per row).
+inline void halo_block(int b, int D, const uint32_t *bj, const std::vector<int64_t> &slptr,
+                       const std::vector<uint8_t> &colour, const std::vector<int32_t> &perm, const double *X,
+                       uint16_t *bl, uint16_t *bk, HaloBlock &out)
+{
+    constexpr int SPB = PDX_BLOCK / PDX_SLICE;
+    const int i0 = b * PDX_BLOCK;
+    const int64_t s_lo = slptr[(size_t)b * SPB], s_hi = slptr[(size_t)(b + 1) * SPB];
+    std::vector<int32_t> o;
+    for (int64_t s = s_lo; s < s_hi; ++s) {
+        const uint32_t e = bj[s];
+        if (e == 0xffffffffu) continue;
+        const int32_t j = (int32_t)(e & ~PDX_BROKEN);
+        if (j - i0 < 0 || j - i0 >= PDX_BLOCK) o.push_back(j);
+    }
+    std::sort(o.begin(), o.end());
+    o.erase(std::unique(o.begin(), o.end()), o.end());
+    const int nh = PDX_BLOCK + (int)o.size();
+    const int nslot = (nh + 15) & ~15, rows = nslot / 16;
+    out.ent.clear();
+    out.slot.clear();
+    out.nslot = nslot;
+    out.over = nslot > PDX_HALO_LIMIT;
+    if (out.over) return;
+    std::vector<int32_t> &ent = out.ent;
+    ent.resize(nh);
+    for (int t = 0; t < PDX_BLOCK; ++t) ent[t] = i0 + t;
+    std::copy(o.begin(), o.end(), ent.begin() + PDX_BLOCK);
+    // slots by colour, then the overflow into the free slots
+    std::vector<int32_t> sl(nslot, -1), slot_of(nh, -1), spill;
+    int cnt[16] = {0};
+    for (int e = 0; e < nh; ++e) {
+        const int cl = colour[ent[e]] & 15;
+        if (cnt[cl] < rows) {
+            const int sidx = 16 * cnt[cl]++ + cl;
+            sl[sidx] = ent[e];
+            slot_of[e] = sidx;
+        } else {
+            spill.push_back(e);
+        }
+    }
+    size_t k = 0;
+    for (int sidx = 0; sidx < nslot && k < spill.size(); ++sidx)
+        if (sl[sidx] < 0) {
+            sl[sidx] = ent[spill[k]];
+            slot_of[spill[k]] = sidx;
+            ++k;
+        }
+    out.slot.assign(slot_of.begin(), slot_of.end());
+    for (int64_t s = s_lo; s < s_hi; ++s) {
+        const uint32_t e = bj[s];
+        if (e == 0xffffffffu) continue;
+        const int32_t j = (int32_t)(e & ~PDX_BROKEN);
+        int32_t l = j - i0;
+        if (l < 0 || l >= PDX_BLOCK) l = PDX_BLOCK + (int32_t)(std::lower_bound(o.begin(), o.end(), j) - o.begin());
+        bl[s] = (uint16_t)(slot_of[l] | ((e & PDX_BROKEN) ? PDX_HBROKEN : 0u));
+    }
+    for (int q = 0; q < SPB; ++q) {
+        const int64_t a0 = slptr[(size_t)b * SPB + q], a1 = slptr[(size_t)b * SPB + q + 1];
+        for (int64_t s = a0; s < a1; ++s) {
+            const int t = q * PDX_SLICE + (int)((s - a0) % PDX_SLICE);
+            const uint16_t e = bl[s];
+            const bool live = e != PDX_HNONE && !(e & PDX_HBROKEN) && perm[i0 + t] >= 0;
+            bk[s] = live ? e : (uint16_t)slot_of[t];
+        }
+    }
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int t = 0; t < PDX_BLOCK; ++t) {
+        const int32_t r = perm[i0 + t];
+        if (r < 0) continue;
+        for (int q = 0; q < D; ++q) {
+            lo[q] = std::min(lo[q], X[(size_t)r * D + q]);
+            hi[q] = std::max(hi[q], X[(size_t)r * D + q]);
+        }
+    }
+    for (int q = 0; q < 3; ++q) out.anchor[q] = 0.0;
+    for (int q = 0; q < D; ++q) out.anchor[q] = lo[q] <= hi[q] ? 0.5 * (lo[q] + hi[q]) : 0.0;
+}
+
+// The invariants of a context's halo tables that the halo kernels rely on
+// for their memory accesses; returns "" or the first violation.  nbk blocks,
+// slots SELL slots (bl has slots words, bk slots + PDX_BK_TAIL), cap the LDS
+// slots a launch provides.
+inline std::string check_halo_tables(int nbk, int ni, int64_t slots, const uint32_t *bj,
+                                     const std::vector<int64_t> &slptr, const std::vector<int32_t> &perm,
+                                     const std::vector<int32_t> &ptr, const std::vector<int32_t> &hl,
+                                     const std::vector<uint16_t> &hs, const std::vector<uint16_t> &bl,
+                                     const std::vector<uint16_t> &bk, int cap)
+{
+    constexpr int SPB = PDX_BLOCK / PDX_SLICE;
+    auto err = [](int b, const std::string &m) { return "block " + std::to_string(b) + ": " + m; };
+    if ((int)ptr.size() != nbk + 1 || ptr[0] != 0) return "hl_ptr size";
+    if ((int64_t)bl.size() != slots || (int64_t)bk.size() != slots + PDX_BK_TAIL) return "bl / bk size";
+    if (cap % 16 != 0 || cap > PDX_HALO_LIMIT) return "slot cap not a multiple of 16 within the limit";
+    std::vector<int32_t> ent_of_slot;
+    for (int b = 0; b < nbk; ++b) {
+        const int i0 = b * PDX_BLOCK, nh = ptr[b + 1] - ptr[b];
+        const int nslot = (nh + 15) & ~15;
+        if (nh < PDX_BLOCK) return err(b, "halo smaller than the block");
+        if (nslot > cap) return err(b, "more slots than the launch provides");
+        ent_of_slot.assign(nslot, -1);
+        for (int e = 0; e < nh; ++e) {
+            const int32_t j = hl[ptr[b] + e];
+            const int sidx = hs[ptr[b] + e];
+            if (e < PDX_BLOCK ? j != i0 + e : (j < 0 || j >= ni || (j >= i0 && j < i0 + PDX_BLOCK)))
+                return err(b, "halo entry out of place");
+            if (e > PDX_BLOCK && j <= hl[ptr[b] + e - 1]) return err(b, "halo not ascending");
+            if (sidx >= nslot) return err(b, "LDS slot beyond the block's slots");
+            if (ent_of_slot[sidx] >= 0) return err(b, "two entries in one LDS slot");
+            ent_of_slot[sidx] = j;
+        }
+        for (int q = 0; q < SPB; ++q) {
+            const int64_t a0 = slptr[(size_t)b * SPB + q], a1 = slptr[(size_t)b * SPB + q + 1];
+            for (int64_t s = a0; s < a1; ++s) {
+                const int t = q * PDX_SLICE + (int)((s - a0) % PDX_SLICE);
+                const uint32_t e = bj[s];
+                const uint16_t l = bl[s], k = bk[s];
+                const int own = hs[ptr[b] + t];
+                if (e == 0xffffffffu) {
+                    if (l != PDX_HNONE) return err(b, "bl of an empty slot");
+                } else {
+                    const int sidx = l & ~PDX_HBROKEN;
+                    if (l == PDX_HNONE || sidx >= nslot || ent_of_slot[sidx] != (int32_t)(e & ~PDX_BROKEN))
+                        return err(b, "bl does not name the bond's particle");
+                    if (((l & PDX_HBROKEN) != 0) != ((e & PDX_BROKEN) != 0)) return err(b, "bl broken bit");
+                }
+                if (k >= nslot) return err(b, "bk beyond the block's slots");
+                const bool live = e != 0xffffffffu && !(e & PDX_BROKEN) && perm[i0 + t] >= 0;
+                if (k != (live ? (int)l : own)) return err(b, "bk neither the bond's nor the row's slot");
+            }
+        }
+    }
+    for (int64_t s = slots; s < slots + PDX_BK_TAIL; ++s)
+        if (bk[s] >= cap) return "bk tail beyond the slots";
+    return "";
+}
+
 // bonds in caller order: for every caller row the neighbours' caller indices
 // sorted ascending, the bj slot of each entry and its position in internal
 // CSR order.  Returns false when a row holds an entry that is not a particle

@@ -95,3 +95,16 @@ def test_layout_bank_colours_and_lattice_quantisation(exe):
                        env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"))
     assert r.returncode == 0, r.stderr[-2000:]
     assert "layout ok" in r.stdout
+
+
+def test_halo_tables_in_bounds(exe):
+    """The halo tables of the LDS-staged kernels from the library's host
+    builder on lattices, a padded three-case batch, a 2-D plate, a coarse
+    vertebral body and a wide horizon, with broken bonds: every table entry
+    within its block's LDS slots, and the kernels' row loop (every bond group
+    and split, with its look-ahead loads) inside the bk array the library
+    allocates (tests/cpu_check/halo_check.cpp, AddressSanitizer / UBSan)."""
+    r = subprocess.run([os.path.join(HERE, "halo_check")], capture_output=True, text=True, timeout=300,
+                       env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0", UBSAN_OPTIONS="halt_on_error=1"))
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "halo ok" in r.stdout

@@ -555,3 +555,81 @@ def test_spectral_mass_falls_back_when_a_case_diverges(gpu):
     r2 = body.solve("adaptive", tol=1e-6, ref=cold, mass="spectral", max_iterations=20000)
     assert r2.converged and r2.iterations == r1.iterations
     assert np.array_equal(body.u, u1)
+
+
+def _tables(body):
+    """The context's halo tables (pdx_get_halo_tables), or None when unbuilt."""
+    from simul_amd import _native as NN
+    sizes = np.zeros(5, np.int64)
+    body.ctx.call("pdx_get_halo_tables", NN.ptr(sizes, NN._i64p), None, None, None, None, None, None)
+    nbk, nent, nbl, nbkw, cap = (int(v) for v in sizes)
+    if nbk == 0:
+        return None
+    ptr = np.zeros(nbk + 1, np.int32)
+    hl = np.zeros(max(nent, 1), np.int32)
+    hs = np.zeros(max(nent, 1), np.uint16)
+    bl = np.zeros(nbl, np.uint16)
+    bk = np.zeros(nbkw, np.uint16)
+    anc = np.zeros(3 * nbk)
+    body.ctx.call("pdx_get_halo_tables", NN.ptr(sizes, NN._i64p), NN.ptr(ptr, NN._i32p), NN.ptr(hl, NN._i32p),
+                  NN.ptr(hs), NN.ptr(bl), NN.ptr(bk), NN.ptr(anc, NN._dp))
+    return {"cap": cap, "ptr": ptr, "hl": hl[:nent], "hs": hs[:nent], "bl": bl, "bk": bk, "anchor": anc}
+
+
+def _layout_pair(X, h, off=None, **kw):
+    n, dim = X.shape
+    bulk, shear = W.bulk_shear(np.full(n, 15000.0), 0.3)
+    out = []
+    for where in (1, 0):  # PDX_LAYOUT_HOST, PDX_LAYOUT_DEVICE
+        b = PDBody(X, 1.0, h, bulk, shear, case_offsets=off, build=False, **kw)
+        b.ctx.call("pdx_set_layout_builder", where)
+        b.build_bonds(h)
+        out.append((b, _tables(b)))
+    return out
+
+
+@pytest.mark.parametrize("case", ["lattice", "batch", "plate_2d", "wide_horizon", "vertebra", "batch64"])
+def test_device_layout_equals_host_layout(gpu, case):
+    """The device builder of the halo tables (k_halo_build) gives the host
+    builder's tables bit for bit: halo lists, LDS slots (the colour classes and
+    the over-full colours' free slots), bl / bk and the FP64 anchors; after
+    bonds break (bl / bk rewritten by the damage kernel) both stay equal, and
+    the forces of the two contexts are identical."""
+    off = None
+    h = 3.015
+    if case == "lattice":
+        X = lattice(20, 14, 11, jitter=0.2, seed=3)
+    elif case == "batch":
+        specs = [lattice(9, 8, 7), lattice(6, 7, 5, jitter=0.2, seed=4) + 50.0, lattice(13, 5, 4, jitter=0.1, seed=5)]
+        X = np.concatenate(specs)
+        off = np.cumsum([0] + [len(s) for s in specs])
+    elif case == "plate_2d":
+        g = np.arange(60.0)
+        X = np.stack(np.meshgrid(g, g[:41], indexing="ij"), -1).reshape(-1, 2)
+        X = X + np.random.default_rng(6).uniform(-0.2, 0.2, X.shape)
+    elif case == "wide_horizon":
+        X = lattice(16, 16, 12, jitter=0.15, seed=7)
+        h = 4.5
+    elif case == "vertebra":
+        spec = W.vertebra(0.4)
+        X, h = spec.X, spec.horizon
+    else:
+        spec, off = W.pack(W.batch())
+        X, h = spec.X, spec.horizon
+    (bh, th), (bd, td) = _layout_pair(X, h, off)
+    assert th is not None and td is not None
+    for k in ("cap", "ptr", "hl", "hs", "bl", "bk", "anchor"):
+        assert np.array_equal(th[k], td[k]), (case, k)
+    if case in ("vertebra", "batch64"):
+        return
+    u = smooth_u(X, amp=0.03, seed=9)
+    for b in (bh, bd):
+        b.u = u
+        assert b.break_bonds(0.01) > 0
+    th, td = _tables(bh), _tables(bd)
+    for k in ("bl", "bk"):
+        assert np.array_equal(th[k], td[k]), (case, k)
+    if X.shape[1] == 3:
+        for b in (bh, bd):
+            b.shape_tensor()
+        assert np.array_equal(bh.compute_forces(), bd.compute_forces())
