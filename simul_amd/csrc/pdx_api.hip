@@ -104,6 +104,22 @@ static HostMirror &mirror(pdx_ctx *c)
     return g_mirror[c];
 }
 
+// fn(i0, i1) over [0, n) in contiguous chunks on host threads (set-up loops
+// over millions of particles)
+template <typename F>
+static void par_for(int64_t n, F &&fn)
+{
+    const int nth = (int)std::max<int64_t>(1, std::min<int64_t>((n + 65535) / 65536,
+                                                                std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+    if (nth == 1) {
+        fn((int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int k = 0; k < nth; ++k) th.emplace_back([&, k] { fn(n * k / nth, n * (k + 1) / nth); });
+    for (auto &t : th) t.join();
+}
+
 // ---------------------------------------------------------------------------
 // scratch of the allocation-free entry point (pdx_cold_residual_noalloc)
 // ---------------------------------------------------------------------------
@@ -301,22 +317,26 @@ static const int KR_OFF_BULK = 12, KR_OFF_FLAGS = 14, KR_OFF_NNB = 15;
 static int push_flags(pdx_ctx *c)
 {
     HostMirror &m = mirror(c);
-    std::vector<float> w(c->ni);
-    // keep the singular bit computed on the device
-    std::vector<KRec> kr;
-    PDX_TRY(download(c, kr, c->d.KR, c->ni));
-    for (int i = 0; i < c->ni; ++i) {
-        uint32_t fl = kr[i].flags & PDX_FL_SING;
-        int r = c->perm[i];
-        if (r >= 0) {
-            fl |= PDX_FL_ACTIVE | (m.fixed[r] & PDX_FL_FIXMASK);
+    std::vector<uint32_t> w(c->ni, 0u);
+    par_for(c->ni, [&](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) {
+            const int r = c->perm[i];
+            if (r < 0) continue;
+            uint32_t fl = PDX_FL_ACTIVE | (m.fixed[r] & PDX_FL_FIXMASK);
             bool loaded = false;
             for (int q = 0; q < c->dim; ++q) loaded |= m.fext[(size_t)r * c->dim + q] != 0.0;
             if (loaded) fl |= PDX_FL_LOADED;
+            w[i] = fl;
         }
-        memcpy(&w[i], &fl, 4);
-    }
-    return kr_put(c, w, KR_OFF_FLAGS, 1);
+    });
+    // the singular bit computed on the device stays
+    uint32_t *tmp = nullptr;
+    PDX_HIP(hipMallocAsync((void **)&tmp, sizeof(uint32_t) * w.size(), c->stream));
+    PDX_HIP(hipMemcpyAsync(tmp, w.data(), sizeof(uint32_t) * w.size(), hipMemcpyHostToDevice, c->stream));
+    int r = pdx_launch_kr_merge(c, tmp, KR_OFF_FLAGS, PDX_FL_SING);
+    hipFreeAsync(tmp, c->stream);
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    return r;
 }
 
 // caller-order n*dim doubles -> internal pdx_d4
@@ -437,17 +457,19 @@ static int push_prec(pdx_ctx *c)
     HostMirror &m = mirror(c);
     const int D = c->dim;
     std::vector<XRec> hp(c->ni);
-    for (int i = 0; i < c->ni; ++i) {
-        XRec p;
-        memset(&p, 0, sizeof(p));
-        int r = c->perm[i];
-        if (r >= 0) {
-            for (int q = 0; q < D; ++q) p.X[q] = c->hX[(size_t)r * D + q];
-            p.V = (float)m.vol[r];
-            p.c = (float)m.cst[r];
+    par_for(c->ni, [&](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) {
+            XRec p;
+            memset(&p, 0, sizeof(p));
+            int r = c->perm[i];
+            if (r >= 0) {
+                for (int q = 0; q < D; ++q) p.X[q] = c->hX[(size_t)r * D + q];
+                p.V = (float)m.vol[r];
+                p.c = (float)m.cst[r];
+            }
+            hp[i] = p;
         }
-        hp[i] = p;
-    }
+    });
     XRec *tmp = nullptr;
     PDX_HIP(hipMallocAsync((void **)&tmp, sizeof(XRec) * hp.size(), c->stream));
     PDX_HIP(hipMemcpyAsync(tmp, hp.data(), sizeof(XRec) * hp.size(), hipMemcpyHostToDevice, c->stream));
@@ -546,7 +568,18 @@ extern "C" int pdx_set_stabilization(pdx_ctx *c, double G_s, const double *cst, 
     if (!c->have_particles) return fail(PDX_ESTATE, "set particles first");
     HostMirror &m = mirror(c);
     for (int r = 0; r < c->n; ++r) m.cst[r] = cst ? cst[r] : c_scalar;
-    int r = push_prec(c);
+    // only the c word of the gather records changes
+    std::vector<float> w(c->ni, 0.f);
+    par_for(c->ni, [&](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i)
+            if (c->perm[i] >= 0) w[i] = (float)m.cst[c->perm[i]];
+    });
+    float *tmp = nullptr;
+    PDX_HIP(hipMallocAsync((void **)&tmp, sizeof(float) * w.size(), c->stream));
+    PDX_HIP(hipMemcpyAsync(tmp, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice, c->stream));
+    int r = pdx_launch_xu_c(c, tmp);
+    hipFreeAsync(tmp, c->stream);
+    PDX_HIP(hipStreamSynchronize(c->stream));
     c->G_s = G_s;
     c->d.G_s = G_s;
     c->have_mass = false;

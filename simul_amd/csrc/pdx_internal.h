@@ -92,6 +92,10 @@ int pdx_halo_lds(const pdx_ctx *c, int entry_bytes, int split, int nv);
 int pdx_launch_resid(pdx_ctx *c);
 int pdx_launch_zero_free(pdx_ctx *c, int ubuf);
 int pdx_launch_kr_put(pdx_ctx *c, const float *d_src, int off, int cnt);
+// KRec word `off` of every particle: bits in keep kept, the rest from d_src
+int pdx_launch_kr_merge(pdx_ctx *c, const uint32_t *d_src, int off, uint32_t keep);
+// the stabilisation modulus c of both gather buffers from d_src (internal order)
+int pdx_launch_xu_c(pdx_ctx *c, const float *d_src);
 int pdx_launch_xu_geom(pdx_ctx *c, const XRec *d_src);
 int pdx_launch_nbr_mean(pdx_ctx *c, const double *d_in, double *d_out, int width);
 // power-iteration steps of pdx_adr_mass_auto: what 0 init v, 1 perturb u, 2 apply

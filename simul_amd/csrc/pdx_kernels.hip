@@ -1091,6 +1091,24 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_kr_put(Dev d, const float *src, i
     for (int q = 0; q < cnt; ++q) w[off + q] = src[(int64_t)i * cnt + q];
 }
 
+// one 32-bit KRec word per particle: the bits in keep stay, the others come from src
+__global__ __launch_bounds__(PDX_BLOCK) void k_kr_merge(Dev d, const uint32_t *src, int off, uint32_t keep)
+{
+    const int i = blockIdx.x * PDX_BLOCK + threadIdx.x;
+    if (i >= d.ni) return;
+    uint32_t *w = reinterpret_cast<uint32_t *>(&d.KR[i]);
+    w[off] = (w[off] & keep) | (src[i] & ~keep);
+}
+
+// the stabilisation modulus word of both gather buffers
+__global__ __launch_bounds__(PDX_BLOCK) void k_xu_c(Dev d, const float *src)
+{
+    const int i = blockIdx.x * PDX_BLOCK + threadIdx.x;
+    if (i >= d.ni) return;
+    d.xu[0][i].c = src[i];
+    d.xu[1][i].c = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -1791,6 +1809,20 @@ int pdx_launch_xu_geom(pdx_ctx *c, const XRec *d_src)
 int pdx_launch_kr_put(pdx_ctx *c, const float *d_src, int off, int cnt)
 {
     hipLaunchKernelGGL(k_kr_put, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_src, off, cnt);
+    PDX_LAUNCH_CHECK();
+    return PDX_OK;
+}
+
+int pdx_launch_kr_merge(pdx_ctx *c, const uint32_t *d_src, int off, uint32_t keep)
+{
+    hipLaunchKernelGGL(k_kr_merge, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_src, off, keep);
+    PDX_LAUNCH_CHECK();
+    return PDX_OK;
+}
+
+int pdx_launch_xu_c(pdx_ctx *c, const float *d_src)
+{
+    hipLaunchKernelGGL(k_xu_c, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_src);
     PDX_LAUNCH_CHECK();
     return PDX_OK;
 }

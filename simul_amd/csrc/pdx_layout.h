@@ -7,7 +7,9 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <atomic>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -81,9 +83,11 @@ inline void morton_order(int D, const double *X, const double *vol, const std::v
     perm.assign(p0[nc], -1);
     inv.assign(off[nc], -1);
     colour.assign(p0[nc], 0);
-    for (int k = 0; k < nc; ++k) {
+    // cases are independent: host threads take them in turn
+    std::atomic<int> next{0};
+    auto one_case = [&](int k) {
         const int64_t r0 = off[k], r1 = off[k + 1], nk = r1 - r0;
-        if (nk == 0) continue;
+        if (nk == 0) return;
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
         for (int64_t r = r0; r < r1; ++r)
             for (int q = 0; q < D; ++q) {
@@ -112,7 +116,15 @@ inline void morton_order(int D, const double *X, const double *vol, const std::v
             inv[key[t].second] = (int32_t)(p0[k] + t);
             colour[p0[k] + t] = col[key[t].second - r0];
         }
-    }
+    };
+    auto worker = [&]() {
+        for (int k = next++; k < nc; k = next++) one_case(k);
+    };
+    const int nth = std::max(1, std::min(nc, (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+    std::vector<std::thread> th;
+    for (int t = 1; t < nth; ++t) th.emplace_back(worker);
+    worker();
+    for (auto &x : th) x.join();
 }
 
 // per-case cell grids of edge 1.01 * horizon (reference neighbor.py:47) over the
