@@ -106,7 +106,7 @@ def main():
     ap.add_argument("--variant", type=str, default="0",
                     help="hot-pass variant (1, 2, 4, 17, 18, 33, 34, 36, 97, 98, 100, 161, 162, 164) or 'F,G' for the F-pass "
                          "and the force pass; 0 = autotune")
-    ap.add_argument("--workload", default="vertebra", choices=["vertebra", "vertebra_j2", "batch"],
+    ap.add_argument("--workload", default="vertebra", choices=["vertebra", "vertebra_j2", "batch", "nn"],
                     help="vertebra: the headline (dx 0.4, Neo-Hookean, pressure); vertebra_j2: J2, 20 "
                          "prescribed increments with breaking (dx 0.25); batch: 64 random cases in one context")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -136,6 +136,9 @@ def main():
         return
     if args.workload == "batch":
         _print(run_batch(args, torch))
+        return
+    if args.workload == "nn":
+        _print(run_nn(args, torch))
         return
 
     t0 = time.perf_counter()
@@ -462,6 +465,81 @@ def run_batch(args, torch):
                  {"workload": packed.name, "cases": len(specs), "n_particles": packed.n, "n_bonds": nb,
                   "variant": variant},
                  time_to_solve=tts, roofline=roof, cpu_baseline=cpu)
+
+
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+
+
+def run_nn(args, torch):
+    """The learned warm start's layer GEMMs on the BF16 MFMA kernels
+    (include/pdx_nn.h): forward, data gradient and weight gradient of every
+    layer of the 64-256-256-256-32 network at the training batch (65 536
+    rows) and at the dx 0.4 vertebral body (365 056 rows, forward only), timed
+    with HIP events on the stream the kernels run on (`--steps` launches
+    each).  value = FLOP/s of the 256 x 256 forward layer at the training batch."""
+    import ctypes as C
+
+    import numpy as np
+
+    from simul_amd import _native as N
+    from simul_amd.learned.net import DIMS
+
+    lib = N.load()
+    st = torch.cuda.current_stream()
+    sp = C.c_void_p(st.cuda_stream)
+    p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+    gen = torch.Generator(device="cuda").manual_seed(1)
+    reps = max(5, args.steps)
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e3  # us
+
+    def bf(shape):
+        x = torch.randn(shape, device="cuda", generator=gen, dtype=torch.float32) * 0.1
+        y = torch.empty(shape, dtype=torch.int16, device="cuda")
+        assert lib.pdx_nn_to_bf16(p(x), x.numel(), p(y), sp) == 0
+        return y
+
+    rows = []
+    for M, grads in ((65536, True), (365056, False)):
+        for li, (k, n) in enumerate(zip(DIMS[:-1], DIMS[1:])):
+            X, W, dY = bf((M, k)), bf((k, n)), bf((M, n))
+            b = torch.zeros(n, dtype=torch.float32, device="cuda")
+            Y = torch.empty((M, n), dtype=torch.int16, device="cuda")
+            flop = 2.0 * M * k * n
+            us = timed(lambda: lib.pdx_nn_linear_fwd(p(X), p(W), p(b), M, k, n, 1, p(Y), None, n, sp))
+            rec = {"rows": M, "k": k, "n": n, "fwd_us": round(us, 2), "fwd_tflops": round(flop / us / 1e6, 1),
+                   "fwd_frac_peak": round(flop / us / 1e6 / BF16_PEAK_TFLOPS, 4),
+                   "fwd_GBs": round((M * (k + n) * 2 + k * n * 2) / us / 1e3, 1)}
+            if grads:
+                dX = torch.empty((M, k), dtype=torch.int16, device="cuda")
+                us_d = timed(lambda: lib.pdx_nn_linear_dgrad(p(dY), p(W), M, k, n, n, p(X), p(dX), sp))
+                ns = (M + 4095) // 4096
+                part = torch.empty((ns * k * n,), dtype=torch.float32, device="cuda")
+                gW = torch.empty((k, n), dtype=torch.float32, device="cuda")
+                gb = torch.empty((n,), dtype=torch.float32, device="cuda")
+                us_w = timed(lambda: lib.pdx_nn_linear_wgrad(p(X), p(dY), M, k, n, n, 4096, p(part), p(gW), p(gb), sp))
+                rec.update({"dgrad_us": round(us_d, 2), "dgrad_tflops": round(flop / us_d / 1e6, 1),
+                            "wgrad_us": round(us_w, 2), "wgrad_tflops": round(flop / us_w / 1e6, 1)})
+            rows.append(rec)
+            log(f"M {M} {k}x{n}: " + ", ".join(f"{a} {b}" for a, b in rec.items() if a not in ("rows", "k", "n")))
+    head = next(r for r in rows if r["rows"] == 65536 and r["k"] == 256 and r["n"] == 256)
+    flop = 2.0 * 65536 * 256 * 256
+    return _line("BF16 MFMA GEMM FLOP/s (learned warm start, 256x256 layer, 65 536 rows)", flop / head["fwd_us"] * 1e6,
+                 "FLOP/s", reps, 3, head["fwd_us"] / 1e3,
+                 {"workload": "warmstart_mlp_64-256-256-256-32", "rows": [65536, 365056]},
+                 roofline={"bound": "mfma", "kernel": "k_nn_gemm_lds", "achieved": head["fwd_tflops"],
+                           "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": head["fwd_frac_peak"],
+                           "traffic": None},
+                 layers=rows, cpu_baseline=None)
 
 
 def copy_calibration(torch):

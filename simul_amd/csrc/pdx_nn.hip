@@ -85,6 +85,262 @@ __global__ __launch_bounds__(256) void k_nn_gemm(const uint16_t *__restrict__ A,
     }
 }
 
+// ---------------------------------------------------------------------------
+// The layer GEMMs of the forward pass and the data gradient on the gfx950
+// 32x32x16 BF16 MFMA, with the whole weight operand resident in LDS.
+//
+// C (M x NC) = A (M x KR) B (KR x NC), KR <= 256 the reduction (a layer's input
+// or output width), NC <= 256 the output columns.  A is row-major with
+// 16-byte aligned rows; B(k, j) = B[k * sb0 + j * sb1] (W, or W^T for the data
+// gradient).  A persistent grid of at most one block per CU stages B once,
+// transposed to column-major with rows of KR + 8 values (the padding spreads
+// the lanes that read one 16-byte slice over the LDS banks), then walks tiles
+// of 128 rows.  Each of the 4 waves (one per SIMD) owns RT row tiles x CT
+// column tiles of 32 x 32 (WC waves across the columns).  A lane's A operands
+// of a whole tile (8 consecutive values of its row per 16-wide k step, 16-byte
+// loads; every row is read once) are loaded into registers before the tile's
+// MFMAs, and those of the next tile while they run, so the loads' latency
+// is hidden behind a tile of MFMA work even with one wave per SIMD.
+// MFMA operand layout (32x32x16): lane l holds row / column l % 32 and k
+// 8 (l / 32) .. 8 (l / 32) + 7; result register v of lane l is row
+// 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32.
+// ---------------------------------------------------------------------------
+typedef __bf16 pdx_bf8 __attribute__((ext_vector_type(8)));
+typedef __bf16 pdx_bf4v __attribute__((ext_vector_type(4)));
+
+template <int RT, int KS>
+__device__ __forceinline__ void nn_load_a(const uint16_t *__restrict__ A, int64_t lda, int M, int row0, int l32, int kh,
+                                          uint4 (&af)[RT][KS])
+{
+#pragma unroll
+    for (int a = 0; a < RT; ++a) {
+        const int r = row0 + a * 32 + l32;
+        const bool in = r < M;
+        const uint16_t *p = A + (int64_t)(in ? r : 0) * lda + kh;
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+            af[a][s] = in ? *reinterpret_cast<const uint4 *>(p + 16 * s) : make_uint4(0, 0, 0, 0);
+    }
+}
+
+template <int EPI, int KR, int NC, int RT, int CT, int WC>
+__global__ __launch_bounds__(256) void k_nn_gemm_lds(const uint16_t *__restrict__ A, int64_t lda,
+                                                     const uint16_t *__restrict__ B, int64_t sb0, int64_t sb1, int M,
+                                                     const float *__restrict__ bias, int relu,
+                                                     const uint16_t *__restrict__ act, int64_t lda_act,
+                                                     uint16_t *__restrict__ ybf, float *__restrict__ yf, int64_t ldy)
+{
+    static_assert(KR % 16 == 0 && KR <= 256 && (4 / WC) * RT * 32 == 128 && WC * CT * 32 == NC, "tile shape");
+    constexpr int KP = KR + 8;  // LDS row of one B column (bf16 values)
+    constexpr int KS = KR / 16; // k steps
+    extern __shared__ uint16_t bt[];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wr = wv / WC, wc = wv % WC;
+    const int l32 = lane & 31, kh = (lane >> 5) * 8;
+    const int ntile = (M + 127) / 128;
+    // this tile's A operands; each is reloaded with the next tile's as soon as
+    // its MFMAs have been issued
+    uint4 af[RT][KS];
+    int tile = blockIdx.x;
+    if (tile < ntile) nn_load_a<RT, KS>(A, lda, M, tile * 128 + wr * RT * 32, l32, kh, af);
+    // B -> LDS, column-major: bt[j * KP + k] = B(k, j), 16-byte loads along the contiguous index
+    // (a thread's loads go out in batches of 8 before their LDS stores)
+    constexpr int NL = (KR * NC / 8 + 255) / 256;
+    static_assert(KR % 8 == 0 && NC % 8 == 0, "8 x 8 staging blocks");
+    if (sb1 == 1 && (sb0 & 7) == 0 && !(reinterpret_cast<uintptr_t>(B) & 15)) {
+        // row-major B: a thread takes 8 x 8 sub-blocks (8 rows of 8 values, 16-byte
+        // loads), transposes them in registers and writes 8 columns of 8 k values
+        // (16-byte LDS stores)
+        constexpr int NB = (KR / 8) * (NC / 8);
+        for (int blk = tid; blk < NB; blk += 256) {
+            const int kb = blk / (NC / 8), jb = blk % (NC / 8);
+            uint32_t w[8][4];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const uint4 r = *reinterpret_cast<const uint4 *>(B + (int64_t)(8 * kb + i) * sb0 + 8 * jb);
+                w[i][0] = r.x;
+                w[i][1] = r.y;
+                w[i][2] = r.z;
+                w[i][3] = r.w;
+            }
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                uint32_t o[4];
+#pragma unroll
+                for (int p2 = 0; p2 < 4; ++p2) {
+                    const uint32_t lo = w[2 * p2][c >> 1], hi = w[2 * p2 + 1][c >> 1];
+                    o[p2] = (c & 1) ? ((lo >> 16) | (hi & 0xffff0000u)) : ((lo & 0xffffu) | (hi << 16));
+                }
+                *reinterpret_cast<uint4 *>(&bt[(8 * jb + c) * KP + 8 * kb]) = make_uint4(o[0], o[1], o[2], o[3]);
+            }
+        }
+    } else if (sb0 == 1 && (sb1 & 7) == 0 && !(reinterpret_cast<uintptr_t>(B) & 15)) {
+        for (int t0 = 0; t0 < NL; t0 += 8) {
+            uint4 v[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int e = tid + 256 * (t0 + t);
+                v[t] = e < NC * (KR / 8)
+                           ? *reinterpret_cast<const uint4 *>(B + (int64_t)(e / (KR / 8)) * sb1 + (e % (KR / 8)) * 8)
+                           : make_uint4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int e = tid + 256 * (t0 + t);
+                if (e < NC * (KR / 8)) *reinterpret_cast<uint4 *>(&bt[(e / (KR / 8)) * KP + (e % (KR / 8)) * 8]) = v[t];
+            }
+        }
+    } else {
+        for (int e = tid; e < KR * NC; e += 256) {
+            const int k = e / NC, j = e % NC;
+            bt[j * KP + k] = B[(int64_t)k * sb0 + (int64_t)j * sb1];
+        }
+    }
+    // the bias in LDS too: an epilogue load from global memory would have to
+    // wait for the next tile's operand loads issued before it
+    float *bl = reinterpret_cast<float *>(bt + NC * KP);
+    if (EPI == NN_EPI_FWD)
+        for (int j = tid; j < NC; j += 256) bl[j] = bias ? bias[j] : 0.f;
+    __syncthreads();
+    for (; tile < ntile; tile += gridDim.x) {
+        const int row_base = tile * 128 + wr * RT * 32;
+        // the next tile's rows: operand s of a row is loaded into the register
+        // of this tile's operand s once that has been used
+        const bool more = tile + (int)gridDim.x < ntile;
+        const uint16_t *nrow[RT];
+        bool nin[RT];
+#pragma unroll
+        for (int a = 0; a < RT; ++a) {
+            const int r = (tile + (int)gridDim.x) * 128 + wr * RT * 32 + a * 32 + l32;
+            nin[a] = more && r < M;
+            nrow[a] = A + (int64_t)(nin[a] ? r : 0) * lda + kh;
+        }
+        // the data gradient's ReLU mask of this tile, loaded before the next
+        // tile's operands so that waiting for it does not wait for them
+        uint2 mk[EPI == NN_EPI_DGRAD ? RT : 1][EPI == NN_EPI_DGRAD ? CT * 4 : 1];
+        if (EPI == NN_EPI_DGRAD) {
+#pragma unroll
+            for (int a = 0; a < RT; ++a) {
+                const int i = row_base + a * 32 + l32;
+#pragma unroll
+                for (int b = 0; b < CT; ++b)
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        const int j = (wc * CT + b) * 32 + 8 * g + 4 * (lane >> 5);
+                        mk[a][b * 4 + g] = (act && i < M) ? *reinterpret_cast<const uint2 *>(act + (int64_t)i * lda_act + j)
+                                                          : make_uint2(0x3f803f80u, 0x3f803f80u);
+                    }
+            }
+        }
+        pdx_f16v acc[RT][CT];
+#pragma unroll
+        for (int a = 0; a < RT; ++a)
+#pragma unroll
+            for (int b = 0; b < CT; ++b)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            pdx_bf8 bfr[CT];
+#pragma unroll
+            for (int b = 0; b < CT; ++b) {
+                const int j = (wc * CT + b) * 32 + l32;
+                bfr[b] = __builtin_bit_cast(pdx_bf8, *reinterpret_cast<const uint4 *>(&bt[j * KP + 16 * s + kh]));
+            }
+#pragma unroll
+            for (int a = 0; a < RT; ++a) {
+                const pdx_bf8 av = __builtin_bit_cast(pdx_bf8, af[a][s]);
+                // operands swapped: the tile comes out transposed, C^T = B^T A^T, so a
+                // lane's result registers are runs of 4 consecutive columns of one row
+#pragma unroll
+                for (int b = 0; b < CT; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bfr[b], av, acc[a][b], 0, 0, 0);
+                if (more)
+                    af[a][s] = nin[a] ? *reinterpret_cast<const uint4 *>(nrow[a] + 16 * s) : make_uint4(0, 0, 0, 0);
+            }
+        }
+        // epilogue: lane l writes row l % 32 of each row tile; register group g
+        // (v = 4 g .. 4 g + 3) is columns 8 g + 4 (l / 32) .. + 3 of the column tile
+#pragma unroll
+        for (int a = 0; a < RT; ++a) {
+            const int i = row_base + a * 32 + l32;
+            if (i >= M) continue;
+#pragma unroll
+            for (int b = 0; b < CT; ++b)
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int j = (wc * CT + b) * 32 + 8 * g + 4 * (lane >> 5);
+                    float y[4] = {acc[a][b][4 * g], acc[a][b][4 * g + 1], acc[a][b][4 * g + 2], acc[a][b][4 * g + 3]};
+                    if (EPI == NN_EPI_FWD) {
+                        const float4 bb = *reinterpret_cast<const float4 *>(bl + j);
+                        y[0] += bb.x;
+                        y[1] += bb.y;
+                        y[2] += bb.z;
+                        y[3] += bb.w;
+                        if (relu)
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) y[q] = fmaxf(y[q], 0.f);
+                    } else {
+                        const uint2 m = mk[EPI == NN_EPI_DGRAD ? a : 0][EPI == NN_EPI_DGRAD ? b * 4 + g : 0];
+                        const uint32_t mw[2] = {m.x, m.y};
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            if (!(bf2f((uint16_t)(mw[q >> 1] >> (16 * (q & 1)))) > 0.f)) y[q] = 0.f;
+                    }
+                    if (ybf) {
+                        pdx_bf4v o;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) o[q] = (__bf16)y[q];  // round to nearest even
+                        *reinterpret_cast<pdx_bf4v *>(ybf + (int64_t)i * ldy + j) = o;
+                    }
+                    if (yf) *reinterpret_cast<float4 *>(yf + (int64_t)i * ldy + j) = make_float4(y[0], y[1], y[2], y[3]);
+                }
+        }
+    }
+}
+
+// persistent blocks: at most one per CU (the LDS-resident B allows one)
+static int nn_persistent_blocks(int ntile)
+{
+    static int ncu = 0;
+    if (ncu == 0) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                     hipSuccess || n <= 0)
+            n = 256;
+        ncu = n;
+    }
+    return ntile < ncu ? ntile : ncu;
+}
+
+// the LDS-resident-B kernel for the shapes of the network's layers; false:
+// the shape is not one of them (the generic kernel runs)
+template <int EPI>
+static bool nn_gemm_lds(const uint16_t *A, int64_t lda, const uint16_t *B, int64_t sb0, int64_t sb1, int M, int KR,
+                        int NC, const float *bias, int relu, const uint16_t *act, int64_t lda_act, uint16_t *ybf,
+                        float *yf, int64_t ldy, hipStream_t st)
+{
+    if ((reinterpret_cast<uintptr_t>(A) & 15) || (lda % 8) || M <= 0 || (ldy % 4) ||
+        (reinterpret_cast<uintptr_t>(ybf) & 7) || (reinterpret_cast<uintptr_t>(yf) & 15) ||
+        (reinterpret_cast<uintptr_t>(bias) & 15) || (reinterpret_cast<uintptr_t>(act) & 7) || (lda_act % 4))
+        return false;
+    const dim3 g(nn_persistent_blocks((M + 127) / 128)), b(256);
+#define NN_LDS_CASE(kr, nc, rt, ct, wc)                                                                          \
+    if (KR == kr && NC == nc) {                                                                                  \
+        const size_t lds = sizeof(uint16_t) * (size_t)nc * (kr + 8) + sizeof(float) * nc;                        \
+        hipLaunchKernelGGL((k_nn_gemm_lds<EPI, kr, nc, rt, ct, wc>), g, b, lds, st, A, lda, B, sb0, sb1, M, bias, \
+                           relu, act, lda_act, ybf, yf, ldy);                                                    \
+        return true;                                                                                             \
+    }
+    NN_LDS_CASE(256, 256, 2, 4, 2)
+    NN_LDS_CASE(64, 256, 2, 4, 2)
+    NN_LDS_CASE(32, 256, 2, 4, 2)
+    NN_LDS_CASE(256, 64, 2, 1, 2)
+    NN_LDS_CASE(256, 32, 1, 1, 1)
+#undef NN_LDS_CASE
+    return false;
+}
+
 // out[i] = sum over slices s of part[s * n + i], in slice order
 __global__ __launch_bounds__(256) void k_nn_sum_slices(const float *__restrict__ part, int nslice, int64_t n,
                                                        float *__restrict__ out)
@@ -175,9 +431,11 @@ extern "C" int pdx_nn_linear_fwd(const uint16_t *X, const uint16_t *W, const flo
 {
     if (!X || !W || M < 0 || K <= 0 || N <= 0 || ldy < N || (!Ybf && !Yf)) return 1;
     if (M == 0) return 0;
-    dim3 g(nn_grid_rows(M), (N + NN_TN - 1) / NN_TN);
-    hipLaunchKernelGGL(k_nn_gemm<NN_EPI_FWD>, g, dim3(256), 0, (hipStream_t)stream, X, (int64_t)K, (int64_t)1, W,
-                       (int64_t)N, (int64_t)1, M, N, K, K, bias, relu, nullptr, (int64_t)0, Ybf, Yf, (int64_t)ldy);
+    if (!nn_gemm_lds<NN_EPI_FWD>(X, K, W, N, 1, M, K, N, bias, relu, nullptr, 0, Ybf, Yf, ldy, (hipStream_t)stream)) {
+        dim3 g(nn_grid_rows(M), (N + NN_TN - 1) / NN_TN);
+        hipLaunchKernelGGL(k_nn_gemm<NN_EPI_FWD>, g, dim3(256), 0, (hipStream_t)stream, X, (int64_t)K, (int64_t)1, W,
+                           (int64_t)N, (int64_t)1, M, N, K, K, bias, relu, nullptr, (int64_t)0, Ybf, Yf, (int64_t)ldy);
+    }
     return nn_check();
 }
 
@@ -187,9 +445,13 @@ extern "C" int pdx_nn_linear_dgrad(const uint16_t *dY, const uint16_t *W, int M,
     if (!dY || !W || !dX || M < 0 || K <= 0 || N <= 0 || ldy < N) return 1;
     if (M == 0) return 0;
     // dX (M x K) = dY (M x N) W^T; B(k = n, j = k_in) = W[k_in * N + n]
-    dim3 g(nn_grid_rows(M), (K + NN_TN - 1) / NN_TN);
-    hipLaunchKernelGGL(k_nn_gemm<NN_EPI_DGRAD>, g, dim3(256), 0, (hipStream_t)stream, dY, (int64_t)ldy, (int64_t)1,
-                       W, (int64_t)1, (int64_t)N, M, K, N, N, nullptr, 0, act, (int64_t)K, dX, nullptr, (int64_t)K);
+    if (!nn_gemm_lds<NN_EPI_DGRAD>(dY, ldy, W, 1, N, M, N, K, nullptr, 0, act, K, dX, nullptr, K,
+                                   (hipStream_t)stream)) {
+        dim3 g(nn_grid_rows(M), (K + NN_TN - 1) / NN_TN);
+        hipLaunchKernelGGL(k_nn_gemm<NN_EPI_DGRAD>, g, dim3(256), 0, (hipStream_t)stream, dY, (int64_t)ldy,
+                           (int64_t)1, W, (int64_t)1, (int64_t)N, M, K, N, N, nullptr, 0, act, (int64_t)K, dX, nullptr,
+                           (int64_t)K);
+    }
     return nn_check();
 }
 
