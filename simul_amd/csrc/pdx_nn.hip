@@ -93,57 +93,128 @@ __global__ __launch_bounds__(256) void k_nn_gemm(const uint16_t *__restrict__ A,
 // or output width), NC <= 256 the output columns.  A is row-major with
 // 16-byte aligned rows; B(k, j) = B[k * sb0 + j * sb1] (W, or W^T for the data
 // gradient).  A persistent grid of at most one block per CU stages B once,
-// transposed to column-major with rows of KR + 8 values (the padding spreads
-// the lanes that read one 16-byte slice over the LDS banks), then walks tiles
-// of 128 rows.  Each of the 4 waves (one per SIMD) owns RT row tiles x CT
-// column tiles of 32 x 32 (WC waves across the columns).  A lane's A operands
-// of a whole tile (8 consecutive values of its row per 16-wide k step, 16-byte
-// loads; every row is read once) are loaded into registers before the tile's
-// MFMAs, and those of the next tile while they run, so the loads' latency
-// is hidden behind a tile of MFMA work even with one wave per SIMD.
+// transposed to column-major (the 16-byte k chunks of each column XOR-swizzled
+// so that staging stores and operand reads both spread over the LDS banks),
+// then walks tiles of 128 rows.  Each of the 4 waves (one per SIMD) owns RT row tiles x CT
+// column tiles of 32 x 32 (WC waves across the columns).
+//
+// The tile loop is software-pipelined so that one wave per SIMD keeps the
+// MFMA unit, the loads and the stores busy at once.  During tile t's k steps a
+// lane
+//   * reloads each A operand register with tile t + G's value (G the grid)
+//     right after its MFMAs are issued (8 consecutive values of a row per 16-wide
+//     k step, 16-byte loads, every row read once),
+//   * stores tile t - G's results, packed in registers at the end of the
+//     previous tile, a few 4-column groups per k step,
+//   * (data gradient) loads tile t's ReLU mask in the first half of the steps,
+//   * reads the next k step's B operands from LDS.
+// Every global access goes through a raw buffer descriptor of the tile's rows
+// (rows [128 t, min(M, 128 t + 128))): loads past the end return zero and
+// stores past it are dropped, so the loop issues the same memory instructions
+// on every tile, without branches, and the wait counts the compiler derives
+// for the MFMAs are those of the steady state (the prologue ends with the
+// first tile's empty-descriptor stores for the same reason).
 // MFMA operand layout (32x32x16): lane l holds row / column l % 32 and k
 // 8 (l / 32) .. 8 (l / 32) + 7; result register v of lane l is row
-// 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32.
+// 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32.  The operands are passed
+// swapped (C^T = B^T A^T), so a lane's registers 4 g .. 4 g + 3 of a tile are
+// columns 8 g + 4 (l / 32) .. + 3 of its row l % 32: 8- or 16-byte stores.
 // ---------------------------------------------------------------------------
 typedef __bf16 pdx_bf8 __attribute__((ext_vector_type(8)));
 typedef __bf16 pdx_bf4v __attribute__((ext_vector_type(4)));
+typedef uint32_t pdx_u4v __attribute__((ext_vector_type(4)));
+typedef uint32_t pdx_u2v __attribute__((ext_vector_type(2)));
 
-template <int RT, int KS>
-__device__ __forceinline__ void nn_load_a(const uint16_t *__restrict__ A, int64_t lda, int M, int row0, int l32, int kh,
-                                          uint4 (&af)[RT][KS])
+enum { NN_OUT_BF16 = 0, NN_OUT_F32 = 1 };
+
+// raw buffer descriptor of `bytes` bytes at p (word 3: the gfx9 family's
+// untyped 32-bit data format); loads past the end return zero, stores past it
+// are dropped
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t nn_rsrc(const void *p, int bytes)
 {
-#pragma unroll
-    for (int a = 0; a < RT; ++a) {
-        const int r = row0 + a * 32 + l32;
-        const bool in = r < M;
-        const uint16_t *p = A + (int64_t)(in ? r : 0) * lda + kh;
-#pragma unroll
-        for (int s = 0; s < KS; ++s)
-            af[a][s] = in ? *reinterpret_cast<const uint4 *>(p + 16 * s) : make_uint4(0, 0, 0, 0);
-    }
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, bytes, 0x00020000);
 }
 
-template <int EPI, int KR, int NC, int RT, int CT, int WC>
+template <int OUT> struct nn_packed;
+template <> struct nn_packed<NN_OUT_BF16> {
+    typedef pdx_u2v type;  // 4 BF16
+};
+template <> struct nn_packed<NN_OUT_F32> {
+    typedef pdx_u4v type;  // 4 FP32
+};
+
+template <int EPI, int OUT, int RELU, int KR, int NC, int RT, int CT, int WC>
 __global__ __launch_bounds__(256) void k_nn_gemm_lds(const uint16_t *__restrict__ A, int64_t lda,
                                                      const uint16_t *__restrict__ B, int64_t sb0, int64_t sb1, int M,
-                                                     const float *__restrict__ bias, int relu,
+                                                     const float *__restrict__ bias,
                                                      const uint16_t *__restrict__ act, int64_t lda_act,
-                                                     uint16_t *__restrict__ ybf, float *__restrict__ yf, int64_t ldy)
+                                                     void *__restrict__ Y, int64_t ldy)
 {
     static_assert(KR % 16 == 0 && KR <= 256 && (4 / WC) * RT * 32 == 128 && WC * CT * 32 == NC, "tile shape");
-    constexpr int KP = KR + 8;  // LDS row of one B column (bf16 values)
-    constexpr int KS = KR / 16; // k steps
+    constexpr int KP = KR;                                      // LDS row of one B column (bf16 values)
+    // the 16-byte k chunks of column j sit XOR-swizzled: chunk c at position
+    // c ^ swz(j), which spreads both the staging stores (8 columns apart) and the
+    // operand reads (consecutive columns) over all LDS banks
+    constexpr int FM = KR / 8 - 1 < 7 ? KR / 8 - 1 : 7;
+    auto swz = [](int j) { return (j + (j >> 3)) & FM; };
+    constexpr int KS = KR / 16;                                 // k steps
+    constexpr int ESZ = OUT == NN_OUT_F32 ? 4 : 2;              // output element size
+    // The results leave through store units of 32 rows x UC columns (128 bytes
+    // of a row, or the wave's width): written to the wave's LDS staging rows in
+    // the MFMA layout, read back 16 bytes per lane, LPR lanes per row, and
+    // stored as whole row segments.
+    constexpr int UC = 128 / ESZ < CT * 32 ? 128 / ESZ : CT * 32;  // columns of a store unit
+    constexpr int UPR = CT * 32 / UC;                           // units per row tile
+    constexpr int NU = RT * UPR;                                // units per tile
+    constexpr int LPR = UC * ESZ / 16;                          // lanes per row of a store
+    constexpr int RPI = 64 / LPR;                               // rows per store instruction
+    constexpr int NP = 32 / RPI;                                // store instructions per unit
+    constexpr int NST = NU * NP;                                // 16-byte stores per lane per tile
+    constexpr int RS = UC * ESZ + 16;                           // staging row stride (bytes)
+    constexpr int GPS = (NST + KS - 1) / KS;                    // stores per k step
+    constexpr int MH = (KS + 1) / 2;                            // k steps that load the mask
+    constexpr int MPS = (NST + MH - 1) / MH;                    // mask loads per such step
+    constexpr int NMK = EPI == NN_EPI_DGRAD ? NST : 1;
+    static_assert(UC % 32 == 0 && (UC * ESZ) % 16 == 0 && 64 % LPR == 0 && 32 % RPI == 0, "store units");
+    // B operands held in registers for the whole kernel when they fit (64
+    // VGPRs); else read from LDS one k step ahead
+    constexpr bool BREG = KS * CT <= 16;
+    typedef typename nn_packed<OUT>::type PV;
     extern __shared__ uint16_t bt[];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = wv / WC, wc = wv % WC;
     const int l32 = lane & 31, kh = (lane >> 5) * 8;
     const int ntile = (M + 127) / 128;
-    // this tile's A operands; each is reloaded with the next tile's as soon as
-    // its MFMAs have been issued
+    const int lrow = wr * RT * 32 + l32;         // the lane's row in row tile 0 of a tile
+    const int jl = wc * CT * 32 + 4 * (lane >> 5);  // first column of the lane's groups
+    // byte offsets within a tile (the host keeps 128 rows of A, Y and the mask under 2^31 bytes)
+    int offa[RT];
+#pragma unroll
+    for (int a = 0; a < RT; ++a) offa[a] = (lrow + 32 * a) * (int)lda * 2 + kh * 2;
+    // store q (unit u = q / NP, piece p = q % NP): row (wr RT + a) 32 + p RPI + lane / LPR of the
+    // tile, columns wc CT 32 + ub UC + (lane % LPR) 16 / ESZ .. + 16 / ESZ - 1 (a = u / UPR, ub = u % UPR)
+    const int srow = wr * RT * 32 + lane / LPR, scol = wc * CT * 32 + (lane % LPR) * (16 / ESZ);
+    const int offy0 = srow * (int)ldy * ESZ + scol * ESZ, offm0 = srow * (int)lda_act * 2 + scol * 2;
+    auto qrow = [](int q) { return (q / NP / UPR) * 32 + (q % NP) * RPI; };
+    auto qcol = [](int q) { return (q / NP % UPR) * UC; };
+    auto rows = [&](int t) { return t < ntile ? min(128, M - 128 * t) : 0; };
+    auto a_rsrc = [&](int t) {
+        return nn_rsrc(A + (int64_t)128 * (t < ntile ? t : 0) * lda, rows(t) * (int)lda * 2);
+    };
+
+    // prologue: the first tile's A operands, in the loop's order
     uint4 af[RT][KS];
     int tile = blockIdx.x;
-    if (tile < ntile) nn_load_a<RT, KS>(A, lda, M, tile * 128 + wr * RT * 32, l32, kh, af);
-    // B -> LDS, column-major: bt[j * KP + k] = B(k, j), 16-byte loads along the contiguous index
+    {
+        const __amdgpu_buffer_rsrc_t ra = a_rsrc(tile);
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+#pragma unroll
+            for (int a = 0; a < RT; ++a)
+                af[a][s] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(ra, offa[a] + 32 * s, 0, 0));
+    }
+    // B -> LDS, column-major: B(k, j) at bt[j * KP + 8 ((k / 8) ^ swz(j)) + k % 8], 16-byte loads along
+    // the contiguous index
     // (a thread's loads go out in batches of 8 before their LDS stores)
     constexpr int NL = (KR * NC / 8 + 255) / 256;
     static_assert(KR % 8 == 0 && NC % 8 == 0, "8 x 8 staging blocks");
@@ -171,7 +242,8 @@ __global__ __launch_bounds__(256) void k_nn_gemm_lds(const uint16_t *__restrict_
                     const uint32_t lo = w[2 * p2][c >> 1], hi = w[2 * p2 + 1][c >> 1];
                     o[p2] = (c & 1) ? ((lo >> 16) | (hi & 0xffff0000u)) : ((lo & 0xffffu) | (hi << 16));
                 }
-                *reinterpret_cast<uint4 *>(&bt[(8 * jb + c) * KP + 8 * kb]) = make_uint4(o[0], o[1], o[2], o[3]);
+                *reinterpret_cast<uint4 *>(&bt[(8 * jb + c) * KP + 8 * (kb ^ swz(8 * jb + c))]) =
+                    make_uint4(o[0], o[1], o[2], o[3]);
             }
         }
     } else if (sb0 == 1 && (sb1 & 7) == 0 && !(reinterpret_cast<uintptr_t>(B) & 15)) {
@@ -187,51 +259,75 @@ __global__ __launch_bounds__(256) void k_nn_gemm_lds(const uint16_t *__restrict_
 #pragma unroll
             for (int t = 0; t < 8; ++t) {
                 const int e = tid + 256 * (t0 + t);
-                if (e < NC * (KR / 8)) *reinterpret_cast<uint4 *>(&bt[(e / (KR / 8)) * KP + (e % (KR / 8)) * 8]) = v[t];
+                const int j = e / (KR / 8);
+                if (e < NC * (KR / 8)) *reinterpret_cast<uint4 *>(&bt[j * KP + 8 * ((e % (KR / 8)) ^ swz(j))]) = v[t];
             }
         }
     } else {
         for (int e = tid; e < KR * NC; e += 256) {
             const int k = e / NC, j = e % NC;
-            bt[j * KP + k] = B[(int64_t)k * sb0 + (int64_t)j * sb1];
+            bt[j * KP + 8 * ((k >> 3) ^ swz(j)) + (k & 7)] = B[(int64_t)k * sb0 + (int64_t)j * sb1];
         }
     }
-    // the bias in LDS too: an epilogue load from global memory would have to
-    // wait for the next tile's operand loads issued before it
+    // the bias in LDS too, read in every tile's epilogue
     float *bl = reinterpret_cast<float *>(bt + NC * KP);
     if (EPI == NN_EPI_FWD)
         for (int j = tid; j < NC; j += 256) bl[j] = bias ? bias[j] : 0.f;
     __syncthreads();
+
+    // the previous tile's packed results and their descriptor: before the first
+    // tile, zeros through an empty descriptor (the stores are dropped)
+    pdx_u4v pv[NST];
+#pragma unroll
+    for (int q = 0; q < NST; ++q) pv[q] = pdx_u4v{};
+    __amdgpu_buffer_rsrc_t ry = nn_rsrc(Y, 0);
+    const int ldyb = (int)ldy * ESZ;
+    auto store_group = [&](int q) {
+        __builtin_amdgcn_raw_buffer_store_b128(pv[q], ry, offy0 + qrow(q) * ldyb + qcol(q) * ESZ, 0, 0);
+    };
+#pragma unroll
+    for (int q = 0; q < NST; ++q) store_group(q);
+    // this wave's staging rows
+    char *stg = reinterpret_cast<char *>(bt + NC * KP) + sizeof(float) * NC + wv * 32 * RS;
+    const bool use_mask = EPI == NN_EPI_DGRAD && act != nullptr;
+    // the bias of the lane's columns, in registers for the whole kernel (read
+    // from LDS in the epilogue where the FP32 results leave no room)
+    constexpr bool BIASREG = EPI == NN_EPI_FWD && (OUT == NN_OUT_BF16 || CT == 1);
+    float4 bv[BIASREG ? CT * 4 : 1];
+    if constexpr (BIASREG)
+#pragma unroll
+        for (int q = 0; q < CT * 4; ++q) bv[q] = *reinterpret_cast<const float4 *>(bl + jl + (q / 4) * 32 + 8 * (q % 4));
+    // the B operands of k step s (the same for every tile): column j_b = (wc CT + b) 32 + l % 32, chunk
+    // 2 s + l / 32, i.e. element offset boff + bsw[b][s % 4] + 64 (s / 4) (the swizzle acts on the low
+    // 3 bits of the chunk)
+    int boff = 0, bsw[CT][4];
+#pragma unroll
+    for (int b = 0; b < CT; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = (wc * CT + b) * 32 + l32;
+            bsw[b][q] = j * KP + 8 * ((2 * q + (kh >> 3)) ^ swz(j));
+        }
+    auto read_b = [&](int s, pdx_bf8 (&bf)[CT]) {
+#pragma unroll
+        for (int b = 0; b < CT; ++b)
+            bf[b] = __builtin_bit_cast(pdx_bf8,
+                                       *reinterpret_cast<const uint4 *>(&bt[boff + bsw[b][s & 3] + 64 * (s >> 2)]));
+    };
+    pdx_bf8 bres[BREG ? KS : 1][CT], bfr[2][CT];
+    if constexpr (BREG) {
+#pragma unroll
+        for (int s = 0; s < KS; ++s) read_b(s, bres[s]);
+    } else {
+        read_b(0, bfr[0]);
+    }
     for (; tile < ntile; tile += gridDim.x) {
-        const int row_base = tile * 128 + wr * RT * 32;
-        // the next tile's rows: operand s of a row is loaded into the register
-        // of this tile's operand s once that has been used
-        const bool more = tile + (int)gridDim.x < ntile;
-        const uint16_t *nrow[RT];
-        bool nin[RT];
-#pragma unroll
-        for (int a = 0; a < RT; ++a) {
-            const int r = (tile + (int)gridDim.x) * 128 + wr * RT * 32 + a * 32 + l32;
-            nin[a] = more && r < M;
-            nrow[a] = A + (int64_t)(nin[a] ? r : 0) * lda + kh;
-        }
-        // the data gradient's ReLU mask of this tile, loaded before the next
-        // tile's operands so that waiting for it does not wait for them
-        uint2 mk[EPI == NN_EPI_DGRAD ? RT : 1][EPI == NN_EPI_DGRAD ? CT * 4 : 1];
-        if (EPI == NN_EPI_DGRAD) {
-#pragma unroll
-            for (int a = 0; a < RT; ++a) {
-                const int i = row_base + a * 32 + l32;
-#pragma unroll
-                for (int b = 0; b < CT; ++b)
-#pragma unroll
-                    for (int g = 0; g < 4; ++g) {
-                        const int j = (wc * CT + b) * 32 + 8 * g + 4 * (lane >> 5);
-                        mk[a][b * 4 + g] = (act && i < M) ? *reinterpret_cast<const uint2 *>(act + (int64_t)i * lda_act + j)
-                                                          : make_uint2(0x3f803f80u, 0x3f803f80u);
-                    }
-            }
-        }
+        // an opaque LDS offset per tile keeps the per-step reads inside the loop
+        if constexpr (!BREG) asm volatile("" : "+v"(boff));
+        const __amdgpu_buffer_rsrc_t ra = a_rsrc(tile + (int)gridDim.x);
+        const __amdgpu_buffer_rsrc_t rm =
+            nn_rsrc(use_mask ? act + (int64_t)128 * tile * lda_act : act, use_mask ? rows(tile) * (int)lda_act * 2 : 0);
+        pdx_u4v mk[NMK];
         pdx_f16v acc[RT][CT];
 #pragma unroll
         for (int a = 0; a < RT; ++a)
@@ -241,62 +337,95 @@ __global__ __launch_bounds__(256) void k_nn_gemm_lds(const uint16_t *__restrict_
                 for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
-            pdx_bf8 bfr[CT];
-#pragma unroll
-            for (int b = 0; b < CT; ++b) {
-                const int j = (wc * CT + b) * 32 + l32;
-                bfr[b] = __builtin_bit_cast(pdx_bf8, *reinterpret_cast<const uint4 *>(&bt[j * KP + 16 * s + kh]));
-            }
+            // the next step's B operands (step 0 of the next tile after the last)
+            if constexpr (!BREG) read_b((s + 1) % KS, bfr[(s + 1) & 1]);
 #pragma unroll
             for (int a = 0; a < RT; ++a) {
                 const pdx_bf8 av = __builtin_bit_cast(pdx_bf8, af[a][s]);
-                // operands swapped: the tile comes out transposed, C^T = B^T A^T, so a
-                // lane's result registers are runs of 4 consecutive columns of one row
 #pragma unroll
                 for (int b = 0; b < CT; ++b)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bfr[b], av, acc[a][b], 0, 0, 0);
-                if (more)
-                    af[a][s] = nin[a] ? *reinterpret_cast<const uint4 *>(nrow[a] + 16 * s) : make_uint4(0, 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(BREG ? bres[BREG ? s : 0][b] : bfr[s & 1][b],
+                                                                        av, acc[a][b], 0, 0, 0);
+                af[a][s] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(ra, offa[a] + 32 * s, 0, 0));
             }
+            if (EPI == NN_EPI_DGRAD)
+#pragma unroll
+                for (int q = s * MPS; q < s * MPS + MPS; ++q)
+                    if (q < NST)
+                        mk[q] = __builtin_amdgcn_raw_buffer_load_b128(rm, offm0 + qrow(q) * (int)lda_act * 2 + qcol(q) * 2,
+                                                                      0, 0);
+#pragma unroll
+            for (int q = s * GPS; q < s * GPS + GPS; ++q)
+                if (q < NST) store_group(q);
+            // each step's loads and stores stay among its MFMAs
+            __builtin_amdgcn_sched_barrier(0);
         }
-        // epilogue: lane l writes row l % 32 of each row tile; register group g
-        // (v = 4 g .. 4 g + 3) is columns 8 g + 4 (l / 32) .. + 3 of the column tile
+        if (!BREG && (KS & 1)) {  // the step-0 operands were read into bfr[KS & 1]; keep them in bfr[0]
 #pragma unroll
-        for (int a = 0; a < RT; ++a) {
-            const int i = row_base + a * 32 + l32;
-            if (i >= M) continue;
+            for (int b = 0; b < CT; ++b) bfr[0][b] = bfr[1][b];
+        }
+        // epilogue of this tile: bias and ReLU (forward), rounding, and through
+        // the staging rows into the store layout, one unit at a time (a wave's
+        // LDS operations complete in order, so its own lanes read back what they
+        // wrote without a barrier)
 #pragma unroll
-            for (int b = 0; b < CT; ++b)
+        for (int u = 0; u < NU; ++u) {
+            const int a = u / UPR, ub = u % UPR;
+#pragma unroll
+            for (int b = ub * UC / 32; b < (ub + 1) * UC / 32; ++b)
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
-                    const int j = (wc * CT + b) * 32 + 8 * g + 4 * (lane >> 5);
                     float y[4] = {acc[a][b][4 * g], acc[a][b][4 * g + 1], acc[a][b][4 * g + 2], acc[a][b][4 * g + 3]};
                     if (EPI == NN_EPI_FWD) {
-                        const float4 bb = *reinterpret_cast<const float4 *>(bl + j);
+                        const float4 bb = BIASREG ? bv[BIASREG ? b * 4 + g : 0]
+                                                  : *reinterpret_cast<const float4 *>(bl + jl + b * 32 + 8 * g);
                         y[0] += bb.x;
                         y[1] += bb.y;
                         y[2] += bb.z;
                         y[3] += bb.w;
-                        if (relu)
+                        if (RELU)
 #pragma unroll
-                            for (int q = 0; q < 4; ++q) y[q] = fmaxf(y[q], 0.f);
-                    } else {
-                        const uint2 m = mk[EPI == NN_EPI_DGRAD ? a : 0][EPI == NN_EPI_DGRAD ? b * 4 + g : 0];
-                        const uint32_t mw[2] = {m.x, m.y};
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-                            if (!(bf2f((uint16_t)(mw[q >> 1] >> (16 * (q & 1)))) > 0.f)) y[q] = 0.f;
+                            for (int r = 0; r < 4; ++r) y[r] = fmaxf(y[r], 0.f);
                     }
-                    if (ybf) {
+                    char *dst = stg + l32 * RS + (b * 32 - ub * UC + 8 * g + 4 * (lane >> 5)) * ESZ;
+                    if constexpr (OUT == NN_OUT_F32) {
+                        *reinterpret_cast<float4 *>(dst) = make_float4(y[0], y[1], y[2], y[3]);
+                    } else {
                         pdx_bf4v o;
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) o[q] = (__bf16)y[q];  // round to nearest even
-                        *reinterpret_cast<pdx_bf4v *>(ybf + (int64_t)i * ldy + j) = o;
+                        for (int r = 0; r < 4; ++r) o[r] = (__bf16)y[r];  // round to nearest even
+                        *reinterpret_cast<pdx_bf4v *>(dst) = o;
                     }
-                    if (yf) *reinterpret_cast<float4 *>(yf + (int64_t)i * ldy + j) = make_float4(y[0], y[1], y[2], y[3]);
+                }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+                pv[u * NP + p] = *reinterpret_cast<const pdx_u4v *>(stg + (p * RPI + lane / LPR) * RS + (lane % LPR) * 16);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        // the data gradient's ReLU derivative: zero where the layer input is not > 0
+        // (a BF16 h is > 0 exactly when h - 1 < 0x7f80 as unsigned: sign clear, not
+        // zero, not NaN)
+        if (use_mask) {
+#pragma unroll
+            for (int q = 0; q < NST; ++q)
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    const uint32_t m = mk[EPI == NN_EPI_DGRAD ? q : 0][w];
+                    const uint32_t keep = (((m & 0xffffu) - 1u) < 0x7f80u ? 0x0000ffffu : 0u) |
+                                          (((m >> 16) - 1u) < 0x7f80u ? 0xffff0000u : 0u);
+                    pv[q][w] &= keep;
                 }
         }
+        ry = nn_rsrc(static_cast<char *>(Y) + (int64_t)128 * tile * ldy * ESZ, rows(tile) * (int)ldy * ESZ);
     }
+    // the last tile's results
+#pragma unroll
+    for (int q = 0; q < NST; ++q) store_group(q);
 }
 
 // persistent blocks: at most one per CU (the LDS-resident B allows one)
@@ -313,23 +442,47 @@ static int nn_persistent_blocks(int ntile)
     return ntile < ncu ? ntile : ncu;
 }
 
-// the LDS-resident-B kernel for the shapes of the network's layers; false:
-// the shape is not one of them (the generic kernel runs)
+// bytes of a store unit's row: 128, or a wave's columns when fewer
+static int nn_unit_bytes(int esz, int nc_wave) { return 128 < nc_wave * esz ? 128 : nc_wave * esz; }
+
+// the LDS-resident-B kernel for the shapes of the network's layers, one
+// output (BF16 or FP32) per launch; false: not one of them (the generic
+// kernel runs)
 template <int EPI>
 static bool nn_gemm_lds(const uint16_t *A, int64_t lda, const uint16_t *B, int64_t sb0, int64_t sb1, int M, int KR,
                         int NC, const float *bias, int relu, const uint16_t *act, int64_t lda_act, uint16_t *ybf,
                         float *yf, int64_t ldy, hipStream_t st)
 {
+    if ((ybf != nullptr) == (yf != nullptr) || (EPI == NN_EPI_DGRAD && !ybf)) return false;
     if ((reinterpret_cast<uintptr_t>(A) & 15) || (lda % 8) || M <= 0 || (ldy % 4) ||
         (reinterpret_cast<uintptr_t>(ybf) & 7) || (reinterpret_cast<uintptr_t>(yf) & 15) ||
         (reinterpret_cast<uintptr_t>(bias) & 15) || (reinterpret_cast<uintptr_t>(act) & 7) || (lda_act % 4))
         return false;
+    // byte offsets within 128 rows of A, Y and the mask stay below 2^31
+    if (lda > (1 << 20) || ldy > (1 << 20) || lda_act > (1 << 20)) return false;
     const dim3 g(nn_persistent_blocks((M + 127) / 128)), b(256);
 #define NN_LDS_CASE(kr, nc, rt, ct, wc)                                                                          \
     if (KR == kr && NC == nc) {                                                                                  \
-        const size_t lds = sizeof(uint16_t) * (size_t)nc * (kr + 8) + sizeof(float) * nc;                        \
-        hipLaunchKernelGGL((k_nn_gemm_lds<EPI, kr, nc, rt, ct, wc>), g, b, lds, st, A, lda, B, sb0, sb1, M, bias, \
-                           relu, act, lda_act, ybf, yf, ldy);                                                    \
+        const size_t lds = sizeof(uint16_t) * (size_t)nc * kr + sizeof(float) * nc +                            \
+                           4 * 32 * (size_t)(nn_unit_bytes(yf ? 4 : 2, nc / wc) + 16);                            \
+        if constexpr (EPI == NN_EPI_FWD) {                                                                       \
+            void *y = yf ? (void *)yf : (void *)ybf;                                                             \
+            if (yf && relu)                                                                                      \
+                hipLaunchKernelGGL((k_nn_gemm_lds<EPI, NN_OUT_F32, 1, kr, nc, rt, ct, wc>), g, b, lds, st, A,     \
+                                   lda, B, sb0, sb1, M, bias, act, lda_act, y, ldy);                             \
+            else if (yf)                                                                                         \
+                hipLaunchKernelGGL((k_nn_gemm_lds<EPI, NN_OUT_F32, 0, kr, nc, rt, ct, wc>), g, b, lds, st, A,     \
+                                   lda, B, sb0, sb1, M, bias, act, lda_act, y, ldy);                             \
+            else if (relu)                                                                                       \
+                hipLaunchKernelGGL((k_nn_gemm_lds<EPI, NN_OUT_BF16, 1, kr, nc, rt, ct, wc>), g, b, lds, st, A,    \
+                                   lda, B, sb0, sb1, M, bias, act, lda_act, y, ldy);                             \
+            else                                                                                                 \
+                hipLaunchKernelGGL((k_nn_gemm_lds<EPI, NN_OUT_BF16, 0, kr, nc, rt, ct, wc>), g, b, lds, st, A,    \
+                                   lda, B, sb0, sb1, M, bias, act, lda_act, y, ldy);                             \
+        } else {                                                                                                 \
+            hipLaunchKernelGGL((k_nn_gemm_lds<EPI, NN_OUT_BF16, 0, kr, nc, rt, ct, wc>), g, b, lds, st, A, lda,   \
+                               B, sb0, sb1, M, bias, act, lda_act, (void *)ybf, ldy);                            \
+        }                                                                                                        \
         return true;                                                                                             \
     }
     NN_LDS_CASE(256, 256, 2, 4, 2)
