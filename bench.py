@@ -482,7 +482,7 @@ def run_nn(args, torch):
     import numpy as np
 
     from simul_amd import _native as N
-    from simul_amd.learned.net import DIMS
+    from simul_amd.learned.net import DIMS, wgrad_rows_per_slice
 
     lib = N.load()
     st = torch.cuda.current_stream()
@@ -522,11 +522,12 @@ def run_nn(args, torch):
             if grads:
                 dX = torch.empty((M, k), dtype=torch.int16, device="cuda")
                 us_d = timed(lambda: lib.pdx_nn_linear_dgrad(p(dY), p(W), M, k, n, n, p(X), p(dX), sp))
-                ns = (M + 4095) // 4096
+                R = wgrad_rows_per_slice(M)
+                ns = (M + R - 1) // R
                 part = torch.empty((ns * k * n,), dtype=torch.float32, device="cuda")
                 gW = torch.empty((k, n), dtype=torch.float32, device="cuda")
                 gb = torch.empty((n,), dtype=torch.float32, device="cuda")
-                us_w = timed(lambda: lib.pdx_nn_linear_wgrad(p(X), p(dY), M, k, n, n, 4096, p(part), p(gW), p(gb), sp))
+                us_w = timed(lambda: lib.pdx_nn_linear_wgrad(p(X), p(dY), M, k, n, n, R, p(part), p(gW), p(gb), sp))
                 rec.update({"dgrad_us": round(us_d, 2), "dgrad_tflops": round(flop / us_d / 1e6, 1),
                             "wgrad_us": round(us_w, 2), "wgrad_tflops": round(flop / us_w / 1e6, 1)})
             rows.append(rec)

@@ -143,14 +143,15 @@ template <> struct nn_packed<NN_OUT_F32> {
     typedef pdx_u4v type;  // 4 FP32
 };
 
-template <int EPI, int OUT, int RELU, int KR, int NC, int RT, int CT, int WC>
+template <int EPI, int OUT, int RELU, int KR, int NC, int RT, int CT, int WC, int PF>
 __global__ __launch_bounds__(256) void k_nn_gemm_lds(const uint16_t *__restrict__ A, int64_t lda,
                                                      const uint16_t *__restrict__ B, int64_t sb0, int64_t sb1, int M,
                                                      const float *__restrict__ bias,
                                                      const uint16_t *__restrict__ act, int64_t lda_act,
                                                      void *__restrict__ Y, int64_t ldy)
 {
-    static_assert(KR % 16 == 0 && KR <= 256 && (4 / WC) * RT * 32 == 128 && WC * CT * 32 == NC, "tile shape");
+    static_assert(KR % 16 == 0 && KR <= 256 && (4 / WC) * RT * 32 == 128 && WC * CT * 32 == NC && PF >= 1,
+                  "tile shape");
     constexpr int KP = KR;                                      // LDS row of one B column (bf16 values)
     // the 16-byte k chunks of column j sit XOR-swizzled: chunk c at position
     // c ^ swz(j), which spreads both the staging stores (8 columns apart) and the
@@ -202,16 +203,19 @@ __global__ __launch_bounds__(256) void k_nn_gemm_lds(const uint16_t *__restrict_
         return nn_rsrc(A + (int64_t)128 * (t < ntile ? t : 0) * lda, rows(t) * (int)lda * 2);
     };
 
-    // prologue: the first tile's A operands, in the loop's order
-    uint4 af[RT][KS];
+    // prologue: the A operands of the block's first PF tiles, in the loop's order
+    // (set p holds the tiles p, p + PF, p + 2 PF, ... of the block)
+    uint4 af[PF][RT][KS];
     int tile = blockIdx.x;
-    {
-        const __amdgpu_buffer_rsrc_t ra = a_rsrc(tile);
+#pragma unroll
+    for (int p = 0; p < PF; ++p) {
+        const __amdgpu_buffer_rsrc_t ra = a_rsrc(tile + p * (int)gridDim.x);
 #pragma unroll
         for (int s = 0; s < KS; ++s)
 #pragma unroll
             for (int a = 0; a < RT; ++a)
-                af[a][s] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(ra, offa[a] + 32 * s, 0, 0));
+                af[p][a][s] =
+                    __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(ra, offa[a] + 32 * s, 0, 0));
     }
     // B -> LDS, column-major: B(k, j) at bt[j * KP + 8 ((k / 8) ^ swz(j)) + k % 8], 16-byte loads along
     // the contiguous index
@@ -282,8 +286,11 @@ __global__ __launch_bounds__(256) void k_nn_gemm_lds(const uint16_t *__restrict_
     for (int q = 0; q < NST; ++q) pv[q] = pdx_u4v{};
     __amdgpu_buffer_rsrc_t ry = nn_rsrc(Y, 0);
     const int ldyb = (int)ldy * ESZ;
+    // results are written once and not read again by this launch: non-temporal
+    // stores (cache-policy bit NT), which measured faster for a single layer and
+    // for the chained network
     auto store_group = [&](int q) {
-        __builtin_amdgcn_raw_buffer_store_b128(pv[q], ry, offy0 + qrow(q) * ldyb + qcol(q) * ESZ, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(pv[q], ry, offy0 + qrow(q) * ldyb + qcol(q) * ESZ, 0, 2);
     };
 #pragma unroll
     for (int q = 0; q < NST; ++q) store_group(q);
@@ -292,7 +299,7 @@ __global__ __launch_bounds__(256) void k_nn_gemm_lds(const uint16_t *__restrict_
     const bool use_mask = EPI == NN_EPI_DGRAD && act != nullptr;
     // the bias of the lane's columns, in registers for the whole kernel (read
     // from LDS in the epilogue where the FP32 results leave no room)
-    constexpr bool BIASREG = EPI == NN_EPI_FWD && (OUT == NN_OUT_BF16 || CT == 1);
+    constexpr bool BIASREG = EPI == NN_EPI_FWD && CT <= 4 && (OUT == NN_OUT_BF16 || CT == 1);
     float4 bv[BIASREG ? CT * 4 : 1];
     if constexpr (BIASREG)
 #pragma unroll
@@ -321,12 +328,14 @@ __global__ __launch_bounds__(256) void k_nn_gemm_lds(const uint16_t *__restrict_
     } else {
         read_b(0, bfr[0]);
     }
-    for (; tile < ntile; tile += gridDim.x) {
+    // one tile t with operand set p: its MFMAs, its operands reloaded with tile t + PF G's, the
+    // previous tile's stores; then its results packed for the stores
+    auto process = [&](const int t, uint4 (&afc)[RT][KS]) {
         // an opaque LDS offset per tile keeps the per-step reads inside the loop
         if constexpr (!BREG) asm volatile("" : "+v"(boff));
-        const __amdgpu_buffer_rsrc_t ra = a_rsrc(tile + (int)gridDim.x);
+        const __amdgpu_buffer_rsrc_t ra = a_rsrc(t + PF * (int)gridDim.x);
         const __amdgpu_buffer_rsrc_t rm =
-            nn_rsrc(use_mask ? act + (int64_t)128 * tile * lda_act : act, use_mask ? rows(tile) * (int)lda_act * 2 : 0);
+            nn_rsrc(use_mask ? act + (int64_t)128 * t * lda_act : act, use_mask ? rows(t) * (int)lda_act * 2 : 0);
         pdx_u4v mk[NMK];
         pdx_f16v acc[RT][CT];
 #pragma unroll
@@ -341,12 +350,12 @@ __global__ __launch_bounds__(256) void k_nn_gemm_lds(const uint16_t *__restrict_
             if constexpr (!BREG) read_b((s + 1) % KS, bfr[(s + 1) & 1]);
 #pragma unroll
             for (int a = 0; a < RT; ++a) {
-                const pdx_bf8 av = __builtin_bit_cast(pdx_bf8, af[a][s]);
+                const pdx_bf8 av = __builtin_bit_cast(pdx_bf8, afc[a][s]);
 #pragma unroll
                 for (int b = 0; b < CT; ++b)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(BREG ? bres[BREG ? s : 0][b] : bfr[s & 1][b],
                                                                         av, acc[a][b], 0, 0, 0);
-                af[a][s] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(ra, offa[a] + 32 * s, 0, 0));
+                afc[a][s] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(ra, offa[a] + 32 * s, 0, 0));
             }
             if (EPI == NN_EPI_DGRAD)
 #pragma unroll
@@ -371,6 +380,16 @@ __global__ __launch_bounds__(256) void k_nn_gemm_lds(const uint16_t *__restrict_
 #pragma unroll
         for (int u = 0; u < NU; ++u) {
             const int a = u / UPR, ub = u % UPR;
+            // the unit's bias, read together ahead of its use unless held in registers
+            // (or, for FP32 results, read at its use: they leave no registers for it)
+            constexpr bool BIASU = EPI == NN_EPI_FWD && !BIASREG && OUT == NN_OUT_BF16;
+            float4 bu[UC / 32][4];
+            if constexpr (BIASU)
+#pragma unroll
+                for (int b = 0; b < UC / 32; ++b)
+#pragma unroll
+                    for (int g = 0; g < 4; ++g)
+                        bu[b][g] = *reinterpret_cast<const float4 *>(bl + jl + (ub * UC / 32 + b) * 32 + 8 * g);
 #pragma unroll
             for (int b = ub * UC / 32; b < (ub + 1) * UC / 32; ++b)
 #pragma unroll
@@ -378,6 +397,7 @@ __global__ __launch_bounds__(256) void k_nn_gemm_lds(const uint16_t *__restrict_
                     float y[4] = {acc[a][b][4 * g], acc[a][b][4 * g + 1], acc[a][b][4 * g + 2], acc[a][b][4 * g + 3]};
                     if (EPI == NN_EPI_FWD) {
                         const float4 bb = BIASREG ? bv[BIASREG ? b * 4 + g : 0]
+                                          : BIASU ? bu[b - ub * UC / 32][g]
                                                   : *reinterpret_cast<const float4 *>(bl + jl + b * 32 + 8 * g);
                         y[0] += bb.x;
                         y[1] += bb.y;
@@ -421,7 +441,14 @@ __global__ __launch_bounds__(256) void k_nn_gemm_lds(const uint16_t *__restrict_
                     pv[q][w] &= keep;
                 }
         }
-        ry = nn_rsrc(static_cast<char *>(Y) + (int64_t)128 * tile * ldy * ESZ, rows(tile) * (int)ldy * ESZ);
+        ry = nn_rsrc(static_cast<char *>(Y) + (int64_t)128 * t * ldy * ESZ, rows(t) * (int)ldy * ESZ);
+    };
+    for (; tile < ntile; tile += PF * (int)gridDim.x) {
+#pragma unroll
+        for (int p = 0; p < PF; ++p) {
+            if (p > 0 && tile + p * (int)gridDim.x >= ntile) break;
+            process(tile + p * (int)gridDim.x, af[p]);
+        }
     }
     // the last tile's results
 #pragma unroll
@@ -460,37 +487,208 @@ static bool nn_gemm_lds(const uint16_t *A, int64_t lda, const uint16_t *B, int64
         return false;
     // byte offsets within 128 rows of A, Y and the mask stay below 2^31
     if (lda > (1 << 20) || ldy > (1 << 20) || lda_act > (1 << 20)) return false;
-    const dim3 g(nn_persistent_blocks((M + 127) / 128)), b(256);
-#define NN_LDS_CASE(kr, nc, rt, ct, wc)                                                                          \
+    const dim3 g(nn_persistent_blocks((M + 127) / 128));
+    void *y = yf ? (void *)yf : (void *)ybf;
+    // one launch: the block shape (RT, CT, WC) and the prefetch depth PF of a variant
+    auto launch = [&](auto kern, int kr, int nc, int wc) {
+        const size_t lds = sizeof(uint16_t) * (size_t)nc * kr + sizeof(float) * nc +
+                           4 * 32 * (size_t)(nn_unit_bytes(yf ? 4 : 2, nc / wc) + 16);
+        hipLaunchKernelGGL(kern, g, dim3(256), lds, st, A, lda, B, sb0, sb1, M, bias, act, lda_act, y, ldy);
+        return true;
+    };
+#define NN_LDS_VARIANTS(kr, nc, f32, bf16, dgrad)                                                                \
     if (KR == kr && NC == nc) {                                                                                  \
-        const size_t lds = sizeof(uint16_t) * (size_t)nc * kr + sizeof(float) * nc +                            \
-                           4 * 32 * (size_t)(nn_unit_bytes(yf ? 4 : 2, nc / wc) + 16);                            \
         if constexpr (EPI == NN_EPI_FWD) {                                                                       \
-            void *y = yf ? (void *)yf : (void *)ybf;                                                             \
-            if (yf && relu)                                                                                      \
-                hipLaunchKernelGGL((k_nn_gemm_lds<EPI, NN_OUT_F32, 1, kr, nc, rt, ct, wc>), g, b, lds, st, A,     \
-                                   lda, B, sb0, sb1, M, bias, act, lda_act, y, ldy);                             \
-            else if (yf)                                                                                         \
-                hipLaunchKernelGGL((k_nn_gemm_lds<EPI, NN_OUT_F32, 0, kr, nc, rt, ct, wc>), g, b, lds, st, A,     \
-                                   lda, B, sb0, sb1, M, bias, act, lda_act, y, ldy);                             \
-            else if (relu)                                                                                       \
-                hipLaunchKernelGGL((k_nn_gemm_lds<EPI, NN_OUT_BF16, 1, kr, nc, rt, ct, wc>), g, b, lds, st, A,    \
-                                   lda, B, sb0, sb1, M, bias, act, lda_act, y, ldy);                             \
-            else                                                                                                 \
-                hipLaunchKernelGGL((k_nn_gemm_lds<EPI, NN_OUT_BF16, 0, kr, nc, rt, ct, wc>), g, b, lds, st, A,    \
-                                   lda, B, sb0, sb1, M, bias, act, lda_act, y, ldy);                             \
+            if (yf && relu) return launch(k_nn_gemm_lds<EPI, NN_OUT_F32, 1, kr, nc, NN_UNPACK f32>, kr, nc, NN_WC f32); \
+            if (yf) return launch(k_nn_gemm_lds<EPI, NN_OUT_F32, 0, kr, nc, NN_UNPACK f32>, kr, nc, NN_WC f32);     \
+            if (relu) return launch(k_nn_gemm_lds<EPI, NN_OUT_BF16, 1, kr, nc, NN_UNPACK bf16>, kr, nc, NN_WC bf16); \
+            return launch(k_nn_gemm_lds<EPI, NN_OUT_BF16, 0, kr, nc, NN_UNPACK bf16>, kr, nc, NN_WC bf16);        \
         } else {                                                                                                 \
-            hipLaunchKernelGGL((k_nn_gemm_lds<EPI, NN_OUT_BF16, 0, kr, nc, rt, ct, wc>), g, b, lds, st, A, lda,   \
-                               B, sb0, sb1, M, bias, act, lda_act, (void *)ybf, ldy);                            \
+            return launch(k_nn_gemm_lds<EPI, NN_OUT_BF16, 0, kr, nc, NN_UNPACK dgrad>, kr, nc, NN_WC dgrad);      \
         }                                                                                                        \
+    }
+#define NN_UNPACK(rt, ct, wc, pf) rt, ct, wc, pf
+#define NN_WC(rt, ct, wc, pf) wc
+    // (RT, CT, WC, PF) of the FP32-output forward, the BF16 forward and the data gradient: one wave
+    // across all columns where its registers allow, so that every A row is loaded once per block
+    NN_LDS_VARIANTS(256, 256, (1, 8, 1, 1), (1, 8, 1, 2), (1, 8, 1, 1))
+    NN_LDS_VARIANTS(64, 256, (2, 4, 2, 1), (2, 4, 2, 1), (2, 4, 2, 1))
+    NN_LDS_VARIANTS(32, 256, (2, 4, 2, 1), (2, 4, 2, 1), (2, 4, 2, 1))
+    NN_LDS_VARIANTS(256, 64, (2, 1, 2, 1), (2, 1, 2, 1), (2, 1, 2, 1))
+    NN_LDS_VARIANTS(256, 32, (1, 1, 1, 1), (1, 1, 1, 1), (1, 1, 1, 1))
+#undef NN_UNPACK
+#undef NN_WC
+#undef NN_LDS_VARIANTS
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// The weight gradient dW (K x N) = X^T dY over a slice of rows, on the 32x32x16
+// BF16 MFMA with the reduction over the rows m.  The operands need 8 consecutive
+// rows of one column per lane; the gfx950 LDS transpose read (ds_read_b64_tr_b16)
+// gives exactly that from rows staged as they lie in memory: within each group
+// of 16 lanes, lane i receives element i % 4 of the 8-byte pieces addressed by
+// lanes 4 q + i / 4, q = 0..3.  Lane j = 4 q + t of group g (lanes 16 g' ..) of
+// half h (lanes 32 h ..) addresses row 8 h + 4 p + q, columns c0 + 16 (g' % 2) +
+// 4 t .. + 3 of the staged rows; two such reads (p = 0, 1) give lane l the rows
+// 8 (l / 32) .. + 7 of column c0 + l % 32, the operand layout of the MFMA with
+// the rows as its k.
+// A block takes one slice of rows and the whole K x N product: WI x WJ waves
+// over the rows x columns of dW, each TI x TJ tiles of 32 x 32.  Stages
+// of 32 rows of X and dY pass through two LDS buffers and two register slots:
+// a stage is loaded (16-byte buffer loads, zero past the slice) three stages
+// ahead, written to LDS one stage ahead, one barrier per stage.  The
+// 16-byte chunks of row r sit XOR-swizzled by 2 (r % 4), so that the four rows
+// a transpose read touches fall on distinct banks.  The FP32 partial of the
+// slice goes to part; the slices are summed in a fixed order by k_nn_sum_slices4.
+// ---------------------------------------------------------------------------
+typedef short pdx_s4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) pdx_s4 *pdx_lds_s4p;
+
+template <int K, int N, int WI, int WJ>
+__global__ __launch_bounds__(64 * WI * WJ) void k_nn_wgrad(const uint16_t *__restrict__ X, const uint16_t *__restrict__ dY,
+                                                  int M, int R, float *__restrict__ part)
+{
+    constexpr int TI = K / WI / 32, TJ = N / WJ / 32;
+    constexpr int NT = 64 * WI * WJ;              // threads
+    static_assert(TI >= 1 && TJ >= 1 && TI * WI * 32 == K && TJ * WJ * 32 == N, "wgrad shape");
+    constexpr int S = 32;                          // rows per stage
+    constexpr int CX = K / 8, CY = N / 8;          // 16-byte chunks per row
+    constexpr int XB = S * K * 2, YB = S * N * 2;  // stage bytes
+    constexpr int NX = (S * CX + NT - 1) / NT, NY = (S * CY + NT - 1) / NT;  // loads per thread per stage
+    constexpr int SB = XB + YB;                    // one LDS buffer
+    extern __shared__ __attribute__((aligned(16))) char wsm[];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wi = wv / WJ, wj = wv % WJ;
+    const int z = blockIdx.x, row0 = z * R;
+    const int nrows = min(R, M - row0);
+    const int ns = (nrows + S - 1) / S;
+    const __amdgpu_buffer_rsrc_t rx = nn_rsrc(X + (int64_t)row0 * K, nrows * K * 2);
+    const __amdgpu_buffer_rsrc_t ryy = nn_rsrc(dY + (int64_t)row0 * N, nrows * N * 2);
+    auto swz = [](int r, int ch) { return (2 * (r & 3)) & (ch - 1); };
+    uint4 sx[2][NX], sy[2][NY];
+    auto load = [&](int st, uint4 (&vx)[NX], uint4 (&vy)[NY]) {
+#pragma unroll
+        for (int u = 0; u < NX; ++u) {
+            const int e = tid + NT * u;  // chunk e of the stage: row e / CX, chunk e % CX
+            vx[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(
+                                                  rx, (st * S + e / CX) * K * 2 + (e % CX) * 16, 0, 0));
+        }
+#pragma unroll
+        for (int u = 0; u < NY; ++u) {
+            const int e = tid + NT * u;
+            vy[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(
+                                                  ryy, (st * S + e / CY) * N * 2 + (e % CY) * 16, 0, 0));
+        }
+    };
+    auto put = [&](int buf, const uint4 (&vx)[NX], const uint4 (&vy)[NY]) {
+        char *b = wsm + buf * SB;
+#pragma unroll
+        for (int u = 0; u < NX; ++u) {
+            const int e = tid + NT * u, r = e / CX, c = e % CX;
+            if (S * CX % NT == 0 || e < S * CX)
+                *reinterpret_cast<uint4 *>(b + r * K * 2 + (c ^ swz(r, CX)) * 16) = vx[u];
+        }
+#pragma unroll
+        for (int u = 0; u < NY; ++u) {
+            const int e = tid + NT * u, r = e / CY, c = e % CY;
+            if (S * CY % NT == 0 || e < S * CY)
+                *reinterpret_cast<uint4 *>(b + XB + r * N * 2 + (c ^ swz(r, CY)) * 16) = vy[u];
+        }
+    };
+    // the transpose-read address of this lane within a stage, for k step ks, part p and the 32 columns from c0
+    const int h = lane >> 5, gq = (lane >> 4) & 1, q = (lane >> 2) & 3, t = lane & 3;
+    auto tr_off = [&](int ks, int p, int c0, int rowbytes, int ch) {
+        const int r = 16 * ks + 8 * h + 4 * p + q, col = c0 + 16 * gq + 4 * t;
+        return r * rowbytes + (((col >> 3) ^ swz(r, ch)) << 4) + ((col >> 2) & 1) * 8;
+    };
+    pdx_f16v acc[TI][TJ];
+#pragma unroll
+    for (int a = 0; a < TI; ++a)
+#pragma unroll
+        for (int b = 0; b < TJ; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+    auto compute = [&](int buf) {
+        const char *bx = wsm + buf * SB, *by = bx + XB;
+#pragma unroll
+        for (int ks = 0; ks < S / 16; ++ks) {
+            pdx_bf8 fa[TI], fb[TJ];
+#pragma unroll
+            for (int a = 0; a < TI; ++a) {
+                const int c0 = (wi * TI + a) * 32;
+                const pdx_s4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((pdx_lds_s4p)(bx + tr_off(ks, 0, c0, K * 2, CX)));
+                const pdx_s4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((pdx_lds_s4p)(bx + tr_off(ks, 1, c0, K * 2, CX)));
+                fa[a] = __builtin_bit_cast(pdx_bf8, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+            }
+#pragma unroll
+            for (int b = 0; b < TJ; ++b) {
+                const int c0 = (wj * TJ + b) * 32;
+                const pdx_s4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((pdx_lds_s4p)(by + tr_off(ks, 0, c0, N * 2, CY)));
+                const pdx_s4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((pdx_lds_s4p)(by + tr_off(ks, 1, c0, N * 2, CY)));
+                fb[b] = __builtin_bit_cast(pdx_bf8, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+            }
+#pragma unroll
+            for (int a = 0; a < TI; ++a)
+#pragma unroll
+                for (int b = 0; b < TJ; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+        }
+    };
+    // prologue: stages 0 and 1 into the register slots, stage 0 into buffer 0, stage 2 into slot 0
+    load(0, sx[0], sy[0]);
+    load(1, sx[1], sy[1]);
+    put(0, sx[0], sy[0]);
+    load(2, sx[0], sy[0]);
+    // stage s: slot s % 2, buffer s % 2; at stage it (after the barrier that ends
+    // everyone's stage it - 1), stage it + 1 goes to LDS and its slot is reloaded
+    // with stage it + 3 (loads past the slice return zero)
+    for (int it = 0; it < ns; it += 2) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            if (it + u >= ns) break;
+            __syncthreads();
+            const int nx = (u + 1) % 2;
+            put(nx, sx[nx], sy[nx]);
+            load(it + u + 3, sx[nx], sy[nx]);
+            compute(u);
+        }
+    }
+    // the slice's partial: lane l, register v of tile (a, b) is row 8 (v / 4) + 4 (l / 32) + v % 4 and
+    // column l % 32 of the tile
+    float *pz = part + (int64_t)z * K * N;
+#pragma unroll
+    for (int a = 0; a < TI; ++a)
+#pragma unroll
+        for (int b = 0; b < TJ; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int i = (wi * TI + a) * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const int j = (wj * TJ + b) * 32 + (lane & 31);
+                pz[i * N + j] = acc[a][b][v];
+            }
+}
+
+// the transpose-read weight gradient for the layers' shapes; false: not one of them
+static bool nn_wgrad_tr(const uint16_t *X, const uint16_t *dY, int M, int K, int N, int R, float *part,
+                        hipStream_t st)
+{
+    if ((reinterpret_cast<uintptr_t>(X) & 15) || (reinterpret_cast<uintptr_t>(dY) & 15) || R % 32 || R <= 0 ||
+        (int64_t)R * 512 >= (1ll << 31))
+        return false;
+    const int ns = (M + R - 1) / R;
+    // (K, N, WI, WJ): two LDS buffers of 32 rows of X and dY
+#define NN_WGRAD_CASE(k, n, wi, wj)                                                                              \
+    if (K == k && N == n) {                                                                                      \
+        hipLaunchKernelGGL((k_nn_wgrad<k, n, wi, wj>), dim3(ns), dim3(64 * wi * wj), 2 * 32 * (k + n) * 2, st, X,   \
+                           dY, M, R, part);                                                                      \
         return true;                                                                                             \
     }
-    NN_LDS_CASE(256, 256, 2, 4, 2)
-    NN_LDS_CASE(64, 256, 2, 4, 2)
-    NN_LDS_CASE(32, 256, 2, 4, 2)
-    NN_LDS_CASE(256, 64, 2, 1, 2)
-    NN_LDS_CASE(256, 32, 1, 1, 1)
-#undef NN_LDS_CASE
+    NN_WGRAD_CASE(256, 256, 4, 2)
+    NN_WGRAD_CASE(64, 256, 1, 4)
+    NN_WGRAD_CASE(256, 64, 2, 2)
+    NN_WGRAD_CASE(256, 32, 4, 1)
+#undef NN_WGRAD_CASE
     return false;
 }
 
@@ -515,6 +713,107 @@ __global__ __launch_bounds__(256) void k_nn_colsum(const uint16_t *__restrict__ 
     float s = 0.f;
     for (int i = r0; i < r1; ++i) s += bf2f(Y[(int64_t)i * N + j]);
     part[(int64_t)blockIdx.y * N + j] = s;
+}
+
+// out[i] = sum over slices z of part[z * n + i], n % 4 == 0, 16-byte aligned: a
+// block takes 256 consecutive outputs (64 float4 columns) x 4 slice groups;
+// group g sums the slices g, g + 4, g + 8, ... in that order (8 loads in
+// flight), and the four group sums are added in group order -- a fixed order,
+// the same on every run
+__global__ __launch_bounds__(256) void k_nn_sum_slices4(const float *__restrict__ part, int nslice, int64_t n,
+                                                        float *__restrict__ out)
+{
+    __shared__ float4 red[4][64];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int64_t i4 = (int64_t)blockIdx.x * 64 + c, n4 = n / 4;
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i4 < n4) {
+        const float4 *p = reinterpret_cast<const float4 *>(part) + i4;
+        int z = g;
+        for (; z + 28 < nslice; z += 32) {
+            float4 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = p[(int64_t)(z + 4 * u) * n4];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                s.x += v[u].x;
+                s.y += v[u].y;
+                s.z += v[u].z;
+                s.w += v[u].w;
+            }
+        }
+        for (; z < nslice; z += 4) {
+            const float4 v = p[(int64_t)z * n4];
+            s.x += v.x;
+            s.y += v.y;
+            s.z += v.z;
+            s.w += v.w;
+        }
+    }
+    red[g][c] = s;
+    __syncthreads();
+    if (g == 0 && i4 < n4) {
+        float4 o = red[0][c];
+#pragma unroll
+        for (int q = 1; q < 4; ++q) {
+            o.x += red[q][c].x;
+            o.y += red[q][c].y;
+            o.z += red[q][c].z;
+            o.w += red[q][c].w;
+        }
+        reinterpret_cast<float4 *>(out)[i4] = o;
+    }
+}
+
+// column sums of a BF16 matrix (M x N, N even, 256 % (N / 2) == 0) in row
+// slices: part[z * N + j].  Block z takes slice z: thread (column pair cp, row
+// group rg) sums rows rg, rg + RG, ... of the slice (8 loads in flight), and the
+// RG row-group sums are added in group order.
+__global__ __launch_bounds__(256) void k_nn_colsum2(const uint16_t *__restrict__ Y, int M, int N, int rslice,
+                                                    float *__restrict__ part)
+{
+    __shared__ float2 red[256];
+    const int cpn = N / 2, RG = 256 / cpn;
+    const int cp = threadIdx.x % cpn, rg = threadIdx.x / cpn;
+    const int r0 = blockIdx.x * rslice, r1 = min(M, r0 + rslice);
+    const uint32_t *y = reinterpret_cast<const uint32_t *>(Y) + cp;
+    float2 s = make_float2(0.f, 0.f);
+    int r = r0 + rg;
+    for (; r + 7 * RG < r1; r += 8 * RG) {
+        uint32_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = y[(int64_t)(r + u * RG) * cpn];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            s.x += __uint_as_float(v[u] << 16);
+            s.y += __uint_as_float(v[u] & 0xffff0000u);
+        }
+    }
+    for (; r < r1; r += RG) {
+        const uint32_t v = y[(int64_t)r * cpn];
+        s.x += __uint_as_float(v << 16);
+        s.y += __uint_as_float(v & 0xffff0000u);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (rg == 0) {
+        float2 o = red[cp];
+        for (int q = 1; q < RG; ++q) {
+            o.x += red[q * cpn + cp].x;
+            o.y += red[q * cpn + cp].y;
+        }
+        part[(int64_t)blockIdx.x * N + 2 * cp] = o.x;
+        part[(int64_t)blockIdx.x * N + 2 * cp + 1] = o.y;
+    }
+}
+
+// sum over the slices of part into out (n values): the fixed-order float4 kernel when n allows it
+static void nn_sum_slices(const float *part, int ns, int64_t n, float *out, hipStream_t st)
+{
+    if (n % 4 == 0 && !(reinterpret_cast<uintptr_t>(part) & 15) && !(reinterpret_cast<uintptr_t>(out) & 15))
+        hipLaunchKernelGGL(k_nn_sum_slices4, dim3((unsigned)((n / 4 + 63) / 64)), dim3(256), 0, st, part, ns, n, out);
+    else
+        hipLaunchKernelGGL(k_nn_sum_slices, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, part, ns, n, out);
 }
 
 // masked mean-squared error of Y (M x ldy, first 3 columns, FP32) against T
@@ -615,20 +914,25 @@ extern "C" int pdx_nn_linear_wgrad(const uint16_t *X, const uint16_t *dY, int M,
         rows_per_slice % 8)
         return 1;
     const int ns = (M + rows_per_slice - 1) / rows_per_slice;
-    // dW (K x N) = X^T (K x M) dY (M x N): A(i = k_in, k = m) = X[m * K + k_in], B(k = m, j) = dY[m * ldy + j]
-    dim3 g(nn_grid_rows(K), (N + NN_TN - 1) / NN_TN, ns);
-    hipLaunchKernelGGL(k_nn_gemm<NN_EPI_WGRAD>, g, dim3(256), 0, (hipStream_t)stream, X, (int64_t)1, (int64_t)K, dY,
-                       (int64_t)ldy, (int64_t)1, K, N, M, rows_per_slice, nullptr, 0, nullptr, (int64_t)0, nullptr,
-                       part, (int64_t)N);
+    // dW (K x N) = X^T (K x M) dY (M x N): A(i = k_in, k = m) = X[m * K + k_in], B(k = m, j) = dY[m * ldy + j];
+    // the transpose-read MFMA kernel for the layers' shapes, else the generic one
+    if (!nn_wgrad_tr(X, dY, M, K, N, rows_per_slice, part, (hipStream_t)stream)) {
+        dim3 g(nn_grid_rows(K), (N + NN_TN - 1) / NN_TN, ns);
+        hipLaunchKernelGGL(k_nn_gemm<NN_EPI_WGRAD>, g, dim3(256), 0, (hipStream_t)stream, X, (int64_t)1, (int64_t)K,
+                           dY, (int64_t)ldy, (int64_t)1, K, N, M, rows_per_slice, nullptr, 0, nullptr, (int64_t)0,
+                           nullptr, part, (int64_t)N);
+    }
     const int64_t nw = (int64_t)K * N;
-    hipLaunchKernelGGL(k_nn_sum_slices, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
-                       part, ns, nw, dW);
+    nn_sum_slices(part, ns, nw, dW, (hipStream_t)stream);
     if (db) {
         // bias gradient: column sums of dY in the same row slices
-        hipLaunchKernelGGL(k_nn_colsum, dim3((N + 255) / 256, ns), dim3(256), 0, (hipStream_t)stream, dY, M, ldy,
-                           rows_per_slice, part);
-        hipLaunchKernelGGL(k_nn_sum_slices, dim3((N + 255) / 256), dim3(256), 0, (hipStream_t)stream, part, ns,
-                           (int64_t)ldy, db);
+        if (N % 2 == 0 && N <= 512 && 256 % (N / 2) == 0 && !(reinterpret_cast<uintptr_t>(dY) & 3))
+            hipLaunchKernelGGL(k_nn_colsum2, dim3(ns), dim3(256), 0, (hipStream_t)stream, dY, M, N, rows_per_slice,
+                               part);
+        else
+            hipLaunchKernelGGL(k_nn_colsum, dim3((N + 255) / 256, ns), dim3(256), 0, (hipStream_t)stream, dY, M, ldy,
+                               rows_per_slice, part);
+        nn_sum_slices(part, ns, (int64_t)N, db, (hipStream_t)stream);
     }
     return nn_check();
 }

@@ -28,6 +28,12 @@ def _ok(rc, what):
         raise RuntimeError(f"{what} failed with status {rc}")
 
 
+def wgrad_rows_per_slice(M, slices=256):
+    """Rows per weight-gradient slice: a multiple of 32 (the transpose-read
+    kernel's stage) giving about `slices` partials, one block each."""
+    return max(32, -(-M // (slices * 32)) * 32)
+
+
 class WarmStartNet:
     def __init__(self, dims=DIMS, seed: int = 0, weights=None):
         import torch
@@ -112,11 +118,15 @@ class WarmStartNet:
                 acts.append(h)
         return (yf, acts) if keep else yf
 
-    def gradients(self, Xbf, T, w, wsum, rows_per_slice=4096):
+    def gradients(self, Xbf, T, w, wsum, rows_per_slice=None):
         """Loss and parameter gradients of sum_i w_i |y_i - t_i|^2 / wsum.
-        T: (M, 3) FP32 targets, w: (M,) FP32 weights (device tensors)."""
+        T: (M, 3) FP32 targets, w: (M,) FP32 weights (device tensors).
+        rows_per_slice: rows of one weight-gradient partial (default: about one
+        slice per CU, see wgrad_rows_per_slice)."""
         torch = self.torch
         M = Xbf.shape[0]
+        if rows_per_slice is None:
+            rows_per_slice = wgrad_rows_per_slice(M)
         y, acts = self.forward(Xbf, keep=True)
         L = len(self.W)
         nout = self.dims[-1]
