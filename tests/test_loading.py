@@ -25,22 +25,44 @@ def _oracle(spec, nthreads=None):
     return p
 
 
-def _oracle_increments(p, idx, u_final, n_inc, tol):
-    its = []
+def _oracle_increments(p, idx, u_final, n_inc, tol, critical_stretch=None, break_rounds=3):
+    """The driver's increments (simul_amd.loading.IncrementalLoading.step) on the
+    oracle: prescribed DOFs at k / n of the final displacement, adaptive DR to
+    tol, then up to break_rounds rounds of breaking, shape tensor and
+    re-equilibration to tol of the increment's start residual, then the J2
+    commit.  Returns (iterations, bonds broken) per increment."""
+    out = []
     for k in range(1, n_inc + 1):
         u = p.u
         u[idx] = (k / n_inc) * u_final
         p.set_u(u)
         r = p.adr_solve(p.adr_mass(1.0), max_it=100000, tol=tol)
         assert r["converged"]
+        its, nb_now, ref = r["iterations"], 0, r["reference_residual"]
+        if critical_stretch is not None:
+            for _ in range(break_rounds):
+                nb = p.update_damage(critical_stretch)
+                if nb == 0:
+                    break
+                nb_now += nb
+                p.shape_tensor("relative", 1e-12)
+                r = p.adr_solve(p.adr_mass(1.0), max_it=100000, tol=tol, ref=ref)
+                assert r["converged"]
+                its += r["iterations"]
         p.deformation_gradient()
         p.stress()
         p.commit()
-        its.append(r["iterations"])
-    return its
+        out.append((its, nb_now))
+    return out
 
 
 def test_incremental_j2_matches_oracle(gpu):
+    """Config 3's J2 loading at dx 1.25 mm (12,080 particles), 4 increments to
+    1e-6, against the oracle's increments to the same criterion.  Measured:
+    the same iteration count in every increment, u within 1.0e-8 and the
+    equivalent plastic strain within 6.9e-8 relative-L2 (the bound below
+    leaves 100x; against an oracle solved to 1e-9 instead, u differs by
+    4.9e-5: the 1e-6 criterion's own error)."""
     spec = W.vertebra_j2(dx=1.25, n_increments=4)
     idx, uf = spec.extra["top_idx"], spec.extra["u_final"]
     body = from_specimen(spec)
@@ -49,12 +71,39 @@ def test_incremental_j2_matches_oracle(gpu):
     assert all(r.converged for r in res)
     prob = _oracle(spec)
     its = _oracle_increments(prob, idx, uf, 4, 1e-6)
-    for r, io in zip(res, its):
-        assert abs(r.iterations - io) <= max(10, 0.15 * io), (r.iterations, io)
+    for r, (io, _) in zip(res, its):
+        assert abs(r.iterations - io) <= max(3, 0.01 * io), (r.iterations, io)
     assert np.allclose(body.u[idx], uf)
-    assert rel_l2(body.u, prob.u) < 1e-4, rel_l2(body.u, prob.u)
+    assert rel_l2(body.u, prob.u) < 1e-6, rel_l2(body.u, prob.u)
     assert prob.epbar.max() > 0  # the cancellous core yields
-    assert rel_l2(body.get("epbar"), prob.epbar) < 1e-3, rel_l2(body.get("epbar"), prob.epbar)
+    assert rel_l2(body.get("epbar"), prob.epbar) < 1e-5, rel_l2(body.get("epbar"), prob.epbar)
+
+
+def test_incremental_breaking_matches_oracle(gpu):
+    """Config 3's loading with bond breaking at dx 1.25 mm: the configured 2 %
+    compression in 3 increments, critical stretch 0.012, up to 3 breaking
+    rounds per increment, against the oracle running the same increments.
+    Breaking cascades here (about 173 k of 1.4 M bonds break), so any
+    difference in the fields would show in the flags.  Measured: broken flags
+    identical, the same iterations and bonds broken in every increment, u
+    within 2.1e-8 and the plastic strain within 9.2e-8 relative-L2."""
+    spec = W.vertebra_j2(dx=1.25, n_increments=3)
+    idx, uf = spec.extra["top_idx"], spec.extra["u_final"]
+    body = from_specimen(spec)
+    drv = IncrementalLoading(body, idx, uf, n_increments=3, critical_stretch=0.012, tol=1e-6, max_break_rounds=3)
+    res = drv.run()
+    assert all(r.converged for r in res)
+    prob = _oracle(spec)
+    out = _oracle_increments(prob, idx, uf, 3, 1e-6, critical_stretch=0.012, break_rounds=3)
+    print("GPU", [(r.iterations, r.broken_now) for r in res], "oracle", out, flush=True)
+    for r, (io, nbo) in zip(res, out):
+        assert r.broken_now == nbo
+        assert abs(r.iterations - io) <= max(3, 0.01 * io), (r.iterations, io)
+    _, _, br = body.bonds()
+    assert br.sum() > 1000
+    assert np.array_equal(br.astype(bool), prob.broken[: prob.nb].astype(bool))
+    assert rel_l2(body.u, prob.u) < 1e-6, rel_l2(body.u, prob.u)
+    assert rel_l2(body.get("epbar"), prob.epbar) < 1e-5, rel_l2(body.get("epbar"), prob.epbar)
 
 
 def test_incremental_breaking_invariants(gpu):
@@ -124,7 +173,8 @@ def test_secant_predictor_and_spectral_mass_same_states(gpu):
     """The J2 loading with the secant predictor and the spectral DR mass ends
     every increment at the state of the plain driver (same tolerance against
     the same plain-start reference) in fewer DR iterations; the oracle's
-    final state agrees to the tolerance of test_incremental_j2_matches_oracle."""
+    final state agrees within 1e-4 (another DR path to the same 1e-6
+    criterion, whose own error is about 5e-5 here)."""
     spec = W.vertebra_j2(dx=1.25, n_increments=6)
     idx, uf = spec.extra["top_idx"], spec.extra["u_final"]
     runs = {}
