@@ -113,6 +113,8 @@ def main():
                     help="run only a plain 256 MiB device copy (scales rocprof FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--headline-only", action="store_true",
                     help="measure only the dx 0.4 vertebral body (no J2-loading and batch blocks)")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "simul_amd", "learned", "weights_vertebra.npz"),
+                    help="the warm start's network weights")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -247,7 +249,7 @@ def main():
         log(f"time-to-solve: {wall:.3f}s, {rs.iterations} iterations, converged={rs.converged}, "
             f"rel={rs.relative_residual:.2e}")
         u_cold = body.u
-        wfile = os.path.join(ROOT, "simul_amd", "learned", "weights_vertebra.npz")
+        wfile = args.weights
         if os.path.exists(wfile):
             from simul_amd.learned import WarmStartNet, warm_solve
             from simul_amd.learned.warmstart import predict_u_device
@@ -532,14 +534,24 @@ def run_nn(args, torch):
                             "wgrad_us": round(us_w, 2), "wgrad_tflops": round(flop / us_w / 1e6, 1)})
             rows.append(rec)
             log(f"M {M} {k}x{n}: " + ", ".join(f"{a} {b}" for a, b in rec.items() if a not in ("rows", "k", "n")))
-    head = next(r for r in rows if r["rows"] == 65536 and r["k"] == 256 and r["n"] == 256)
-    flop = 2.0 * 65536 * 256 * 256
-    return _line("BF16 MFMA GEMM FLOP/s (learned warm start, 256x256 layer, 65 536 rows)", flop / head["fwd_us"] * 1e6,
+    # the value: a 256 x 256 layer's forward at the dx 0.4 vertebral body (the warm start's inference there)
+    M = 365056
+    head = min((r for r in rows if r["rows"] == M and r["k"] == 256 and r["n"] == 256), key=lambda r: r["fwd_us"])
+    flop = 2.0 * M * 256 * 256
+    byts = M * (256 + 256) * 2 + 256 * 256 * 2  # BF16 rows in and out, the weight
+    # one training step of a 65 536-row batch: forward, data gradients (layers 3..1), weight gradients
+    tr = [r for r in rows if r["rows"] == 65536]
+    step_us = sum(r["fwd_us"] + r["wgrad_us"] for r in tr) + sum(r["dgrad_us"] for r in tr[1:])
+    return _line("BF16 MFMA GEMM FLOP/s (learned warm start, 256x256 layer, 365 056 rows)", flop / head["fwd_us"] * 1e6,
                  "FLOP/s", reps, 3, head["fwd_us"] / 1e3,
                  {"workload": "warmstart_mlp_64-256-256-256-32", "rows": [65536, 365056]},
                  roofline={"bound": "mfma", "kernel": "k_nn_gemm_lds", "achieved": head["fwd_tflops"],
                            "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": head["fwd_frac_peak"],
-                           "traffic": None},
+                           "traffic": None,
+                           "hbm": {"achieved": round(byts / head["fwd_us"] / 1e3, 1), "peak": HBM_PEAK_GBS,
+                                   "unit": "GB/s", "frac": round(byts / head["fwd_us"] / 1e3 / HBM_PEAK_GBS, 3)}},
+                 training_step={"rows": 65536, "us": round(step_us, 1),
+                                "rows_per_s": round(65536 / step_us * 1e6)},
                  layers=rows, cpu_baseline=None)
 
 

@@ -33,44 +33,50 @@ def _group_body(specs):
     return packed, off, body
 
 
-def generate(seeds, dx, group, tol, max_iterations):
-    """Features, targets and weights of the cases ``seeds`` at spacing dx (host float32)."""
+def generate(seeds, dx, group, tol, max_iterations, to_bf16, mass=None):
+    """Training data of the cases ``seeds`` at spacing dx, kept on the device:
+    BF16 features (int16 bits), FP32 targets u / s and FP32 weights (1 on free
+    particles), as lists of CUDA tensors per group of ``group`` cases.  The
+    cases are solved by the plain cold-start adaptive DR to tol (``mass``:
+    PDBody.solve's DR mass)."""
+    import torch
     from ..workloads import random_vertebra
-    from .features import feature_matrix, target_scale
-    Fs, Ts, Ws, stats = [], [], [], []
+    from .features import feature_tensor, target_scale_tensor
+    Xs, Ts, Ws, stats = [], [], [], []
     for g0 in range(0, len(seeds), group):
         t0 = time.perf_counter()
         specs = [random_vertebra(int(s), dx=dx) for s in seeds[g0:g0 + group]]
         packed, off, body = _group_body(specs)
-        rs = body.solve("adaptive", tol=tol, max_iterations=max_iterations, slice_iterations=2048,
+        rs = body.solve("adaptive", tol=tol, max_iterations=max_iterations, slice_iterations=2048, mass=mass,
                         progress=lambda d, r: log(f"    {d} iterations: {sum(x.converged for x in r)}/{len(r)} converged"))
-        u = body.u
-        F = feature_matrix(body, packed, off)
-        s = target_scale(packed, off)
-        Fs.append(F)
-        Ts.append((u / s[:, None]).astype(np.float32))
-        Ws.append((packed.fixed == 0).astype(np.float32))
+        u = torch.from_numpy(body.u).cuda()
+        F = feature_tensor(body, packed, off)
+        Xs.append(to_bf16(F))
+        del F
+        s = target_scale_tensor(packed, off)
+        Ts.append((u / s[:, None]).to(torch.float32))
+        Ws.append(torch.from_numpy((packed.fixed == 0).astype(np.float32)).cuda())
         its = [r.iterations for r in rs]
         stats.append({"cases": len(specs), "particles": packed.n, "converged": int(sum(r.converged for r in rs)),
                       "iterations_max": int(max(its)), "seconds": time.perf_counter() - t0})
         log(f"  data group {g0 // group}: {len(specs)} cases, {packed.n} particles, iterations "
             f"{min(its)}..{max(its)}, {stats[-1]['seconds']:.1f}s")
         del body
-    return np.concatenate(Fs), np.concatenate(Ts), np.concatenate(Ws), stats
+    return Xs, Ts, Ws, stats
 
 
-def train(net, F, T, Wt, epochs, batch_rows, lr0, lr_min, seed=0):
+def train(net, Xbf, T, Wd, epochs, batch_rows, lr0, lr_min, seed=0):
+    """Adam over the device-resident data (BF16 features Xbf, FP32 targets T
+    and weights Wd), shuffled once on the device, cosine learning rate."""
     import torch
-    rng = np.random.default_rng(seed)
-    order = rng.permutation(len(F))
-    Fd = torch.from_numpy(F[order]).cuda()
-    Xbf = net.to_bf16(Fd)
-    del Fd
-    Td = torch.from_numpy(np.ascontiguousarray(T[order])).cuda()
-    Wd = torch.from_numpy(np.ascontiguousarray(Wt[order])).cuda()
-    n = len(F)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    order = torch.randperm(Xbf.shape[0], device="cuda", generator=gen)
+    Xbf, T, Wd = Xbf[order], T[order], Wd[order]
+    del order
+    n = Xbf.shape[0]
     nb = max(1, n // batch_rows)
     total = epochs * nb
+    wsums = [max(float(Wd[bi * batch_rows:min(n, (bi + 1) * batch_rows)].sum()), 1.0) for bi in range(nb)]
     hist = []
     step = 0
     t0 = time.perf_counter()
@@ -79,8 +85,7 @@ def train(net, F, T, Wt, epochs, batch_rows, lr0, lr_min, seed=0):
         for bi in range(nb):
             a, b = bi * batch_rows, min(n, (bi + 1) * batch_rows)
             lr = lr_min + 0.5 * (lr0 - lr_min) * (1 + np.cos(np.pi * step / total))
-            wsum = float(Wt[order[a:b]].sum())
-            loss = net.train_step(Xbf[a:b], Td[a:b], Wd[a:b], max(wsum, 1.0), lr)
+            loss = net.train_step(Xbf[a:b], T[a:b], Wd[a:b], wsums[bi], lr)
             ep_loss += loss
             step += 1
         hist.append(ep_loss / nb)
@@ -155,6 +160,8 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=5000)
     ap.add_argument("--mass", default=None, choices=["gershgorin", "spectral"],
                     help="adaptive-DR mass of the held-out cold and warm solves")
+    ap.add_argument("--data-mass", default=None, choices=["gershgorin", "spectral"],
+                    help="adaptive-DR mass of the training-data solves")
     ap.add_argument("--out", default=None)
     ap.add_argument("--report", default=None)
     args = ap.parse_args(argv)
@@ -166,17 +173,25 @@ def main(argv=None):
     train_seeds = np.arange(args.seed, args.seed + args.train)
     held_seeds = np.arange(args.seed + 100000, args.seed + 100000 + args.heldout)
     parts = np.array_split(train_seeds, len(args.dx))
-    Fs, Ts, Ws, gstats = [], [], [], []
+    net = WarmStartNet(seed=args.seed)
+    Xs, Ts, Ws, gstats = [], [], [], []
+    t_gen = time.perf_counter()
     for dx, seeds in zip(args.dx, parts):
         log(f"generating {len(seeds)} training cases at dx {dx}")
-        F_, T_, W_, g_ = generate(seeds, dx, args.group, args.tol, args.max_iterations)
-        Fs.append(F_), Ts.append(T_), Ws.append(W_), gstats.extend(g_)
-    F, T, Wt = np.concatenate(Fs), np.concatenate(Ts), np.concatenate(Ws)
-    del Fs, Ts, Ws
-    log(f"training set: {len(F)} particles, {int(Wt.sum())} free")
-    net = WarmStartNet(seed=args.seed)
-    hist = train(net, F, T, Wt, args.epochs, args.batch_rows, args.lr, args.lr_min)
-    del F, T, Wt
+        X_, T_, W_, g_ = generate(seeds, dx, args.group, args.tol, args.max_iterations, net.to_bf16,
+                                  mass=args.data_mass)
+        Xs += X_
+        Ts += T_
+        Ws += W_
+        gstats += g_
+    t_gen = time.perf_counter() - t_gen
+    Xbf, T, Wd = torch.cat(Xs), torch.cat(Ts), torch.cat(Ws)
+    del Xs, Ts, Ws
+    log(f"training set: {Xbf.shape[0]} particles, {int(Wd.sum())} free ({t_gen:.0f}s to generate)")
+    t_tr = time.perf_counter()
+    hist = train(net, Xbf, T, Wd, args.epochs, args.batch_rows, args.lr, args.lr_min)
+    t_tr = time.perf_counter() - t_tr
+    del Xbf, T, Wd
     if args.out:
         net.save(args.out, dx=np.array(args.dx), train_cases=args.train, epochs=args.epochs, final_loss=hist[-1])
         log(f"weights -> {args.out}")
@@ -187,7 +202,8 @@ def main(argv=None):
     ic = [i for e in ev for i in e["cold_iterations"]]
     iw = [i for e in ev for i in e["warm_iterations"]]
     rep = {"train_cases": args.train, "heldout_cases": args.heldout, "dx": list(args.dx), "epochs": args.epochs,
-           "loss_history": hist, "data_groups": gstats, "heldout": ev,
+           "train_rows": int(sum(g["particles"] for g in gstats)), "data_s": t_gen, "train_s": t_tr,
+           "mass": args.mass, "loss_history": hist, "data_groups": gstats, "heldout": ev,
            "summary": {"cold_s": cold, "warm_s": warm, "speedup_wall": cold / max(warm, 1e-9),
                        "cold_iterations_mean": float(np.mean(ic)), "warm_iterations_mean": float(np.mean(iw)),
                        "speedup_iterations_mean": float(np.mean(ic) / max(np.mean(iw), 1e-9)),

@@ -269,19 +269,20 @@ def test_device_features_refuse_bad_arguments(gpu):
 
 
 def test_trained_network_beats_the_same_mass_cold_start(gpu):
-    """The committed network (trained on random vertebral bodies at dx 0.8 and
-    0.6 mm) on 16 held-out specimens (seeds never used in training) at dx 0.8,
-    batched as in training: DR from its prediction reaches the cold start's
-    criterion (1e-6 of each case's cold residual, the same spectral DR mass)
-    in fewer iterations, and converges to the cold solution.  Measured gain on
-    64 held-out cases: 1.32x in iterations; the dx 0.4 headline body, outside
-    the training spacings, gains nothing (0.96x)."""
+    """The network trained at config 5's scale (4,096 random vertebral bodies at
+    dx 0.8 mm, 164 M particle rows, simul_amd/learned/weights_vertebra_dx08.npz)
+    on 16 of its held-out specimens (seeds never used in training), batched as
+    in training: DR from its prediction reaches the cold start's criterion
+    (1e-6 of each case's cold residual, the same spectral DR mass) in fewer
+    iterations, and converges to the cold solution.  Measured on all 256
+    held-out cases: 1.39x in iterations (every case at least 1.24x), 1.33x in
+    wall time with features and inference included."""
     import os
     import numpy as np
     from simul_amd.learned import WarmStartNet
     from simul_amd.learned.train import evaluate
     w = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "simul_amd", "learned",
-                     "weights_vertebra.npz")
+                     "weights_vertebra_dx08.npz")
     net = WarmStartNet.load(w)
     seeds = np.arange(105000, 105016)  # the held-out range of train.py (seed + 100000)
     ev = evaluate(net, seeds, 0.8, 16, 1e-6, 60000, mass="spectral")
@@ -292,4 +293,5 @@ def test_trained_network_beats_the_same_mass_cold_start(gpu):
           f"(iterations {np.mean(ic) / np.mean(iw):.2f}x, wall {cold_s / warm_s:.2f}x)", flush=True)
     assert all(e["all_converged"] for e in ev)
     assert max(x for e in ev for x in e["final_rel_l2_vs_cold"]) < 5e-5  # two solutions at 1e-6
-    assert np.mean(ic) / np.mean(iw) > 1.15
+    assert min(c / w for c, w in zip(ic, iw)) > 1.1
+    assert np.mean(ic) / np.mean(iw) > 1.25
