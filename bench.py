@@ -472,6 +472,17 @@ def run_batch(args, torch):
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 
 
+def _nn_traffic():
+    """HBM bytes per launch of the 256 x 256 forward at 365 056 rows, from the
+    committed FETCH_SIZE / WRITE_SIZE summary (profiles/), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic_warmstart_mlp_gemm.json")) as fh:
+            t = json.load(fh)["fwd_365056"]
+        return int(t["fetch"] + t["write"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def run_nn(args, torch):
     """The learned warm start's layer GEMMs on the BF16 MFMA kernels
     (include/pdx_nn.h): forward, data gradient and weight gradient of every
@@ -547,7 +558,7 @@ def run_nn(args, torch):
                  {"workload": "warmstart_mlp_64-256-256-256-32", "rows": [65536, 365056]},
                  roofline={"bound": "mfma", "kernel": "k_nn_gemm_lds", "achieved": head["fwd_tflops"],
                            "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": head["fwd_frac_peak"],
-                           "traffic": None,
+                           "traffic": _nn_traffic(),
                            "hbm": {"achieved": round(byts / head["fwd_us"] / 1e3, 1), "peak": HBM_PEAK_GBS,
                                    "unit": "GB/s", "frac": round(byts / head["fwd_us"] / 1e3 / HBM_PEAK_GBS, 3)}},
                  training_step={"rows": 65536, "us": round(step_us, 1),
