@@ -3,13 +3,17 @@
 // on these kernels; host code (Python) owns the device buffers and calls the
 // C ABI below with device pointers on a given HIP stream.
 //
-// Matrices are row-major.  Every GEMM is one kernel, C = A B with A and B read
-// through element strides, so transposed operands (W^T in the data gradient,
-// X^T in the weight gradient) are views, not copies.  Each wave computes a
-// 32x32 tile of C with v_mfma_f32_32x32x8_bf16 (FP32 accumulation); a block of
-// 4 waves covers 128 rows x 32 columns.  The weight gradient reduces over the
-// particle dimension in fixed-size slices (grid z) written as FP32 partials and
-// summed in slice order by a second kernel: training is deterministic.
+// Matrices are row-major; transposed operands (W^T in the data gradient, X^T in
+// the weight gradient) are read through strides or transpose reads, never
+// copied.  The layers' shapes run on gfx950's v_mfma_f32_32x32x16_bf16 with
+// FP32 accumulation:
+//   k_nn_gemm_lds  forward and data gradient, the weight resident in LDS;
+//   k_nn_wgrad     weight gradient, the reduction over the particle rows with
+//                  LDS transpose reads, in fixed-size row slices whose FP32
+//                  partials are summed in a fixed order (k_nn_sum_slices4):
+//                  training is deterministic.
+// k_nn_gemm, one 32x32 tile per wave on v_mfma_f32_32x32x8_bf16 with operands
+// loaded element by element, serves every other shape or alignment.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -95,17 +99,18 @@ __global__ __launch_bounds__(256) void k_nn_gemm(const uint16_t *__restrict__ A,
 // gradient).  A persistent grid of at most one block per CU stages B once,
 // transposed to column-major (the 16-byte k chunks of each column XOR-swizzled
 // so that staging stores and operand reads both spread over the LDS banks),
-// then walks tiles of 128 rows.  Each of the 4 waves (one per SIMD) owns RT row tiles x CT
-// column tiles of 32 x 32 (WC waves across the columns).
+// then walks tiles of 128 rows.  Each of the 4 waves (one per SIMD) owns RT
+// row tiles x CT column tiles of 32 x 32 (WC waves across the columns).
 //
 // The tile loop is software-pipelined so that one wave per SIMD keeps the
 // MFMA unit, the loads and the stores busy at once.  During tile t's k steps a
 // lane
-//   * reloads each A operand register with tile t + G's value (G the grid)
-//     right after its MFMAs are issued (8 consecutive values of a row per 16-wide
-//     k step, 16-byte loads, every row read once),
-//   * stores tile t - G's results, packed in registers at the end of the
-//     previous tile, a few 4-column groups per k step,
+//   * reloads each A operand register with tile t + PF G's value (G the grid;
+//     PF sets of operands are in flight) right after its MFMAs are issued (8
+//     consecutive values of a row per 16-wide k step, 16-byte loads, every row
+//     read once),
+//   * stores the previous tile's results, packed in the store layout at its
+//     end (below), a few 16-byte row pieces per k step,
 //   * (data gradient) loads tile t's ReLU mask in the first half of the steps,
 //   * reads the next k step's B operands from LDS.
 // Every global access goes through a raw buffer descriptor of the tile's rows
@@ -118,12 +123,12 @@ __global__ __launch_bounds__(256) void k_nn_gemm(const uint16_t *__restrict__ A,
 // 8 (l / 32) .. 8 (l / 32) + 7; result register v of lane l is row
 // 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32.  The operands are passed
 // swapped (C^T = B^T A^T), so a lane's registers 4 g .. 4 g + 3 of a tile are
-// columns 8 g + 4 (l / 32) .. + 3 of its row l % 32: 8- or 16-byte stores.
+// columns 8 g + 4 (l / 32) .. + 3 of its row l % 32: 8- or 16-byte writes
+// into the staging rows.
 // ---------------------------------------------------------------------------
 typedef __bf16 pdx_bf8 __attribute__((ext_vector_type(8)));
 typedef __bf16 pdx_bf4v __attribute__((ext_vector_type(4)));
 typedef uint32_t pdx_u4v __attribute__((ext_vector_type(4)));
-typedef uint32_t pdx_u2v __attribute__((ext_vector_type(2)));
 
 enum { NN_OUT_BF16 = 0, NN_OUT_F32 = 1 };
 
@@ -134,14 +139,6 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t nn_rsrc(const void *p, int byt
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, bytes, 0x00020000);
 }
-
-template <int OUT> struct nn_packed;
-template <> struct nn_packed<NN_OUT_BF16> {
-    typedef pdx_u2v type;  // 4 BF16
-};
-template <> struct nn_packed<NN_OUT_F32> {
-    typedef pdx_u4v type;  // 4 FP32
-};
 
 template <int EPI, int OUT, int RELU, int KR, int NC, int RT, int CT, int WC, int PF>
 __global__ __launch_bounds__(256) void k_nn_gemm_lds(const uint16_t *__restrict__ A, int64_t lda,
@@ -180,7 +177,6 @@ __global__ __launch_bounds__(256) void k_nn_gemm_lds(const uint16_t *__restrict_
     // B operands held in registers for the whole kernel when they fit (64
     // VGPRs); else read from LDS one k step ahead
     constexpr bool BREG = KS * CT <= 16;
-    typedef typename nn_packed<OUT>::type PV;
     extern __shared__ uint16_t bt[];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = wv / WC, wc = wv % WC;
