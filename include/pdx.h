@@ -226,8 +226,8 @@ int pdx_adr_mass(pdx_ctx *ctx, double beta);
 /* adaptive-DR mass from the spectrum (no reference counterpart): the Gershgorin mass of
  * pdx_adr_mass(ctx, 1) scaled per case by beta = safety * rho / 4 (at most 1), rho the
  * largest eigenvalue of M^-1 K estimated by `iters` power iterations (force differences
- * at a perturbation of 1e-4 horizons around the current u; 2 force passes each).  With
- * safety 2 the explicit update keeps a factor-2 margin to its stability limit and DR
+ * at a perturbation of 1e-4 horizons around the current u; 2 force passes each).  safety
+ * is the margin to the explicit update's stability limit (the Python layer uses 1.4); DR
  * needs about sqrt(beta) of the Gershgorin iterations.  beta_out: ncase values or NULL.
  * The solver state is unchanged except the second displacement buffer and the forces. */
 int pdx_adr_mass_auto(pdx_ctx *ctx, int32_t iters, double safety, double *beta_out);
