@@ -45,10 +45,14 @@ METRIC = "PD bond-updates/s; time-to-solve at rel. residual 1e-6 (plain vs warm-
 HBM_PEAK_GBS = 8000.0
 VARIANT_COOP, VARIANT_HALO = 16, 32  # PDX_VARIANT_COOP / PDX_VARIANT_HALO of include/pdx.h
 VARIANT_SPLIT2, VARIANT_SPLIT4 = 64, 128
+VARIANT_LATTICE = 256
 
 
 def variant_names(v):
     """(label, F-pass kernel, force kernel) of a hot-pass variant code."""
+    if v & VARIANT_LATTICE:
+        split = 4 if v & VARIANT_SPLIT4 else (2 if v & VARIANT_SPLIT2 else 1)
+        return f"lattice stencil over the LDS-staged halo, rows split over {split} waves", "k_fpass_l", "k_force_l"
     if v & VARIANT_HALO:
         split = 4 if v & VARIANT_SPLIT4 else (2 if v & VARIANT_SPLIT2 else 1)
         return f"LDS-staged halo, bond group {v & 15}, rows split over {split} waves", "k_fpass_h", "k_force_h"
@@ -104,7 +108,7 @@ def main():
     ap.add_argument("--predictor", choices=["secant", "none"], default="secant",
                     help="start of each load increment of the J2 loading (secant: 2 u_(k-1) - u_(k-2))")
     ap.add_argument("--variant", type=str, default="0",
-                    help="hot-pass variant (1, 2, 4, 17, 18, 33, 34, 36, 97, 98, 100, 161, 162, 164) or 'F,G' for the F-pass "
+                    help="hot-pass variant (1, 2, 4, 17, 18, 33, 34, 36, 97, 98, 100, 161, 162, 164, 256, 320, 384) or 'F,G' for the F-pass "
                          "and the force pass; 0 = autotune")
     ap.add_argument("--workload", default="vertebra", choices=["vertebra", "vertebra_j2", "batch", "nn"],
                     help="vertebra: the headline (dx 0.4, Neo-Hookean, pressure); vertebra_j2: J2, 20 "

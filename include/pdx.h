@@ -287,6 +287,19 @@ int pdx_time_kernels(pdx_ctx *ctx, int32_t iters, double *ms);
  * block's halo (partial sums added in a fixed order) */
 #define PDX_VARIANT_SPLIT2 64
 #define PDX_VARIANT_SPLIT4 128
+/* PDX_VARIANT_LATTICE, optionally | PDX_VARIANT_SPLIT2 or PDX_VARIANT_SPLIT4 (no bond
+ * group): the halo staging of PDX_VARIANT_HALO, with each row walking the 122 offsets o
+ * of the lattice stencil (0 < |o|^2 <= 9) instead of its bond list, the bond vector
+ * xi = h o, |xi| and the influence weight being constants of the offset.  Available in
+ * 3-D NOSB contexts whose every bond is h o for one lattice spacing h (to 1e-8 h) and
+ * whose particles share one volume (voxel specimens at the reference's horizon
+ * 3.015 h); elsewhere, and for the diagnostic and J2 commit passes, the pass runs
+ * PDX_VARIANT_HALO with the same split.  Results agree with the halo variants to
+ * FP32 rounding.  pdx_lattice_spacing reports whether a context qualifies. */
+#define PDX_VARIANT_LATTICE 256
+/* h of the lattice kernels (0 when the context does not qualify for them); no
+ * reference counterpart */
+int pdx_lattice_spacing(pdx_ctx *ctx, double *h);
 /* launch tuning (no reference counterpart): times `iters` adaptive-DR iterations of
  * every hot-pass variant on this context's own data, keeps the fastest for each pass
  * and leaves the solver state as it was; group_out = the force pass's variant */

@@ -98,6 +98,7 @@ SIGNATURES = {
     "pdx_explicit_run": [_vp, C.c_double, C.c_double, C.c_int64, C.c_double, _i64p, _dp],
     "pdx_time_kernels": [_vp, C.c_int32, _dp],
     "pdx_stats": [_vp, _i64p],
+    "pdx_lattice_spacing": [_vp, _dp],
     "pdx_autotune": [_vp, C.c_int32, _i32p],
     "pdx_set_group": [_vp, C.c_int32],
     "pdx_set_variants": [_vp, C.c_int32, C.c_int32],

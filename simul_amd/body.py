@@ -385,6 +385,13 @@ class PDBody:
         self.ctx.call("pdx_get_variants", C.byref(f), C.byref(g))
         return f.value, g.value
 
+    def lattice_spacing(self):
+        """Lattice spacing h of the PDX_VARIANT_LATTICE kernels, or 0.0 when the
+        context does not qualify for them (pdx_lattice_spacing)."""
+        h = C.c_double()
+        self.ctx.call("pdx_lattice_spacing", C.byref(h))
+        return h.value
+
     def stats(self):
         o = np.zeros(4, np.int64)
         self.ctx.call("pdx_stats", N.ptr(o, N._i64p))

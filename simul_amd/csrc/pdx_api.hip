@@ -72,6 +72,7 @@ static void free_all(pdx_ctx *c)
     dfree(d.Fd); dfree(d.Pd); dfree(d.damage); dfree(d.init_nb);
     dfree(d.sl_ptr); dfree(d.bj); dfree(d.case_of_blk); dfree(d.case_blk0);
     dfree(d.hl_ptr); dfree(d.hl); dfree(d.hs); dfree(d.bl); dfree(d.bk); dfree(d.anchor); dfree(d.m64);
+    dfree(d.bks); dfree(d.lk);
     dfree(d.sc); dfree(d.part); dfree(d.grid); dfree(d.case_of_p); dfree(d.perm);
     dfree(d.cell_start); dfree(d.cell_idx); dfree(d.counts);
     if (c->gexec) hipGraphExecDestroy(c->gexec);
@@ -789,6 +790,8 @@ static int build_halo(pdx_ctx *c)
 {
     Dev &d = c->d;
     dfree(d.hl_ptr); dfree(d.hl); dfree(d.hs); dfree(d.bl); dfree(d.bk); dfree(d.anchor); dfree(d.m64);
+    dfree(d.bks); dfree(d.lk);
+    c->have_lat = false;
     d.halo_cap = 0;
     c->have_m = false;
     invalidate_graph(c);
@@ -803,6 +806,84 @@ static int build_halo(pdx_ctx *c)
         if (r != PDX_OK || !fallback) return r;
     }
     return build_halo_host(c);
+}
+
+// Lattice tables of the PDX_VARIANT_LATTICE kernels (d.bks, d.lk, c->lat),
+// built when the context qualifies: 3-D, halo tables built, one volume for
+// every particle, a horizon above 3 h, and every bond h o for an offset o of
+// the stencil (to 1e-8 h per component), h being the shortest bond.  Left
+// unbuilt (have_lat false) otherwise.
+static int build_lattice(pdx_ctx *c)
+{
+    Dev &d = c->d;
+    dfree(d.bks);
+    dfree(d.lk);
+    c->have_lat = false;
+    invalidate_graph(c);
+    if (c->dim != 3 || !d.hl_ptr || c->slots <= 0 || c->nb <= 0) return PDX_OK;
+    HostMirror &m = mirror(c);
+    const double V = m.vol[0];
+    for (int r = 1; r < c->n; ++r)
+        if (m.vol[r] != V) return PDX_OK;
+    unsigned long long *dmin = nullptr;
+    PDX_HIP(hipMallocAsync((void **)&dmin, 16, c->stream));
+    unsigned long long init[2] = {~0ull, 0ull};
+    hipError_t e = hipMemcpyAsync(dmin, init, 16, hipMemcpyHostToDevice, c->stream);
+    int r = e == hipSuccess ? pdx_launch_lat_minlen(c, dmin) : fail(PDX_EDEVICE, "hipMemcpyAsync failed");
+    unsigned long long hmin = 0;
+    if (r == PDX_OK && (hipMemcpyAsync(&hmin, dmin, 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                        hipStreamSynchronize(c->stream) != hipSuccess))
+        r = fail(PDX_EDEVICE, "reading the shortest bond");
+    double h2 = 0.0;
+    memcpy(&h2, &hmin, 8);
+    const double h = sqrt(h2);
+    const double horizon = d.h;
+    const size_t nbks = (size_t)(c->ni / PDX_SLICE) * PDX_NST * PDX_SLICE;
+    bool ok = r == PDX_OK && h > 0.0 && isfinite(h) && horizon > 3.0 * h * (1.0 + 1e-9);
+    if (ok) {
+        r = dalloc(c->stream, &d.bks, nbks);
+        if (r == PDX_OK) r = dalloc(c->stream, &d.lk, (size_t)c->slots);
+        unsigned *flag = reinterpret_cast<unsigned *>(dmin + 1);
+        if (r == PDX_OK) r = pdx_launch_lat_rows(c, h, 1e-8, V, flag);
+        unsigned viol = 0;
+        if (r == PDX_OK && (hipMemcpyAsync(&viol, flag, 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                            hipStreamSynchronize(c->stream) != hipSuccess))
+            r = fail(PDX_EDEVICE, "reading the lattice check");
+        ok = r == PDX_OK && viol == 0;
+    }
+    hipFreeAsync(dmin, c->stream);
+    hipStreamSynchronize(c->stream);
+    if (!ok) {
+        dfree(d.bks);
+        dfree(d.lk);
+        return r;
+    }
+    LatC &L = c->lat;
+    L = LatC{};
+    for (int n = 1; n <= 9; ++n) {
+        const double len = sqrt((double)n);
+        const double w0 = std::max(0.0, 1.0 - len * h / horizon);
+        L.w1[n] = w0;
+        L.w2[n] = 2.0 * w0;
+        L.w3[n] = 3.0 * w0;
+        L.c1[n] = (float)(h * w0);
+        L.c2[n] = (float)(2.0 * h * w0);
+        L.c3[n] = (float)(3.0 * h * w0);
+        L.wl[n] = (float)(w0 / len);
+    }
+    L.inv_h = 1.0 / h;
+    L.fscale = V * h;
+    L.V = V;
+    c->have_lat = true;
+    return PDX_OK;
+}
+
+extern "C" int pdx_lattice_spacing(pdx_ctx *c, double *h)
+{
+    CHECK_CTX(c);
+    if (!h) return fail(PDX_EINVAL, "h is NULL");
+    *h = c->have_lat ? 1.0 / c->lat.inv_h : 0.0;
+    return PDX_OK;
 }
 
 extern "C" int pdx_build_bonds(pdx_ctx *c, double horizon, int64_t *n_bonds)
@@ -839,6 +920,7 @@ extern "C" int pdx_build_bonds(pdx_ctx *c, double horizon, int64_t *n_bonds)
     }
     PDX_TRY(kr_put(c, w, KR_OFF_NNB, 1));
     PDX_TRY(build_halo(c));
+    PDX_TRY(build_lattice(c));
     c->horizon = horizon;
     c->have_bonds = true;
     c->have_K = c->have_mass = c->have_A = false;
@@ -1756,14 +1838,17 @@ extern "C" int pdx_time_kernels(pdx_ctx *c, int32_t iters, double *ms)
 
 static bool valid_variant(int32_t g)
 {
-    bool ok = g == 1 || g == 2 || g == 4 || g == (PDX_VARIANT_COOP | 1) || g == (PDX_VARIANT_COOP | 2);
+    bool ok = g == 1 || g == 2 || g == 4 || g == (PDX_VARIANT_COOP | 1) || g == (PDX_VARIANT_COOP | 2) ||
+              g == PDX_VARIANT_LATTICE || g == (PDX_VARIANT_LATTICE | PDX_VARIANT_SPLIT2) ||
+              g == (PDX_VARIANT_LATTICE | PDX_VARIANT_SPLIT4);
 #define PDX_HALO_OK(v) ok = ok || g == (v);
     PDX_HALO_VARIANTS(PDX_HALO_OK)
 #undef PDX_HALO_OK
     return ok;
 }
 
-static const char *k_variant_msg = "variant must be 1, 2, 4, PDX_VARIANT_COOP | 1 or 2, or PDX_VARIANT_HALO | 1, 2 or 4 "
+static const char *k_variant_msg = "variant must be 1, 2, 4, PDX_VARIANT_COOP | 1 or 2, PDX_VARIANT_HALO | 1, 2 or 4 "
+                                   "with optional PDX_VARIANT_SPLIT2 or PDX_VARIANT_SPLIT4, or PDX_VARIANT_LATTICE "
                                    "with optional PDX_VARIANT_SPLIT2 or PDX_VARIANT_SPLIT4";
 
 extern "C" int pdx_set_group(pdx_ctx *c, int32_t g)
@@ -1818,6 +1903,10 @@ extern "C" int pdx_autotune(pdx_ctx *c, int32_t iters, int32_t *group_out)
 #define PDX_HALO_CAND(v) cand.push_back(v);
     if (c->d.hl_ptr) { PDX_HALO_VARIANTS(PDX_HALO_CAND) }
 #undef PDX_HALO_CAND
+    if (c->have_lat)
+        for (int v : {PDX_VARIANT_LATTICE, PDX_VARIANT_LATTICE | PDX_VARIANT_SPLIT2,
+                      PDX_VARIANT_LATTICE | PDX_VARIANT_SPLIT4})
+            cand.push_back(v);
     for (int g : cand) {
         c->tune_f = c->tune_g = g;
         c->have_m = false;  // each candidate times its own m64 (pdx_time_kernels builds it)
@@ -1987,7 +2076,9 @@ extern "C" int pdx_set(pdx_ctx *c, int32_t field, const void *in)
         if (field == PDX_FIELD_VOLUME) {
             c->have_K = c->have_mass = false;
             invalidate_graph(c);
-            return push_prec(c);
+            PDX_TRY(push_prec(c));
+            // the lattice kernels take one volume for every particle
+            return c->have_bonds ? build_lattice(c) : PDX_OK;
         }
         return put_scalar(c, c->d.mass, m.mass.data());
     }

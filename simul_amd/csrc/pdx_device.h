@@ -161,6 +161,14 @@ struct Dev {
     // kept so that the pass need not sum it (valid while the context's
     // have_m is set)
     pdx_d4 *m64;
+    // lattice tables (null unless every bond of the context is an offset of
+    // the 122-site stencil on one lattice spacing, with one particle volume):
+    // bks[(slice * PDX_NST + k) * PDX_SLICE + lane] is the LDS slot of the
+    // row's neighbour at stencil offset k (the row's own slot when there is
+    // none, the bond is broken or the row inactive), lk[s] the stencil offset
+    // of bond slot s of bj
+    uint16_t *bks;
+    uint8_t *lk;
     int32_t halo_cap;  // LDS slots per block: the largest block's slot count (a multiple of 16)
     int32_t halo_pad;
     int32_t *case_of_blk;
@@ -180,6 +188,110 @@ struct Dev {
 };
 
 PDX_HD int64_t slot0(const Dev &d, int i) { return d.sl_ptr[i >> 6] + (i & 63); }
+
+// ---------------------------------------------------------------------------
+// lattice stencil.  A voxel specimen puts every particle on a cubic lattice
+// of spacing h; at horizons in (3 h, sqrt(10) h] (the reference's 3.015 h,
+// pd_adapter.py:86) a particle's bonds are the lattice offsets o with
+// 0 < |o|^2 <= 9 that land on another particle: 122 of them.  With the
+// offset known the bond vector is xi = h o exactly, |xi| = h sqrt(|o|^2) and
+// the influence weight w = 1 - |xi| / delta depends on |o|^2 alone, so the
+// lattice kernels take xi, |xi| and w as constants of the stencil entry
+// instead of arithmetic on positions.  Entries are ordered by which
+// components are non-zero (x, y, z, xy, xz, yz, xyz), each class in
+// lexicographic order of o; idx maps o + 3 (7 x 7 x 7) to the entry or -1.
+// The stencil is symmetric (-o is an entry whenever o is), so
+// sum_k w(o_k) o_k = 0 over the whole stencil.
+// ---------------------------------------------------------------------------
+#define PDX_NST 122
+
+struct LatStencil {
+    int8_t o[PDX_NST][3];
+    int8_t idx[343];
+};
+
+constexpr LatStencil lat_make()
+{
+    LatStencil s{};
+    for (int t = 0; t < 343; ++t) s.idx[t] = -1;
+    const int order[7] = {1, 2, 4, 3, 5, 6, 7};
+    int k = 0;
+    for (int pi = 0; pi < 7; ++pi)
+        for (int a = -3; a <= 3; ++a)
+            for (int b = -3; b <= 3; ++b)
+                for (int c = -3; c <= 3; ++c) {
+                    const int n = a * a + b * b + c * c;
+                    const int p = (a != 0 ? 1 : 0) | (b != 0 ? 2 : 0) | (c != 0 ? 4 : 0);
+                    if (n < 1 || n > 9 || p != order[pi]) continue;
+                    s.o[k][0] = (int8_t)a;
+                    s.o[k][1] = (int8_t)b;
+                    s.o[k][2] = (int8_t)c;
+                    s.idx[(a + 3) * 49 + (b + 3) * 7 + (c + 3)] = (int8_t)k;
+                    ++k;
+                }
+    return s;
+}
+
+static constexpr LatStencil kLat = lat_make();
+
+// per-context constants of the lattice kernels, indexed by n = |o|^2 (1..9),
+// w0(n) = 1 - sqrt(n) h / delta
+struct LatC {
+    double w1[10], w2[10], w3[10];  // w0, 2 w0, 3 w0 (FP64): the F-pass's weights of o_c = 1, 2, 3
+    float c1[10], c2[10], c3[10];   // h w0, 2 h w0, 3 h w0: the force pass's correspondence weights
+    float wl[10];                   // w0 / sqrt(n): the stabilisation term's weight
+    double inv_h;                   // 1 / h
+    double fscale;                  // V h: the F-pass sum -> sum_j w V_j du (x) xi
+    double V;                       // the particles' common volume
+};
+
+PDX_HD int64_t lat_at(int i, int k) { return ((int64_t)(i >> 6) * PDX_NST + k) * PDX_SLICE + (i & 63); }
+
+// lattice tables of row i: lk of each bond slot and the row's PDX_NST bks
+// entries.  Returns 0, or bit 1 when a bond is not h o for a stencil offset o
+// within tol h per component, bit 2 when two bonds share an offset, bit 4
+// when an active row's volume differs from V.  own is the row's LDS slot.
+PDX_HD unsigned lat_row_one(const Dev &d, int i, double h, double tol, double V, uint16_t own)
+{
+    const KRec kr = d.KR[i];
+    const bool act = (kr.flags & PDX_FL_ACTIVE) != 0;
+    for (int k = 0; k < PDX_NST; ++k) d.bks[lat_at(i, k)] = own;
+    unsigned viol = 0;
+    if (act && d.V64[i] != V) viol |= 4u;
+    const XRec pi = d.xu[0][i];
+    const int64_t base = slot0(d, i);
+    uint64_t seen0 = 0, seen1 = 0;
+    const double ih = 1.0 / h;
+    for (int s = 0; s < (int)kr.nnb; ++s) {
+        const int64_t a = base + (int64_t)s * PDX_SLICE;
+        const uint32_t e = d.bj[a];
+        const int j = (int)(e & ~PDX_BROKEN);
+        const XRec pj = d.xu[0][j];
+        int o[3];
+        bool on = true;
+        for (int q = 0; q < 3; ++q) {
+            const double t = pj.X[q] - pi.X[q];
+            const double r = t * ih;
+            o[q] = (int)(r < 0.0 ? -floor(-r + 0.5) : floor(r + 0.5));
+            if (!(fabs(t - o[q] * h) <= tol * h) || o[q] < -3 || o[q] > 3) on = false;
+        }
+        const int k = on ? (int)kLat.idx[(o[0] + 3) * 49 + (o[1] + 3) * 7 + (o[2] + 3)] : -1;
+        if (k < 0) {
+            viol |= 1u;
+            d.lk[a] = 0xFF;
+            continue;
+        }
+        const uint64_t bit = 1ull << (k & 63);
+        uint64_t &seen = k < 64 ? seen0 : seen1;
+        if (seen & bit) viol |= 2u;
+        seen |= bit;
+        d.lk[a] = (uint8_t)k;
+        const uint16_t l = d.bl[a];
+        const bool live = act && !(e & PDX_BROKEN) && !(l & PDX_HBROKEN);
+        d.bks[lat_at(i, k)] = live ? (uint16_t)(l & ~PDX_HBROKEN) : own;
+    }
+    return viol;
+}
 
 // ---------------------------------------------------------------------------
 // small tensor algebra
@@ -1139,7 +1251,9 @@ PDX_HD unsigned damage_one(const Dev &d, int i, int ubuf, double s_c)
             d.bj[a] = e | PDX_BROKEN;
             if (d.bl) {
                 d.bl[a] |= PDX_HBROKEN;
-                d.bk[a] = d.hs[d.hl_ptr[i / PDX_BLOCK] + i % PDX_BLOCK];
+                const uint16_t own = d.hs[d.hl_ptr[i / PDX_BLOCK] + i % PDX_BLOCK];
+                d.bk[a] = own;
+                if (d.bks) d.bks[lat_at(i, d.lk[a])] = own;
             }
             nb++;
         }

@@ -36,6 +36,10 @@ struct pdx_ctx {
     // where a halo does not fit)
     int tune_g = PDX_VARIANT_HALO | PDX_VARIANT_SPLIT4 | 4, tune_f = PDX_VARIANT_HALO | PDX_VARIANT_SPLIT4 | 1;
     int64_t lds_dyn_max = 0;  // dynamic LDS bytes a block may use on this device
+    // the lattice kernels' constants; have_lat: d.bks / d.lk hold this
+    // context's lattice tables (PDX_VARIANT_LATTICE runs)
+    LatC lat{};
+    bool have_lat = false;
     Dev d{};
     // host mirror of slice pointers
     std::vector<int64_t> h_slptr;
@@ -77,6 +81,10 @@ int pdx_launch_case_reduce(pdx_ctx *c, int ubuf, int upd, bool solve_mode);
 int pdx_build_m64(pdx_ctx *c);
 int pdx_launch_kd_move(pdx_ctx *c, int ubuf_in, bool solve_mode, double dt, bool apply_reset);
 int pdx_launch_halo_bits(pdx_ctx *c);
+// lattice tables: the smallest squared bond length (as FP64 bits, atomicMin into
+// d_out) and every row's lk / bks entries (violations ORed into d_flag)
+int pdx_launch_lat_minlen(pdx_ctx *c, unsigned long long *d_out);
+int pdx_launch_lat_rows(pdx_ctx *c, double h, double tol, double V, unsigned *d_flag);
 // device builder of the halo tables: mode 0 counts each block's halo into d_nh
 // (-1: too many neighbours for the device builder), mode 1 writes the tables
 // (hl / hs at d_ptr[b], bl, bk, anchor)

@@ -16,6 +16,8 @@
 // adaptive-DR mass and its spectral scaling, dt estimate, damage.
 #include "pdx_internal.h"
 
+#include <utility>
+
 // blocks are dispatched round-robin over the 8 XCDs; give every XCD one
 // contiguous range of Morton blocks so the neighbour halo of a block mostly
 // hits the same L2
@@ -778,6 +780,390 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
 }
 
 // ---------------------------------------------------------------------------
+// lattice kernels (3-D NOSB on the 122-site stencil, one particle volume V).
+// The same halo staging and split rows as k_fpass_h / k_force_h, but a row
+// walks the PDX_NST stencil entries of bks instead of its bond list, and
+// every entry k is code of its own, compiled with o_k: the bond vector, its
+// length and the influence weight are constants, the components of o_k that
+// are zero cost nothing, and the per-context weights come from the LatC
+// kernel argument (scalar registers).  Part p of S takes the entries
+// k = p, p + S, ... (each part gets a share of every class).  An entry
+// without a live bond reads the row's own slot: its F-pass term u_i (x) w o
+// cancels against the opposite entry's over the symmetric stencil, and its
+// force term is exactly zero (u_j - u_i = 0, A_j - A_i = 0).
+// ---------------------------------------------------------------------------
+template <int S, int P>
+constexpr int lat_count() { return (PDX_NST - P + S - 1) / S; }
+
+// |o| for |o|^2 = n (the stencil's lengths in lattice units)
+constexpr float lat_len(int n)
+{
+    return n == 1 ? 1.0f : n == 2 ? 1.41421356f : n == 3 ? 1.73205081f : n == 4 ? 2.0f : n == 5 ? 2.23606798f
+         : n == 6 ? 2.44948974f : n == 8 ? 2.82842712f : 3.0f;
+}
+
+// stage u of block lb's halo for the lattice F-pass: lanes 2 e and 2 e + 1 of
+// a wave load quarters 2 (u_x, u_y) and 3 (u_z) of entry 32 g + e into the
+// arrays u01 (16 B per slot) and u2 (8 B per slot)
+template <int NT>
+__device__ __forceinline__ void lat_stage_f(const Dev &d, int ubuf, int lb, pdx_q4 *lds)
+{
+    constexpr int NW = NT / 64, U = 4;
+    const int cap = d.halo_cap;
+    const int h0 = d.hl_ptr[lb], nh = d.hl_ptr[lb + 1] - h0;
+    const pdx_q4 *__restrict__ Xq = reinterpret_cast<const pdx_q4 *>(d.xu[ubuf]);
+    pdx_q4 *u01 = lds;
+    double *u2 = reinterpret_cast<double *>(lds + cap);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int half = lane & 1, sub = lane >> 1;
+    for (int g0 = wv * 32; g0 < nh; g0 += U * NW * 32) {
+        int j[U], sl[U];
+        bool ok[U];
+        pdx_q4 r[U];
+#pragma unroll
+        for (int v = 0; v < U; ++v) {
+            const int e = g0 + v * NW * 32 + sub;
+            ok[v] = e < nh;
+            const uint32_t he = (uint32_t)(h0 + (ok[v] ? e : 0));
+            j[v] = at32<int32_t>(d.hl, he * 4u);
+            sl[v] = ok[v] ? (int)at32<uint16_t>(d.hs, he * 2u) : 0;
+        }
+#pragma unroll
+        for (int v = 0; v < U; ++v) r[v] = at32<pdx_q4>(Xq, (uint32_t)j[v] * 64u + (uint32_t)(2 + half) * 16u);
+#pragma unroll
+        for (int v = 0; v < U; ++v) {
+            if (!ok[v]) continue;
+            if (half == 0) u01[sl[v]] = r[v];
+            else u2[sl[v]] = __builtin_bit_cast(double, r[v].xy);
+        }
+    }
+    __syncthreads();
+}
+
+// F-pass term of stencil entry K: S[r][c] += u_j[r] w0 o_c (FP64)
+template <int OC, int N, int C>
+__device__ __forceinline__ void lat_facc(const double (&uj)[3], const LatC &lc, double (&S)[9])
+{
+    if constexpr (OC != 0) {
+        constexpr int m = OC < 0 ? -OC : OC;
+        const double w = m == 1 ? lc.w1[N] : (m == 2 ? lc.w2[N] : lc.w3[N]);
+        const double ws = OC < 0 ? -w : w;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) S[r * 3 + C] = fma(uj[r], ws, S[r * 3 + C]);
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void lat_fbond(const uint16_t *__restrict__ bp, const pdx_q4 *u01, const double *u2,
+                                          const LatC &lc, double (&S)[9])
+{
+    constexpr int o0 = kLat.o[K][0], o1 = kLat.o[K][1], o2 = kLat.o[K][2];
+    constexpr int n = o0 * o0 + o1 * o1 + o2 * o2;
+    const uint32_t e = bp[K * PDX_SLICE];
+    const PdxD2 a = __builtin_bit_cast(PdxD2, u01[e]);
+    const double uj[3] = {a.a, a.b, u2[e]};
+    lat_facc<o0, n, 0>(uj, lc, S);
+    lat_facc<o1, n, 1>(uj, lc, S);
+    lat_facc<o2, n, 2>(uj, lc, S);
+}
+
+template <int S, int P, size_t... I>
+__device__ __forceinline__ void lat_fpart(const uint16_t *__restrict__ bp, const pdx_q4 *u01, const double *u2,
+                                          const LatC &lc, double (&Sd)[9], std::index_sequence<I...>)
+{
+    (lat_fbond<P + (int)I * S>(bp, u01, u2, lc, Sd), ...);
+}
+
+template <int S, int P>
+__device__ __forceinline__ void lat_fpart_p(const uint16_t *bp, const pdx_q4 *u01, const double *u2, const LatC &lc,
+                                            double (&Sd)[9])
+{
+    lat_fpart<S, P>(bp, u01, u2, lc, Sd, std::make_index_sequence<lat_count<S, P>()>{});
+}
+
+// RED as in k_fpass_h
+template <int MODEL, int S, bool RED = false>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_l(Dev d, LatC lc, int ubuf, int solve_mode, int red_ubuf)
+{
+    extern __shared__ pdx_q4 hlds[];
+    if constexpr (RED) {
+        if ((int)blockIdx.x >= d.nblk) {
+            case_reduce_block<1, PDX_BLOCK * S>(d, (int)blockIdx.x - d.nblk, red_ubuf, solve_mode);
+            return;
+        }
+    }
+    const int lb = logical_block_n(d.nblk);
+    if (solve_mode &&
+        __hip_atomic_load(&d.sc[d.case_of_blk[lb]].converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        return;
+    if (lb * PDX_BLOCK >= d.ni) return;  // grid padding blocks (whole block)
+    lat_stage_f<PDX_BLOCK * S>(d, ubuf, lb, hlds);
+    const int cap = d.halo_cap;
+    const pdx_q4 *u01 = hlds;
+    const double *u2 = reinterpret_cast<const double *>(hlds + cap);
+    const int t = threadIdx.x % PDX_BLOCK, part = threadIdx.x / PDX_BLOCK;
+    const int i = lb * PDX_BLOCK + t;
+    const uint16_t *__restrict__ bp = d.bks + lat_at(i, 0);
+    double Sd[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) Sd[q] = 0.0;
+    if (part == 0) lat_fpart_p<S, 0>(bp, u01, u2, lc, Sd);
+    if constexpr (S > 1) {
+        if (part == 1) lat_fpart_p<S, 1>(bp, u01, u2, lc, Sd);
+    }
+    if constexpr (S > 2) {
+        if (part == 2) lat_fpart_p<S, 2>(bp, u01, u2, lc, Sd);
+        if (part == 3) lat_fpart_p<S, 3>(bp, u01, u2, lc, Sd);
+    }
+    split_sum<9, S>(hlds, Sd);
+    if (part != 0) return;
+    const KRec kr = d.KR[i];
+    if (!(kr.flags & PDX_FL_ACTIVE)) return;
+    // sum_j w V_j (u_j - u_i) (x) xi = V h sum_k u_j(k) (x) w0 o_k: the own
+    // slot's u_i (x) w0 o_k of the entries without a live bond and the live
+    // entries' u_i (x) w0 o_k add up to u_i (x) sum_k w0 o_k = 0
+    double Hd[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) Hd[q] = Sd[q] * lc.fscale;
+    fpass_finish<3, MODEL, false, false>(d, i, kr, Hd);
+}
+
+// stage block lb's halo for the lattice force pass, one 80-byte LDS record
+// per slot: u / h in FP64 (x, y at 0; z at 16), A_zz at 24, c' = c G_s / 2
+// at 28, A(0..3) at 32, A(4..7) at 48.  Four lanes per entry as in
+// halo_stage: lane q loads quarter q of the XRec and quarter min(q, 2) of the
+// ARec; lane 3 takes A_zz from lane 2 and c' from lane 1 by DPP.
+template <int NT>
+__device__ __forceinline__ void lat_stage_g(const Dev &d, int ubuf, int lb, pdx_q4 *lds, double ih)
+{
+    constexpr int NW = NT / 64, U = 2;
+    const int h0 = d.hl_ptr[lb], nh = d.hl_ptr[lb + 1] - h0;
+    const pdx_q4 *__restrict__ Xq = reinterpret_cast<const pdx_q4 *>(d.xu[ubuf]);
+    const pdx_q4 *__restrict__ Aq = reinterpret_cast<const pdx_q4 *>(d.AR);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int q = lane & 3, sub = lane >> 2;
+    const int qa = q < 3 ? q : 2;
+    const float csc = (float)(0.5 * d.G_s);
+    char *hb = reinterpret_cast<char *>(lds);
+    for (int g0 = wv * 16; g0 < nh; g0 += U * NW * 16) {
+        int j[U], sl[U];
+        bool ok[U];
+        pdx_q4 r[U], a[U];
+#pragma unroll
+        for (int v = 0; v < U; ++v) {
+            const int e = g0 + v * NW * 16 + sub;
+            ok[v] = e < nh;
+            const uint32_t he = (uint32_t)(h0 + (ok[v] ? e : 0));
+            j[v] = at32<int32_t>(d.hl, he * 4u);
+            sl[v] = ok[v] ? (int)at32<uint16_t>(d.hs, he * 2u) : 0;
+        }
+#pragma unroll
+        for (int v = 0; v < U; ++v) {
+            r[v] = at32<pdx_q4>(Xq, (uint32_t)j[v] * 64u + (uint32_t)q * 16u);
+            a[v] = at32<pdx_q4>(Aq, (uint32_t)j[v] * 48u + (uint32_t)qa * 16u);
+        }
+#pragma unroll
+        for (int v = 0; v < U; ++v) {
+            const float azz = dpp_quad<0, 1, 2, 2>(a[v].x);      // lane 3: lane 2's A_zz
+            const float cc = dpp_quad<0, 1, 2, 1>(r[v].w * csc);  // lane 3: lane 1's c'
+            const PdxD2 x = __builtin_bit_cast(PdxD2, r[v]);
+            pdx_q4 out;
+            int off;
+            if (q == 0) {
+                out = a[v];
+                off = 32;
+            } else if (q == 1) {
+                out = a[v];
+                off = 48;
+            } else if (q == 2) {
+                out = __builtin_bit_cast(pdx_q4, PdxD2{x.a * ih, x.b * ih});
+                off = 0;
+            } else {
+                const pdx_q2 z = __builtin_bit_cast(pdx_q2, x.a * ih);
+                out = pdx_q4{z.x, z.y, azz, cc};
+                off = 16;
+            }
+            if (ok[v]) *reinterpret_cast<pdx_q4 *>(hb + sl[v] * 80 + off) = out;
+        }
+    }
+    __syncthreads();
+}
+
+// force term of stencil entry K into the FP32 group sum fg:
+// (A_j - A_i) h w0 o + w0 / |o| (c'_i + c'_j) dn / ((|eta| + |o|) |eta|) eta,
+// in lattice units (eta = o + (u_j - u_i) / h, dn = |eta|^2 - |o|^2); the
+// caller multiplies the particle's sum by V.  |eta|^2 is floored at 1e-12
+// (a bond squeezed to 1e-6 h has an undefined direction; the halo kernels
+// drop its stabilisation term instead).
+template <int K>
+__device__ __forceinline__ void lat_gbond(const uint16_t *__restrict__ bp, const char *hb, const LatC &lc,
+                                          const double (&ui)[3], const float (&Ai)[9], float ci, float (&fg)[3])
+{
+    constexpr int o[3] = {kLat.o[K][0], kLat.o[K][1], kLat.o[K][2]};
+    constexpr int n = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+    constexpr float L = lat_len(n);
+    const unsigned e = bp[K * PDX_SLICE];
+    const char *rp = hb + __umul24(e, 80u);
+    const pdx_q4 f0 = *reinterpret_cast<const pdx_q4 *>(rp);
+    const pdx_q4 f1 = *reinterpret_cast<const pdx_q4 *>(rp + 16);
+    const pdx_q4 f2 = *reinterpret_cast<const pdx_q4 *>(rp + 32);
+    const pdx_q4 f3 = *reinterpret_cast<const pdx_q4 *>(rp + 48);
+    const PdxD2 u01 = __builtin_bit_cast(PdxD2, f0);
+    const double u2 = __builtin_bit_cast(double, f1.xy);
+    const float du[3] = {(float)(u01.a - ui[0]), (float)(u01.b - ui[1]), (float)(u2 - ui[2])};
+    float dn = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dn = fmaf(du[c], o[c] != 0 ? du[c] + (float)(2 * o[c]) : du[c], dn);
+    const float E2 = fmaxf(dn + (float)n, 1e-12f);
+    const float el = bond_sqrt(E2);
+    const float gam = dn * bond_rcp((el + L) * el);
+    const float wg = (ci + f1.w) * lc.wl[n] * gam;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) fg[r] = fmaf(wg, o[r] != 0 ? du[r] + (float)o[r] : du[r], fg[r]);
+    const float Aj[9] = {f2.x, f2.y, f2.z, f2.w, f3.x, f3.y, f3.z, f3.w, f1.z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (o[c] == 0) continue;
+        const int m = o[c] < 0 ? -o[c] : o[c];
+        const float cw = m == 1 ? lc.c1[n] : (m == 2 ? lc.c2[n] : lc.c3[n]);
+        const float cs = o[c] < 0 ? -cw : cw;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) fg[r] = fmaf(Aj[r * 3 + c] - Ai[r * 3 + c], cs, fg[r]);
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void lat_gbond_if(const uint16_t *bp, const char *hb, const LatC &lc, const double (&ui)[3],
+                                             const float (&Ai)[9], float ci, float (&fg)[3])
+{
+    if constexpr (K < PDX_NST) lat_gbond<K>(bp, hb, lc, ui, Ai, ci, fg);
+}
+
+// group g of part P: entries P + (4 g + m) S, m < 4, summed in FP32, the
+// group sum added to the FP64 particle sum
+template <int S, int P, int GI, size_t... M>
+__device__ __forceinline__ void lat_ggroup(const uint16_t *bp, const char *hb, const LatC &lc, const double (&ui)[3],
+                                           const float (&Ai)[9], float ci, double (&fa)[3], std::index_sequence<M...>)
+{
+    float fg[3] = {0.f, 0.f, 0.f};
+    (lat_gbond_if<P + (GI * 4 + (int)M) * S>(bp, hb, lc, ui, Ai, ci, fg), ...);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) fa[r] += (double)fg[r];
+}
+
+template <int S, int P, size_t... GI>
+__device__ __forceinline__ void lat_gpart(const uint16_t *bp, const char *hb, const LatC &lc, const double (&ui)[3],
+                                          const float (&Ai)[9], float ci, double (&fa)[3], std::index_sequence<GI...>)
+{
+    (lat_ggroup<S, P, (int)GI>(bp, hb, lc, ui, Ai, ci, fa, std::make_index_sequence<4>{}), ...);
+}
+
+template <int S, int P>
+__device__ __forceinline__ void lat_gpart_p(const uint16_t *bp, const char *hb, const LatC &lc, const double (&ui)[3],
+                                            const float (&Ai)[9], float ci, double (&fa)[3])
+{
+    lat_gpart<S, P>(bp, hb, lc, ui, Ai, ci, fa, std::make_index_sequence<(lat_count<S, P>() + 3) / 4>{});
+}
+
+template <int UPD, int S>
+__global__ __launch_bounds__(PDX_BLOCK * S) void k_force_l(Dev d, LatC lc, int ubuf, int solve_mode, double dt,
+                                                           double visc)
+{
+    extern __shared__ pdx_q4 hlds[];
+    const int lb = logical_block();
+    const int cs = d.case_of_blk[lb];
+    if (solve_mode && d.sc[cs].converged) return;
+    double red[3] = {0.0, 0.0, 0.0};
+    if (lb * PDX_BLOCK < d.ni) {  // grid padding blocks only join the reduction
+        lat_stage_g<PDX_BLOCK * S>(d, ubuf, lb, hlds, lc.inv_h);
+        const char *hb = reinterpret_cast<const char *>(hlds);
+        const int t = threadIdx.x % PDX_BLOCK, part = threadIdx.x / PDX_BLOCK;
+        const int i = lb * PDX_BLOCK + t;
+        const int so = d.hs[d.hl_ptr[lb] + t];
+        const char *rp = hb + so * 80;
+        const pdx_q4 f0 = *reinterpret_cast<const pdx_q4 *>(rp);
+        const pdx_q4 f1 = *reinterpret_cast<const pdx_q4 *>(rp + 16);
+        const pdx_q4 f2 = *reinterpret_cast<const pdx_q4 *>(rp + 32);
+        const pdx_q4 f3 = *reinterpret_cast<const pdx_q4 *>(rp + 48);
+        const PdxD2 u01 = __builtin_bit_cast(PdxD2, f0);
+        const double ui[3] = {u01.a, u01.b, __builtin_bit_cast(double, f1.xy)};
+        const float Ai[9] = {f2.x, f2.y, f2.z, f2.w, f3.x, f3.y, f3.z, f3.w, f1.z};
+        const float ci = f1.w;
+        const uint16_t *__restrict__ bp = d.bks + lat_at(i, 0);
+        double fa[3] = {0.0, 0.0, 0.0};
+        if (part == 0) lat_gpart_p<S, 0>(bp, hb, lc, ui, Ai, ci, fa);
+        if constexpr (S > 1) {
+            if (part == 1) lat_gpart_p<S, 1>(bp, hb, lc, ui, Ai, ci, fa);
+        }
+        if constexpr (S > 2) {
+            if (part == 2) lat_gpart_p<S, 2>(bp, hb, lc, ui, Ai, ci, fa);
+            if (part == 3) lat_gpart_p<S, 3>(bp, hb, lc, ui, Ai, ci, fa);
+        }
+        // part 0 loads the update's inputs while the other parts end
+        const KRec kr = d.KR[i];
+        const bool act = (kr.flags & PDX_FL_ACTIVE) != 0;
+        FinishIn fin{};
+        XRec pi{};
+        if (part == 0 && act) {
+            fin = finish_in<UPD>(d, i, kr);
+            const XRec &xr = d.xu[ubuf][i];
+            pi.u[0] = xr.u[0];
+            pi.u[1] = xr.u[1];
+            pi.u[2] = xr.u[2];
+        }
+        split_sum<3, S>(hlds, fa);
+        if (part == 0 && act) {
+            ARec ai{};
+#pragma unroll
+            for (int q = 0; q < 9; ++q) ai.A[q] = Ai[q];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) fa[r] *= lc.V;
+            force_finish<3, 0, UPD>(d, i, kr, pi, ai, ubuf, cs, dt, visc, fa, red, fin);
+        }
+    }
+    block_partials<3, PDX_BLOCK * S, PDX_BLOCK / 64>(d, lb, red);  // part 0's waves
+}
+
+// lattice tables of every row (lat_row_one); violations ORed into *flag
+__global__ __launch_bounds__(PDX_BLOCK) void k_lat_rows(Dev d, double h, double tol, double V, unsigned *flag)
+{
+    const int i = blockIdx.x * PDX_BLOCK + threadIdx.x;
+    if (i >= d.ni) return;
+    const uint16_t own = d.hs[d.hl_ptr[i / PDX_BLOCK] + i % PDX_BLOCK];
+    const unsigned v = lat_row_one(d, i, h, tol, V, own);
+    if (v) atomicOr(flag, v);
+}
+
+// smallest squared bond length of the context (FP64 bits: positive doubles
+// order as unsigned integers)
+__global__ __launch_bounds__(PDX_BLOCK) void k_lat_minlen(Dev d, unsigned long long *out)
+{
+    const int i = blockIdx.x * PDX_BLOCK + threadIdx.x;
+    double m = 1e300;
+    if (i < d.ni) {
+        const KRec kr = d.KR[i];
+        const XRec pi = d.xu[0][i];
+        const int64_t base = slot0(d, i);
+        for (int s = 0; s < (int)kr.nnb; ++s) {
+            const int j = (int)(d.bj[base + (int64_t)s * PDX_SLICE] & ~PDX_BROKEN);
+            const XRec pj = d.xu[0][j];
+            double l2 = 0.0;
+            for (int q = 0; q < 3; ++q) {
+                const double t = pj.X[q] - pi.X[q];
+                l2 += t * t;
+            }
+            m = l2 < m ? l2 : m;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double t = __shfl_xor(m, o, 64);
+        m = t < m ? t : m;
+    }
+    if ((threadIdx.x & 63) == 0 && m < 1e300) atomicMin(out, (unsigned long long)__double_as_longlong(m));
+}
+
+// ---------------------------------------------------------------------------
 // device builder of the halo tables: the tables of the host builder
 // (pdx_layout::halo_block), one workgroup of PDX_BLOCK threads per block of
 // rows.  The block's distinct out-of-block neighbours go into an LDS hash set
@@ -1005,6 +1391,13 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_halo_bits(Dev d)
         d.bl[s] = (uint16_t)(l | (br ? PDX_HBROKEN : 0u));
         d.bk[s] = (act && !br) ? l : own;
     }
+    if (!d.bks) return;
+    // the lattice slots follow the same rule, at each bond's stencil offset
+    for (int k = 0; k < PDX_NST; ++k) d.bks[lat_at(i, k)] = own;
+    for (int k = 0; k < width; ++k) {
+        const int64_t s = s0 + (int64_t)k * PDX_SLICE + (i & (PDX_SLICE - 1));
+        if (d.bl[s] != PDX_HNONE) d.bks[lat_at(i, d.lk[s])] = d.bk[s];
+    }
 }
 
 template <int D>
@@ -1214,10 +1607,44 @@ static void launch_fpass_g(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm,
     }
 }
 
+// the lattice F-pass with rows split over S waves; false when it cannot run
+// (no lattice tables, 2-D, a diagnostic or commit pass, or a halo the
+// device's LDS does not take)
+template <int D, int MODEL, int S>
+static bool launch_fpass_lat(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm, int red_ubuf)
+{
+    if constexpr (D != 3) {
+        return false;
+    } else {
+        if (diag || commit || !c->have_lat || !c->d.bks || c->form != PDX_FORM_NOSB) return false;
+        const int lds = pdx_halo_lds(c, 24, S, 9);
+        if (!lds) return false;
+        const dim3 bh(PDX_BLOCK * S);
+        if (red_ubuf >= 0)
+            hipLaunchKernelGGL((k_fpass_l<MODEL, S, true>), dim3(c->nblk + c->ncase), bh, lds, c->stream, c->d, c->lat,
+                               ubuf, sm, red_ubuf);
+        else
+            hipLaunchKernelGGL((k_fpass_l<MODEL, S>), dim3(c->nblk), bh, lds, c->stream, c->d, c->lat, ubuf, sm, -1);
+        return true;
+    }
+}
+
 template <int D, int MODEL>
 static void launch_fpass_t(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm, int red)
 {
     switch (c->tune_f) {
+    case PDX_VARIANT_LATTICE:
+        if (!launch_fpass_lat<D, MODEL, 1>(c, ubuf, diag, commit, sm, red))
+            launch_fpass_g<D, MODEL, PDX_VARIANT_HALO | 1>(c, ubuf, diag, commit, sm, red);
+        break;
+    case PDX_VARIANT_LATTICE | PDX_VARIANT_SPLIT2:
+        if (!launch_fpass_lat<D, MODEL, 2>(c, ubuf, diag, commit, sm, red))
+            launch_fpass_g<D, MODEL, PDX_VARIANT_HALO | PDX_VARIANT_SPLIT2 | 1>(c, ubuf, diag, commit, sm, red);
+        break;
+    case PDX_VARIANT_LATTICE | PDX_VARIANT_SPLIT4:
+        if (!launch_fpass_lat<D, MODEL, 4>(c, ubuf, diag, commit, sm, red))
+            launch_fpass_g<D, MODEL, PDX_VARIANT_HALO | PDX_VARIANT_SPLIT4 | 1>(c, ubuf, diag, commit, sm, red);
+        break;
 #define PDX_HALO_CASE(v) \
     case v: launch_fpass_g<D, MODEL, v>(c, ubuf, diag, commit, sm, red); break;
     PDX_HALO_VARIANTS(PDX_HALO_CASE)
@@ -1314,10 +1741,45 @@ static void launch_force_g(pdx_ctx *c, int ubuf, int upd, int sm, double dt, dou
     }
 }
 
+// the lattice force pass with rows split over S waves; false when it cannot
+// run (no lattice tables, 2-D, the bond form, or a halo the device's LDS does
+// not take)
+template <int D, int FORM, int S>
+static bool launch_force_lat(pdx_ctx *c, int ubuf, int upd, int sm, double dt, double visc)
+{
+    if constexpr (D != 3 || FORM != PDX_FORM_NOSB) {
+        return false;
+    } else {
+        if (!c->have_lat || !c->d.bks) return false;
+        const int lds = pdx_halo_lds(c, 80, S, 3);
+        if (!lds) return false;
+        const dim3 g(c->nblk), bh(PDX_BLOCK * S);
+        if (upd == 1)
+            hipLaunchKernelGGL((k_force_l<1, S>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc);
+        else if (upd == 2)
+            hipLaunchKernelGGL((k_force_l<2, S>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc);
+        else
+            hipLaunchKernelGGL((k_force_l<0, S>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc);
+        return true;
+    }
+}
+
 template <int D, int FORM>
 static void launch_force_t(pdx_ctx *c, int ubuf, int upd, int sm, double dt, double visc)
 {
     switch (c->tune_g) {
+    case PDX_VARIANT_LATTICE:
+        if (!launch_force_lat<D, FORM, 1>(c, ubuf, upd, sm, dt, visc))
+            launch_force_g<D, FORM, PDX_VARIANT_HALO | 4>(c, ubuf, upd, sm, dt, visc);
+        break;
+    case PDX_VARIANT_LATTICE | PDX_VARIANT_SPLIT2:
+        if (!launch_force_lat<D, FORM, 2>(c, ubuf, upd, sm, dt, visc))
+            launch_force_g<D, FORM, PDX_VARIANT_HALO | PDX_VARIANT_SPLIT2 | 4>(c, ubuf, upd, sm, dt, visc);
+        break;
+    case PDX_VARIANT_LATTICE | PDX_VARIANT_SPLIT4:
+        if (!launch_force_lat<D, FORM, 4>(c, ubuf, upd, sm, dt, visc))
+            launch_force_g<D, FORM, PDX_VARIANT_HALO | PDX_VARIANT_SPLIT4 | 4>(c, ubuf, upd, sm, dt, visc);
+        break;
 #define PDX_HALO_CASE(v) \
     case v: launch_force_g<D, FORM, v>(c, ubuf, upd, sm, dt, visc); break;
     PDX_HALO_VARIANTS(PDX_HALO_CASE)
@@ -1347,6 +1809,21 @@ int pdx_launch_force_pass(pdx_ctx *c, int ubuf, int upd, bool solve_mode, double
         if (c->form == PDX_FORM_BOND) launch_force_t<2, PDX_FORM_BOND>(c, ubuf, upd, sm, dt, visc);
         else launch_force_t<2, PDX_FORM_NOSB>(c, ubuf, upd, sm, dt, visc);
     }
+    PDX_LAUNCH_CHECK();
+    return PDX_OK;
+}
+
+int pdx_launch_lat_minlen(pdx_ctx *c, unsigned long long *d_out)
+{
+    hipLaunchKernelGGL(k_lat_minlen, dim3((unsigned)grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_out);
+    PDX_LAUNCH_CHECK();
+    return PDX_OK;
+}
+
+int pdx_launch_lat_rows(pdx_ctx *c, double h, double tol, double V, unsigned *d_flag)
+{
+    hipLaunchKernelGGL(k_lat_rows, dim3((unsigned)grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, h, tol, V,
+                       d_flag);
     PDX_LAUNCH_CHECK();
     return PDX_OK;
 }
