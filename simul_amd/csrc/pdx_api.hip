@@ -72,7 +72,7 @@ static void free_all(pdx_ctx *c)
     dfree(d.Fd); dfree(d.Pd); dfree(d.damage); dfree(d.init_nb);
     dfree(d.sl_ptr); dfree(d.bj); dfree(d.case_of_blk); dfree(d.case_blk0);
     dfree(d.hl_ptr); dfree(d.hl); dfree(d.hs); dfree(d.bl); dfree(d.bk); dfree(d.anchor); dfree(d.m64);
-    dfree(d.bks); dfree(d.lk);
+    dfree(d.bks); dfree(d.lk); dfree(d.fr);
     dfree(d.sc); dfree(d.part); dfree(d.grid); dfree(d.case_of_p); dfree(d.perm);
     dfree(d.cell_start); dfree(d.cell_idx); dfree(d.counts);
     if (c->gexec) hipGraphExecDestroy(c->gexec);
@@ -790,8 +790,9 @@ static int build_halo(pdx_ctx *c)
 {
     Dev &d = c->d;
     dfree(d.hl_ptr); dfree(d.hl); dfree(d.hs); dfree(d.bl); dfree(d.bk); dfree(d.anchor); dfree(d.m64);
-    dfree(d.bks); dfree(d.lk);
+    dfree(d.bks); dfree(d.lk); dfree(d.fr);
     c->have_lat = false;
+    c->fr_ubuf = -1;
     d.halo_cap = 0;
     c->have_m = false;
     invalidate_graph(c);
@@ -818,7 +819,9 @@ static int build_lattice(pdx_ctx *c)
     Dev &d = c->d;
     dfree(d.bks);
     dfree(d.lk);
+    dfree(d.fr);
     c->have_lat = false;
+    c->fr_ubuf = -1;
     invalidate_graph(c);
     if (c->dim != 3 || !d.hl_ptr || c->slots <= 0 || c->nb <= 0) return PDX_OK;
     HostMirror &m = mirror(c);
@@ -843,6 +846,7 @@ static int build_lattice(pdx_ctx *c)
     if (ok) {
         r = dalloc(c->stream, &d.bks, nbks);
         if (r == PDX_OK) r = dalloc(c->stream, &d.lk, (size_t)c->slots);
+        if (r == PDX_OK) r = dalloc(c->stream, &d.fr, (size_t)c->ni);
         unsigned *flag = reinterpret_cast<unsigned *>(dmin + 1);
         if (r == PDX_OK) r = pdx_launch_lat_rows(c, h, 1e-8, V, flag);
         unsigned viol = 0;
@@ -856,6 +860,7 @@ static int build_lattice(pdx_ctx *c)
     if (!ok) {
         dfree(d.bks);
         dfree(d.lk);
+        dfree(d.fr);
         return r;
     }
     LatC &L = c->lat;

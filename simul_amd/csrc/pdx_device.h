@@ -85,6 +85,15 @@ struct alignas(16) ARec {
     float pad[3];
 };
 
+// record of a particle for the lattice force pass's halo, 64 bytes, written
+// by the lattice F-pass: u / h (FP64), A = P K^-1 (FP32), c' = c G_s / 2
+struct alignas(64) FRec {
+    double u[3];
+    float A8;
+    float c;
+    float A[8];
+};
+
 struct alignas(16) J2Par {
     float sy0, hiso, hkin, pad;
 };
@@ -169,6 +178,7 @@ struct Dev {
     // of bond slot s of bj
     uint16_t *bks;
     uint8_t *lk;
+    FRec *fr;  // the lattice F-pass's records for the lattice force pass (with bks)
     int32_t halo_cap;  // LDS slots per block: the largest block's slot count (a multiple of 16)
     int32_t halo_pad;
     int32_t *case_of_blk;
@@ -554,8 +564,8 @@ PDX_HD uint32_t bond_entry(const uint32_t *bp, int k, int nnb)
     return k < nnb ? bp[(int64_t)k * PDX_SLICE] : PDX_BROKEN;
 }
 
-template <int D, int MODEL, bool DIAG, bool COMMIT>
-PDX_HD void fpass_finish(const Dev &d, int i, const KRec &kr, const double *Hd);
+template <int D, int MODEL, bool DIAG, bool COMMIT, bool STORE = true>
+PDX_HD ARec fpass_finish(const Dev &d, int i, const KRec &kr, const double *Hd);
 
 template <int D, int MODEL, bool DIAG, bool COMMIT, int G>
 PDX_HD void fpass_one(const Dev &d, int i, int ubuf)
@@ -592,9 +602,10 @@ PDX_HD void fpass_one(const Dev &d, int i, int ubuf)
 }
 
 // per-particle tensor algebra of the F-pass in FP64 with the FP64 inverse:
-// H = Hd K^-1, P(H), A = P K^-1; A is rounded to FP32 only when stored
-template <int D, int MODEL, bool DIAG, bool COMMIT>
-PDX_HD void fpass_finish(const Dev &d, int i, const KRec &kr, const double *Hd)
+// H = Hd K^-1, P(H), A = P K^-1; A is rounded to FP32 when stored (into
+// d.AR with STORE, else only returned)
+template <int D, int MODEL, bool DIAG, bool COMMIT, bool STORE>
+PDX_HD ARec fpass_finish(const Dev &d, int i, const KRec &kr, const double *Hd)
 {
     const double *Ki = d.Kinv64 + (int64_t)i * 9;
     const bool sing = (kr.flags & PDX_FL_SING) != 0;
@@ -634,13 +645,14 @@ PDX_HD void fpass_finish(const Dev &d, int i, const KRec &kr, const double *Hd)
         }
     for (int q = D * D; q < 9; ++q) ar.A[q] = 0.f;
     ar.pad[0] = ar.pad[1] = ar.pad[2] = 0.f;
-    d.AR[i] = ar;
+    if (STORE) d.AR[i] = ar;
     if (DIAG) {
         for (int q = 0; q < D * D; ++q) {
             d.Fd[(int64_t)i * 9 + q] = (float)H[q];  // F - I: FP32 keeps the gradient's own precision
             d.Pd[(int64_t)i * 9 + q] = (float)P[q];
         }
     }
+    return ar;
 }
 
 // ---------------------------------------------------------------------------

@@ -31,15 +31,20 @@ struct pdx_ctx {
     int64_t nb = 0, slots = 0;
     int ucur = 0;
     // hot-pass variants of the force pass (tune_g) and the F-pass (tune_f):
-    // bond group 1, 2, 4, PDX_VARIANT_COOP | 1, 2 or a PDX_HALO_VARIANTS code
-    // (default: the LDS-staged kernels, which fall back to PDX_VARIANT_COOP | 2
-    // where a halo does not fit)
-    int tune_g = PDX_VARIANT_HALO | PDX_VARIANT_SPLIT4 | 4, tune_f = PDX_VARIANT_HALO | PDX_VARIANT_SPLIT4 | 1;
+    // bond group 1, 2, 4, PDX_VARIANT_COOP | 1, 2, a PDX_HALO_VARIANTS code or
+    // a PDX_VARIANT_LATTICE code (default: the lattice kernels, which run the
+    // LDS-staged halo kernels of the same split where the context is not a
+    // lattice, and those the cooperative group-2 kernels where a halo does not
+    // fit)
+    int tune_g = PDX_VARIANT_LATTICE | PDX_VARIANT_SPLIT4, tune_f = PDX_VARIANT_LATTICE | PDX_VARIANT_SPLIT2;
     int64_t lds_dyn_max = 0;  // dynamic LDS bytes a block may use on this device
     // the lattice kernels' constants; have_lat: d.bks / d.lk hold this
     // context's lattice tables (PDX_VARIANT_LATTICE runs)
     LatC lat{};
     bool have_lat = false;
+    // buffer whose d.fr records the last F-pass (a lattice F-pass followed by
+    // a lattice force pass) wrote, or -1: the F-pass then wrote d.AR instead
+    int fr_ubuf = -1;
     Dev d{};
     // host mirror of slice pointers
     std::vector<int64_t> h_slptr;

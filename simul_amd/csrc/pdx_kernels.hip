@@ -881,8 +881,9 @@ __device__ __forceinline__ void lat_fpart_p(const uint16_t *bp, const pdx_q4 *u0
     lat_fpart<S, P>(bp, u01, u2, lc, Sd, std::make_index_sequence<lat_count<S, P>()>{});
 }
 
-// RED as in k_fpass_h
-template <int MODEL, int S, bool RED = false>
+// RED as in k_fpass_h.  FR: A goes into the lattice force pass's records
+// d.fr (with u / h and c') instead of d.AR.
+template <int MODEL, int S, bool RED = false, bool FR = false>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_l(Dev d, LatC lc, int ubuf, int solve_mode, int red_ubuf)
 {
     extern __shared__ pdx_q4 hlds[];
@@ -915,6 +916,15 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_l(Dev d, LatC lc, int u
         if (part == 2) lat_fpart_p<S, 2>(bp, u01, u2, lc, Sd);
         if (part == 3) lat_fpart_p<S, 3>(bp, u01, u2, lc, Sd);
     }
+    // the own u for the force pass's record, read before the split sums
+    // reuse the halo's LDS
+    PdxD2 uo{0.0, 0.0};
+    double uz = 0.0;
+    if (FR && part == 0) {
+        const int so = d.hs[d.hl_ptr[lb] + t];
+        uo = __builtin_bit_cast(PdxD2, u01[so]);
+        uz = u2[so];
+    }
     split_sum<9, S>(hlds, Sd);
     if (part != 0) return;
     const KRec kr = d.KR[i];
@@ -925,18 +935,31 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_l(Dev d, LatC lc, int u
     double Hd[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) Hd[q] = Sd[q] * lc.fscale;
-    fpass_finish<3, MODEL, false, false>(d, i, kr, Hd);
+    if constexpr (!FR) {
+        fpass_finish<3, MODEL, false, false>(d, i, kr, Hd);
+    } else {
+        const float cc = d.xu[ubuf][i].c * (float)(0.5 * d.G_s);
+        const ARec ar = fpass_finish<3, MODEL, false, false, false>(d, i, kr, Hd);
+        pdx_q4 *o = reinterpret_cast<pdx_q4 *>(d.fr + i);
+        const double ih = lc.inv_h;
+        o[0] = __builtin_bit_cast(pdx_q4, PdxD2{uo.a * ih, uo.b * ih});
+        const pdx_q2 z = __builtin_bit_cast(pdx_q2, uz * ih);
+        o[1] = pdx_q4{z.x, z.y, ar.A[8], cc};
+        o[2] = pdx_q4{ar.A[0], ar.A[1], ar.A[2], ar.A[3]};
+        o[3] = pdx_q4{ar.A[4], ar.A[5], ar.A[6], ar.A[7]};
+    }
 }
 
 // stage block lb's halo for the lattice force pass, one 80-byte LDS record
 // per slot: u / h in FP64 (x, y at 0; z at 16), A_zz at 24, c' = c G_s / 2
-// at 28, A(0..3) at 32, A(4..7) at 48.  Four lanes per entry as in
-// halo_stage: lane q loads quarter q of the XRec and quarter min(q, 2) of the
-// ARec; lane 3 takes A_zz from lane 2 and c' from lane 1 by DPP.
-template <int NT>
+// at 28, A(0..3) at 32, A(4..7) at 48 -- the FRec layout.  Four lanes per
+// entry as in halo_stage.  FR: lane q copies quarter q of the entry's FRec.
+// Else lane q loads quarter q of the XRec and quarter min(q, 2) of the ARec;
+// lane 3 takes A_zz from lane 2 and c' from lane 1 by DPP.
+template <int NT, bool FR>
 __device__ __forceinline__ void lat_stage_g(const Dev &d, int ubuf, int lb, pdx_q4 *lds, double ih)
 {
-    constexpr int NW = NT / 64, U = 2;
+    constexpr int NW = NT / 64, U = FR ? 4 : 2;
     const int h0 = d.hl_ptr[lb], nh = d.hl_ptr[lb + 1] - h0;
     const pdx_q4 *__restrict__ Xq = reinterpret_cast<const pdx_q4 *>(d.xu[ubuf]);
     const pdx_q4 *__restrict__ Aq = reinterpret_cast<const pdx_q4 *>(d.AR);
@@ -956,6 +979,15 @@ __device__ __forceinline__ void lat_stage_g(const Dev &d, int ubuf, int lb, pdx_
             const uint32_t he = (uint32_t)(h0 + (ok[v] ? e : 0));
             j[v] = at32<int32_t>(d.hl, he * 4u);
             sl[v] = ok[v] ? (int)at32<uint16_t>(d.hs, he * 2u) : 0;
+        }
+        if constexpr (FR) {
+            const pdx_q4 *__restrict__ Fq = reinterpret_cast<const pdx_q4 *>(d.fr);
+#pragma unroll
+            for (int v = 0; v < U; ++v) r[v] = at32<pdx_q4>(Fq, (uint32_t)j[v] * 64u + (uint32_t)q * 16u);
+#pragma unroll
+            for (int v = 0; v < U; ++v)
+                if (ok[v]) *reinterpret_cast<pdx_q4 *>(hb + sl[v] * 80 + q * 16) = r[v];
+            continue;
         }
 #pragma unroll
         for (int v = 0; v < U; ++v) {
@@ -1065,7 +1097,9 @@ __device__ __forceinline__ void lat_gpart_p(const uint16_t *bp, const char *hb, 
     lat_gpart<S, P>(bp, hb, lc, ui, Ai, ci, fa, std::make_index_sequence<(lat_count<S, P>() + 3) / 4>{});
 }
 
-template <int UPD, int S>
+// FR: the halo comes from the FRec records of the lattice F-pass that ran on
+// this buffer, else from the XRec and ARec records
+template <int UPD, int S, bool FR>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_l(Dev d, LatC lc, int ubuf, int solve_mode, double dt,
                                                            double visc)
 {
@@ -1075,7 +1109,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_l(Dev d, LatC lc, int u
     if (solve_mode && d.sc[cs].converged) return;
     double red[3] = {0.0, 0.0, 0.0};
     if (lb * PDX_BLOCK < d.ni) {  // grid padding blocks only join the reduction
-        lat_stage_g<PDX_BLOCK * S>(d, ubuf, lb, hlds, lc.inv_h);
+        lat_stage_g<PDX_BLOCK * S, FR>(d, ubuf, lb, hlds, lc.inv_h);
         const char *hb = reinterpret_cast<const char *>(hlds);
         const int t = threadIdx.x % PDX_BLOCK, part = threadIdx.x / PDX_BLOCK;
         const int i = lb * PDX_BLOCK + t;
@@ -1607,6 +1641,21 @@ static void launch_fpass_g(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm,
     }
 }
 
+// LDS bytes of the lattice force pass with split S, or 0 when it cannot run
+static int lat_force_lds(const pdx_ctx *c, int split)
+{
+    if (!c->have_lat || !c->d.bks || c->dim != 3 || c->form != PDX_FORM_NOSB) return 0;
+    return pdx_halo_lds(c, 80, split, 3);
+}
+
+// the force pass's variant is a lattice variant that runs
+static bool lat_force_runs(const pdx_ctx *c)
+{
+    if (!(c->tune_g & PDX_VARIANT_LATTICE)) return false;
+    const int split = (c->tune_g & PDX_VARIANT_SPLIT4) ? 4 : ((c->tune_g & PDX_VARIANT_SPLIT2) ? 2 : 1);
+    return lat_force_lds(c, split) > 0;
+}
+
 // the lattice F-pass with rows split over S waves; false when it cannot run
 // (no lattice tables, 2-D, a diagnostic or commit pass, or a halo the
 // device's LDS does not take)
@@ -1619,12 +1668,19 @@ static bool launch_fpass_lat(pdx_ctx *c, int ubuf, bool diag, bool commit, int s
         if (diag || commit || !c->have_lat || !c->d.bks || c->form != PDX_FORM_NOSB) return false;
         const int lds = pdx_halo_lds(c, 24, S, 9);
         if (!lds) return false;
-        const dim3 bh(PDX_BLOCK * S);
-        if (red_ubuf >= 0)
-            hipLaunchKernelGGL((k_fpass_l<MODEL, S, true>), dim3(c->nblk + c->ncase), bh, lds, c->stream, c->d, c->lat,
-                               ubuf, sm, red_ubuf);
+        const dim3 bh(PDX_BLOCK * S), g(c->nblk), gr(c->nblk + c->ncase);
+        // with a lattice force pass to follow, A goes into its records
+        const bool fr = lat_force_runs(c);
+        if (red_ubuf >= 0 && fr)
+            hipLaunchKernelGGL((k_fpass_l<MODEL, S, true, true>), gr, bh, lds, c->stream, c->d, c->lat, ubuf, sm,
+                               red_ubuf);
+        else if (red_ubuf >= 0)
+            hipLaunchKernelGGL((k_fpass_l<MODEL, S, true>), gr, bh, lds, c->stream, c->d, c->lat, ubuf, sm, red_ubuf);
+        else if (fr)
+            hipLaunchKernelGGL((k_fpass_l<MODEL, S, false, true>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, -1);
         else
-            hipLaunchKernelGGL((k_fpass_l<MODEL, S>), dim3(c->nblk), bh, lds, c->stream, c->d, c->lat, ubuf, sm, -1);
+            hipLaunchKernelGGL((k_fpass_l<MODEL, S>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, -1);
+        c->fr_ubuf = fr ? ubuf : -1;
         return true;
     }
 }
@@ -1659,6 +1715,7 @@ static void launch_fpass_t(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm,
 
 int pdx_launch_fpass(pdx_ctx *c, int ubuf, bool diag, bool commit, bool solve_mode, int red_ubuf)
 {
+    c->fr_ubuf = -1;  // set again by a lattice F-pass that writes the force pass's records
     const int sm = solve_mode ? 1 : 0;
     const int red = red_ubuf;
     if (c->dim == 3) {
@@ -1698,10 +1755,17 @@ static int build_m64_v(pdx_ctx *c)
 int pdx_build_m64(pdx_ctx *c)
 {
     c->have_m = false;
-    // only the halo F-pass reads m64
-    if (!c->d.m64 || !c->d.hl_ptr || c->form != PDX_FORM_NOSB || !(c->tune_f & PDX_VARIANT_HALO)) return PDX_OK;
+    // only the halo F-pass reads m64; a lattice variant runs the halo F-pass
+    // of its split (group 1) where the lattice F-pass cannot run
+    int tf = c->tune_f;
+    if (tf & PDX_VARIANT_LATTICE) {
+        const int split = (tf & PDX_VARIANT_SPLIT4) ? 4 : ((tf & PDX_VARIANT_SPLIT2) ? 2 : 1);
+        if (c->dim == 3 && c->have_lat && c->d.bks && pdx_halo_lds(c, 24, split, 9)) return PDX_OK;
+        tf = PDX_VARIANT_HALO | (tf & (PDX_VARIANT_SPLIT2 | PDX_VARIANT_SPLIT4)) | 1;
+    }
+    if (!c->d.m64 || !c->d.hl_ptr || c->form != PDX_FORM_NOSB || !(tf & PDX_VARIANT_HALO)) return PDX_OK;
 #define PDX_M64_CASE(v)                                                          \
-    if (c->tune_f == (v)) return c->dim == 3 ? build_m64_v<3, (v)>(c) : build_m64_v<2, (v)>(c);
+    if (tf == (v)) return c->dim == 3 ? build_m64_v<3, (v)>(c) : build_m64_v<2, (v)>(c);
     PDX_HALO_VARIANTS(PDX_M64_CASE)
 #undef PDX_M64_CASE
     return PDX_OK;
@@ -1750,16 +1814,25 @@ static bool launch_force_lat(pdx_ctx *c, int ubuf, int upd, int sm, double dt, d
     if constexpr (D != 3 || FORM != PDX_FORM_NOSB) {
         return false;
     } else {
-        if (!c->have_lat || !c->d.bks) return false;
-        const int lds = pdx_halo_lds(c, 80, S, 3);
+        const int lds = lat_force_lds(c, S);
         if (!lds) return false;
         const dim3 g(c->nblk), bh(PDX_BLOCK * S);
-        if (upd == 1)
-            hipLaunchKernelGGL((k_force_l<1, S>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc);
-        else if (upd == 2)
-            hipLaunchKernelGGL((k_force_l<2, S>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc);
-        else
-            hipLaunchKernelGGL((k_force_l<0, S>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc);
+        // the records of the lattice F-pass that ran last, on this buffer
+        if (c->fr_ubuf == ubuf) {
+            if (upd == 1)
+                hipLaunchKernelGGL((k_force_l<1, S, true>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc);
+            else if (upd == 2)
+                hipLaunchKernelGGL((k_force_l<2, S, true>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc);
+            else
+                hipLaunchKernelGGL((k_force_l<0, S, true>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc);
+        } else {
+            if (upd == 1)
+                hipLaunchKernelGGL((k_force_l<1, S, false>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc);
+            else if (upd == 2)
+                hipLaunchKernelGGL((k_force_l<2, S, false>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc);
+            else
+                hipLaunchKernelGGL((k_force_l<0, S, false>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc);
+        }
         return true;
     }
 }

@@ -43,6 +43,18 @@ def _lattice_batch(model="neo_hookean", dx=0.5, seed=1, **kw):
     return X, body
 
 
+def _pass_forces(body, u):
+    """Internal forces at u through the solver's own F-pass and force pass (a
+    kinetic-DR step from rest leaves u in place and stores f), where
+    compute_forces runs the diagnostic F-pass."""
+    body.u = u
+    body.set("v", np.zeros_like(u))
+    body.set("a", np.zeros_like(u))
+    body.kd_step(1e-3, 0.0, 0.0, 1)
+    assert np.array_equal(body.u, u)
+    return body.get("f")
+
+
 def test_lattice_qualification(gpu):
     """An exact lattice at 3.015 h qualifies with its spacing; jitter, two
     volumes, a horizon at or below 3 h, a later change of volumes and 2-D do
@@ -85,6 +97,8 @@ def test_lattice_forces_match_oracle(gpu, model):
         body.set_variants(v, v)
         f = body.compute_forces()
         assert rel_l2(f, prob.f) < 5e-5, (v, rel_l2(f, prob.f))
+        f = _pass_forces(body, u)
+        assert rel_l2(f, prob.f) < 5e-5, (v, rel_l2(f, prob.f))
 
 
 def test_lattice_forces_match_halo(gpu):
@@ -107,14 +121,20 @@ def test_lattice_forces_match_halo(gpu):
         body.u = u1
         body.set_variants(HALO_REF, HALO_REF)
         f0 = body.compute_forces().copy()
+        g0 = _pass_forces(body, u1).copy()
+        assert rel_l2(g0, f0) < 2e-6, step
         for v in LAT:
             body.set_variants(v, v)
             f = body.compute_forces()
             assert rel_l2(f, f0) < 2e-6, (step, v, rel_l2(f, f0))
+            # the solver's passes: lattice F-pass writing the force pass's
+            # records, and each lattice pass beside a halo pass
+            assert rel_l2(_pass_forces(body, u1), g0) < 2e-6, (step, v, "both")
             body.set_variants(v, HALO_REF)
-            assert rel_l2(body.compute_forces(), f0) < 2e-6, (step, v, "F-pass")
+            assert rel_l2(_pass_forces(body, u1), g0) < 2e-6, (step, v, "F-pass")
             body.set_variants(HALO_REF, v)
-            assert rel_l2(body.compute_forces(), f0) < 2e-6, (step, v, "force")
+            assert rel_l2(_pass_forces(body, u1), g0) < 2e-6, (step, v, "force")
+            assert rel_l2(body.compute_forces(), f0) < 2e-6, (step, v, "force after the diagnostic F-pass")
 
 
 def test_lattice_j2_forces_match_oracle(gpu):
@@ -128,10 +148,10 @@ def test_lattice_j2_forces_match_oracle(gpu):
     prob.set_u(u)
     prob.compute_forces()
     body.set_variants(HALO_REF, HALO_REF)
-    fh = body.compute_forces().copy()
+    fh = _pass_forces(body, u).copy()
     for v in LAT:
         body.set_variants(v, v)
-        f = body.compute_forces()
+        f = _pass_forces(body, u)
         assert rel_l2(f, prob.f) < 5e-5, (v, rel_l2(f, prob.f))
         assert rel_l2(f, fh) < 2e-6, (v, rel_l2(f, fh))
 
