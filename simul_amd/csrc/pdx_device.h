@@ -209,9 +209,10 @@ PDX_HD int64_t slot0(const Dev &d, int i) { return d.sl_ptr[i >> 6] + (i & 63); 
 // lattice kernels take xi, |xi| and w as constants of the stencil entry
 // instead of arithmetic on positions.  Entries are ordered by which
 // components are non-zero (x, y, z, xy, xz, yz, xyz), each class in
-// lexicographic order of o; idx maps o + 3 (7 x 7 x 7) to the entry or -1.
-// The stencil is symmetric (-o is an entry whenever o is), so
-// sum_k w(o_k) o_k = 0 over the whole stencil.
+// lexicographic order of the offsets whose first non-zero component is
+// positive, each such o followed by -o: entries 2 m and 2 m + 1 are a pair
+// o, -o.  idx maps o + 3 (7 x 7 x 7) to the entry or -1.  The stencil is
+// symmetric, so sum_k w(o_k) o_k = 0 over the whole stencil.
 // ---------------------------------------------------------------------------
 #define PDX_NST 122
 
@@ -232,12 +233,15 @@ constexpr LatStencil lat_make()
                 for (int c = -3; c <= 3; ++c) {
                     const int n = a * a + b * b + c * c;
                     const int p = (a != 0 ? 1 : 0) | (b != 0 ? 2 : 0) | (c != 0 ? 4 : 0);
-                    if (n < 1 || n > 9 || p != order[pi]) continue;
-                    s.o[k][0] = (int8_t)a;
-                    s.o[k][1] = (int8_t)b;
-                    s.o[k][2] = (int8_t)c;
-                    s.idx[(a + 3) * 49 + (b + 3) * 7 + (c + 3)] = (int8_t)k;
-                    ++k;
+                    const int lead = a != 0 ? a : (b != 0 ? b : c);
+                    if (n < 1 || n > 9 || p != order[pi] || lead < 0) continue;
+                    for (int sg = 1; sg >= -1; sg -= 2) {
+                        s.o[k][0] = (int8_t)(sg * a);
+                        s.o[k][1] = (int8_t)(sg * b);
+                        s.o[k][2] = (int8_t)(sg * c);
+                        s.idx[(sg * a + 3) * 49 + (sg * b + 3) * 7 + (sg * c + 3)] = (int8_t)k;
+                        ++k;
+                    }
                 }
     return s;
 }

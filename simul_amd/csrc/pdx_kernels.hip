@@ -792,9 +792,6 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_h(Dev d, int ubuf, int 
 // cancels against the opposite entry's over the symmetric stencil, and its
 // force term is exactly zero (u_j - u_i = 0, A_j - A_i = 0).
 // ---------------------------------------------------------------------------
-template <int S, int P>
-constexpr int lat_count() { return (PDX_NST - P + S - 1) / S; }
-
 // |o| for |o|^2 = n (the stencil's lengths in lattice units)
 constexpr float lat_len(int n)
 {
@@ -840,45 +837,49 @@ __device__ __forceinline__ void lat_stage_f(const Dev &d, int ubuf, int lb, pdx_
     __syncthreads();
 }
 
-// F-pass term of stencil entry K: S[r][c] += u_j[r] w0 o_c (FP64)
+// F-pass terms of stencil pair M (entries 2 M: o, 2 M + 1: -o):
+// S[r][c] += (u_+ - u_-)[r] w0 o_c (FP64), the two entries' u_j (x) w0 (+-o)
 template <int OC, int N, int C>
-__device__ __forceinline__ void lat_facc(const double (&uj)[3], const LatC &lc, double (&S)[9])
+__device__ __forceinline__ void lat_facc(const double (&du)[3], const LatC &lc, double (&S)[9])
 {
     if constexpr (OC != 0) {
         constexpr int m = OC < 0 ? -OC : OC;
         const double w = m == 1 ? lc.w1[N] : (m == 2 ? lc.w2[N] : lc.w3[N]);
         const double ws = OC < 0 ? -w : w;
 #pragma unroll
-        for (int r = 0; r < 3; ++r) S[r * 3 + C] = fma(uj[r], ws, S[r * 3 + C]);
+        for (int r = 0; r < 3; ++r) S[r * 3 + C] = fma(du[r], ws, S[r * 3 + C]);
     }
 }
 
-template <int K>
-__device__ __forceinline__ void lat_fbond(const uint16_t *__restrict__ bp, const pdx_q4 *u01, const double *u2,
+template <int M>
+__device__ __forceinline__ void lat_fpair(const uint16_t *__restrict__ bp, const pdx_q4 *u01, const double *u2,
                                           const LatC &lc, double (&S)[9])
 {
+    constexpr int K = 2 * M;
     constexpr int o0 = kLat.o[K][0], o1 = kLat.o[K][1], o2 = kLat.o[K][2];
+    static_assert(kLat.o[K + 1][0] == -o0 && kLat.o[K + 1][1] == -o1 && kLat.o[K + 1][2] == -o2, "stencil pair");
     constexpr int n = o0 * o0 + o1 * o1 + o2 * o2;
-    const uint32_t e = bp[K * PDX_SLICE];
-    const PdxD2 a = __builtin_bit_cast(PdxD2, u01[e]);
-    const double uj[3] = {a.a, a.b, u2[e]};
-    lat_facc<o0, n, 0>(uj, lc, S);
-    lat_facc<o1, n, 1>(uj, lc, S);
-    lat_facc<o2, n, 2>(uj, lc, S);
+    const uint32_t ep = bp[K * PDX_SLICE], em = bp[(K + 1) * PDX_SLICE];
+    const PdxD2 a = __builtin_bit_cast(PdxD2, u01[ep]), b = __builtin_bit_cast(PdxD2, u01[em]);
+    const double du[3] = {a.a - b.a, a.b - b.b, u2[ep] - u2[em]};
+    lat_facc<o0, n, 0>(du, lc, S);
+    lat_facc<o1, n, 1>(du, lc, S);
+    lat_facc<o2, n, 2>(du, lc, S);
 }
 
 template <int S, int P, size_t... I>
 __device__ __forceinline__ void lat_fpart(const uint16_t *__restrict__ bp, const pdx_q4 *u01, const double *u2,
                                           const LatC &lc, double (&Sd)[9], std::index_sequence<I...>)
 {
-    (lat_fbond<P + (int)I * S>(bp, u01, u2, lc, Sd), ...);
+    (lat_fpair<P + (int)I * S>(bp, u01, u2, lc, Sd), ...);
 }
 
+// the pairs of part P of S: P, P + S, ...
 template <int S, int P>
 __device__ __forceinline__ void lat_fpart_p(const uint16_t *bp, const pdx_q4 *u01, const double *u2, const LatC &lc,
                                             double (&Sd)[9])
 {
-    lat_fpart<S, P>(bp, u01, u2, lc, Sd, std::make_index_sequence<lat_count<S, P>()>{});
+    lat_fpart<S, P>(bp, u01, u2, lc, Sd, std::make_index_sequence<(PDX_NST / 2 - P + S - 1) / S>{});
 }
 
 // RED as in k_fpass_h.  FR: A goes into the lattice force pass's records
@@ -931,7 +932,8 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_l(Dev d, LatC lc, int u
     if (!(kr.flags & PDX_FL_ACTIVE)) return;
     // sum_j w V_j (u_j - u_i) (x) xi = V h sum_k u_j(k) (x) w0 o_k: the own
     // slot's u_i (x) w0 o_k of the entries without a live bond and the live
-    // entries' u_i (x) w0 o_k add up to u_i (x) sum_k w0 o_k = 0
+    // entries' u_i (x) w0 o_k add up to u_i (x) sum_k w0 o_k = 0 (the sum runs
+    // over the pairs, as (u_+ - u_-) (x) w0 o)
     double Hd[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) Hd[q] = Sd[q] * lc.fscale;
@@ -1021,38 +1023,55 @@ __device__ __forceinline__ void lat_stage_g(const Dev &d, int ubuf, int lb, pdx_
     __syncthreads();
 }
 
-// force term of stencil entry K into the FP32 group sum fg:
-// (A_j - A_i) h w0 o + w0 / |o| (c'_i + c'_j) dn / ((|eta| + |o|) |eta|) eta,
-// in lattice units (eta = o + (u_j - u_i) / h, dn = |eta|^2 - |o|^2); the
-// caller multiplies the particle's sum by V.  |eta|^2 is floored at 1e-12
+// force terms of stencil pair M (entries 2 M: o and 2 M + 1: -o, the
+// neighbours at x_i + h o and x_i - h o) into the FP32 group sum fg.
+// Stabilisation, per entry, in lattice units (eta = +-o + (u_j - u_i) / h,
+// dn = |eta|^2 - |o|^2): w0 / |o| (c'_i + c'_j) dn / ((|eta| + |o|) |eta|) eta.
+// Correspondence: the two entries' h w0 (A_j - A_i) (+-o) add up to
+// h w0 (A_+ - A_-) o, A_i cancelling (still a difference of neighbouring
+// tensors, so its FP32 products are rounded relative to that difference).
+// The caller multiplies the particle's sum by V.  |eta|^2 is floored at 1e-12
 // (a bond squeezed to 1e-6 h has an undefined direction; the halo kernels
 // drop its stabilisation term instead).
-template <int K>
-__device__ __forceinline__ void lat_gbond(const uint16_t *__restrict__ bp, const char *hb, const LatC &lc,
-                                          const double (&ui)[3], const float (&Ai)[9], float ci, float (&fg)[3])
+template <int O0, int O1, int O2, int N>
+__device__ __forceinline__ void lat_stab(const pdx_q4 &f0, const pdx_q4 &f1, const LatC &lc, const double (&ui)[3],
+                                         float ci, float (&fg)[3])
 {
-    constexpr int o[3] = {kLat.o[K][0], kLat.o[K][1], kLat.o[K][2]};
-    constexpr int n = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
-    constexpr float L = lat_len(n);
-    const unsigned e = bp[K * PDX_SLICE];
-    const char *rp = hb + __umul24(e, 80u);
-    const pdx_q4 f0 = *reinterpret_cast<const pdx_q4 *>(rp);
-    const pdx_q4 f1 = *reinterpret_cast<const pdx_q4 *>(rp + 16);
-    const pdx_q4 f2 = *reinterpret_cast<const pdx_q4 *>(rp + 32);
-    const pdx_q4 f3 = *reinterpret_cast<const pdx_q4 *>(rp + 48);
+    constexpr int o[3] = {O0, O1, O2};
+    constexpr float L = lat_len(N);
     const PdxD2 u01 = __builtin_bit_cast(PdxD2, f0);
     const double u2 = __builtin_bit_cast(double, f1.xy);
     const float du[3] = {(float)(u01.a - ui[0]), (float)(u01.b - ui[1]), (float)(u2 - ui[2])};
     float dn = 0.f;
 #pragma unroll
     for (int c = 0; c < 3; ++c) dn = fmaf(du[c], o[c] != 0 ? du[c] + (float)(2 * o[c]) : du[c], dn);
-    const float E2 = fmaxf(dn + (float)n, 1e-12f);
+    const float E2 = fmaxf(dn + (float)N, 1e-12f);
     const float el = bond_sqrt(E2);
     const float gam = dn * bond_rcp((el + L) * el);
-    const float wg = (ci + f1.w) * lc.wl[n] * gam;
+    const float wg = (ci + f1.w) * lc.wl[N] * gam;
 #pragma unroll
     for (int r = 0; r < 3; ++r) fg[r] = fmaf(wg, o[r] != 0 ? du[r] + (float)o[r] : du[r], fg[r]);
-    const float Aj[9] = {f2.x, f2.y, f2.z, f2.w, f3.x, f3.y, f3.z, f3.w, f1.z};
+}
+
+template <int M>
+__device__ __forceinline__ void lat_gpair(const uint16_t *__restrict__ bp, const char *hb, const LatC &lc,
+                                          const double (&ui)[3], float ci, float (&fg)[3])
+{
+    constexpr int K = 2 * M;
+    constexpr int o0 = kLat.o[K][0], o1 = kLat.o[K][1], o2 = kLat.o[K][2];
+    static_assert(kLat.o[K + 1][0] == -o0 && kLat.o[K + 1][1] == -o1 && kLat.o[K + 1][2] == -o2, "stencil pair");
+    constexpr int o[3] = {o0, o1, o2};
+    constexpr int n = o0 * o0 + o1 * o1 + o2 * o2;
+    const unsigned ep = bp[K * PDX_SLICE], em = bp[(K + 1) * PDX_SLICE];
+    const char *rp = hb + __umul24(ep, 80u), *rm = hb + __umul24(em, 80u);
+    const pdx_q4 p0 = *reinterpret_cast<const pdx_q4 *>(rp), p1 = *reinterpret_cast<const pdx_q4 *>(rp + 16);
+    const pdx_q4 p2 = *reinterpret_cast<const pdx_q4 *>(rp + 32), p3 = *reinterpret_cast<const pdx_q4 *>(rp + 48);
+    const pdx_q4 m0 = *reinterpret_cast<const pdx_q4 *>(rm), m1 = *reinterpret_cast<const pdx_q4 *>(rm + 16);
+    const pdx_q4 m2 = *reinterpret_cast<const pdx_q4 *>(rm + 32), m3 = *reinterpret_cast<const pdx_q4 *>(rm + 48);
+    lat_stab<o0, o1, o2, n>(p0, p1, lc, ui, ci, fg);
+    lat_stab<-o0, -o1, -o2, n>(m0, m1, lc, ui, ci, fg);
+    const float Ap[9] = {p2.x, p2.y, p2.z, p2.w, p3.x, p3.y, p3.z, p3.w, p1.z};
+    const float Am[9] = {m2.x, m2.y, m2.z, m2.w, m3.x, m3.y, m3.z, m3.w, m1.z};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         if (o[c] == 0) continue;
@@ -1060,45 +1079,45 @@ __device__ __forceinline__ void lat_gbond(const uint16_t *__restrict__ bp, const
         const float cw = m == 1 ? lc.c1[n] : (m == 2 ? lc.c2[n] : lc.c3[n]);
         const float cs = o[c] < 0 ? -cw : cw;
 #pragma unroll
-        for (int r = 0; r < 3; ++r) fg[r] = fmaf(Aj[r * 3 + c] - Ai[r * 3 + c], cs, fg[r]);
+        for (int r = 0; r < 3; ++r) fg[r] = fmaf(Ap[r * 3 + c] - Am[r * 3 + c], cs, fg[r]);
     }
 }
 
-template <int K>
-__device__ __forceinline__ void lat_gbond_if(const uint16_t *bp, const char *hb, const LatC &lc, const double (&ui)[3],
-                                             const float (&Ai)[9], float ci, float (&fg)[3])
+template <int M>
+__device__ __forceinline__ void lat_gpair_if(const uint16_t *bp, const char *hb, const LatC &lc, const double (&ui)[3],
+                                             float ci, float (&fg)[3])
 {
-    if constexpr (K < PDX_NST) lat_gbond<K>(bp, hb, lc, ui, Ai, ci, fg);
+    if constexpr (M < PDX_NST / 2) lat_gpair<M>(bp, hb, lc, ui, ci, fg);
 }
 
-// group g of part P: entries P + (4 g + m) S, m < 4, summed in FP32, the
-// group sum added to the FP64 particle sum
-template <int S, int P, int GI, size_t... M>
+// group g of part P: pairs P + (2 g + m) S, m < 2 (four bonds), summed in
+// FP32, the group sum added to the FP64 particle sum
+template <int S, int P, int GI, size_t... Q>
 __device__ __forceinline__ void lat_ggroup(const uint16_t *bp, const char *hb, const LatC &lc, const double (&ui)[3],
-                                           const float (&Ai)[9], float ci, double (&fa)[3], std::index_sequence<M...>)
+                                           float ci, double (&fa)[3], std::index_sequence<Q...>)
 {
     float fg[3] = {0.f, 0.f, 0.f};
-    (lat_gbond_if<P + (GI * 4 + (int)M) * S>(bp, hb, lc, ui, Ai, ci, fg), ...);
+    (lat_gpair_if<P + (GI * 2 + (int)Q) * S>(bp, hb, lc, ui, ci, fg), ...);
 #pragma unroll
     for (int r = 0; r < 3; ++r) fa[r] += (double)fg[r];
 }
 
 template <int S, int P, size_t... GI>
 __device__ __forceinline__ void lat_gpart(const uint16_t *bp, const char *hb, const LatC &lc, const double (&ui)[3],
-                                          const float (&Ai)[9], float ci, double (&fa)[3], std::index_sequence<GI...>)
+                                          float ci, double (&fa)[3], std::index_sequence<GI...>)
 {
-    (lat_ggroup<S, P, (int)GI>(bp, hb, lc, ui, Ai, ci, fa, std::make_index_sequence<4>{}), ...);
+    (lat_ggroup<S, P, (int)GI>(bp, hb, lc, ui, ci, fa, std::make_index_sequence<2>{}), ...);
 }
 
+// the pairs of part P of S: P, P + S, ...
 template <int S, int P>
 __device__ __forceinline__ void lat_gpart_p(const uint16_t *bp, const char *hb, const LatC &lc, const double (&ui)[3],
-                                            const float (&Ai)[9], float ci, double (&fa)[3])
+                                            float ci, double (&fa)[3])
 {
-    lat_gpart<S, P>(bp, hb, lc, ui, Ai, ci, fa, std::make_index_sequence<(lat_count<S, P>() + 3) / 4>{});
+    constexpr int np = (PDX_NST / 2 - P + S - 1) / S;
+    lat_gpart<S, P>(bp, hb, lc, ui, ci, fa, std::make_index_sequence<(np + 1) / 2>{});
 }
 
-// FR: the halo comes from the FRec records of the lattice F-pass that ran on
-// this buffer, else from the XRec and ARec records
 template <int UPD, int S, bool FR>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_l(Dev d, LatC lc, int ubuf, int solve_mode, double dt,
                                                            double visc)
@@ -1125,13 +1144,13 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_l(Dev d, LatC lc, int u
         const float ci = f1.w;
         const uint16_t *__restrict__ bp = d.bks + lat_at(i, 0);
         double fa[3] = {0.0, 0.0, 0.0};
-        if (part == 0) lat_gpart_p<S, 0>(bp, hb, lc, ui, Ai, ci, fa);
+        if (part == 0) lat_gpart_p<S, 0>(bp, hb, lc, ui, ci, fa);
         if constexpr (S > 1) {
-            if (part == 1) lat_gpart_p<S, 1>(bp, hb, lc, ui, Ai, ci, fa);
+            if (part == 1) lat_gpart_p<S, 1>(bp, hb, lc, ui, ci, fa);
         }
         if constexpr (S > 2) {
-            if (part == 2) lat_gpart_p<S, 2>(bp, hb, lc, ui, Ai, ci, fa);
-            if (part == 3) lat_gpart_p<S, 3>(bp, hb, lc, ui, Ai, ci, fa);
+            if (part == 2) lat_gpart_p<S, 2>(bp, hb, lc, ui, ci, fa);
+            if (part == 3) lat_gpart_p<S, 3>(bp, hb, lc, ui, ci, fa);
         }
         // part 0 loads the update's inputs while the other parts end
         const KRec kr = d.KR[i];
