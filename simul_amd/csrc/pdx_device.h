@@ -569,7 +569,13 @@ PDX_HD uint32_t bond_entry(const uint32_t *bp, int k, int nnb)
 }
 
 template <int D, int MODEL, bool DIAG, bool COMMIT, bool STORE = true>
-PDX_HD ARec fpass_finish(const Dev &d, int i, const KRec &kr, const double *Hd);
+PDX_HD ARec fpass_finish_k(const Dev &d, int i, const KRec &kr, const double *Hd, const double *Ki);
+
+template <int D, int MODEL, bool DIAG, bool COMMIT, bool STORE = true>
+PDX_HD ARec fpass_finish(const Dev &d, int i, const KRec &kr, const double *Hd)
+{
+    return fpass_finish_k<D, MODEL, DIAG, COMMIT, STORE>(d, i, kr, Hd, d.Kinv64 + (int64_t)i * 9);
+}
 
 template <int D, int MODEL, bool DIAG, bool COMMIT, int G>
 PDX_HD void fpass_one(const Dev &d, int i, int ubuf)
@@ -605,13 +611,13 @@ PDX_HD void fpass_one(const Dev &d, int i, int ubuf)
     fpass_finish<D, MODEL, DIAG, COMMIT>(d, i, kr, Hd);
 }
 
-// per-particle tensor algebra of the F-pass in FP64 with the FP64 inverse:
+// per-particle tensor algebra of the F-pass in FP64 with the FP64 inverse
+// Ki (row i's K^-1, from d.Kinv64 or loaded ahead by the caller):
 // H = Hd K^-1, P(H), A = P K^-1; A is rounded to FP32 when stored (into
 // d.AR with STORE, else only returned)
 template <int D, int MODEL, bool DIAG, bool COMMIT, bool STORE>
-PDX_HD ARec fpass_finish(const Dev &d, int i, const KRec &kr, const double *Hd)
+PDX_HD ARec fpass_finish_k(const Dev &d, int i, const KRec &kr, const double *Hd, const double *Ki)
 {
-    const double *Ki = d.Kinv64 + (int64_t)i * 9;
     const bool sing = (kr.flags & PDX_FL_SING) != 0;
     double H[D * D];
     if (sing) {

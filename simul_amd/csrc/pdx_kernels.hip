@@ -799,9 +799,9 @@ constexpr float lat_len(int n)
          : n == 6 ? 2.44948974f : n == 8 ? 2.82842712f : 3.0f;
 }
 
-// stage u of block lb's halo for the lattice F-pass: lanes 2 e and 2 e + 1 of
-// a wave load quarters 2 (u_x, u_y) and 3 (u_z) of entry 32 g + e into the
-// arrays u01 (16 B per slot) and u2 (8 B per slot)
+// stage u of block lb's halo for the lattice F-pass into the arrays u01
+// (u_x, u_y: 16 B per slot) and u2 (u_z: 8 B per slot): lane L of a wave
+// takes entry 64 g + L, loading the record's bytes 32..47 and 48..55
 template <int NT>
 __device__ __forceinline__ void lat_stage_f(const Dev &d, int ubuf, int lb, pdx_q4 *lds)
 {
@@ -812,26 +812,29 @@ __device__ __forceinline__ void lat_stage_f(const Dev &d, int ubuf, int lb, pdx_
     pdx_q4 *u01 = lds;
     double *u2 = reinterpret_cast<double *>(lds + cap);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int half = lane & 1, sub = lane >> 1;
-    for (int g0 = wv * 32; g0 < nh; g0 += U * NW * 32) {
+    for (int g0 = wv * 64; g0 < nh; g0 += U * NW * 64) {
         int j[U], sl[U];
         bool ok[U];
-        pdx_q4 r[U];
+        pdx_q4 a[U];
+        pdx_q2 b[U];
 #pragma unroll
         for (int v = 0; v < U; ++v) {
-            const int e = g0 + v * NW * 32 + sub;
+            const int e = g0 + v * NW * 64 + lane;
             ok[v] = e < nh;
             const uint32_t he = (uint32_t)(h0 + (ok[v] ? e : 0));
             j[v] = at32<int32_t>(d.hl, he * 4u);
             sl[v] = ok[v] ? (int)at32<uint16_t>(d.hs, he * 2u) : 0;
         }
 #pragma unroll
-        for (int v = 0; v < U; ++v) r[v] = at32<pdx_q4>(Xq, (uint32_t)j[v] * 64u + (uint32_t)(2 + half) * 16u);
+        for (int v = 0; v < U; ++v) {
+            a[v] = at32<pdx_q4>(Xq, (uint32_t)j[v] * 64u + 32u);
+            b[v] = at32<pdx_q2>(Xq, (uint32_t)j[v] * 64u + 48u);
+        }
 #pragma unroll
         for (int v = 0; v < U; ++v) {
             if (!ok[v]) continue;
-            if (half == 0) u01[sl[v]] = r[v];
-            else u2[sl[v]] = __builtin_bit_cast(double, r[v].xy);
+            u01[sl[v]] = a[v];
+            u2[sl[v]] = __builtin_bit_cast(double, b[v]);
         }
     }
     __syncthreads();
@@ -899,12 +902,20 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_l(Dev d, LatC lc, int u
         __hip_atomic_load(&d.sc[d.case_of_blk[lb]].converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
         return;
     if (lb * PDX_BLOCK >= d.ni) return;  // grid padding blocks (whole block)
+    const int t = threadIdx.x % PDX_BLOCK, part = threadIdx.x / PDX_BLOCK;
+    const int i = lb * PDX_BLOCK + t;
+    // the finish's per-row inputs, loaded by part 0 while the block stages
+    KRec kr{};
+    double Kp[9];
+    if (part == 0) {
+        kr = d.KR[i];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) Kp[q] = d.Kinv64[(int64_t)i * 9 + q];
+    }
     lat_stage_f<PDX_BLOCK * S>(d, ubuf, lb, hlds);
     const int cap = d.halo_cap;
     const pdx_q4 *u01 = hlds;
     const double *u2 = reinterpret_cast<const double *>(hlds + cap);
-    const int t = threadIdx.x % PDX_BLOCK, part = threadIdx.x / PDX_BLOCK;
-    const int i = lb * PDX_BLOCK + t;
     const uint16_t *__restrict__ bp = d.bks + lat_at(i, 0);
     double Sd[9];
 #pragma unroll
@@ -928,7 +939,6 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_l(Dev d, LatC lc, int u
     }
     split_sum<9, S>(hlds, Sd);
     if (part != 0) return;
-    const KRec kr = d.KR[i];
     if (!(kr.flags & PDX_FL_ACTIVE)) return;
     // sum_j w V_j (u_j - u_i) (x) xi = V h sum_k u_j(k) (x) w0 o_k: the own
     // slot's u_i (x) w0 o_k of the entries without a live bond and the live
@@ -938,10 +948,10 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_l(Dev d, LatC lc, int u
 #pragma unroll
     for (int q = 0; q < 9; ++q) Hd[q] = Sd[q] * lc.fscale;
     if constexpr (!FR) {
-        fpass_finish<3, MODEL, false, false>(d, i, kr, Hd);
+        fpass_finish_k<3, MODEL, false, false>(d, i, kr, Hd, Kp);
     } else {
         const float cc = d.xu[ubuf][i].c * (float)(0.5 * d.G_s);
-        const ARec ar = fpass_finish<3, MODEL, false, false, false>(d, i, kr, Hd);
+        const ARec ar = fpass_finish_k<3, MODEL, false, false, false>(d, i, kr, Hd, Kp);
         pdx_q4 *o = reinterpret_cast<pdx_q4 *>(d.fr + i);
         const double ih = lc.inv_h;
         o[0] = __builtin_bit_cast(pdx_q4, PdxD2{uo.a * ih, uo.b * ih});
@@ -961,7 +971,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_l(Dev d, LatC lc, int u
 template <int NT, bool FR>
 __device__ __forceinline__ void lat_stage_g(const Dev &d, int ubuf, int lb, pdx_q4 *lds, double ih)
 {
-    constexpr int NW = NT / 64, U = FR ? 4 : 2;
+    constexpr int NW = NT / 64, U = FR ? 8 : 2;
     const int h0 = d.hl_ptr[lb], nh = d.hl_ptr[lb + 1] - h0;
     const pdx_q4 *__restrict__ Xq = reinterpret_cast<const pdx_q4 *>(d.xu[ubuf]);
     const pdx_q4 *__restrict__ Aq = reinterpret_cast<const pdx_q4 *>(d.AR);
