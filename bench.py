@@ -62,7 +62,13 @@ def variant_names(v):
 
 
 def apply_variant(body, spec, iters):
-    """--variant: one code for both passes, 'F,G' per pass, or 0 = autotune."""
+    """--variant: one code for both passes, 'F,G' per pass, 0 = autotune, or
+    auto: the library's default variants (the lattice kernels) where the
+    context is a lattice, else autotune."""
+    if str(spec) == "auto":
+        if body.lattice_spacing() > 0.0:
+            return
+        spec = "0"
     v = [int(x) for x in str(spec).split(",")]
     if v == [0]:
         body.autotune(iters)
@@ -107,9 +113,10 @@ def main():
                          "the Gershgorin fallback should a case diverge)")
     ap.add_argument("--predictor", choices=["secant", "none"], default="secant",
                     help="start of each load increment of the J2 loading (secant: 2 u_(k-1) - u_(k-2))")
-    ap.add_argument("--variant", type=str, default="0",
+    ap.add_argument("--variant", type=str, default="auto",
                     help="hot-pass variant (1, 2, 4, 17, 18, 33, 34, 36, 97, 98, 100, 161, 162, 164, 256, 320, 384) or 'F,G' for the F-pass "
-                         "and the force pass; 0 = autotune")
+                         "and the force pass; 0 = autotune; auto = the library's default lattice kernels on a "
+                         "lattice specimen, else autotune")
     ap.add_argument("--workload", default="vertebra", choices=["vertebra", "vertebra_j2", "batch", "nn"],
                     help="vertebra: the headline (dx 0.4, Neo-Hookean, pressure); vertebra_j2: J2, 20 "
                          "prescribed increments with breaking (dx 0.25); batch: 64 random cases in one context")
