@@ -36,7 +36,7 @@ struct pdx_ctx {
     // LDS-staged halo kernels of the same split where the context is not a
     // lattice, and those the cooperative group-2 kernels where a halo does not
     // fit)
-    int tune_g = PDX_VARIANT_LATTICE | PDX_VARIANT_SPLIT4, tune_f = PDX_VARIANT_LATTICE | PDX_VARIANT_SPLIT2;
+    int tune_g = PDX_VARIANT_LATTICE | PDX_VARIANT_SPLIT4, tune_f = PDX_VARIANT_LATTICE;
     int64_t lds_dyn_max = 0;  // dynamic LDS bytes a block may use on this device
     // the lattice kernels' constants; have_lat: d.bks / d.lk hold this
     // context's lattice tables (PDX_VARIANT_LATTICE runs)
