@@ -1,7 +1,10 @@
 // pdx_kernels.hip -- gfx950 kernels of the NOSB peridynamics solver step.
 //
 // The per-particle and per-bond arithmetic lives in pdx_device.h (one thread
-// per particle, wave64).  Hot loop per dynamic-relaxation iteration:
+// per particle, wave64).  Hot loop per dynamic-relaxation iteration, on a
+// voxel lattice k_fpass_l / k_force_l (each row walks the 122-offset lattice
+// stencil, the bond vectors, lengths and weights compile-time constants of
+// the offset), elsewhere:
 //   F-pass (k_fpass_h; gathering variants k_fpass, k_fpass_w):
 //              H = [sum_j w V_j du (x) xi] K^-1 (FP32 terms, FP64 sums), P(H)
 //              (LE, NH and the J2 radial return in FP64), A = P K^-1 stored in FP32
