@@ -14,9 +14,10 @@ Raw features (20 per particle), all dimensionless:
   18-19 the specimen's stiffness eccentricity: the E-weighted centroid minus
         the geometric centre, over the lateral half-widths (x, y); an axially
         loaded column bends in proportion to it
-Two message-passing hops over the bonds (weighted neighbour means, libpdx
-pdx_neighbor_mean) of the raw vector give 20 + 20 more; 4 zeros pad the
-input to 64 columns (the GEMM K granularity).  Column 3 enters the network
+``hops`` message-passing hops over the bonds (weighted neighbour means,
+libpdx pdx_neighbor_mean; 2 by default, up to 4) of the raw vector give 20
+more columns each; zeros pad the input to a multiple of 16 columns (the GEMM
+K granularity): 64 columns for 2 hops, 112 for 4.  Column 3 enters the network
 only through the two hops: the particle's own modulus is i.i.d. noise in the
 cancellous core, and a prediction that follows it is rough at the particle
 scale.
@@ -27,6 +28,19 @@ import numpy as np
 
 FEAT = 64
 RAW = 20
+
+
+def feature_width(hops: int) -> int:
+    """Input columns of the network for ``hops`` neighbour-mean hops."""
+    return -(-RAW * (hops + 1) // 16) * 16
+
+
+def hops_of(width: int) -> int:
+    """The hop count whose feature_width is ``width``."""
+    for h in range(0, 5):
+        if feature_width(h) == width:
+            return h
+    raise ValueError(f"no hop count gives {width} input columns")
 
 
 def _per_case(spec, off):
@@ -99,26 +113,28 @@ def raw_features_tensor(body, spec):
     return out
 
 
-def feature_tensor(body, spec, case_offsets=None):
-    """(n, 64) float32 device tensor: raw features and two neighbour-mean hops
-    through the bonds of ``body`` (a libpdx context of the same particles),
-    built on the device (pdx_warm_raw_features, pdx_neighbor_features_device)
-    from one upload of the specimen arrays."""
+def feature_tensor(body, spec, case_offsets=None, hops=2):
+    """(n, feature_width(hops)) float32 device tensor: raw features and
+    ``hops`` neighbour-mean hops through the bonds of ``body`` (a libpdx
+    context of the same particles), built on the device
+    (pdx_warm_raw_features, pdx_neighbor_features_device) from one upload of
+    the specimen arrays."""
     import ctypes as C
 
     import torch
     raw = raw_features_tensor(body, spec)
-    out = torch.zeros((spec.n, FEAT), dtype=torch.float32, device="cuda")
+    width = feature_width(hops)
+    out = torch.zeros((spec.n, width), dtype=torch.float32, device="cuda")
     torch.cuda.current_stream().synchronize()
-    body.ctx.call("pdx_neighbor_features_device", RAW, 2, C.c_void_p(raw.data_ptr()), C.c_void_p(out.data_ptr()),
-                  FEAT)
+    body.ctx.call("pdx_neighbor_features_device", RAW, int(hops), C.c_void_p(raw.data_ptr()),
+                  C.c_void_p(out.data_ptr()), width)
     out[:, 3] = 0.0  # the local modulus only through its neighbour means
     return out
 
 
-def feature_matrix(body, spec, case_offsets=None) -> np.ndarray:
+def feature_matrix(body, spec, case_offsets=None, hops=2) -> np.ndarray:
     """feature_tensor on the host (the training data)."""
-    return feature_tensor(body, spec, case_offsets).cpu().numpy()
+    return feature_tensor(body, spec, case_offsets, hops).cpu().numpy()
 
 
 def target_scale_tensor(spec, case_offsets=None):

@@ -33,7 +33,7 @@ def _group_body(specs):
     return packed, off, body
 
 
-def generate(seeds, dx, group, tol, max_iterations, to_bf16, mass=None):
+def generate(seeds, dx, group, tol, max_iterations, to_bf16, mass=None, hops=2):
     """Training data of the cases ``seeds`` at spacing dx, kept on the device:
     BF16 features (int16 bits), FP32 targets u / s and FP32 weights (1 on free
     particles), as lists of CUDA tensors per group of ``group`` cases.  The
@@ -50,7 +50,7 @@ def generate(seeds, dx, group, tol, max_iterations, to_bf16, mass=None):
         rs = body.solve("adaptive", tol=tol, max_iterations=max_iterations, slice_iterations=2048, mass=mass,
                         progress=lambda d, r: log(f"    {d} iterations: {sum(x.converged for x in r)}/{len(r)} converged"))
         u = torch.from_numpy(body.u).cuda()
-        F = feature_tensor(body, packed, off)
+        F = feature_tensor(body, packed, off, hops)
         Xs.append(to_bf16(F))
         del F
         s = target_scale_tensor(packed, off)
@@ -162,24 +162,26 @@ def main(argv=None):
                     help="adaptive-DR mass of the held-out cold and warm solves")
     ap.add_argument("--data-mass", default=None, choices=["gershgorin", "spectral"],
                     help="adaptive-DR mass of the training-data solves")
+    ap.add_argument("--hops", type=int, default=2, help="neighbour-mean hops of the input features (0..4)")
     ap.add_argument("--out", default=None)
     ap.add_argument("--report", default=None)
     args = ap.parse_args(argv)
 
     import torch
     assert torch.cuda.is_available(), "training runs on the GPU"
-    from .net import WarmStartNet
+    from .features import feature_width
+    from .net import DIMS, WarmStartNet
     t_all = time.perf_counter()
     train_seeds = np.arange(args.seed, args.seed + args.train)
     held_seeds = np.arange(args.seed + 100000, args.seed + 100000 + args.heldout)
     parts = np.array_split(train_seeds, len(args.dx))
-    net = WarmStartNet(seed=args.seed)
+    net = WarmStartNet(dims=(feature_width(args.hops),) + DIMS[1:], seed=args.seed)
     Xs, Ts, Ws, gstats = [], [], [], []
     t_gen = time.perf_counter()
     for dx, seeds in zip(args.dx, parts):
         log(f"generating {len(seeds)} training cases at dx {dx}")
         X_, T_, W_, g_ = generate(seeds, dx, args.group, args.tol, args.max_iterations, net.to_bf16,
-                                  mass=args.data_mass)
+                                  mass=args.data_mass, hops=args.hops)
         Xs += X_
         Ts += T_
         Ws += W_
@@ -202,6 +204,7 @@ def main(argv=None):
     ic = [i for e in ev for i in e["cold_iterations"]]
     iw = [i for e in ev for i in e["warm_iterations"]]
     rep = {"train_cases": args.train, "heldout_cases": args.heldout, "dx": list(args.dx), "epochs": args.epochs,
+           "hops": args.hops,
            "train_rows": int(sum(g["particles"] for g in gstats)), "data_s": t_gen, "train_s": t_tr,
            "mass": args.mass, "loss_history": hist, "data_groups": gstats, "heldout": ev,
            "summary": {"cold_s": cold, "warm_s": warm, "speedup_wall": cold / max(warm, 1e-9),

@@ -8,14 +8,15 @@ import time
 
 import numpy as np
 
-from .features import feature_tensor, target_scale_tensor
+from .features import feature_tensor, hops_of, target_scale_tensor
 
 
 def predict_u_device(net, body, spec, case_offsets=None):
     """The predicted displacement as a (n, 3) float64 CUDA tensor (features,
-    inference and scaling on the GPU); prescribed DOFs keep spec.u0."""
+    inference and scaling on the GPU); prescribed DOFs keep spec.u0.  The
+    network's input width gives its neighbour-mean hop count."""
     import torch
-    y = net.predict_device(feature_tensor(body, spec, case_offsets))
+    y = net.predict_device(feature_tensor(body, spec, case_offsets, hops_of(net.dims[0])))
     u = y.to(torch.float64) * target_scale_tensor(spec, case_offsets)[:, None]
     fixed = torch.from_numpy(np.ascontiguousarray(spec.fixed, np.uint8)).cuda().to(torch.int32)
     u0 = torch.from_numpy(np.ascontiguousarray(spec.u0, np.float64)).cuda()
