@@ -1349,8 +1349,8 @@ PDX_HD void nbr_mean_one(const Dev &d, int i, const double *in, double *out, int
             const double L = sqrt(s);
             const double w = (L > 0 ? fmax(0.0, 1.0 - L / d.h) : 1.0) * d.V64[(int)e];
             const double *row = in + (int64_t)e * width + q0;
-            if constexpr (CH == 20) {
-                // launched for width 20 only: the 160-byte row in 16-byte loads
+            if constexpr (CH == 20 || CH == 10) {
+                // launched for width CH only: the 160- or 80-byte row in 16-byte loads
                 struct alignas(16) Pair {
                     double a, b;
                 };

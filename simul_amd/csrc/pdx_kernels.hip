@@ -2047,10 +2047,13 @@ int pdx_launch_zero_free(pdx_ctx *c, int ubuf)
 
 int pdx_launch_nbr_mean(pdx_ctx *c, const double *d_in, double *d_out, int width)
 {
-    // the warm-start features (20 columns) in one pass over the bonds; other
-    // widths four columns per pass
+    // the warm-start features (20 columns, or the 10 that vary within a case)
+    // in one pass over the bonds; other widths four columns per pass
     if (width == 20)
         hipLaunchKernelGGL(k_nbr_mean<20>, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_in, d_out,
+                           width);
+    else if (width == 10)
+        hipLaunchKernelGGL(k_nbr_mean<10>, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_in, d_out,
                            width);
     else
         hipLaunchKernelGGL(k_nbr_mean<4>, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_in, d_out,

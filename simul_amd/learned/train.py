@@ -7,6 +7,10 @@ Data: random vertebral bodies (simul_amd.workloads.random_vertebra: size,
 cancellous field, pressure magnitude and direction within 20 deg), solved by
 the plain cold-start adaptive DR to 1e-6 in batched contexts of ``group``
 cases.  The network learns u / s (s = p H / 1000 MPa) on the free particles.
+``--family j2``: J2 vertebral bodies compressed between platens in one
+increment to a nominal strain of 0.1-2 % (workloads.random_vertebra_j2); the
+features see the prescribed displacement as the load (features.load_view)
+and s is the prescribed |u|.
 Evaluation: the held-out cases solved cold and warm-started (same batched
 context layout, same criterion: residual relative to each case's cold-start
 residual), iterations and wall time including feature building and inference.
@@ -33,27 +37,41 @@ def _group_body(specs):
     return packed, off, body
 
 
-def generate(seeds, dx, group, tol, max_iterations, to_bf16, mass=None, hops=2):
-    """Training data of the cases ``seeds`` at spacing dx, kept on the device:
-    BF16 features (int16 bits), FP32 targets u / s and FP32 weights (1 on free
-    particles), as lists of CUDA tensors per group of ``group`` cases.  The
-    cases are solved by the plain cold-start adaptive DR to tol (``mass``:
-    PDBody.solve's DR mass)."""
+FAMILIES = ("pressure", "j2")
+
+
+def _specimen(family, seed, dx):
+    """One case of a training / held-out family: "pressure" is
+    workloads.random_vertebra (Neo-Hookean, pressure on the top layer), "j2"
+    workloads.random_vertebra_j2 (J2, compressed between platens in one
+    increment)."""
+    from ..workloads import random_vertebra, random_vertebra_j2
+    if family == "j2":
+        return random_vertebra_j2(int(seed), dx=dx)
+    if family == "pressure":
+        return random_vertebra(int(seed), dx=dx)
+    raise ValueError(f"unknown family {family!r}")
+
+
+def generate(seeds, dx, group, tol, max_iterations, to_bf16, mass=None, hops=2, family="pressure"):
+    """Training data of the cases ``seeds`` of ``family`` at spacing dx, kept
+    on the device: BF16 features (int16 bits), FP32 targets u / s and FP32
+    weights (1 on free particles), as lists of CUDA tensors per group of
+    ``group`` cases.  The cases are solved by the plain cold-start adaptive DR
+    to tol (``mass``: PDBody.solve's DR mass)."""
     import torch
-    from ..workloads import random_vertebra
-    from .features import feature_tensor, target_scale_tensor
+    from .features import features_and_scale
     Xs, Ts, Ws, stats = [], [], [], []
     for g0 in range(0, len(seeds), group):
         t0 = time.perf_counter()
-        specs = [random_vertebra(int(s), dx=dx) for s in seeds[g0:g0 + group]]
+        specs = [_specimen(family, s, dx) for s in seeds[g0:g0 + group]]
         packed, off, body = _group_body(specs)
         rs = body.solve("adaptive", tol=tol, max_iterations=max_iterations, slice_iterations=2048, mass=mass,
                         progress=lambda d, r: log(f"    {d} iterations: {sum(x.converged for x in r)}/{len(r)} converged"))
         u = torch.from_numpy(body.u).cuda()
-        F = feature_tensor(body, packed, off, hops)
+        F, s = features_and_scale(body, packed, off, hops)
         Xs.append(to_bf16(F))
         del F
-        s = target_scale_tensor(packed, off)
         Ts.append((u / s[:, None]).to(torch.float32))
         Ws.append(torch.from_numpy((packed.fixed == 0).astype(np.float32)).cuda())
         its = [r.iterations for r in rs]
@@ -93,17 +111,18 @@ def train(net, Xbf, T, Wd, epochs, batch_rows, lr0, lr_min, seed=0):
     return hist
 
 
-def evaluate(net, seeds, dx, group, tol, max_iterations, mass=None):
-    """Cold vs warm-started batched solves of the held-out cases (the same DR
-    mass for both: PDBody.solve's ``mass``).  Features and inference run on
-    the device (predict_u_device, pdx_set_u_device); one untimed inference
-    on the first group loads their GPU code first."""
+def evaluate(net, seeds, dx, group, tol, max_iterations, mass=None, family="pressure", ref_tol=None):
+    """Cold vs warm-started batched solves of the held-out cases of ``family``
+    (the same DR mass for both: PDBody.solve's ``mass``).  Features and
+    inference run on the device (predict_u_device, pdx_set_u_device); one
+    untimed inference on the first group loads their GPU code first.  With
+    ``ref_tol``, each group is also solved cold to ref_tol, and both solutions'
+    distance to that one is reported (the error the tol criterion leaves)."""
     import torch
-    from ..workloads import random_vertebra
     from .warmstart import predict_u_device
     out = []
     for g0 in range(0, len(seeds), group):
-        specs = [random_vertebra(int(s), dx=dx) for s in seeds[g0:g0 + group]]
+        specs = [_specimen(family, s, dx) for s in seeds[g0:g0 + group]]
         packed, off, body = _group_body(specs)
         if g0 == 0:
             predict_u_device(net, body, packed, off)
@@ -137,6 +156,15 @@ def evaluate(net, seeds, dx, group, tol, max_iterations, mass=None):
                     "cold_iterations": ic, "warm_iterations": iw,
                     "all_converged": bool(all(r.converged for r in rc) and all(r.converged for r in rw)),
                     "prediction_rel_l2": err_pred, "final_rel_l2_vs_cold": err_final})
+        if ref_tol is not None:
+            body.u = packed.u0
+            rr = body.solve("adaptive", tol=ref_tol, max_iterations=max_iterations, refs=cold,
+                            slice_iterations=2048, progress=lambda d, r: None, mass=mass)
+            u_ref = body.u
+            dist = lambda u: [float(np.linalg.norm(u[a:b] - u_ref[a:b]) / np.linalg.norm(u_ref[a:b]))
+                              for a, b in zip(off[:-1], off[1:])]
+            out[-1].update({"ref_tol": ref_tol, "ref_converged": bool(all(r.converged for r in rr)),
+                            "cold_rel_l2_vs_ref": dist(u_cold), "warm_rel_l2_vs_ref": dist(u_warm)})
         log(f"  held-out group {g0 // group}: cold {t_cold:.2f}s ({max(ic)} it max, {np.mean(ic):.0f} mean), "
             f"warm {t_warm:.2f}s incl. {t_inf:.2f}s inference ({max(iw)} it max, {np.mean(iw):.0f} mean), "
             f"prediction error {np.median(err_pred):.3f} median")
@@ -163,6 +191,10 @@ def main(argv=None):
     ap.add_argument("--data-mass", default=None, choices=["gershgorin", "spectral"],
                     help="adaptive-DR mass of the training-data solves")
     ap.add_argument("--hops", type=int, default=2, help="neighbour-mean hops of the input features (0..4)")
+    ap.add_argument("--family", default="pressure", choices=FAMILIES,
+                    help="pressure: Neo-Hookean bodies under a top pressure; j2: J2 bodies compressed between platens")
+    ap.add_argument("--ref-tol", type=float, default=None,
+                    help="also solve each held-out group cold to this tolerance and report both solutions' distance")
     ap.add_argument("--out", default=None)
     ap.add_argument("--report", default=None)
     args = ap.parse_args(argv)
@@ -181,7 +213,7 @@ def main(argv=None):
     for dx, seeds in zip(args.dx, parts):
         log(f"generating {len(seeds)} training cases at dx {dx}")
         X_, T_, W_, g_ = generate(seeds, dx, args.group, args.tol, args.max_iterations, net.to_bf16,
-                                  mass=args.data_mass, hops=args.hops)
+                                  mass=args.data_mass, hops=args.hops, family=args.family)
         Xs += X_
         Ts += T_
         Ws += W_
@@ -195,16 +227,18 @@ def main(argv=None):
     t_tr = time.perf_counter() - t_tr
     del Xbf, T, Wd
     if args.out:
-        net.save(args.out, dx=np.array(args.dx), train_cases=args.train, epochs=args.epochs, final_loss=hist[-1])
+        net.save(args.out, dx=np.array(args.dx), train_cases=args.train, epochs=args.epochs, final_loss=hist[-1],
+                 family=args.family)
         log(f"weights -> {args.out}")
     log(f"evaluating on {args.heldout} held-out cases")
-    ev = evaluate(net, held_seeds, args.dx[0], args.group, args.tol, args.max_iterations, mass=args.mass)
+    ev = evaluate(net, held_seeds, args.dx[0], args.group, args.tol, args.max_iterations, mass=args.mass,
+                  family=args.family, ref_tol=args.ref_tol)
     cold = sum(e["cold_s"] for e in ev)
     warm = sum(e["warm_s"] for e in ev)
     ic = [i for e in ev for i in e["cold_iterations"]]
     iw = [i for e in ev for i in e["warm_iterations"]]
     rep = {"train_cases": args.train, "heldout_cases": args.heldout, "dx": list(args.dx), "epochs": args.epochs,
-           "hops": args.hops,
+           "hops": args.hops, "family": args.family,
            "train_rows": int(sum(g["particles"] for g in gstats)), "data_s": t_gen, "train_s": t_tr,
            "mass": args.mass, "loss_history": hist, "data_groups": gstats, "heldout": ev,
            "summary": {"cold_s": cold, "warm_s": warm, "speedup_wall": cold / max(warm, 1e-9),

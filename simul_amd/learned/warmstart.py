@@ -8,21 +8,27 @@ import time
 
 import numpy as np
 
-from .features import feature_tensor, hops_of, target_scale_tensor
+from .features import features_and_scale, hops_of
 
 
 def predict_u_device(net, body, spec, case_offsets=None):
     """The predicted displacement as a (n, 3) float64 CUDA tensor (features,
     inference and scaling on the GPU); prescribed DOFs keep spec.u0.  The
-    network's input width gives its neighbour-mean hop count."""
+    network's input width gives its neighbour-mean hop count; a case loaded
+    through prescribed displacements is seen through features.load_view."""
     import torch
-    y = net.predict_device(feature_tensor(body, spec, case_offsets, hops_of(net.dims[0])))
-    u = y.to(torch.float64) * target_scale_tensor(spec, case_offsets)[:, None]
-    fixed = torch.from_numpy(np.ascontiguousarray(spec.fixed, np.uint8)).cuda().to(torch.int32)
-    u0 = torch.from_numpy(np.ascontiguousarray(spec.u0, np.float64)).cuda()
-    for q in range(3):
-        m = ((fixed >> q) & 1).bool()
-        u[:, q] = torch.where(m, u0[:, q], u[:, q])
+    F, s = features_and_scale(body, spec, case_offsets, hops_of(net.dims[0]))
+    u = net.predict_device(F).to(torch.float64) * s[:, None]
+    del F
+    rows = np.flatnonzero(spec.fixed)
+    if len(rows):
+        r = torch.from_numpy(rows).cuda()
+        bits = torch.from_numpy(np.ascontiguousarray(spec.fixed[rows])).cuda().to(torch.int32)
+        u0 = torch.from_numpy(np.ascontiguousarray(spec.u0[rows], np.float64)).cuda()
+        sub = u[r]
+        for q in range(3):
+            sub[:, q] = torch.where(((bits >> q) & 1).bool(), u0[:, q], sub[:, q])
+        u[r] = sub
     return u
 
 
@@ -46,3 +52,19 @@ def warm_solve(body, spec, net, tol=1e-6, ref=None, case_offsets=None, max_itera
     torch.cuda.synchronize()
     t_all = time.perf_counter() - t0
     return r, {"inference_s": t_inf, "total_s": t_all}
+
+
+def loading_start(net, body, spec, indices, u_final, n_increments):
+    """The ``start`` of loading.IncrementalLoading from a network trained on
+    prescribed-displacement cases: increment k's first guess is the network's
+    prediction for the specimen with its prescribed rows ``indices`` at
+    (k / n_increments) x u_final (the other prescribed DOFs at spec.u0)."""
+    import dataclasses
+    idx = np.asarray(indices, np.int64)
+    uf = np.broadcast_to(np.asarray(u_final, np.float64), (len(idx), spec.X.shape[1]))
+
+    def start(k):
+        u0 = spec.u0.copy()
+        u0[idx] = (k / n_increments) * uf
+        return predict_u_device(net, body, dataclasses.replace(spec, u0=u0))
+    return start

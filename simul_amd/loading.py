@@ -41,7 +41,7 @@ class IncrementalLoading:
     def __init__(self, body, indices, u_final, n_increments: int = 20, critical_stretch: Optional[float] = None,
                  tol: float = 1e-6, max_iterations: int = 200000, method: str = "adaptive",
                  max_break_rounds: int = 3, dt: float = 0.0, viscous_damping: float = 0.0,
-                 progress=None, slice_iterations: int = 4096, mass=None, predictor=None):
+                 progress=None, slice_iterations: int = 4096, mass=None, predictor=None, start=None):
         if n_increments < 1:
             raise ValueError("n_increments must be >= 1")
         self.body = body
@@ -55,17 +55,21 @@ class IncrementalLoading:
         self.progress, self.slice_iterations = progress, slice_iterations
         self.mass = mass  # adaptive-DR mass of every solve (PDBody.solve's mass=)
         # start of an increment: None = the previous state with the new
-        # boundary values; "secant" = 2 u_(k-1) - u_(k-2) from the third
-        # increment on (the convergence reference stays the residual of the
-        # plain start, so both reach the same tolerance)
+        # boundary values; "secant" = 2 u_(k-1) - u_(k-2) from the second
+        # increment on, u_0 being the state the loading starts from; ``start``
+        # = a callable k -> (n, dim) float64 CUDA tensor, the first guess of
+        # an increment that has no secant guess (a learned warm start).  The
+        # convergence reference stays the residual of the plain start, so
+        # every start reaches the same tolerance.
         if predictor not in (None, "secant"):
             raise ValueError("predictor must be None or 'secant'")
         self.predictor = predictor
+        self.start = start
         # displacement states stay on the GPU (pdx_get_u_device / pdx_set_u_device)
         import torch
         self._idx = torch.from_numpy(self.indices).cuda()
         self._u_final = torch.from_numpy(self.u_final).cuda()
-        self._hist: list = []
+        self._hist: list = [body.get_u_device()] if predictor == "secant" else []
         # broken bonds so far: counted once here, then from each break_bonds
         # call (pdx_stats reads the whole bond list back)
         st = body.stats()
@@ -86,12 +90,16 @@ class IncrementalLoading:
         u = b.get_u_device()
         u[self._idx] = (k / self.n_increments) * self._u_final
         b.set_u_device(u)
+        guess = None
         if self.predictor == "secant" and len(self._hist) == 2:
+            guess = 2.0 * self._hist[1] - self._hist[0]
+        elif self.start is not None:
+            guess = self.start(k)
+        if guess is not None:
             b.compute_forces(download=False)
             ref0 = float(np.atleast_1d(b.residual_norm())[0])
-            up = 2.0 * self._hist[1] - self._hist[0]
-            up[self._idx] = u[self._idx]
-            b.set_u_device(up)
+            guess[self._idx] = u[self._idx]
+            b.set_u_device(guess)
             r = self._solve(ref0 if ref0 > 0.0 else 0.0)
         else:
             r = self._solve()

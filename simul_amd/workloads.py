@@ -11,6 +11,9 @@
 * vertebra_j2: the same geometry with J2 plasticity under a prescribed 2 %
   endplate compression in 20 increments, bonds breaking at stretch 0.02.
 * batch: several random box / cylinder / truncated-ellipsoid specimens at dx = 0.5 mm.
+* random_vertebra / random_vertebra_j2: the learned warm start's families of
+  random vertebral bodies (pressure-loaded Neo-Hookean; J2 compressed between
+  platens in one increment).
 
 The lateral distance to the ellipse uses the first-order estimate
 (1 - r) / |grad r| with r = sqrt(x^2/a^2 + y^2/b^2).
@@ -208,9 +211,15 @@ def pack(specs):
     off = np.zeros(len(specs) + 1, np.int64)
     off[1:] = np.cumsum([s.n for s in specs])
     cat = lambda name: np.concatenate([getattr(s, name) for s in specs])
+    extra = {"cases": [s.name for s in specs]}
+    if specs[0].model == "j2_plasticity":
+        # the per-particle J2 parameters travel with their particles
+        for key in ("yield", "h_iso"):
+            extra[key] = np.concatenate([np.broadcast_to(np.asarray(s.extra[key], np.float64), (s.n,))
+                                         for s in specs])
     packed = Specimen(f"batch{len(specs)}_dx{dx}", cat("X"), dx, specs[0].horizon, cat("E"), specs[0].nu,
                       specs[0].rho, cat("fixed"), cat("u0"), cat("fext"), specs[0].model, cat("material_id"),
-                      {"cases": [s.name for s in specs]})
+                      extra)
     return packed, off
 
 
@@ -230,4 +239,23 @@ def random_vertebra(seed, dx=0.8, p_range=(0.5, 3.0), cone_deg=20.0, model="neo_
     spec = vertebra(dx=dx, a=a, b=b, height=h, pressure=p, seed=int(rng.integers(1 << 31)), model=model,
                     direction=d)
     spec.name = f"rvert{seed}_dx{dx}"
+    return spec
+
+
+def random_vertebra_j2(seed, dx=0.8, strain_range=(1e-3, 2e-2), **kw):
+    """A vertebral body of random size (as random_vertebra) with the J2
+    material and the platens of vertebra_j2, compressed in one increment to a
+    nominal strain drawn log-uniform from ``strain_range``: u0 holds the
+    prescribed top displacement (0, 0, -strain x height).  The J2 family of
+    the learned warm start."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(14.0, 24.0)
+    b = a * rng.uniform(0.6, 0.9)
+    h = rng.uniform(18.0, 30.0)
+    eps = math.exp(rng.uniform(math.log(strain_range[0]), math.log(strain_range[1])))
+    spec = vertebra_j2(dx=dx, a=a, b=b, height=h, strain=-eps, n_increments=1, seed=int(rng.integers(1 << 31)),
+                       **kw)
+    spec.u0[spec.extra["top_idx"]] = spec.extra["u_final"]
+    spec.name = f"rvertj2_{seed}_dx{dx}"
+    spec.extra["strain"] = eps
     return spec

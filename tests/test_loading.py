@@ -5,6 +5,8 @@ through size-independent checks: neighbour sets and broken flags bit-exact
 with the oracle on the same displacement field, J2 state of that field within
 the J2 tolerance.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -190,3 +192,38 @@ def test_secant_predictor_and_spectral_mass_same_states(gpu):
     _oracle_increments(prob, idx, uf, 6, 1e-6)
     assert rel_l2(runs["fast"][1], prob.u) < 1e-4
     assert rel_l2(runs["fast"][2], prob.epbar) < 1e-3
+
+
+def test_secant_from_the_start_state_and_learned_first_increment(gpu):
+    """The secant predictor takes the state the loading starts from as u_0, so
+    the second increment already starts from 2 u_1 - u_0 (cheaper than the
+    plain start where the response is elastic).  A ``start`` callable (here
+    the learned J2 warm start, loading_start) supplies the first increment's
+    guess, the only one without a secant guess: every run ends at the plain
+    driver's state (same tolerance against the same plain-start reference)."""
+    from simul_amd.learned import WarmStartNet
+    from simul_amd.learned.warmstart import loading_start
+    spec = W.vertebra_j2(dx=1.25, n_increments=4)
+    idx, uf = spec.extra["top_idx"], spec.extra["u_final"]
+    net = WarmStartNet.load(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "simul_amd",
+                                         "learned", "weights_vertebra_j2.npz"))
+    runs = {}
+    for key in ("plain", "secant", "secant+start"):
+        body = from_specimen(spec)
+        calls = []
+        start = None
+        if key == "secant+start":
+            s0 = loading_start(net, body, spec, idx, uf, 4)
+            start = lambda k: (calls.append(k), s0(k))[1]
+        drv = IncrementalLoading(body, idx, uf, n_increments=4, critical_stretch=None, tol=1e-6,
+                                 predictor=None if key == "plain" else "secant", start=start)
+        res = drv.run()
+        assert all(r.converged for r in res), key
+        runs[key] = ([r.iterations for r in res], body.u.copy())
+        if key == "secant+start":
+            assert calls == [1]
+    print({k: v[0] for k, v in runs.items()}, flush=True)
+    assert runs["secant"][0][0] == runs["plain"][0][0]
+    assert runs["secant"][0][1] < runs["plain"][0][1]
+    for key in ("secant", "secant+start"):
+        assert rel_l2(runs[key][1], runs["plain"][1]) < 1e-4, key
