@@ -29,6 +29,9 @@ event means of its hot kernels against the same per-unit model; J2 adds
   batch64_dx0.5       64 random specimens (7,045,942 particles) in one context,
                       every case solved to 1e-6 (setup included in the
                       reported time-to-solve)
+  warmstart_j2_...    the learned warm start: 64 held-out J2 vertebral bodies
+                      (dx 0.75) in one context, cold and warm-started solves
+                      to 1e-6 with the same DR mass
 """
 from __future__ import annotations
 
@@ -117,9 +120,10 @@ def main():
                     help="hot-pass variant (1, 2, 4, 17, 18, 33, 34, 36, 97, 98, 100, 161, 162, 164, 256, 320, 384) or 'F,G' for the F-pass "
                          "and the force pass; 0 = autotune; auto = the library's default lattice kernels on a "
                          "lattice specimen, else autotune")
-    ap.add_argument("--workload", default="vertebra", choices=["vertebra", "vertebra_j2", "batch", "nn"],
+    ap.add_argument("--workload", default="vertebra", choices=["vertebra", "vertebra_j2", "batch", "warmstart", "nn"],
                     help="vertebra: the headline (dx 0.4, Neo-Hookean, pressure); vertebra_j2: J2, 20 "
-                         "prescribed increments with breaking (dx 0.25); batch: 64 random cases in one context")
+                         "prescribed increments with breaking (dx 0.25); batch: 64 random cases in one context; "
+                         "warmstart: the learned warm start on 64 held-out J2 cases, cold vs warm")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device copy (scales rocprof FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--headline-only", action="store_true",
@@ -152,6 +156,9 @@ def main():
         return
     if args.workload == "nn":
         _print(run_nn(args, torch))
+        return
+    if args.workload == "warmstart":
+        _print(run_warmstart(args, torch))
         return
 
     t0 = time.perf_counter()
@@ -298,7 +305,7 @@ def main():
         del body
         gc.collect()
         configs = {}
-        for fn in (run_loading, run_batch):
+        for fn in (run_loading, run_batch, run_warmstart):
             blk = fn(args, torch)
             configs[blk["config"]["workload"]] = blk
             gc.collect()
@@ -480,6 +487,85 @@ def run_batch(args, torch):
                  time_to_solve=tts, roofline=roof, cpu_baseline=cpu)
 
 
+def run_warmstart(args, torch):
+    """The learned warm start on its J2 family: 64 held-out random J2
+    vertebral bodies (dx 0.75 mm, about 49 k particles each, compressed
+    between platens in one increment to 0.1-2 %; seeds never used in
+    training) packed into one context, each solved by adaptive DR to 1e-6 of
+    its cold-start residual from rest (cold) and from the network's
+    prediction (warm), the same DR mass for both.  value = intact bonds x
+    batched iterations / DR wall time of the warm solve."""
+    import numpy as np
+
+    from simul_amd.learned import WarmStartNet
+    from simul_amd.learned.train import _group_body, _specimen
+    from simul_amd.learned.warmstart import predict_u_device
+
+    net = WarmStartNet.load(os.path.join(ROOT, "simul_amd", "learned", "weights_vertebra_j2.npz"))
+    dx = 0.75
+    specs = [_specimen("j2", s, dx) for s in range(105000, 105064)]
+    t0 = time.perf_counter()
+    packed, off, body = _group_body(specs)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    nb = body.stats()["n_intact"]
+    log(f"warm-start family: {len(specs)} held-out J2 cases, {packed.n} particles, {nb} bonds, setup {setup:.2f}s")
+    apply_variant(body, args.variant, 5)
+    predict_u_device(net, body, packed, off)  # loads the path's GPU code (untimed, nothing kept)
+    cold = np.atleast_1d(body.cold_residual())
+
+    def solve():
+        return body.solve("adaptive", tol=args.tol, max_iterations=args.max_solve_iters, refs=cold,
+                          slice_iterations=2048, progress=lambda d, r: None, mass=args.dr_mass)
+
+    body.u = packed.u0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rc = solve()
+    torch.cuda.synchronize()
+    t_cold = time.perf_counter() - t0
+    u_cold = body.u
+    body.u = packed.u0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    body.set_u_device(predict_u_device(net, body, packed, off))
+    torch.cuda.synchronize()
+    t_inf = time.perf_counter() - t0
+    rw = solve()
+    torch.cuda.synchronize()
+    t_warm = time.perf_counter() - t0
+    u_warm = body.u
+    ic, iw = [r.iterations for r in rc], [r.iterations for r in rw]
+    du = [float(np.linalg.norm(u_warm[a:b] - u_cold[a:b]) / np.linalg.norm(u_cold[a:b]))
+          for a, b in zip(off[:-1], off[1:])]
+    tts = {"cold_start_s": round(t_cold, 4), "warm_start_s": round(t_warm, 4), "warm_inference_s": round(t_inf, 4),
+           "warm_speedup": round(t_cold / t_warm, 3), "cold_iterations_mean": float(np.mean(ic)),
+           "warm_iterations_mean": float(np.mean(iw)), "warm_iteration_ratio": round(np.mean(ic) / np.mean(iw), 3),
+           "min_case_iteration_ratio": round(min(c / w for c, w in zip(ic, iw)), 3),
+           "cold_iterations_max": max(ic), "warm_iterations_max": max(iw),
+           "converged": int(sum(r.converged for r in rc) + sum(r.converged for r in rw)), "solves": 2 * len(specs),
+           "warm_vs_cold_u_rel_l2_max": max(du), "dr_mass": mass_label(args.dr_mass == "spectral"),
+           "setup_s": round(setup, 3),
+           "warm_start": "J2 network (simul_amd/learned/weights_vertebra_j2.npz, 4,096 training cases); features, "
+                         "inference and DR included in warm_start_s; one untimed inference before it loads the "
+                         "process's GPU code for that path"}
+    log(f"warm-start family: cold {t_cold:.3f}s ({np.mean(ic):.0f} it mean), warm {t_warm:.3f}s incl. "
+        f"{t_inf:.3f}s features+inference ({np.mean(iw):.0f} it mean): {t_cold / t_warm:.2f}x wall, "
+        f"{np.mean(ic) / np.mean(iw):.2f}x iterations")
+    body.u = packed.u0
+    roof = hot_roofline(body, packed.n, nb, args.kernel_iters, state_bytes=52.0)
+    variant = pass_variants(body)[0]
+    del body
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_cases(specs, args.cpu_seconds, what="the first held-out J2 cases")
+    return _line(METRIC, nb * max(iw) / (t_warm - t_inf), "bond-updates/s", max(iw), 0,
+                 1e3 * (t_warm - t_inf) / max(max(iw), 1),
+                 {"workload": f"warmstart_j2_dx{dx}_64heldout", "cases": len(specs), "n_particles": packed.n,
+                  "n_bonds": nb, "variant": variant},
+                 time_to_solve=tts, roofline=roof, cpu_baseline=cpu)
+
+
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 
 
@@ -636,7 +722,7 @@ def cpu_baseline(spec, seconds, u_start=None, what="kinetic-DR iterations of the
                       f"about 300 000 iterations for 1e-4, so 1e-6 is not reached within the sample"}
 
 
-def cpu_baseline_cases(specs, seconds):
+def cpu_baseline_cases(specs, seconds, what="the first cases of the batch"):
     """The oracle on the batch's cases one after another (a case at a time:
     the reference solves one body per solver), about `seconds` of kinetic-DR
     iterations spread over the first cases."""
@@ -663,7 +749,7 @@ def cpu_baseline_cases(specs, seconds):
     v = upd / el_tot
     log(f"cpu baseline (batch): {cores} threads, {len(used)} cases in {el_tot:.1f}s -> {v / 1e6:.1f} M bond-updates/s")
     return {"value": v, "unit": "bond-updates/s", "cores": cores, "kind": "port",
-            "sample": "kinetic-DR iterations from the cold start of the first cases of the batch, one case at a "
+            "sample": f"kinetic-DR iterations from the cold start of {what}, one case at a "
                       "time, reference-faithful FP64 C/OpenMP restatement (setup untimed): " + "; ".join(used)}
 
 
