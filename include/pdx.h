@@ -213,8 +213,11 @@ int pdx_neighbor_features_device(pdx_ctx *ctx, int32_t width, int32_t hops, cons
 /* raw inputs of the learned warm start (no reference counterpart): 20 FP64 columns per
  * particle (n x 20, caller order) into the DEVICE array d_out, from the host arrays of the
  * specimen: X (n x 3), E (n), fext (n x 3 force density), fixed (n DOF masks), material_id
- * (n, 0 = cortical; NULL: none cortical) and the particle spacing dx.  Per-case box, load
- * resultant and modulus statistics over the context's cases.  3-D contexts only. */
+ * (n, 0 = cortical; NULL: none cortical) and the particle spacing dx.  X, fext or fixed
+ * NULL: the context's own positions, external forces or DOF masks (as pdx_set_particles and
+ * pdx_set_bc stored them), gathered on the device instead of copied from the host.
+ * Per-case box, load resultant and modulus statistics over the context's cases.  3-D
+ * contexts only. */
 int pdx_warm_raw_features(pdx_ctx *ctx, const double *X, const double *E, const double *fext, const uint8_t *fixed,
                           const int32_t *material_id, double dx, double *d_out);
 

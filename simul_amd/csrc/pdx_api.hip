@@ -1330,7 +1330,8 @@ extern "C" int pdx_warm_raw_features(pdx_ctx *c, const double *X, const double *
     CHECK_CTX(c);
     if (!c->have_particles) return fail(PDX_ESTATE, "particles not set");
     if (c->dim != 3) return fail(PDX_EINVAL, "raw features are defined for 3-D specimens");
-    if (!X || !E || !fext || !fixed || !d_out) return fail(PDX_EINVAL, "X/E/fext/fixed/d_out NULL");
+    if (!E || !d_out) return fail(PDX_EINVAL, "E/d_out NULL");
+    // X, fext, fixed NULL: the context's own (pdx_set_particles / pdx_set_bc)
     if (!(dx > 0.0)) return fail(PDX_EINVAL, "dx must be > 0");
     if (!is_device_ptr(c, d_out)) return fail(PDX_EINVAL, "d_out is not device memory of the context's device");
     // row chunks that never span two cases
@@ -1356,9 +1357,16 @@ extern "C" int pdx_warm_raw_features(pdx_ctx *c, const double *X, const double *
     const size_t total = off_fx + n;
     char *buf = nullptr;
     PDX_HIP(hipMallocAsync((void **)&buf, total, c->stream));
-    hipError_t e = hipMemcpyAsync(buf + off_X, X, 24 * n, hipMemcpyHostToDevice, c->stream);
+    hipError_t e = hipSuccess;
+    if (!X || !fext || !fixed) {
+        const int rl = pdx_launch_ctx_rows(c, X ? nullptr : (double *)(buf + off_X),
+                                           fext ? nullptr : (double *)(buf + off_f),
+                                           fixed ? nullptr : (uint8_t *)(buf + off_fx));
+        if (rl != PDX_OK) e = hipErrorLaunchFailure;
+    }
+    if (e == hipSuccess && X) e = hipMemcpyAsync(buf + off_X, X, 24 * n, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(buf + off_E, E, 8 * n, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(buf + off_f, fext, 24 * n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && fext) e = hipMemcpyAsync(buf + off_f, fext, 24 * n, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(buf + off_co, c->case_off.data(), 8 * (size_t)(c->ncase + 1), hipMemcpyHostToDevice,
                            c->stream);
@@ -1368,7 +1376,7 @@ extern "C" int pdx_warm_raw_features(pdx_ctx *c, const double *X, const double *
         e = hipMemcpyAsync(buf + off_m, material_id, 4 * n, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(buf + off_fi, first.data(), 4 * (size_t)(c->ncase + 1), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(buf + off_fx, fixed, n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && fixed) e = hipMemcpyAsync(buf + off_fx, fixed, n, hipMemcpyHostToDevice, c->stream);
     int r = PDX_EDEVICE;
     if (e == hipSuccess) {
         PdxRawIn in{(const double *)(buf + off_X), (const double *)(buf + off_E), (const double *)(buf + off_f),

@@ -122,6 +122,7 @@ struct PdxRawIn {
     const uint8_t *fixed;
     const int32_t *mat;
 };
+int pdx_launch_ctx_rows(pdx_ctx *c, double *d_X, double *d_fext, uint8_t *d_fixed);
 int pdx_launch_raw_features(pdx_ctx *c, const PdxRawIn &in, const int64_t *d_chunk, int nchunk,
                             const int32_t *d_chunk_first, const int64_t *d_case_off, double dx, double *d_part,
                             double *d_cc, double *d_out);

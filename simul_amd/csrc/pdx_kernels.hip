@@ -2045,8 +2045,85 @@ int pdx_launch_zero_free(pdx_ctx *c, int ubuf)
     return PDX_OK;
 }
 
+// weighted neighbour means of W FP64 columns on a lattice context: the rows
+// of the block's halo are staged in LDS (W 8 bytes per slot), each stencil
+// offset k weighs w0(|o_k|) V, and a bond counts when bks names another slot
+// than the row's own (an intact bond of an active row; the own slot stands
+// for no bond, a broken one or an inactive row).  A row without bonds keeps
+// its value.  The slot numbers are read NG offsets at a time, the next
+// group's while the current one is summed.
+template <int W>
+__global__ __launch_bounds__(PDX_BLOCK) void k_nbr_mean_l(Dev d, LatC lc, const double *__restrict__ in,
+                                                         double *__restrict__ out)
+{
+    static_assert(W % 2 == 0, "16-byte row pieces");
+    constexpr int NG = 16, NK = (PDX_NST + NG - 1) / NG * NG;
+    extern __shared__ pdx_q4 hlds[];
+    __shared__ double wk[NK];
+    const int lb = logical_block();
+    if (lb * PDX_BLOCK >= d.ni) return;  // grid padding: the whole block leaves
+    const int t = threadIdx.x, i = lb * PDX_BLOCK + t;
+    const uint16_t *__restrict__ bp = d.bks + lat_at(i, 0);
+    const int h0 = d.hl_ptr[lb], nh = d.hl_ptr[lb + 1] - h0;
+    const int so = d.hs[h0 + t];
+    int cur[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) cur[g] = bp[g * PDX_SLICE];
+    for (int e = t; e < nh; e += PDX_BLOCK) {
+        const pdx_q4 *src = reinterpret_cast<const pdx_q4 *>(in + (int64_t)d.hl[h0 + e] * W);
+        pdx_q4 *dst = hlds + (int)d.hs[h0 + e] * (W / 2);
+#pragma unroll
+        for (int q = 0; q < W / 2; ++q) dst[q] = src[q];
+    }
+    if (t < NK) {
+        double w = 0.0;  // offsets past the stencil weigh nothing
+        if (t < PDX_NST) {
+            const int8_t *o = kLat.o[t];
+            w = lc.w1[o[0] * o[0] + o[1] * o[1] + o[2] * o[2]] * lc.V;
+        }
+        wk[t] = w;
+    }
+    __syncthreads();
+    const double *hb = reinterpret_cast<const double *>(hlds);
+    double acc[W], ws = 0.0;
+#pragma unroll
+    for (int q = 0; q < W; ++q) acc[q] = 0.0;
+    for (int k0 = 0; k0 < NK; k0 += NG) {
+        int nxt[NG];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const int k = k0 + NG + g;
+            nxt[g] = k < PDX_NST ? bp[k * PDX_SLICE] : so;
+        }
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const int sl = k0 + g < PDX_NST ? cur[g] : so;
+            const double w = sl == so ? 0.0 : wk[k0 + g];
+            const double *row = hb + sl * W;
+#pragma unroll
+            for (int q = 0; q < W; ++q) acc[q] += w * row[q];
+            ws += w;
+        }
+#pragma unroll
+        for (int g = 0; g < NG; ++g) cur[g] = nxt[g];
+    }
+#pragma unroll
+    for (int q = 0; q < W; ++q) out[(int64_t)i * W + q] = ws > 0.0 ? acc[q] / ws : hb[so * W + q];
+}
+
 int pdx_launch_nbr_mean(pdx_ctx *c, const double *d_in, double *d_out, int width)
 {
+    // the warm-start features' varying columns (10) on a lattice context:
+    // the lattice kernel, when its halo fits the LDS
+    if (width == 10 && c->have_lat && c->d.bks && c->dim == 3) {
+        const int lds = pdx_halo_lds(c, 80, 1, 0);
+        if (lds) {
+            hipLaunchKernelGGL(k_nbr_mean_l<10>, dim3(c->nblk), dim3(PDX_BLOCK), lds, c->stream, c->d, c->lat, d_in,
+                               d_out);
+            PDX_LAUNCH_CHECK();
+            return PDX_OK;
+        }
+    }
     // the warm-start features (20 columns, or the 10 that vary within a case)
     // in one pass over the bonds; other widths four columns per pass
     if (width == 20)
@@ -2063,18 +2140,21 @@ int pdx_launch_nbr_mean(pdx_ctx *c, const double *d_in, double *d_out, int width
 }
 
 // caller-order rows (n x width FP64) into internal order (padding rows 0)
+// one thread per value, consecutive threads along a row
 __global__ __launch_bounds__(PDX_BLOCK) void k_rows_internal(Dev d, const double *in, double *out, int width)
 {
-    const int i = blockIdx.x * PDX_BLOCK + threadIdx.x;
-    if (i >= d.ni) return;
+    const int64_t e = (int64_t)blockIdx.x * PDX_BLOCK + threadIdx.x;
+    if (e >= (int64_t)d.ni * width) return;
+    const int i = (int)(e / width), q = (int)(e - (int64_t)i * width);
     const int r = d.perm[i];
-    for (int q = 0; q < width; ++q)
-        out[(int64_t)i * width + q] = r >= 0 ? in[(int64_t)r * width + q] : 0.0;
+    out[e] = r >= 0 ? in[(int64_t)r * width + q] : 0.0;
 }
 
 int pdx_launch_rows_internal(pdx_ctx *c, const double *d_in, double *d_out, int width)
 {
-    hipLaunchKernelGGL(k_rows_internal, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_in, d_out, width);
+    const int64_t total = (int64_t)c->ni * width;
+    hipLaunchKernelGGL(k_rows_internal, dim3((unsigned)((total + PDX_BLOCK - 1) / PDX_BLOCK)), dim3(PDX_BLOCK), 0,
+                       c->stream, c->d, d_in, d_out, width);
     PDX_LAUNCH_CHECK();
     return PDX_OK;
 }
@@ -2104,23 +2184,28 @@ int pdx_launch_u_device(pdx_ctx *c, int ubuf, double *out, const double *in)
 
 // [hop 0 | hop 1 | ...] of internal-order FP64 blocks (ni x width each) into
 // the caller-order FP32 rows of out (stride ldo)
+// one thread per packed value, consecutive threads along a row's columns
+// (width x (hops + 1) of them), so the reads and writes of a wave cover whole
+// row pieces
 __global__ __launch_bounds__(PDX_BLOCK) void k_feat_pack(Dev d, const double *hv, const int32_t *perm, float *out,
                                                         int width, int hops, int ldo)
 {
-    const int i = blockIdx.x * PDX_BLOCK + threadIdx.x;
-    if (i >= d.ni) return;
+    const int nc = width * (hops + 1);
+    const int64_t e = (int64_t)blockIdx.x * PDX_BLOCK + threadIdx.x;
+    if (e >= (int64_t)d.ni * nc) return;
+    const int i = (int)(e / nc), col = (int)(e - (int64_t)i * nc);
     const int r = perm[i];
     if (r < 0) return;
-    for (int k = 0; k <= hops; ++k)
-        for (int q = 0; q < width; ++q)
-            out[(int64_t)r * ldo + k * width + q] = (float)hv[((int64_t)k * d.ni + i) * width + q];
+    const int k = col / width, q = col - k * width;
+    out[(int64_t)r * ldo + col] = (float)hv[((int64_t)k * d.ni + i) * width + q];
 }
 
 int pdx_launch_feat_pack(pdx_ctx *c, const double *d_hv, const int32_t *d_perm, float *d_out, int width, int hops,
                          int ldo)
 {
-    hipLaunchKernelGGL(k_feat_pack, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_hv, d_perm, d_out,
-                       width, hops, ldo);
+    const int64_t total = (int64_t)c->ni * width * (hops + 1);
+    hipLaunchKernelGGL(k_feat_pack, dim3((unsigned)((total + PDX_BLOCK - 1) / PDX_BLOCK)), dim3(PDX_BLOCK), 0,
+                       c->stream, c->d, d_hv, d_perm, d_out, width, hops, ldo);
     PDX_LAUNCH_CHECK();
     return PDX_OK;
 }
@@ -2251,6 +2336,34 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_raw_fill(PdxRawIn in, const int64
     y[17] = o[RC_CORT];
     y[18] = o[RC_ECC];
     y[19] = o[RC_ECC + 1];
+}
+
+// the context's positions, external forces and fixed-DOF bits in caller
+// order (the arrays pdx_set_particles / pdx_set_bc stored), for the raw
+// features of a caller that passes none of its own
+__global__ __launch_bounds__(PDX_BLOCK) void k_ctx_rows(Dev d, double *X, double *fext, uint8_t *fixed)
+{
+    const int i = blockIdx.x * PDX_BLOCK + threadIdx.x;
+    if (i >= d.ni) return;
+    const int r = d.perm[i];
+    if (r < 0) return;
+    const XRec &x = d.xu[0][i];
+    const pdx_d4 f = d.fext[i];
+    if (X)
+        for (int q = 0; q < 3; ++q) X[(int64_t)r * 3 + q] = x.X[q];
+    if (fext) {
+        fext[(int64_t)r * 3] = f.x;
+        fext[(int64_t)r * 3 + 1] = f.y;
+        fext[(int64_t)r * 3 + 2] = f.z;
+    }
+    if (fixed) fixed[r] = (uint8_t)(d.KR[i].flags & PDX_FL_FIXMASK);
+}
+
+int pdx_launch_ctx_rows(pdx_ctx *c, double *d_X, double *d_fext, uint8_t *d_fixed)
+{
+    hipLaunchKernelGGL(k_ctx_rows, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, d_X, d_fext, d_fixed);
+    PDX_LAUNCH_CHECK();
+    return PDX_OK;
 }
 
 int pdx_launch_raw_features(pdx_ctx *c, const PdxRawIn &in, const int64_t *d_chunk, int nchunk,

@@ -64,11 +64,15 @@ def _platens(spec, off):
     """The moving prescribed rows (fixed, nonzero u0), their cases, and each
     case's count of them and mean prescribed displacement."""
     ncase = len(off) - 1
-    fixed_rows = np.flatnonzero(spec.fixed)
-    rows = fixed_rows[(spec.u0[fixed_rows] != 0).any(1)]
+    rows = np.flatnonzero(spec.fixed != 0)
+    u0r = spec.u0[rows]
+    moving = u0r[:, 0] != 0
+    for q in range(1, u0r.shape[1]):
+        moving |= u0r[:, q] != 0
+    rows, u0r = rows[moving], u0r[moving]
     case = np.searchsorted(off, rows, side="right") - 1
     cnt = np.bincount(case, minlength=ncase)
-    um = np.stack([np.bincount(case, weights=spec.u0[rows, q], minlength=ncase) for q in range(3)], 1)
+    um = np.stack([np.bincount(case, weights=u0r[:, q], minlength=ncase) for q in range(3)], 1)
     return rows, case, cnt, um / np.maximum(cnt, 1)[:, None]
 
 
@@ -146,19 +150,23 @@ def raw_features(spec, case_offsets=None) -> np.ndarray:
     return np.ascontiguousarray(F.T)
 
 
-def raw_features_tensor(body, spec):
+def raw_features_tensor(body, spec, from_context=False):
     """raw_features computed on the GPU (libpdx pdx_warm_raw_features over the
     cases of ``body``, a libpdx context of the same particles): (n, 20)
-    float64 CUDA tensor, the same values to FP64 rounding."""
+    float64 CUDA tensor, the same values to FP64 rounding.  from_context:
+    positions, external forces and DOF masks are the context's own (no copy
+    of them from the host; only the moduli and phases are uploaded)."""
     import ctypes as C
 
     import torch
     from .. import _native as N
     n = spec.n
-    X = np.ascontiguousarray(spec.X, np.float64)
     E = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.E, np.float64), (n,)))
-    fext = np.ascontiguousarray(spec.fext, np.float64)
-    fixed = np.ascontiguousarray(spec.fixed, np.uint8)
+    X = fext = fixed = None
+    if not from_context:
+        X = np.ascontiguousarray(spec.X, np.float64)
+        fext = np.ascontiguousarray(spec.fext, np.float64)
+        fixed = np.ascontiguousarray(spec.fixed, np.uint8)
     mat = None if spec.material_id is None else np.ascontiguousarray(spec.material_id, np.int32)
     out = torch.empty((n, RAW), dtype=torch.float64, device="cuda")
     torch.cuda.current_stream().synchronize()
@@ -203,13 +211,14 @@ def scale_from_raw(raw):
 
 def raw_view_tensor(body, spec, case_offsets=None):
     """raw_features_tensor of load_view(spec) without building the view: the
-    raw features of the specimen's own arrays, then, on the device, columns
+    raw features of the context's positions, loads and DOF masks (those of
+    ``spec``) with the specimen's moduli and phases, then, on the device, columns
     8-12 of each case that load_view would load through its platens (no force
     resultant, moving prescribed rows): 1 on the moving rows, the unit mean
     prescribed displacement and log10 of 1000 |u| / H.  Equal to the view's
     features to FP64 rounding."""
     import torch
-    raw = raw_features_tensor(body, spec)
+    raw = raw_features_tensor(body, spec, from_context=True)
     off = _offsets(spec, case_offsets)
     rows, case, cnt, um = _platens(spec, off)
     if len(rows) == 0:

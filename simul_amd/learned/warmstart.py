@@ -20,7 +20,7 @@ def predict_u_device(net, body, spec, case_offsets=None):
     F, s = features_and_scale(body, spec, case_offsets, hops_of(net.dims[0]))
     u = net.predict_device(F).to(torch.float64) * s[:, None]
     del F
-    rows = np.flatnonzero(spec.fixed)
+    rows = np.flatnonzero(spec.fixed != 0)
     if len(rows):
         r = torch.from_numpy(rows).cuda()
         bits = torch.from_numpy(np.ascontiguousarray(spec.fixed[rows])).cuda().to(torch.int32)

@@ -180,8 +180,9 @@ def test_device_features_match_host_hops(gpu):
     columns 0-9, FP32 pack) against the same two hops through
     pdx_neighbor_mean on host arrays, for a two-case batch with padding
     between the cases: bit for bit in the hopped columns; columns 10-19 are
-    constant over each case, so their neighbour means equal them to FP64
-    rounding and the input repeats them.  The host-input pdx_neighbor_features
+    constant over each case, so their neighbour means (all 20 columns through
+    the bond-list kernel) equal them to FP64 rounding and the input repeats
+    them.  The host-input pdx_neighbor_features
     gives the device path's bits, and the target scale from the raw features
     equals target_scale."""
     import ctypes as C
@@ -197,13 +198,19 @@ def test_device_features_match_host_hops(gpu):
     body = from_specimen(packed, case_offsets=off)
     raw = raw_features_tensor(body, packed).cpu().numpy()
     assert np.allclose(raw, raw_features(packed, off), rtol=1e-12, atol=1e-12)
+    # positions, loads and DOF masks gathered from the context: the same bits
+    assert np.array_equal(raw_features_tensor(body, packed, from_context=True).cpu().numpy(), raw)
     Ft, st = features_and_scale(body, packed, off)
     F = Ft.cpu().numpy()
     assert np.allclose(st.cpu().numpy(), target_scale(packed, off), rtol=1e-12, atol=0)
-    m1 = body.neighbor_mean(raw)
+    # the hopped columns through the same kernel (width 10: the lattice one),
+    # and all 20 through the bond-list kernel: the same means to FP64
+    # rounding, the case constants unchanged
+    m1 = body.neighbor_mean(raw[:, :HOPPED])
     m2 = body.neighbor_mean(m1)
-    assert np.allclose(m1[:, HOPPED:], raw[:, HOPPED:], rtol=1e-14, atol=1e-14)
-    assert np.allclose(m2[:, HOPPED:], raw[:, HOPPED:], rtol=1e-14, atol=1e-14)
+    g1 = body.neighbor_mean(raw)
+    assert np.allclose(g1[:, :HOPPED], m1, rtol=1e-13, atol=1e-13)
+    assert np.allclose(g1[:, HOPPED:], raw[:, HOPPED:], rtol=1e-14, atol=1e-14)
     ref = np.zeros((packed.n, FEAT), np.float32)
     ref[:, :RAW] = raw
     ref[:, 3] = 0.0
@@ -220,6 +227,37 @@ def test_device_features_match_host_hops(gpu):
                   3 * HOPPED)
     assert torch.equal(Fh, Fd)
     assert np.array_equal(Fd.cpu().numpy()[:, HOPPED:2 * HOPPED], m1[:, :HOPPED].astype(np.float32))
+
+
+def test_neighbor_mean_matches_numpy(gpu):
+    """pdx_neighbor_mean against a numpy restatement (omega V weighted mean
+    over each particle's intact bonds, the particle's own value when it has
+    none) on a two-case lattice batch with broken bonds: width 10 runs the
+    lattice kernel (k_nbr_mean_l), widths 20 and 4 the bond-list kernel."""
+    from simul_amd import workloads as W
+    from simul_amd.body import from_specimen
+    specs = [W.random_vertebra(81, dx=1.6), W.random_vertebra(82, dx=1.6)]
+    packed, off = W.pack(specs)
+    body = from_specimen(packed, case_offsets=off)
+    assert body.lattice_spacing() > 0  # the lattice kernel's context
+    row, col, br = body.bonds()
+    br = br.copy()
+    br[::5] = 1
+    body.set_broken(br)
+    n = packed.n
+    i = np.repeat(np.arange(n), np.diff(row))
+    keep = br == 0
+    i, j = i[keep], col[keep].astype(np.int64)
+    L = np.linalg.norm(packed.X[j] - packed.X[i], axis=1)
+    w = np.maximum(0.0, 1.0 - L / packed.horizon) * packed.vol[j]
+    ws = np.bincount(i, weights=w, minlength=n)
+    rng = np.random.default_rng(5)
+    for width in (10, 20, 4):
+        v = rng.standard_normal((n, width))
+        ref = np.stack([np.bincount(i, weights=w * v[j, q], minlength=n) for q in range(width)], 1)
+        ref = np.where(ws[:, None] > 0, ref / np.where(ws > 0, ws, 1.0)[:, None], v)
+        got = body.neighbor_mean(v)
+        assert np.allclose(got, ref, rtol=1e-12, atol=1e-12), (width, np.abs(got - ref).max())
 
 
 def test_load_view_of_prescribed_displacement_cases(gpu):
@@ -273,8 +311,9 @@ def test_device_raw_features_single_case(gpu, which):
 
 def test_device_features_refuse_bad_arguments(gpu):
     """pdx_warm_raw_features and pdx_neighbor_features_device check their
-    arguments before any launch: NULL arrays, host memory where device memory
-    is required, a non-positive spacing, widths and hops out of range."""
+    arguments before any launch: NULL moduli or output (NULL positions, loads
+    and DOF masks mean the context's own), host memory where device memory is
+    required, a non-positive spacing, widths and hops out of range."""
     import ctypes as C
 
     import torch
@@ -291,12 +330,14 @@ def test_device_features_refuse_bad_arguments(gpu):
     out = torch.zeros((n, 20), dtype=torch.float64, device="cuda")
     host = np.zeros((n, 64))
 
-    def raw(x, dx, dst):
-        body.ctx.call("pdx_warm_raw_features", x, N.ptr(E, N._dp), N.ptr(fext, N._dp), N.ptr(fixed, N._u8p),
+    def raw(x, dx, dst, e=N.ptr(E, N._dp)):
+        body.ctx.call("pdx_warm_raw_features", x, e, N.ptr(fext, N._dp), N.ptr(fixed, N._u8p),
                       None, dx, dst)
 
     with pytest.raises(N.FEAValidationError, match="NULL"):
-        raw(None, spec.dx, C.c_void_p(out.data_ptr()))
+        raw(N.ptr(X, N._dp), spec.dx, C.c_void_p(out.data_ptr()), e=None)
+    with pytest.raises(N.FEAValidationError, match="NULL"):
+        raw(N.ptr(X, N._dp), spec.dx, None)
     with pytest.raises(N.FEAValidationError, match="dx"):
         raw(N.ptr(X, N._dp), 0.0, C.c_void_p(out.data_ptr()))
     with pytest.raises(N.FEAValidationError, match="not device memory"):
