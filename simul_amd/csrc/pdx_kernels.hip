@@ -972,7 +972,7 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_fpass_l(Dev d, LatC lc, int u
 // Else lane q loads quarter q of the XRec and quarter min(q, 2) of the ARec;
 // lane 3 takes A_zz from lane 2 and c' from lane 1 by DPP.
 template <int NT, bool FR>
-__device__ __forceinline__ void lat_stage_g(const Dev &d, int ubuf, int lb, pdx_q4 *lds, double ih)
+__device__ __forceinline__ void lat_stage_g(const Dev &d, int ubuf, int lb, pdx_q4 *lds, double ih, int sb)
 {
     constexpr int NW = NT / 64, U = FR ? 8 : 2;
     const int h0 = d.hl_ptr[lb], nh = d.hl_ptr[lb + 1] - h0;
@@ -1001,7 +1001,7 @@ __device__ __forceinline__ void lat_stage_g(const Dev &d, int ubuf, int lb, pdx_
             for (int v = 0; v < U; ++v) r[v] = at32<pdx_q4>(Fq, (uint32_t)j[v] * 64u + (uint32_t)q * 16u);
 #pragma unroll
             for (int v = 0; v < U; ++v)
-                if (ok[v]) *reinterpret_cast<pdx_q4 *>(hb + sl[v] * 80 + q * 16) = r[v];
+                if (ok[v]) *reinterpret_cast<pdx_q4 *>(hb + sl[v] * sb + q * 16) = r[v];
             continue;
         }
 #pragma unroll
@@ -1030,7 +1030,7 @@ __device__ __forceinline__ void lat_stage_g(const Dev &d, int ubuf, int lb, pdx_
                 out = pdx_q4{z.x, z.y, azz, cc};
                 off = 16;
             }
-            if (ok[v]) *reinterpret_cast<pdx_q4 *>(hb + sl[v] * 80 + off) = out;
+            if (ok[v]) *reinterpret_cast<pdx_q4 *>(hb + sl[v] * sb + off) = out;
         }
     }
     __syncthreads();
@@ -1067,7 +1067,7 @@ __device__ __forceinline__ void lat_stab(const pdx_q4 &f0, const pdx_q4 &f1, con
 }
 
 template <int M>
-__device__ __forceinline__ void lat_gpair(const uint16_t *__restrict__ bp, const char *hb, const LatC &lc,
+__device__ __forceinline__ void lat_gpair(const uint16_t *__restrict__ bp, const char *hb, int sb, const LatC &lc,
                                           const double (&ui)[3], float ci, float (&fg)[3])
 {
     constexpr int K = 2 * M;
@@ -1076,7 +1076,7 @@ __device__ __forceinline__ void lat_gpair(const uint16_t *__restrict__ bp, const
     constexpr int o[3] = {o0, o1, o2};
     constexpr int n = o0 * o0 + o1 * o1 + o2 * o2;
     const unsigned ep = bp[K * PDX_SLICE], em = bp[(K + 1) * PDX_SLICE];
-    const char *rp = hb + __umul24(ep, 80u), *rm = hb + __umul24(em, 80u);
+    const char *rp = hb + __umul24(ep, (unsigned)sb), *rm = hb + __umul24(em, (unsigned)sb);
     const pdx_q4 p0 = *reinterpret_cast<const pdx_q4 *>(rp), p1 = *reinterpret_cast<const pdx_q4 *>(rp + 16);
     const pdx_q4 p2 = *reinterpret_cast<const pdx_q4 *>(rp + 32), p3 = *reinterpret_cast<const pdx_q4 *>(rp + 48);
     const pdx_q4 m0 = *reinterpret_cast<const pdx_q4 *>(rm), m1 = *reinterpret_cast<const pdx_q4 *>(rm + 16);
@@ -1097,43 +1097,45 @@ __device__ __forceinline__ void lat_gpair(const uint16_t *__restrict__ bp, const
 }
 
 template <int M>
-__device__ __forceinline__ void lat_gpair_if(const uint16_t *bp, const char *hb, const LatC &lc, const double (&ui)[3],
-                                             float ci, float (&fg)[3])
+__device__ __forceinline__ void lat_gpair_if(const uint16_t *bp, const char *hb, int sb, const LatC &lc,
+                                             const double (&ui)[3], float ci, float (&fg)[3])
 {
-    if constexpr (M < PDX_NST / 2) lat_gpair<M>(bp, hb, lc, ui, ci, fg);
+    if constexpr (M < PDX_NST / 2) lat_gpair<M>(bp, hb, sb, lc, ui, ci, fg);
 }
 
 // group g of part P: pairs P + (2 g + m) S, m < 2 (four bonds), summed in
 // FP32, the group sum added to the FP64 particle sum
 template <int S, int P, int GI, size_t... Q>
-__device__ __forceinline__ void lat_ggroup(const uint16_t *bp, const char *hb, const LatC &lc, const double (&ui)[3],
-                                           float ci, double (&fa)[3], std::index_sequence<Q...>)
+__device__ __forceinline__ void lat_ggroup(const uint16_t *bp, const char *hb, int sb, const LatC &lc,
+                                           const double (&ui)[3], float ci, double (&fa)[3],
+                                           std::index_sequence<Q...>)
 {
     float fg[3] = {0.f, 0.f, 0.f};
-    (lat_gpair_if<P + (GI * 2 + (int)Q) * S>(bp, hb, lc, ui, ci, fg), ...);
+    (lat_gpair_if<P + (GI * 2 + (int)Q) * S>(bp, hb, sb, lc, ui, ci, fg), ...);
 #pragma unroll
     for (int r = 0; r < 3; ++r) fa[r] += (double)fg[r];
 }
 
 template <int S, int P, size_t... GI>
-__device__ __forceinline__ void lat_gpart(const uint16_t *bp, const char *hb, const LatC &lc, const double (&ui)[3],
-                                          float ci, double (&fa)[3], std::index_sequence<GI...>)
+__device__ __forceinline__ void lat_gpart(const uint16_t *bp, const char *hb, int sb, const LatC &lc,
+                                          const double (&ui)[3], float ci, double (&fa)[3],
+                                          std::index_sequence<GI...>)
 {
-    (lat_ggroup<S, P, (int)GI>(bp, hb, lc, ui, ci, fa, std::make_index_sequence<2>{}), ...);
+    (lat_ggroup<S, P, (int)GI>(bp, hb, sb, lc, ui, ci, fa, std::make_index_sequence<2>{}), ...);
 }
 
 // the pairs of part P of S: P, P + S, ...
 template <int S, int P>
-__device__ __forceinline__ void lat_gpart_p(const uint16_t *bp, const char *hb, const LatC &lc, const double (&ui)[3],
-                                            float ci, double (&fa)[3])
+__device__ __forceinline__ void lat_gpart_p(const uint16_t *bp, const char *hb, int sb, const LatC &lc,
+                                            const double (&ui)[3], float ci, double (&fa)[3])
 {
     constexpr int np = (PDX_NST / 2 - P + S - 1) / S;
-    lat_gpart<S, P>(bp, hb, lc, ui, ci, fa, std::make_index_sequence<(np + 1) / 2>{});
+    lat_gpart<S, P>(bp, hb, sb, lc, ui, ci, fa, std::make_index_sequence<(np + 1) / 2>{});
 }
 
 template <int UPD, int S, bool FR>
 __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_l(Dev d, LatC lc, int ubuf, int solve_mode, double dt,
-                                                           double visc)
+                                                           double visc, int lds_bytes)
 {
     extern __shared__ pdx_q4 hlds[];
     const int lb = logical_block();
@@ -1141,12 +1143,16 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_l(Dev d, LatC lc, int u
     if (solve_mode && d.sc[cs].converged) return;
     double red[3] = {0.0, 0.0, 0.0};
     if (lb * PDX_BLOCK < d.ni) {  // grid padding blocks only join the reduction
-        lat_stage_g<PDX_BLOCK * S, FR>(d, ubuf, lb, hlds, lc.inv_h);
+        // the staged records' stride: 80 bytes (bank-coloured slots on distinct
+        // banks), 64 for a block whose halo takes more LDS than that
+        const int nslot = ((d.hl_ptr[lb + 1] - d.hl_ptr[lb]) + 15) & ~15;
+        const int sb = nslot * 80 <= lds_bytes ? 80 : 64;
+        lat_stage_g<PDX_BLOCK * S, FR>(d, ubuf, lb, hlds, lc.inv_h, sb);
         const char *hb = reinterpret_cast<const char *>(hlds);
         const int t = threadIdx.x % PDX_BLOCK, part = threadIdx.x / PDX_BLOCK;
         const int i = lb * PDX_BLOCK + t;
         const int so = d.hs[d.hl_ptr[lb] + t];
-        const char *rp = hb + so * 80;
+        const char *rp = hb + so * sb;
         const pdx_q4 f0 = *reinterpret_cast<const pdx_q4 *>(rp);
         const pdx_q4 f1 = *reinterpret_cast<const pdx_q4 *>(rp + 16);
         const pdx_q4 f2 = *reinterpret_cast<const pdx_q4 *>(rp + 32);
@@ -1157,13 +1163,13 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_l(Dev d, LatC lc, int u
         const float ci = f1.w;
         const uint16_t *__restrict__ bp = d.bks + lat_at(i, 0);
         double fa[3] = {0.0, 0.0, 0.0};
-        if (part == 0) lat_gpart_p<S, 0>(bp, hb, lc, ui, ci, fa);
+        if (part == 0) lat_gpart_p<S, 0>(bp, hb, sb, lc, ui, ci, fa);
         if constexpr (S > 1) {
-            if (part == 1) lat_gpart_p<S, 1>(bp, hb, lc, ui, ci, fa);
+            if (part == 1) lat_gpart_p<S, 1>(bp, hb, sb, lc, ui, ci, fa);
         }
         if constexpr (S > 2) {
-            if (part == 2) lat_gpart_p<S, 2>(bp, hb, lc, ui, ci, fa);
-            if (part == 3) lat_gpart_p<S, 3>(bp, hb, lc, ui, ci, fa);
+            if (part == 2) lat_gpart_p<S, 2>(bp, hb, sb, lc, ui, ci, fa);
+            if (part == 3) lat_gpart_p<S, 3>(bp, hb, sb, lc, ui, ci, fa);
         }
         // part 0 loads the update's inputs while the other parts end
         const KRec kr = d.KR[i];
@@ -1674,10 +1680,18 @@ static void launch_fpass_g(pdx_ctx *c, int ubuf, bool diag, bool commit, int sm,
 }
 
 // LDS bytes of the lattice force pass with split S, or 0 when it cannot run
+// dynamic LDS of the lattice force pass (0: it cannot run).  Its staged
+// records take 64 bytes; a block lays them 80 bytes apart (the 16 lanes of a
+// read then meet distinct banks, with the bank-coloured slots) when its halo
+// fits that way, else 64 apart (4-way bank conflicts; a block whose rows
+// reach more than about 2 000 particles), so the launch takes the 80-byte
+// size of the widest halo, or the most the device gives when only 64 bytes
+// per record fit it.
 static int lat_force_lds(const pdx_ctx *c, int split)
 {
     if (!c->have_lat || !c->d.bks || c->dim != 3 || c->form != PDX_FORM_NOSB) return 0;
-    return pdx_halo_lds(c, 80, split, 3);
+    if (const int lds = pdx_halo_lds(c, 80, split, 3)) return lds;
+    return pdx_halo_lds(c, 64, split, 3) ? c->lds_dyn_max : 0;
 }
 
 // the force pass's variant is a lattice variant that runs
@@ -1837,6 +1851,18 @@ static void launch_force_g(pdx_ctx *c, int ubuf, int upd, int sm, double dt, dou
     }
 }
 
+template <int S, bool FR>
+static void launch_force_l(pdx_ctx *c, int lds, int ubuf, int upd, int sm, double dt, double visc)
+{
+    const dim3 g(c->nblk), bh(PDX_BLOCK * S);
+    if (upd == 1)
+        hipLaunchKernelGGL((k_force_l<1, S, FR>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc, lds);
+    else if (upd == 2)
+        hipLaunchKernelGGL((k_force_l<2, S, FR>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc, lds);
+    else
+        hipLaunchKernelGGL((k_force_l<0, S, FR>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc, lds);
+}
+
 // the lattice force pass with rows split over S waves; false when it cannot
 // run (no lattice tables, 2-D, the bond form, or a halo the device's LDS does
 // not take)
@@ -1848,23 +1874,11 @@ static bool launch_force_lat(pdx_ctx *c, int ubuf, int upd, int sm, double dt, d
     } else {
         const int lds = lat_force_lds(c, S);
         if (!lds) return false;
-        const dim3 g(c->nblk), bh(PDX_BLOCK * S);
         // the records of the lattice F-pass that ran last, on this buffer
-        if (c->fr_ubuf == ubuf) {
-            if (upd == 1)
-                hipLaunchKernelGGL((k_force_l<1, S, true>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc);
-            else if (upd == 2)
-                hipLaunchKernelGGL((k_force_l<2, S, true>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc);
-            else
-                hipLaunchKernelGGL((k_force_l<0, S, true>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc);
-        } else {
-            if (upd == 1)
-                hipLaunchKernelGGL((k_force_l<1, S, false>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc);
-            else if (upd == 2)
-                hipLaunchKernelGGL((k_force_l<2, S, false>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc);
-            else
-                hipLaunchKernelGGL((k_force_l<0, S, false>), g, bh, lds, c->stream, c->d, c->lat, ubuf, sm, dt, visc);
-        }
+        if (c->fr_ubuf == ubuf)
+            launch_force_l<S, true>(c, lds, ubuf, upd, sm, dt, visc);
+        else
+            launch_force_l<S, false>(c, lds, ubuf, upd, sm, dt, visc);
         return true;
     }
 }

@@ -219,3 +219,34 @@ def test_autotune_considers_lattice(gpu):
     f, g = body.variants()
     ok = {1, 2, 4, 16 | 1, 16 | 2} | {32 | s | q for s in (0, 64, 128) for q in (1, 2, 4)} | set(LAT)
     assert f in ok and g in ok
+
+
+def test_lattice_force_pass_with_a_wide_halo(gpu):
+    """A lattice body one of whose blocks reaches more particles than the
+    80-byte staged records fit in LDS (a random J2 vertebral body at dx 0.75,
+    2 170 slots in its widest block) still runs the lattice force pass, with
+    64-byte records: its forces match the gathering kernels (2e-6) for both
+    the F-pass's records and the force pass's own staging, and a few adaptive
+    DR iterations agree."""
+    import ctypes as C
+
+    from simul_amd import _native as NN
+    from simul_amd.body import from_specimen
+    spec = W.random_vertebra_j2(105229, dx=0.75)
+    body = from_specimen(spec)
+    assert abs(body.lattice_spacing() - 0.75) < 1e-12
+    sizes = np.zeros(5, np.int64)
+    body.ctx.call("pdx_get_halo_tables", NN.ptr(sizes, NN._i64p), None, None, None, None, None, None)
+    cap = int(sizes[4])
+    assert cap * 80 > 160 * 1024 - 1024 >= cap * 64, cap  # only 64-byte records fit
+    u = smooth_u(spec.X, amp=2e-3, seed=3)
+    u[spec.fixed != 0] = spec.u0[spec.fixed != 0]
+    body.set_variants(1, 1)
+    g0 = _pass_forces(body, u).copy()
+    f0 = body.compute_forces().copy()
+    for v in LAT:
+        body.set_variants(v, v)
+        assert rel_l2(_pass_forces(body, u), g0) < 2e-6, v
+        body.set_variants(HALO_REF, v)  # the force pass stages the records itself
+        assert rel_l2(_pass_forces(body, u), g0) < 2e-6, v
+        assert rel_l2(body.compute_forces(), f0) < 2e-6, v
