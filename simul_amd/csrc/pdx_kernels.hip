@@ -1048,7 +1048,7 @@ __device__ __forceinline__ void lat_stage_g(const Dev &d, int ubuf, int lb, pdx_
 // drop its stabilisation term instead).
 template <int O0, int O1, int O2, int N>
 __device__ __forceinline__ void lat_stab(const pdx_q4 &f0, const pdx_q4 &f1, const LatC &lc, const double (&ui)[3],
-                                         float ci, float (&fg)[3])
+                                         const float (&ciw)[10], float (&fg)[3])
 {
     constexpr int o[3] = {O0, O1, O2};
     constexpr float L = lat_len(N);
@@ -1060,15 +1060,15 @@ __device__ __forceinline__ void lat_stab(const pdx_q4 &f0, const pdx_q4 &f1, con
     for (int c = 0; c < 3; ++c) dn = fmaf(du[c], o[c] != 0 ? du[c] + (float)(2 * o[c]) : du[c], dn);
     const float E2 = fmaxf(dn + (float)N, 1e-12f);
     const float el = bond_sqrt(E2);
-    const float gam = dn * bond_rcp((el + L) * el);
-    const float wg = (ci + f1.w) * lc.wl[N] * gam;
+    const float gam = dn * bond_rcp(fmaf(L, el, E2));          // dn / ((|eta| + L) |eta|)
+    const float wg = fmaf(f1.w, lc.wl[N], ciw[N]) * gam;       // (c'_i + c'_j) w0 / sqrt(n)
 #pragma unroll
     for (int r = 0; r < 3; ++r) fg[r] = fmaf(wg, o[r] != 0 ? du[r] + (float)o[r] : du[r], fg[r]);
 }
 
 template <int M>
 __device__ __forceinline__ void lat_gpair(const uint16_t *__restrict__ bp, const char *hb, int sb, const LatC &lc,
-                                          const double (&ui)[3], float ci, float (&fg)[3])
+                                          const double (&ui)[3], const float (&ciw)[10], float (&fg)[3])
 {
     constexpr int K = 2 * M;
     constexpr int o0 = kLat.o[K][0], o1 = kLat.o[K][1], o2 = kLat.o[K][2];
@@ -1081,8 +1081,8 @@ __device__ __forceinline__ void lat_gpair(const uint16_t *__restrict__ bp, const
     const pdx_q4 p2 = *reinterpret_cast<const pdx_q4 *>(rp + 32), p3 = *reinterpret_cast<const pdx_q4 *>(rp + 48);
     const pdx_q4 m0 = *reinterpret_cast<const pdx_q4 *>(rm), m1 = *reinterpret_cast<const pdx_q4 *>(rm + 16);
     const pdx_q4 m2 = *reinterpret_cast<const pdx_q4 *>(rm + 32), m3 = *reinterpret_cast<const pdx_q4 *>(rm + 48);
-    lat_stab<o0, o1, o2, n>(p0, p1, lc, ui, ci, fg);
-    lat_stab<-o0, -o1, -o2, n>(m0, m1, lc, ui, ci, fg);
+    lat_stab<o0, o1, o2, n>(p0, p1, lc, ui, ciw, fg);
+    lat_stab<-o0, -o1, -o2, n>(m0, m1, lc, ui, ciw, fg);
     const float Ap[9] = {p2.x, p2.y, p2.z, p2.w, p3.x, p3.y, p3.z, p3.w, p1.z};
     const float Am[9] = {m2.x, m2.y, m2.z, m2.w, m3.x, m3.y, m3.z, m3.w, m1.z};
 #pragma unroll
@@ -1098,39 +1098,39 @@ __device__ __forceinline__ void lat_gpair(const uint16_t *__restrict__ bp, const
 
 template <int M>
 __device__ __forceinline__ void lat_gpair_if(const uint16_t *bp, const char *hb, int sb, const LatC &lc,
-                                             const double (&ui)[3], float ci, float (&fg)[3])
+                                             const double (&ui)[3], const float (&ciw)[10], float (&fg)[3])
 {
-    if constexpr (M < PDX_NST / 2) lat_gpair<M>(bp, hb, sb, lc, ui, ci, fg);
+    if constexpr (M < PDX_NST / 2) lat_gpair<M>(bp, hb, sb, lc, ui, ciw, fg);
 }
 
 // group g of part P: pairs P + (2 g + m) S, m < 2 (four bonds), summed in
 // FP32, the group sum added to the FP64 particle sum
 template <int S, int P, int GI, size_t... Q>
 __device__ __forceinline__ void lat_ggroup(const uint16_t *bp, const char *hb, int sb, const LatC &lc,
-                                           const double (&ui)[3], float ci, double (&fa)[3],
+                                           const double (&ui)[3], const float (&ciw)[10], double (&fa)[3],
                                            std::index_sequence<Q...>)
 {
     float fg[3] = {0.f, 0.f, 0.f};
-    (lat_gpair_if<P + (GI * 2 + (int)Q) * S>(bp, hb, sb, lc, ui, ci, fg), ...);
+    (lat_gpair_if<P + (GI * 2 + (int)Q) * S>(bp, hb, sb, lc, ui, ciw, fg), ...);
 #pragma unroll
     for (int r = 0; r < 3; ++r) fa[r] += (double)fg[r];
 }
 
 template <int S, int P, size_t... GI>
 __device__ __forceinline__ void lat_gpart(const uint16_t *bp, const char *hb, int sb, const LatC &lc,
-                                          const double (&ui)[3], float ci, double (&fa)[3],
+                                          const double (&ui)[3], const float (&ciw)[10], double (&fa)[3],
                                           std::index_sequence<GI...>)
 {
-    (lat_ggroup<S, P, (int)GI>(bp, hb, sb, lc, ui, ci, fa, std::make_index_sequence<2>{}), ...);
+    (lat_ggroup<S, P, (int)GI>(bp, hb, sb, lc, ui, ciw, fa, std::make_index_sequence<2>{}), ...);
 }
 
 // the pairs of part P of S: P, P + S, ...
 template <int S, int P>
 __device__ __forceinline__ void lat_gpart_p(const uint16_t *bp, const char *hb, int sb, const LatC &lc,
-                                            const double (&ui)[3], float ci, double (&fa)[3])
+                                            const double (&ui)[3], const float (&ciw)[10], double (&fa)[3])
 {
     constexpr int np = (PDX_NST / 2 - P + S - 1) / S;
-    lat_gpart<S, P>(bp, hb, sb, lc, ui, ci, fa, std::make_index_sequence<(np + 1) / 2>{});
+    lat_gpart<S, P>(bp, hb, sb, lc, ui, ciw, fa, std::make_index_sequence<(np + 1) / 2>{});
 }
 
 template <int UPD, int S, bool FR>
@@ -1160,16 +1160,19 @@ __global__ __launch_bounds__(PDX_BLOCK * S) void k_force_l(Dev d, LatC lc, int u
         const PdxD2 u01 = __builtin_bit_cast(PdxD2, f0);
         const double ui[3] = {u01.a, u01.b, __builtin_bit_cast(double, f1.xy)};
         const float Ai[9] = {f2.x, f2.y, f2.z, f2.w, f3.x, f3.y, f3.z, f3.w, f1.z};
-        const float ci = f1.w;
+        // c'_i w0(n) / sqrt(n) of each stencil shell n
+        float ciw[10];
+#pragma unroll
+        for (int n = 0; n < 10; ++n) ciw[n] = f1.w * lc.wl[n];
         const uint16_t *__restrict__ bp = d.bks + lat_at(i, 0);
         double fa[3] = {0.0, 0.0, 0.0};
-        if (part == 0) lat_gpart_p<S, 0>(bp, hb, sb, lc, ui, ci, fa);
+        if (part == 0) lat_gpart_p<S, 0>(bp, hb, sb, lc, ui, ciw, fa);
         if constexpr (S > 1) {
-            if (part == 1) lat_gpart_p<S, 1>(bp, hb, sb, lc, ui, ci, fa);
+            if (part == 1) lat_gpart_p<S, 1>(bp, hb, sb, lc, ui, ciw, fa);
         }
         if constexpr (S > 2) {
-            if (part == 2) lat_gpart_p<S, 2>(bp, hb, sb, lc, ui, ci, fa);
-            if (part == 3) lat_gpart_p<S, 3>(bp, hb, sb, lc, ui, ci, fa);
+            if (part == 2) lat_gpart_p<S, 2>(bp, hb, sb, lc, ui, ciw, fa);
+            if (part == 3) lat_gpart_p<S, 3>(bp, hb, sb, lc, ui, ciw, fa);
         }
         // part 0 loads the update's inputs while the other parts end
         const KRec kr = d.KR[i];
