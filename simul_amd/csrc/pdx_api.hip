@@ -63,12 +63,16 @@ static void dfree(T *&p)
     p = nullptr;
 }
 
+static int build_lam_g(pdx_ctx *c);
+
 static void free_all(pdx_ctx *c)
 {
     Dev &d = c->d;
     dfree(d.xu[0]); dfree(d.xu[1]); dfree(d.KR); dfree(d.AR); dfree(d.j2p); dfree(d.j2s);
     dfree(d.v); dfree(d.fp); dfree(d.f); dfree(d.fext);
     dfree(d.mass); dfree(d.lam); dfree(d.V64); dfree(d.K64); dfree(d.Kinv64);
+    dfree(c->lam_g);
+    c->have_lam_g = false;
     dfree(d.Fd); dfree(d.Pd); dfree(d.damage); dfree(d.init_nb);
     dfree(d.sl_ptr); dfree(d.bj); dfree(d.case_of_blk); dfree(d.case_blk0);
     dfree(d.hl_ptr); dfree(d.hl); dfree(d.hs); dfree(d.bl); dfree(d.bk); dfree(d.anchor); dfree(d.m64);
@@ -517,7 +521,7 @@ extern "C" int pdx_set_particles(pdx_ctx *c, const double *X, const double *vol,
     PDX_HIP(hipMemsetAsync(c->d.KR, 0, sizeof(KRec) * c->ni, c->stream));
     c->ucur = 0;
     c->have_particles = true;
-    c->have_bonds = c->have_K = c->have_mass = c->have_A = false;
+    c->have_bonds = c->have_K = c->have_mass = c->have_A = c->have_lam_g = false;
     invalidate_graph(c);
     PDX_TRY(push_flags(c));
     return PDX_OK;
@@ -584,6 +588,7 @@ extern "C" int pdx_set_stabilization(pdx_ctx *c, double G_s, const double *cst, 
     c->G_s = G_s;
     c->d.G_s = G_s;
     c->have_mass = false;
+    c->have_lam_g = false;
     invalidate_graph(c);
     return r;
 }
@@ -595,6 +600,7 @@ extern "C" int pdx_set_formulation(pdx_ctx *c, int32_t form)
     c->form = form;
     c->d.form = form;
     c->have_mass = false;
+    c->have_lam_g = false;
     invalidate_graph(c);
     return PDX_OK;
 }
@@ -928,7 +934,7 @@ extern "C" int pdx_build_bonds(pdx_ctx *c, double horizon, int64_t *n_bonds)
     PDX_TRY(build_lattice(c));
     c->horizon = horizon;
     c->have_bonds = true;
-    c->have_K = c->have_mass = c->have_A = false;
+    c->have_K = c->have_mass = c->have_A = c->have_lam_g = false;
     invalidate_graph(c);
     if (n_bonds) *n_bonds = tot;
     return PDX_OK;
@@ -1035,6 +1041,7 @@ extern "C" int pdx_set_broken(pdx_ctx *c, const uint8_t *broken)
     PDX_TRY(pdx_launch_halo_bits(c));
     PDX_HIP(hipStreamSynchronize(c->stream));
     c->have_mass = false;
+    c->have_lam_g = false;
     return PDX_OK;
 }
 
@@ -1084,7 +1091,10 @@ extern "C" int pdx_shape_tensor(pdx_ctx *c, int32_t mode, double tol)
     PDX_HIP(hipStreamSynchronize(c->stream));
     c->have_K = true;
     c->have_mass = false;
-    return PDX_OK;
+    // the Gershgorin mass is fixed from here until the geometry, the moduli
+    // or the bonds change: built now, with the set-up, so that every solve's
+    // mass (and the spectral tuning, which starts from it) reads the cache
+    return build_lam_g(c);
 }
 
 static int need_K(pdx_ctx *c)
@@ -1205,7 +1215,7 @@ extern "C" int pdx_break_bonds(pdx_ctx *c, double s_c, int64_t *n_new)
     }
     hipFree(cnt);
     if (n_new) *n_new = (int64_t)h;
-    if (h) c->have_mass = false;
+    if (h) c->have_mass = c->have_lam_g = false;
     return r;
 }
 
@@ -1445,12 +1455,23 @@ extern "C" int pdx_estimate_dt(pdx_ctx *c, double bulk, double shear, double rho
     return PDX_OK;
 }
 
+// the Gershgorin mass's raw values into the context's cache
+static int build_lam_g(pdx_ctx *c)
+{
+    if (!c->lam_g) PDX_TRY(dalloc(c->stream, &c->lam_g, (size_t)c->ni));
+    PDX_TRY(pdx_launch_adr_mass(c, 1.0, c->lam_g));
+    PDX_HIP(hipStreamSynchronize(c->stream));
+    c->have_lam_g = true;
+    return PDX_OK;
+}
+
 extern "C" int pdx_adr_mass(pdx_ctx *c, double beta)
 {
     CHECK_CTX(c);
     PDX_TRY(need_K(c));
     if (!(beta > 0.0)) return fail(PDX_EINVAL, "beta must be positive");
-    PDX_TRY(pdx_launch_adr_mass(c, beta));
+    if (!c->have_lam_g) PDX_TRY(build_lam_g(c));
+    PDX_TRY(pdx_launch_mass_scale(c, c->lam_g, beta));
     PDX_HIP(hipStreamSynchronize(c->stream));
     c->have_mass = true;
     return PDX_OK;
@@ -1794,7 +1815,7 @@ extern "C" int pdx_explicit_run(pdx_ctx *c, double dt, double damping, int64_t n
     if (cnt) hipFree(cnt);
     if (r != PDX_OK) return r;
     if (n_broken) *n_broken = (int64_t)h;
-    if (h) c->have_mass = false;
+    if (h) c->have_mass = c->have_lam_g = false;
     if (ke)
         for (int k = 0; k < c->ncase; ++k) ke[k] = n_steps > 0 ? c->h_sc[k].num : 0.0;
     c->have_A = n_steps > 0;
@@ -2087,7 +2108,7 @@ extern "C" int pdx_set(pdx_ctx *c, int32_t field, const void *in)
             else m.rho[r] = x[r];
         }
         if (field == PDX_FIELD_VOLUME) {
-            c->have_K = c->have_mass = false;
+            c->have_K = c->have_mass = c->have_lam_g = false;
             invalidate_graph(c);
             PDX_TRY(push_prec(c));
             // the lattice kernels take one volume for every particle
@@ -2110,7 +2131,7 @@ extern "C" int pdx_set(pdx_ctx *c, int32_t field, const void *in)
         }
         m.bulk = b;
         m.shear = s;
-        c->have_mass = false;
+        c->have_mass = c->have_lam_g = false;
         return kr_put(c, w, KR_OFF_BULK, 2);
     }
     case PDX_FIELD_CSTAB: return pdx_set_stabilization(c, c->G_s, x, 0.0);

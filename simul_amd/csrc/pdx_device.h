@@ -1157,11 +1157,15 @@ PDX_HD void adr_mass_a_one(const Dev &d, int i, double *cs_out, double *g_out)
 }
 
 template <int D>
-PDX_HD void adr_mass_b_one(const Dev &d, int i, const double *cs_in, const double *g_in, double beta)
+PDX_HD void adr_mass_b_one(const Dev &d, int i, const double *cs_in, const double *g_in, double beta,
+                           double *raw = nullptr)
 {
     const KRec kr = d.KR[i];
     if (!(kr.flags & PDX_FL_ACTIVE)) {
-        d.lam[i] = 1.0;
+        if (raw)
+            raw[i] = NAN;  // the scaled mass of an inactive row is 1 whatever beta
+        else
+            d.lam[i] = 1.0;
         return;
     }
     const XRec pi = d.xu[0][i];
@@ -1194,7 +1198,21 @@ PDX_HD void adr_mass_b_one(const Dev &d, int i, const double *cs_in, const doubl
     }
     const double gs = nosb ? d.G_s : 1.0;
     const double r3 = 1.7320508075688772;
+    if (raw) {
+        // the Gershgorin mass before beta: beta * raw rounds as 0.25 beta (...) does
+        raw[i] = 0.25 * (r3 * kc + 2.0 * r3 * gs * ks);
+        return;
+    }
     double lam = 0.25 * beta * (r3 * kc + 2.0 * r3 * gs * ks);
+    if (!(lam > 0.0)) lam = 1.0;
+    d.lam[i] = lam;
+}
+
+// the adaptive-DR mass beta x the cached Gershgorin mass (adr_mass_b_one's raw
+// values), 1 where that is not positive, as adr_mass_b_one computes it
+PDX_HD void mass_scale_one(const Dev &d, int i, const double *raw, double beta)
+{
+    double lam = beta * raw[i];
     if (!(lam > 0.0)) lam = 1.0;
     d.lam[i] = lam;
 }

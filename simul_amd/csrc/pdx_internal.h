@@ -24,6 +24,11 @@ struct pdx_ctx {
     std::vector<double> hX;            // caller-order reference positions
     bool have_particles = false, have_bonds = false, have_K = false, have_mass = false;
     bool have_A = false;
+    // the Gershgorin adaptive-DR mass before beta (adr_mass_b_one's raw
+    // values): a function of positions, volumes, moduli, K^-1 and the intact
+    // bonds, kept until one of them changes (wherever have_mass is cleared)
+    double *lam_g = nullptr;
+    bool have_lam_g = false;
     bool have_m = false;  // d.m64 holds the halo F-pass's sum_j wx for the current bonds
     bool layout_host = false;  // build the halo tables on the host (pdx_set_layout_builder)
     int32_t model = PDX_MODEL_LINEAR_ELASTIC, form = PDX_FORM_NOSB;
@@ -128,7 +133,8 @@ int pdx_launch_raw_features(pdx_ctx *c, const PdxRawIn &in, const int64_t *d_chu
                             double *d_cc, double *d_out);
 int pdx_launch_feat_pack(pdx_ctx *c, const double *d_hv, const int32_t *d_perm, float *d_out, int width, int hops,
                          int ldo);
-int pdx_launch_adr_mass(pdx_ctx *c, double beta);
+int pdx_launch_adr_mass(pdx_ctx *c, double beta, double *raw = nullptr);
+int pdx_launch_mass_scale(pdx_ctx *c, const double *raw, double beta);
 int pdx_launch_dt_est(pdx_ctx *c, double modulus, double *d_out);
 int pdx_launch_damage(pdx_ctx *c, double s_c, unsigned long long *d_count);
 int pdx_launch_local_damage(pdx_ctx *c);

@@ -1509,8 +1509,9 @@ __global__ __launch_bounds__(PDX_BLOCK) void k_resid(Dev d)
 template <int D, bool FILL> PDX_PER_PARTICLE(k_neighbors(Dev d), (neighbors_one<D, FILL>(d, i)))
 template <int D> PDX_PER_PARTICLE(k_shape(Dev d, int mode, double tol), (shape_one<D>(d, i, mode, tol)))
 template <int D> PDX_PER_PARTICLE(k_adr_mass_a(Dev d, double *cs, double *g), (adr_mass_a_one<D>(d, i, cs, g)))
-template <int D> PDX_PER_PARTICLE(k_adr_mass_b(Dev d, const double *cs, const double *g, double beta),
-                                  (adr_mass_b_one<D>(d, i, cs, g, beta)))
+template <int D> PDX_PER_PARTICLE(k_adr_mass_b(Dev d, const double *cs, const double *g, double beta, double *raw),
+                                  (adr_mass_b_one<D>(d, i, cs, g, beta, raw)))
+PDX_PER_PARTICLE(k_mass_scale(Dev d, const double *raw, double beta), (mass_scale_one(d, i, raw, beta)))
 template <int D> PDX_PER_PARTICLE(k_dt_est(Dev d, double modulus, double *out), (dt_est_one<D>(d, i, modulus, out)))
 template <int D> PDX_PER_PARTICLE(k_bond_data(Dev d, double *xi, double *len, double *w, const int64_t *rowi),
                                   (bond_data_one<D>(d, i, xi, len, w, rowi)))
@@ -1971,20 +1972,29 @@ int pdx_launch_kd_move(pdx_ctx *c, int ubuf_in, bool solve_mode, double dt, bool
     return PDX_OK;
 }
 
-int pdx_launch_adr_mass(pdx_ctx *c, double beta)
+// the Gershgorin mass times beta into d.lam, or (raw != null) its values
+// before beta into raw
+int pdx_launch_adr_mass(pdx_ctx *c, double beta, double *raw)
 {
     double *tmp = nullptr;
     PDX_HIP(hipMallocAsync((void **)&tmp, sizeof(double) * 2 * (size_t)c->ni, c->stream));
     dim3 g(grid1(c->ni)), b(PDX_BLOCK);
     if (c->dim == 3) {
         hipLaunchKernelGGL((k_adr_mass_a<3>), g, b, 0, c->stream, c->d, tmp, tmp + c->ni);
-        hipLaunchKernelGGL((k_adr_mass_b<3>), g, b, 0, c->stream, c->d, tmp, tmp + c->ni, beta);
+        hipLaunchKernelGGL((k_adr_mass_b<3>), g, b, 0, c->stream, c->d, tmp, tmp + c->ni, beta, raw);
     } else {
         hipLaunchKernelGGL((k_adr_mass_a<2>), g, b, 0, c->stream, c->d, tmp, tmp + c->ni);
-        hipLaunchKernelGGL((k_adr_mass_b<2>), g, b, 0, c->stream, c->d, tmp, tmp + c->ni, beta);
+        hipLaunchKernelGGL((k_adr_mass_b<2>), g, b, 0, c->stream, c->d, tmp, tmp + c->ni, beta, raw);
     }
     PDX_LAUNCH_CHECK();
     PDX_HIP(hipFreeAsync(tmp, c->stream));
+    return PDX_OK;
+}
+
+int pdx_launch_mass_scale(pdx_ctx *c, const double *raw, double beta)
+{
+    hipLaunchKernelGGL(k_mass_scale, dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d, raw, beta);
+    PDX_LAUNCH_CHECK();
     return PDX_OK;
 }
 
