@@ -128,8 +128,9 @@ def main():
                     help="run only a plain 256 MiB device copy (scales rocprof FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--headline-only", action="store_true",
                     help="measure only the dx 0.4 vertebral body (no J2-loading and batch blocks)")
-    ap.add_argument("--weights", default=os.path.join(ROOT, "simul_amd", "learned", "weights_vertebra.npz"),
-                    help="the warm start's network weights")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "simul_amd", "learned", "weights_vertebra_mc.npz"),
+                    help="the headline warm start's network weights (default: the network trained on the headline's "
+                         "shape and load with 512 other cancellous fields)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -289,7 +290,8 @@ def main():
                         "warm_speedup": round(wall / tw["total_s"], 3),
                         "warm_iteration_ratio": round(rs.iterations / max(rw.iterations, 1), 3),
                         "warm_vs_cold_u_rel_l2": du,
-                        "warm_start": "learned per-particle network (BF16 MFMA kernels), features, inference and "
+                        "warm_start": "learned per-particle network (BF16 MFMA kernels; "
+                                      + os.path.basename(wfile) + "), features, inference and "
                                       "DR included in warm_start_s; one untimed inference before it loads the "
                                       "process's GPU code for that path"})
             log(f"warm start: {tw['total_s']:.3f}s ({tw['inference_s']:.3f}s features+inference), "

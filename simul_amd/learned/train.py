@@ -37,17 +37,23 @@ def _group_body(specs):
     return packed, off, body
 
 
-FAMILIES = ("pressure", "j2")
+FAMILIES = ("pressure", "j2", "vertebra", "vertebra_mc")
 
 
 def _specimen(family, seed, dx):
     """One case of a training / held-out family: "pressure" is
     workloads.random_vertebra (Neo-Hookean, pressure on the top layer), "j2"
     workloads.random_vertebra_j2 (J2, compressed between platens in one
-    increment)."""
-    from ..workloads import random_vertebra, random_vertebra_j2
+    increment), "vertebra" workloads.vertebra_family (the headline's shape,
+    random cancellous field and load), "vertebra_mc" the same with the
+    headline's load (only the cancellous field random)."""
+    from ..workloads import random_vertebra, random_vertebra_j2, vertebra_family
     if family == "j2":
         return random_vertebra_j2(int(seed), dx=dx)
+    if family == "vertebra":
+        return vertebra_family(int(seed), dx=dx)
+    if family == "vertebra_mc":
+        return vertebra_family(int(seed), dx=dx, fixed_load=True)
     if family == "pressure":
         return random_vertebra(int(seed), dx=dx)
     raise ValueError(f"unknown family {family!r}")
@@ -192,7 +198,9 @@ def main(argv=None):
                     help="adaptive-DR mass of the training-data solves")
     ap.add_argument("--hops", type=int, default=2, help="neighbour-mean hops of the input features (0..4)")
     ap.add_argument("--family", default="pressure", choices=FAMILIES,
-                    help="pressure: Neo-Hookean bodies under a top pressure; j2: J2 bodies compressed between platens")
+                    help="pressure: Neo-Hookean bodies under a top pressure; j2: J2 bodies compressed between platens; "
+                         "vertebra: the headline's shape with random cancellous fields and loads; "
+                         "vertebra_mc: the headline's shape and load with random cancellous fields")
     ap.add_argument("--ref-tol", type=float, default=None,
                     help="also solve each held-out group cold to this tolerance and report both solutions' distance")
     ap.add_argument("--out", default=None)

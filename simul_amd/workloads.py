@@ -11,9 +11,10 @@
 * vertebra_j2: the same geometry with J2 plasticity under a prescribed 2 %
   endplate compression in 20 increments, bonds breaking at stretch 0.02.
 * batch: several random box / cylinder / truncated-ellipsoid specimens at dx = 0.5 mm.
-* random_vertebra / random_vertebra_j2: the learned warm start's families of
-  random vertebral bodies (pressure-loaded Neo-Hookean; J2 compressed between
-  platens in one increment).
+* random_vertebra / random_vertebra_j2 / vertebra_family: the learned warm
+  start's families of vertebral bodies (random shapes, pressure-loaded
+  Neo-Hookean; random shapes, J2 compressed between platens in one
+  increment; the headline's shape with random cancellous fields and loads).
 
 The lateral distance to the ellipse uses the first-order estimate
 (1 - r) / |grad r| with r = sqrt(x^2/a^2 + y^2/b^2).
@@ -239,6 +240,25 @@ def random_vertebra(seed, dx=0.8, p_range=(0.5, 3.0), cone_deg=20.0, model="neo_
     spec = vertebra(dx=dx, a=a, b=b, height=h, pressure=p, seed=int(rng.integers(1 << 31)), model=model,
                     direction=d)
     spec.name = f"rvert{seed}_dx{dx}"
+    return spec
+
+
+def vertebra_family(seed, dx=0.4, p_range=(0.5, 3.0), cone_deg=20.0, model="neo_hookean", fixed_load=False):
+    """The headline vertebral body's geometry (20 x 15 x 25 mm) with a random
+    cancellous field and a pressure of random magnitude and direction within
+    cone_deg of the axis: the family of the headline's learned warm start
+    (the headline itself is seed 2000 at 2 MPa along -z).  fixed_load: the
+    headline's own load (2 MPa along -z), only the cancellous field random
+    (a Monte Carlo study over the material)."""
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(*p_range)
+    th = math.radians(cone_deg) * math.sqrt(rng.uniform())
+    ph = rng.uniform(0, 2 * math.pi)
+    d = -np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+    if fixed_load:
+        p, d = 2.0, None
+    spec = vertebra(dx=dx, pressure=p, seed=int(rng.integers(1 << 31)), model=model, direction=d)
+    spec.name = f"vfam{seed}_dx{dx}"
     return spec
 
 
