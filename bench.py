@@ -130,7 +130,7 @@ def main():
                     help="measure only the dx 0.4 vertebral body (no J2-loading and batch blocks)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "simul_amd", "learned", "weights_vertebra_mc.npz"),
                     help="the headline warm start's network weights (default: the network trained on the headline's "
-                         "shape and load with 512 other cancellous fields)")
+                         "shape and load with 1,024 other cancellous fields)")
     args = ap.parse_args()
 
     if args.gpus != 1:

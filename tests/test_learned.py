@@ -422,13 +422,13 @@ def test_trained_j2_network_beats_the_same_mass_cold_start(gpu):
 
 def test_headline_family_network_beats_the_same_mass_cold_start(gpu):
     """The network of the headline's Monte Carlo family (the dx 0.4 vertebral
-    body's shape and 2 MPa axial load with 512 other cancellous fields,
+    body's shape and 2 MPa axial load with 1,024 other cancellous fields,
     simul_amd/learned/weights_vertebra_mc.npz, the bench's headline warm
     start) on 8 held-out fields, batched, and on the headline itself (seed
     2000, never used in training): fewer DR iterations than the same-mass
-    cold start to the same criterion, the same solution.  Measured: 1.22x in
-    iterations on 16 held-out fields, 1.26x on the headline (735 against 929
-    iterations)."""
+    cold start to the same criterion, the same solution.  Measured: 1.26x in
+    iterations on 16 held-out fields (every one at least 1.19x), 1.30x on the
+    headline (713 against 929 iterations)."""
     import os
     import numpy as np
     from simul_amd import workloads as W
