@@ -280,15 +280,25 @@ class PDBody:
 
         mass (adaptive DR): None keeps the context's mass (the Gershgorin mass
         unless set otherwise); "gershgorin" recomputes it; "spectral" scales it
-        to each case's spectrum first (tune_mass) and, should a case diverge
-        (relative residual above 1 or not finite at the end), solves again
-        from the same start with the Gershgorin mass."""
+        to each case's spectrum first (tune_mass, its beta kept as
+        ``last_beta``) and, should a case diverge (relative residual above 1 or
+        not finite at the end), solves again from the same start with the
+        Gershgorin mass; a number beta: the Gershgorin mass times beta (a beta
+        an earlier tuning of the same body found, reused without tuning),
+        and, should a case diverge, solves again from the same start with
+        mass="spectral"."""
         if mass is not None and method == "adaptive" and not resume:
-            if mass == "gershgorin":
+            reuse = not isinstance(mass, str)
+            if not reuse and mass == "gershgorin":
                 self.adr_mass(1.0)
-            elif mass == "spectral":
+            elif reuse or mass == "spectral":
                 u_start = self.get_u_device()
-                self.tune_mass()
+                if reuse:
+                    if not (float(mass) > 0.0):
+                        raise ValueError("a reused mass factor beta must be positive")
+                    self.adr_mass(float(mass))
+                else:
+                    self.last_beta = self.tune_mass()
 
                 def watch(done, rs):  # a diverging case ends the attempt at the slice
                     for x in (rs if isinstance(rs, list) else [rs]):
@@ -308,11 +318,14 @@ class PDBody:
                 except _Diverged:
                     pass
                 self.set_u_device(u_start)
+                if reuse:
+                    return self.solve(method, tol, max_iterations, ref, dt, viscous_damping, cmax, use_graph,
+                                      chunk, False, refs, progress, slice_iterations, mass="spectral")
                 self.adr_mass(1.0)
                 return self.solve(method, tol, max_iterations, ref, dt, viscous_damping, cmax, use_graph, chunk,
                                   False, refs, progress, slice_iterations)
             else:
-                raise ValueError(f"mass must be None, 'gershgorin' or 'spectral', not {mass!r}")
+                raise ValueError(f"mass must be None, 'gershgorin', 'spectral' or a beta, not {mass!r}")
         if slice_iterations is None:
             return self._solve_once(method, tol, max_iterations, ref, dt, viscous_damping, cmax, use_graph,
                                     chunk, resume, refs)

@@ -53,7 +53,13 @@ class IncrementalLoading:
         self.max_break_rounds = int(max_break_rounds)
         self.dt, self.viscous_damping = float(dt), float(viscous_damping)
         self.progress, self.slice_iterations = progress, slice_iterations
-        self.mass = mass  # adaptive-DR mass of every solve (PDBody.solve's mass=)
+        # adaptive-DR mass of the solves (PDBody.solve's mass=).  "spectral"
+        # tunes the first solve's mass to the spectrum and reuses its beta for
+        # the later solves (the elastic stiffness, which bounds the spectrum,
+        # does not grow with plastic flow; a solve that diverges with the
+        # reused beta tunes again)
+        self.mass = mass
+        self._beta = None
         # start of an increment: None = the previous state with the new
         # boundary values; "secant" = 2 u_(k-1) - u_(k-2) from the second
         # increment on, u_0 being the state the loading starts from; ``start``
@@ -78,10 +84,15 @@ class IncrementalLoading:
 
     def _solve(self, ref=0.0):
         b = self.body
-        return b.solve(self.method, tol=self.tol, max_iterations=self.max_iterations, ref=ref, dt=self.dt,
-                       viscous_damping=self.viscous_damping, progress=self.progress,
-                       slice_iterations=self.slice_iterations if self.progress is not None else None,
-                       mass=self.mass if self.method == "adaptive" else None)
+        mass = self.mass if self.method == "adaptive" else None
+        reuse = mass == "spectral" and self._beta is not None
+        r = b.solve(self.method, tol=self.tol, max_iterations=self.max_iterations, ref=ref, dt=self.dt,
+                    viscous_damping=self.viscous_damping, progress=self.progress,
+                    slice_iterations=self.slice_iterations if self.progress is not None else None,
+                    mass=self._beta if reuse else mass)
+        if mass == "spectral":
+            self._beta = getattr(b, "last_beta", None)
+        return r
 
     def step(self, k: int) -> IncrementResult:
         """Increment k (1-based)."""
