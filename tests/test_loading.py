@@ -227,3 +227,31 @@ def test_secant_from_the_start_state_and_learned_first_increment(gpu):
     assert runs["secant"][0][1] < runs["plain"][0][1]
     for key in ("secant", "secant+start"):
         assert rel_l2(runs[key][1], runs["plain"][1]) < 1e-4, key
+
+
+def test_reused_mass_factor(gpu):
+    """solve(mass=beta) with the beta a spectral tuning returned runs the
+    spectral solve's iterates bit for bit (beta x the cached Gershgorin mass
+    rounds as the tuning's scaling does); a beta far below the stability
+    limit diverges and the solve falls back to a fresh spectral tuning, which
+    converges to the same state."""
+    spec = W.vertebra_j2(dx=1.25, n_increments=4)
+    idx, uf = spec.extra["top_idx"], spec.extra["u_final"]
+    body = from_specimen(spec)
+    u = spec.u0.copy()
+    u[idx] = 0.25 * uf
+    body.u = u
+    cold = body.cold_residual()
+    body.u = u
+    r1 = body.solve("adaptive", tol=1e-6, ref=cold, mass="spectral", slice_iterations=512)
+    u1 = body.u.copy()
+    beta = body.last_beta
+    assert 0.0 < beta <= 1.0
+    body.u = u
+    r2 = body.solve("adaptive", tol=1e-6, ref=cold, mass=beta, slice_iterations=512)
+    assert r2.iterations == r1.iterations and np.array_equal(body.u, u1)
+    body.u = u
+    r3 = body.solve("adaptive", tol=1e-6, ref=cold, mass=beta * 1e-3, slice_iterations=512)
+    assert r3.converged and rel_l2(body.u, u1) < 1e-4
+    with pytest.raises(ValueError):
+        body.solve("adaptive", tol=1e-6, ref=cold, mass=-1.0)
