@@ -28,7 +28,7 @@ event means of its hot kernels against the same per-unit model; J2 adds
                       20 increments solved to 1e-6 with bond breaking
   batch64_dx0.5       64 random specimens (7,045,942 particles) in one context,
                       every case solved to 1e-6 (setup included in the
-                      reported time-to-solve)
+                      reported time-to-solve), cold and warm-started
   warmstart_j2_...    the learned warm start: 64 held-out J2 vertebral bodies
                       (dx 0.75) in one context, cold and warm-started solves
                       to 1e-6 with the same DR mass
@@ -475,6 +475,40 @@ def run_batch(args, torch):
                "iterations_min": min(its), "iterations_max": max(its), "setup_s": round(setup, 3)}
         log(f"time-to-solve (all cases): {wall:.2f}s (+ setup {setup:.2f}s), iterations {min(its)}..{max(its)}, "
             f"{tts['converged']}/{len(rs)} converged")
+        wfile = os.path.join(ROOT, "simul_amd", "learned", "weights_batch.npz")
+        if os.path.exists(wfile):
+            # the warm-started batch: the network of the batch's family (other
+            # seeds) predicts every case's start, the same DR mass, the same
+            # criterion (each case's cold-start residual)
+            import numpy as np
+
+            from simul_amd.learned import WarmStartNet
+            from simul_amd.learned.warmstart import predict_u_device
+            net = WarmStartNet.load(wfile)
+            u_cold = body.u
+            predict_u_device(net, body, packed, off)  # loads the path's GPU code (untimed, nothing kept)
+            refs = np.atleast_1d(body.cold_residual())
+            body.u = packed.u0
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            body.set_u_device(predict_u_device(net, body, packed, off))
+            torch.cuda.synchronize()
+            t_inf = time.perf_counter() - t0
+            rw = body.solve("adaptive", tol=args.tol, max_iterations=args.max_solve_iters, refs=refs, chunk=128,
+                            slice_iterations=2048, progress=progress, mass=args.dr_mass)
+            torch.cuda.synchronize()
+            wall_w = time.perf_counter() - t0
+            iw = [r.iterations for r in rw]
+            du = max(float(np.linalg.norm(body.u[a:b] - u_cold[a:b]) / np.linalg.norm(u_cold[a:b]))
+                     for a, b in zip(off[:-1], off[1:]))
+            tts.update({"warm_all_cases_s": round(wall_w, 3), "warm_inference_s": round(t_inf, 4),
+                        "warm_speedup": round(wall / wall_w, 3), "warm_iterations_max": max(iw),
+                        "warm_iteration_ratio_mean": round(float(np.mean(its)) / float(np.mean(iw)), 3),
+                        "warm_converged": sum(r.converged for r in rw), "warm_vs_cold_u_rel_l2_max": du,
+                        "warm_start": "network of the batch's family (simul_amd/learned/weights_batch.npz, "
+                                      "1,024 other random specimens), features, inference and DR included"})
+            log(f"warm-started batch: {wall_w:.2f}s incl. {t_inf:.3f}s features+inference, iterations max {max(iw)} "
+                f"({wall / wall_w:.2f}x wall)")
     body.u = packed.u0
     roof = hot_roofline(body, packed.n, nb, args.kernel_iters)
     log(f"kernels: {roof['kernels']}")

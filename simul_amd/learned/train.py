@@ -37,7 +37,7 @@ def _group_body(specs):
     return packed, off, body
 
 
-FAMILIES = ("pressure", "j2", "vertebra", "vertebra_mc")
+FAMILIES = ("pressure", "j2", "vertebra", "vertebra_mc", "batch")
 
 
 def _specimen(family, seed, dx):
@@ -46,14 +46,18 @@ def _specimen(family, seed, dx):
     workloads.random_vertebra_j2 (J2, compressed between platens in one
     increment), "vertebra" workloads.vertebra_family (the headline's shape,
     random cancellous field and load), "vertebra_mc" the same with the
-    headline's load (only the cancellous field random)."""
-    from ..workloads import random_vertebra, random_vertebra_j2, vertebra_family
+    headline's load (only the cancellous field random), "batch"
+    workloads.batch_family (the random boxes, cylinders and truncated
+    ellipsoids of the 64-case batch, other seeds)."""
+    from ..workloads import batch_family, random_vertebra, random_vertebra_j2, vertebra_family
     if family == "j2":
         return random_vertebra_j2(int(seed), dx=dx)
     if family == "vertebra":
         return vertebra_family(int(seed), dx=dx)
     if family == "vertebra_mc":
         return vertebra_family(int(seed), dx=dx, fixed_load=True)
+    if family == "batch":
+        return batch_family(int(seed), dx=dx)
     if family == "pressure":
         return random_vertebra(int(seed), dx=dx)
     raise ValueError(f"unknown family {family!r}")
@@ -200,7 +204,8 @@ def main(argv=None):
     ap.add_argument("--family", default="pressure", choices=FAMILIES,
                     help="pressure: Neo-Hookean bodies under a top pressure; j2: J2 bodies compressed between platens; "
                          "vertebra: the headline's shape with random cancellous fields and loads; "
-                         "vertebra_mc: the headline's shape and load with random cancellous fields")
+                         "vertebra_mc: the headline's shape and load with random cancellous fields; "
+                         "batch: random boxes, cylinders and truncated ellipsoids of 10-200 k particles")
     ap.add_argument("--ref-tol", type=float, default=None,
                     help="also solve each held-out group cold to this tolerance and report both solutions' distance")
     ap.add_argument("--out", default=None)

@@ -203,6 +203,14 @@ def batch(n_cases=64, seed=4000, n_lo=10_000, n_hi=200_000, dx=0.5, model="neo_h
     return [random_specimen(seed + 1 + k, int(sizes[k]), dx=dx, model=model) for k in range(n_cases)]
 
 
+def batch_family(seed, dx=0.5, n_lo=10_000, n_hi=200_000):
+    """One case of the 64-case batch's family: random_specimen with a size
+    drawn uniformly from [n_lo, n_hi] particles (the batch itself is
+    batch(seed=4000): cases 4001-4064)."""
+    n = int(np.random.default_rng([seed, 7]).integers(n_lo, n_hi + 1))
+    return random_specimen(seed, n, dx=dx)
+
+
 def pack(specs):
     """Several specimens of one spacing packed caller-order into one batched
     input: the concatenated specimen and the case offsets (ncase + 1)."""
