@@ -231,14 +231,14 @@ class PDBody:
     def adr_mass(self, beta=1.0):
         self.ctx.call("pdx_adr_mass", float(beta))
 
-    def tune_mass(self, iters=20, safety=1.4):
+    def tune_mass(self, iters=20, safety=1.2):
         """Adaptive-DR mass scaled to the spectrum of each case
         (pdx_adr_mass_auto); returns beta per case (a float for one case).
-        safety 1.4: a 1.4 margin on the estimated stability limit (DR
-        converged with every factor down to 1.2 on the dx 0.4 vertebral body,
-        the 64-case batch and 128 random vertebral bodies, and diverges on the
-        vertebra near 0.9, beta 0.07); solve(mass="spectral") falls back to the
-        Gershgorin mass should a case diverge all the same."""
+        safety 1.2: a 1.2 margin on the estimated stability limit (DR
+        converged with every factor down to 1.1 on the dx 0.4 vertebral body,
+        the 64-case batch and 400 cases of the warm start's families, and
+        diverges on the vertebra near 0.9, beta 0.07); solve(mass="spectral")
+        falls back to the Gershgorin mass should a case diverge all the same."""
         b = np.zeros(self.ncase)
         self.ctx.call("pdx_adr_mass_auto", int(iters), float(safety), N.ptr(b, N._dp))
         return b if self.ncase > 1 else float(b[0])
