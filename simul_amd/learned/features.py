@@ -72,7 +72,7 @@ def _platens(spec, off):
     rows, u0r = rows[moving], u0r[moving]
     case = np.searchsorted(off, rows, side="right") - 1
     cnt = np.bincount(case, minlength=ncase)
-    um = np.stack([np.bincount(case, weights=u0r[:, q], minlength=ncase) for q in range(3)], 1)
+    um = np.stack([np.bincount(case, weights=u0r[:, q], minlength=ncase) for q in range(u0r.shape[1])], 1)
     return rows, case, cnt, um / np.maximum(cnt, 1)[:, None]
 
 
