@@ -529,8 +529,9 @@ def run_warmstart(args, torch):
     between platens in one increment to 0.1-2 %; seeds never used in
     training) packed into one context, each solved by adaptive DR to 1e-6 of
     its cold-start residual from rest (cold) and from the network's
-    prediction (warm), the same DR mass for both.  value = intact bonds x
-    batched iterations / DR wall time of the warm solve."""
+    prediction (warm), the same DR mass for both.  value = bond updates of
+    the warm solve (each case's share of the intact bonds times its
+    iterations) / its DR wall time."""
     import numpy as np
 
     from simul_amd.learned import WarmStartNet
@@ -595,7 +596,12 @@ def run_warmstart(args, torch):
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline_cases(specs, args.cpu_seconds, what="the first held-out J2 cases")
-    return _line(METRIC, nb * max(iw) / (t_warm - t_inf), "bond-updates/s", max(iw), 0,
+    # bond updates of the warm solve: each case's bonds (its share of the
+    # particles, one spacing) times its own iterations, converged cases
+    # leaving the batched launches
+    sizes = np.diff(off)
+    upd = nb * float(np.dot(sizes, iw)) / float(sizes.sum())
+    return _line(METRIC, upd / (t_warm - t_inf), "bond-updates/s", max(iw), 0,
                  1e3 * (t_warm - t_inf) / max(max(iw), 1),
                  {"workload": f"warmstart_j2_dx{dx}_64heldout", "cases": len(specs), "n_particles": packed.n,
                   "n_bonds": nb, "variant": variant},
