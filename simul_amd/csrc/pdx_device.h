@@ -1039,6 +1039,10 @@ PDX_HD double det3(const double *M)
            M[2] * (M[3] * M[7] - M[4] * M[6]);
 }
 
+// bonds whose data the FP64 bond loops of the set-up passes (shape tensor,
+// adaptive-DR mass) gather at a time
+#define PDX_MASS_BATCH 4
+
 // shape tensor K = sum_intact w xi (x) xi V_j in FP64 with its inverse
 // (nosb.py:59-94); mode 0: |det| > tol, mode 1: |det| > tol (tr K/dim)^dim;
 // also m_i = sum_intact w V_j xi for the force pass
@@ -1054,25 +1058,38 @@ PDX_HD void shape_one(const Dev &d, int i, int mode, double tol)
     double K[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     double m[3] = {0, 0, 0};
     const int64_t base = slot0(d, i);
-    for (int k = 0; k < (int)kr.nnb; ++k) {
-        const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
-        if (e & PDX_BROKEN) continue;
-        const XRec pj = d.xu[0][(int)e];
-        double xi[3] = {0, 0, 0}, s = 0.0;
-        for (int q = 0; q < D; ++q) {
-            xi[q] = pj.X[q] - pi.X[q];
-            s = s + xi[q] * xi[q];
+    const int nnb = (int)kr.nnb;
+    // PDX_MASS_BATCH bonds per step, their slots, positions and volumes
+    // requested before any is used (a broken bond or one past the row reads
+    // the particle's own record and is skipped); the sums keep the bond order
+    for (int k0 = 0; k0 < nnb; k0 += PDX_MASS_BATCH) {
+        uint32_t e[PDX_MASS_BATCH];
+        for (int b = 0; b < PDX_MASS_BATCH; ++b)
+            e[b] = k0 + b < nnb ? d.bj[base + (int64_t)(k0 + b) * PDX_SLICE] : PDX_BROKEN;
+        double X[PDX_MASS_BATCH][3], Vb[PDX_MASS_BATCH];
+        for (int b = 0; b < PDX_MASS_BATCH; ++b) {
+            const int j = (e[b] & PDX_BROKEN) ? i : (int)e[b];
+            for (int q = 0; q < 3; ++q) X[b][q] = d.xu[0][j].X[q];
+            Vb[b] = d.V64[j];
         }
-        const double L = sqrt(s);
-        double w = 1.0;
-        if (L > 0) {
-            w = 1.0 - L / d.h;
-            w = w > 0.0 ? w : 0.0;
-        }
-        const double Vj = d.V64[(int)e];
-        for (int r = 0; r < D; ++r) {
-            for (int c = 0; c < D; ++c) K[r * D + c] += w * xi[r] * xi[c] * Vj;
-            m[r] += w * Vj * xi[r];
+        for (int b = 0; b < PDX_MASS_BATCH; ++b) {
+            if (e[b] & PDX_BROKEN) continue;
+            double xi[3] = {0, 0, 0}, s = 0.0;
+            for (int q = 0; q < D; ++q) {
+                xi[q] = X[b][q] - pi.X[q];
+                s = s + xi[q] * xi[q];
+            }
+            const double L = sqrt(s);
+            double w = 1.0;
+            if (L > 0) {
+                w = 1.0 - L / d.h;
+                w = w > 0.0 ? w : 0.0;
+            }
+            const double Vj = Vb[b];
+            for (int r = 0; r < D; ++r) {
+                for (int c = 0; c < D; ++c) K[r * D + c] += w * xi[r] * xi[c] * Vj;
+                m[r] += w * Vj * xi[r];
+            }
         }
     }
     double tr = 0.0;
@@ -1127,26 +1144,40 @@ PDX_HD void adr_mass_a_one(const Dev &d, int i, double *cs_out, double *g_out)
     const double *Ki = d.Kinv64 + (int64_t)i * 9;
     double G[3] = {0, 0, 0}, sabs = 0.0;
     const int64_t base = slot0(d, i);
-    for (int k = 0; k < (int)kr.nnb; ++k) {
-        const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
-        if (e & PDX_BROKEN) continue;
-        const XRec pj = d.xu[0][(int)e];
-        double xi[3] = {0, 0, 0}, s = 0.0;
-        for (int q = 0; q < D; ++q) {
-            xi[q] = pj.X[q] - pi.X[q];
-            s += xi[q] * xi[q];
+    const int nnb = (int)kr.nnb;
+    // PDX_MASS_BATCH bonds per step: their slots, then their positions and
+    // volumes, are all requested before any is used (a broken bond or one
+    // past the row reads the particle's own record and is skipped); the
+    // sums keep the bond order
+    for (int k0 = 0; k0 < nnb; k0 += PDX_MASS_BATCH) {
+        uint32_t e[PDX_MASS_BATCH];
+        for (int b = 0; b < PDX_MASS_BATCH; ++b)
+            e[b] = k0 + b < nnb ? d.bj[base + (int64_t)(k0 + b) * PDX_SLICE] : PDX_BROKEN;
+        double X[PDX_MASS_BATCH][3], Vj[PDX_MASS_BATCH];
+        for (int b = 0; b < PDX_MASS_BATCH; ++b) {
+            const int j = (e[b] & PDX_BROKEN) ? i : (int)e[b];
+            for (int q = 0; q < 3; ++q) X[b][q] = d.xu[0][j].X[q];
+            Vj[b] = d.V64[j];
         }
-        const double L = sqrt(s);
-        const double w = L > 0 ? fmax(0.0, 1.0 - L / d.h) : 1.0;
-        double gn = 0.0;
-        for (int r = 0; r < D; ++r) {
-            double t = 0.0;
-            for (int c = 0; c < D; ++c) t += Ki[r * D + c] * xi[c];
-            t *= w * d.V64[(int)e];
-            G[r] += t;
-            gn += t * t;
+        for (int b = 0; b < PDX_MASS_BATCH; ++b) {
+            if (e[b] & PDX_BROKEN) continue;
+            double xi[3] = {0, 0, 0}, s = 0.0;
+            for (int q = 0; q < D; ++q) {
+                xi[q] = X[b][q] - pi.X[q];
+                s += xi[q] * xi[q];
+            }
+            const double L = sqrt(s);
+            const double w = L > 0 ? fmax(0.0, 1.0 - L / d.h) : 1.0;
+            double gn = 0.0;
+            for (int r = 0; r < D; ++r) {
+                double t = 0.0;
+                for (int c = 0; c < D; ++c) t += Ki[r * D + c] * xi[c];
+                t *= w * Vj[b];
+                G[r] += t;
+                gn += t * t;
+            }
+            sabs += sqrt(gn);
         }
-        sabs += sqrt(gn);
     }
     double gl = 0.0;
     for (int r = 0; r < D; ++r) gl += G[r] * G[r];
@@ -1172,29 +1203,43 @@ PDX_HD void adr_mass_b_one(const Dev &d, int i, const double *cs_in, const doubl
     const bool nosb = d.form == 0;
     double kc = nosb ? cs_in[i] * g_in[i] : 0.0, ks = 0.0;
     const int64_t base = slot0(d, i);
-    for (int k = 0; k < (int)kr.nnb; ++k) {
-        const uint32_t e = d.bj[base + (int64_t)k * PDX_SLICE];
-        if (e & PDX_BROKEN) continue;
-        const int j = (int)e;
-        const XRec pj = d.xu[0][j];
-        double xi[3] = {0, 0, 0}, s = 0.0;
-        for (int q = 0; q < D; ++q) {
-            xi[q] = pj.X[q] - pi.X[q];
-            s += xi[q] * xi[q];
+    const int nnb = (int)kr.nnb;
+    // PDX_MASS_BATCH bonds per step, their data requested before any is used
+    // (as in adr_mass_a_one); the sums keep the bond order
+    for (int k0 = 0; k0 < nnb; k0 += PDX_MASS_BATCH) {
+        uint32_t e[PDX_MASS_BATCH];
+        for (int b = 0; b < PDX_MASS_BATCH; ++b)
+            e[b] = k0 + b < nnb ? d.bj[base + (int64_t)(k0 + b) * PDX_SLICE] : PDX_BROKEN;
+        double X[PDX_MASS_BATCH][3], Vj[PDX_MASS_BATCH], csj[PDX_MASS_BATCH], Kj[PDX_MASS_BATCH][9];
+        float cj[PDX_MASS_BATCH];
+        for (int b = 0; b < PDX_MASS_BATCH; ++b) {
+            const int j = (e[b] & PDX_BROKEN) ? i : (int)e[b];
+            for (int q = 0; q < 3; ++q) X[b][q] = d.xu[0][j].X[q];
+            cj[b] = d.xu[0][j].c;
+            Vj[b] = d.V64[j];
+            csj[b] = nosb ? cs_in[j] : 0.0;
+            for (int q = 0; q < D * D; ++q) Kj[b][q] = nosb ? d.Kinv64[(int64_t)j * 9 + q] : 0.0;
         }
-        const double L = sqrt(s);
-        const double w = L > 0 ? fmax(0.0, 1.0 - L / d.h) : 1.0;
-        if (nosb) {
-            const double *Kj = d.Kinv64 + (int64_t)j * 9;
-            double gn = 0.0;
-            for (int r = 0; r < D; ++r) {
-                double t = 0.0;
-                for (int c = 0; c < D; ++c) t -= Kj[r * D + c] * xi[c];
-                gn += t * t;
+        for (int b = 0; b < PDX_MASS_BATCH; ++b) {
+            if (e[b] & PDX_BROKEN) continue;
+            double xi[3] = {0, 0, 0}, s = 0.0;
+            for (int q = 0; q < D; ++q) {
+                xi[q] = X[b][q] - pi.X[q];
+                s += xi[q] * xi[q];
             }
-            kc += w * d.V64[j] * cs_in[j] * sqrt(gn);
+            const double L = sqrt(s);
+            const double w = L > 0 ? fmax(0.0, 1.0 - L / d.h) : 1.0;
+            if (nosb) {
+                double gn = 0.0;
+                for (int r = 0; r < D; ++r) {
+                    double t = 0.0;
+                    for (int c = 0; c < D; ++c) t -= Kj[b][r * D + c] * xi[c];
+                    gn += t * t;
+                }
+                kc += w * Vj[b] * csj[b] * sqrt(gn);
+            }
+            if (L > 1e-10) ks += 0.5 * ((double)pi.c + (double)cj[b]) * w * Vj[b] / L;
         }
-        if (L > 1e-10) ks += 0.5 * ((double)pi.c + (double)pj.c) * w * d.V64[j] / L;
     }
     const double gs = nosb ? d.G_s : 1.0;
     const double r3 = 1.7320508075688772;
@@ -1270,32 +1315,46 @@ PDX_HD unsigned damage_one(const Dev &d, int i, int ubuf, double s_c)
     const double xi0[3] = {pi.X[0] + pi.u[0], pi.X[1] + pi.u[1], pi.X[2] + pi.u[2]};
     const int64_t base = slot0(d, i);
     unsigned nb = 0;
-    for (int k = 0; k < (int)kr.nnb; ++k) {
-        const int64_t a = base + (int64_t)k * PDX_SLICE;
-        const uint32_t e = d.bj[a];
-        if (e & PDX_BROKEN) continue;
-        const int j = (int)e;
-        const XRec pj = X[j];
-        const double xj[3] = {pj.X[0] + pj.u[0], pj.X[1] + pj.u[1], pj.X[2] + pj.u[2]};
-        double se = 0.0, sl = 0.0;
-        for (int q = 0; q < D; ++q) {
-            const double t = xj[q] - xi0[q];
-            se = se + t * t;
-            const double r = pj.X[q] - pi.X[q];
-            sl = sl + r * r;
-        }
-        const double el = sqrt(se), L = sqrt(sl);
-        double st = 0.0;
-        if (L > 1e-10) st = (el - L) / L;
-        if (st > s_c) {
-            d.bj[a] = e | PDX_BROKEN;
-            if (d.bl) {
-                d.bl[a] |= PDX_HBROKEN;
-                const uint16_t own = d.hs[d.hl_ptr[i / PDX_BLOCK] + i % PDX_BLOCK];
-                d.bk[a] = own;
-                if (d.bks) d.bks[lat_at(i, d.lk[a])] = own;
+    const int nnb = (int)kr.nnb;
+    // PDX_MASS_BATCH bonds per step, their slots and the neighbours' X and u
+    // requested before any is used (a broken bond or one past the row reads
+    // the particle's own record and is skipped)
+    for (int k0 = 0; k0 < nnb; k0 += PDX_MASS_BATCH) {
+        uint32_t e[PDX_MASS_BATCH];
+        for (int b = 0; b < PDX_MASS_BATCH; ++b)
+            e[b] = k0 + b < nnb ? d.bj[base + (int64_t)(k0 + b) * PDX_SLICE] : PDX_BROKEN;
+        double Xj[PDX_MASS_BATCH][3], Uj[PDX_MASS_BATCH][3];
+        for (int b = 0; b < PDX_MASS_BATCH; ++b) {
+            const int j = (e[b] & PDX_BROKEN) ? i : (int)e[b];
+            for (int q = 0; q < 3; ++q) {
+                Xj[b][q] = X[j].X[q];
+                Uj[b][q] = X[j].u[q];
             }
-            nb++;
+        }
+        for (int b = 0; b < PDX_MASS_BATCH; ++b) {
+            if (e[b] & PDX_BROKEN) continue;
+            const int64_t a = base + (int64_t)(k0 + b) * PDX_SLICE;
+            const double xj[3] = {Xj[b][0] + Uj[b][0], Xj[b][1] + Uj[b][1], Xj[b][2] + Uj[b][2]};
+            double se = 0.0, sl = 0.0;
+            for (int q = 0; q < D; ++q) {
+                const double t = xj[q] - xi0[q];
+                se = se + t * t;
+                const double r = Xj[b][q] - pi.X[q];
+                sl = sl + r * r;
+            }
+            const double el = sqrt(se), L = sqrt(sl);
+            double st = 0.0;
+            if (L > 1e-10) st = (el - L) / L;
+            if (st > s_c) {
+                d.bj[a] = e[b] | PDX_BROKEN;
+                if (d.bl) {
+                    d.bl[a] |= PDX_HBROKEN;
+                    const uint16_t own = d.hs[d.hl_ptr[i / PDX_BLOCK] + i % PDX_BLOCK];
+                    d.bk[a] = own;
+                    if (d.bks) d.bks[lat_at(i, d.lk[a])] = own;
+                }
+                nb++;
+            }
         }
     }
     return nb;
