@@ -915,7 +915,11 @@ extern "C" int pdx_build_bonds(pdx_ctx *c, double horizon, int64_t *n_bonds)
     PDX_TRY(dalloc(c->stream, &c->d.bj, (size_t)std::max<int64_t>(c->slots, 1)));
     PDX_HIP(hipMemsetAsync(c->d.bj, 0xff, sizeof(uint32_t) * (size_t)std::max<int64_t>(c->slots, 1), c->stream));
     PDX_HIP(hipMemsetAsync(c->d.init_nb, 0, sizeof(int32_t) * c->ni, c->stream));
-    PDX_TRY(pdx_launch_fill(c));
+    uint64_t *keys = nullptr;  // the fill's sort keys, one per bond slot
+    PDX_HIP(hipMallocAsync((void **)&keys, sizeof(uint64_t) * (size_t)std::max<int64_t>(c->slots, 1), c->stream));
+    const int rf = pdx_launch_fill(c, keys);
+    hipFreeAsync(keys, c->stream);
+    PDX_TRY(rf);
     std::vector<int32_t> filled;
     PDX_TRY(download(c, filled, c->d.init_nb, c->ni));
     for (int i = 0; i < c->ni; ++i)

@@ -192,6 +192,7 @@ struct Dev {
     int32_t *cell_start; // per cell start into cell_idx (+1 sentinel per case grid)
     int32_t *cell_idx;
     int32_t *counts;
+    uint64_t *nkey;       // neighbour fill only: each row's bond sort keys, in its bond slots
     double h, inv_h, h2;  // horizon, 1/horizon, horizon^2
     double G_s;
     int32_t model, form;
@@ -947,9 +948,8 @@ PDX_HD void kd_move_one(const Dev &d, int i, int ubuf_in, double dt, bool reset)
 // ---------------------------------------------------------------------------
 // sort key of a bond: quantised (z, y, x) offset in 10 bits each, then the index
 template <int D>
-PDX_HD uint64_t stencil_key(const Dev &d, const XRec &pi, uint32_t j, double hk)
+PDX_HD uint64_t stencil_key(const XRec &pi, const XRec &pj, uint32_t j, double hk)
 {
-    const XRec pj = d.xu[0][j];
     uint64_t key = 0;
     for (int q = D - 1; q >= 0; --q) {
         long long t = llround((pj.X[q] - pi.X[q]) * hk) + 512;
@@ -984,6 +984,7 @@ PDX_HD void neighbors_one(const Dev &d, int i)
     int cnt = 0;
     int64_t base = 0;
     if (FILL) base = slot0(d, i);
+    const double hk = 256.0 / sqrt(d.h2);
     for (int a = c[0] - 1; a <= c[0] + 1; ++a) {
         if (a < 0 || a >= g.nc[0]) continue;
         for (int b = c[1] - 1; b <= c[1] + 1; ++b) {
@@ -1002,7 +1003,7 @@ PDX_HD void neighbors_one(const Dev &d, int i)
                         s = s + t * t;
                     }
                     if (s < d.h2) {
-                        if (FILL) d.bj[base + (int64_t)cnt * PDX_SLICE] = (uint32_t)j;
+                        if (FILL) d.nkey[base + (int64_t)cnt * PDX_SLICE] = stencil_key<D>(pi, pj, (uint32_t)j, hk);
                         cnt++;
                     }
                 }
@@ -1016,20 +1017,22 @@ PDX_HD void neighbors_one(const Dev &d, int i)
     // rows in stencil order: by the bond vector quantised to horizon/256
     // (z, then y, then x), ties by index -- on a lattice every interior
     // particle lists its bonds in the same order, so the k-th gathers of a
-    // wave land on one shifted Morton brick
-    const double hk = 256.0 / sqrt(d.h2);
+    // wave land on one shifted Morton brick.  The keys (index in the low 31
+    // bits, so no two are equal) were stored as the scan found the bonds;
+    // insertion sort, then the indices into the row
+    uint64_t *__restrict__ kp = d.nkey + base;
     for (int p = 1; p < cnt; ++p) {
-        const uint32_t ej = d.bj[base + (int64_t)p * PDX_SLICE];
-        const uint64_t key = stencil_key<D>(d, pi, ej, hk);
+        const uint64_t key = kp[(int64_t)p * PDX_SLICE];
         int q = p - 1;
         while (q >= 0) {
-            const uint32_t eq = d.bj[base + (int64_t)q * PDX_SLICE];
-            if (stencil_key<D>(d, pi, eq, hk) <= key) break;
-            d.bj[base + (int64_t)(q + 1) * PDX_SLICE] = eq;
+            const uint64_t kq = kp[(int64_t)q * PDX_SLICE];
+            if (kq <= key) break;
+            kp[(int64_t)(q + 1) * PDX_SLICE] = kq;
             --q;
         }
-        d.bj[base + (int64_t)(q + 1) * PDX_SLICE] = ej;
+        kp[(int64_t)(q + 1) * PDX_SLICE] = key;
     }
+    for (int p = 0; p < cnt; ++p) d.bj[base + (int64_t)p * PDX_SLICE] = (uint32_t)(kp[(int64_t)p * PDX_SLICE] & 0x7fffffffu);
     d.init_nb[i] = cnt;
 }
 

@@ -77,7 +77,8 @@ void pdx_set_error(const std::string &msg);
 
 // kernel launchers (pdx_kernels.hip) ---------------------------------------------
 int pdx_launch_count(pdx_ctx *c, int32_t *d_counts);
-int pdx_launch_fill(pdx_ctx *c);
+// the row fill; d_keys: one 8-byte sort key per bond slot (c->slots)
+int pdx_launch_fill(pdx_ctx *c, uint64_t *d_keys);
 int pdx_launch_shape(pdx_ctx *c, int mode, double tol);
 // red_ubuf >= 0: the adaptive-DR case reduction of the force pass that read buffer red_ubuf
 // runs first (fused into the halo F-pass where it runs)

@@ -1604,12 +1604,14 @@ int pdx_launch_count(pdx_ctx *c, int32_t *d_counts)
     return PDX_OK;
 }
 
-int pdx_launch_fill(pdx_ctx *c)
+int pdx_launch_fill(pdx_ctx *c, uint64_t *d_keys)
 {
+    Dev d = c->d;
+    d.nkey = d_keys;
     if (c->dim == 3)
-        hipLaunchKernelGGL((k_neighbors<3, true>), dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d);
+        hipLaunchKernelGGL((k_neighbors<3, true>), dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, d);
     else
-        hipLaunchKernelGGL((k_neighbors<2, true>), dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, c->d);
+        hipLaunchKernelGGL((k_neighbors<2, true>), dim3(grid1(c->ni)), dim3(PDX_BLOCK), 0, c->stream, d);
     PDX_LAUNCH_CHECK();
     return PDX_OK;
 }

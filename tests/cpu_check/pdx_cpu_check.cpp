@@ -255,6 +255,7 @@ int main(int argc, char **argv)
     memcpy(d.sl_ptr, slptr.data(), sizeof(int64_t) * slptr.size());
     d.bj = alloc<uint32_t>(R.slots);
     memset(d.bj, 0xff, sizeof(uint32_t) * (R.slots > 0 ? R.slots : 1));
+    d.nkey = alloc<uint64_t>(R.slots);
     for (int i = 0; i < ni; ++i) {
         if (D == 3) neighbors_one<3, true>(d, i);
         else neighbors_one<2, true>(d, i);
