@@ -1044,7 +1044,9 @@ PDX_HD double det3(const double *M)
 
 // bonds whose data the FP64 bond loops of the set-up passes (shape tensor,
 // adaptive-DR mass) gather at a time
-#define PDX_MASS_BATCH 4
+#ifndef PDX_MASS_BATCH
+#define PDX_MASS_BATCH 8
+#endif
 
 // shape tensor K = sum_intact w xi (x) xi V_j in FP64 with its inverse
 // (nosb.py:59-94); mode 0: |det| > tol, mode 1: |det| > tol (tr K/dim)^dim;
